@@ -1,0 +1,330 @@
+// libfgx: context, errors, memory/stream/graph wrappers, fills, ID assignment,
+// spatial metadata and scratch sizing.
+#include "fgx_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+#include <limits>
+
+namespace fgx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int make_var_table(uint32_t nvars, const void* const* in, void* const* out, const uint32_t* bytes, VarTable* t) {
+    if (nvars > FGX_MAX_VARS) return fail(FGX_ERR_ARG, "too many variables (max " + std::to_string(FGX_MAX_VARS) + ")");
+    std::memset(t, 0, sizeof(*t));
+    t->n = nvars;
+    for (uint32_t v = 0; v < nvars; ++v) {
+        if (!bytes[v]) return fail(FGX_ERR_ARG, "variable with zero element size");
+        t->bytes[v] = bytes[v];
+        t->in[v] = static_cast<const char*>(in ? in[v] : nullptr);
+        t->out[v] = static_cast<char*>(out ? out[v] : nullptr);
+    }
+    return FGX_OK;
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ void k_fill_u32(uint32_t* __restrict__ dst, uint32_t value, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = value;
+}
+
+struct FillArgs {
+    uint32_t nvars;
+    uint32_t n;
+    uint32_t bytes[FGX_MAX_VARS];
+    char* ptr[FGX_MAX_VARS];
+    uint32_t words[FGX_MAX_VARS][4];  // default value, up to 16 bytes per element
+};
+
+// One launch fills every variable (grid.y indexes the variable).
+__global__ void k_fill_vars(FillArgs a) {
+    const uint32_t v = blockIdx.y;
+    const uint32_t b = a.bytes[v];
+    char* base = a.ptr[v];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+        char* d = base + (size_t)i * b;
+        if ((b & 3u) == 0) {
+            for (uint32_t w = 0; w < b / 4; ++w) reinterpret_cast<uint32_t*>(d)[w] = a.words[v][w & 3];
+        } else {
+            const char* src = reinterpret_cast<const char*>(a.words[v]);
+            for (uint32_t w = 0; w < b; ++w) d[w] = src[w & 15];
+        }
+    }
+}
+
+__global__ void k_assign_ids(uint32_t* __restrict__ ids, uint32_t n, uint32_t next_id) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && ids[i] == 0u) ids[i] = next_id + i;
+}
+
+}  // namespace fgx
+
+using namespace fgx;
+
+struct fgx_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+const char* fgx_last_error(void) { return g_last_error.c_str(); }
+
+int fgx_ctx_create(int device, fgx_ctx** out) {
+    if (!out) return fail(FGX_ERR_ARG, "fgx_ctx_create: out is NULL");
+    int count = 0;
+    FGX_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(FGX_ERR_ARG, "fgx_ctx_create: device " + std::to_string(device) + " out of range");
+    FGX_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    FGX_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(FGX_ERR_ARG, std::string("fgx_ctx_create: built for gfx950, device is ") + prop.gcnArchName);
+    auto* c = new fgx_ctx();
+    c->c.device = device;
+    c->c.cu_count = prop.multiProcessorCount;
+    c->c.hbm_bytes = prop.totalGlobalMem;
+    c->c.xcd_count = 8;
+    *out = c;
+    return FGX_OK;
+}
+
+int fgx_ctx_destroy(fgx_ctx* ctx) {
+    delete ctx;
+    return FGX_OK;
+}
+
+int fgx_device_info(fgx_ctx* ctx, int* cu_count, int* xcd_count, size_t* hbm_bytes) {
+    if (!ctx) return fail(FGX_ERR_ARG, "fgx_device_info: NULL context");
+    if (cu_count) *cu_count = ctx->c.cu_count;
+    if (xcd_count) *xcd_count = ctx->c.xcd_count;
+    if (hbm_bytes) *hbm_bytes = ctx->c.hbm_bytes;
+    return FGX_OK;
+}
+
+int fgx_malloc(fgx_ctx* ctx, size_t bytes, void** out) {
+    if (!out) return fail(FGX_ERR_ARG, "fgx_malloc: out is NULL");
+    if (ctx) FGX_HIP(hipSetDevice(ctx->c.device));
+    *out = nullptr;
+    if (!bytes) return FGX_OK;
+    FGX_HIP(hipMalloc(out, bytes));
+    return FGX_OK;
+}
+
+int fgx_free(fgx_ctx* ctx, void* p) {
+    (void)ctx;
+    if (p) FGX_HIP(hipFree(p));
+    return FGX_OK;
+}
+
+int fgx_host_alloc(size_t bytes, void** out) {
+    if (!out) return fail(FGX_ERR_ARG, "fgx_host_alloc: out is NULL");
+    FGX_HIP(hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault));
+    return FGX_OK;
+}
+
+int fgx_host_free(void* p) {
+    if (p) FGX_HIP(hipHostFree(p));
+    return FGX_OK;
+}
+
+int fgx_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!bytes) return FGX_OK;
+    FGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    return FGX_OK;
+}
+
+int fgx_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!bytes) return FGX_OK;
+    FGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    return FGX_OK;
+}
+
+int fgx_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!bytes) return FGX_OK;
+    FGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return FGX_OK;
+}
+
+int fgx_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* stream) {
+    if (!n) return FGX_OK;
+    const uint32_t blocks = std::min<uint32_t>(ceil_div(n, 256), 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, (hipStream_t)stream, dst, value, n);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+int fgx_fill_vars(uint32_t nvars, void* const* vars, const uint32_t* var_bytes, const void* defaults,
+                  const uint32_t* def_off, uint32_t n, void* stream) {
+    if (!nvars || !n) return FGX_OK;
+    if (nvars > FGX_MAX_VARS) return fail(FGX_ERR_ARG, "fgx_fill_vars: too many variables");
+    FillArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nvars = nvars;
+    a.n = n;
+    for (uint32_t v = 0; v < nvars; ++v) {
+        a.bytes[v] = var_bytes[v];
+        a.ptr[v] = static_cast<char*>(vars[v]);
+        const uint32_t b = var_bytes[v];
+        // Element default: up to 16 bytes are taken verbatim; larger (array) elements
+        // repeat their first 16 bytes, which covers the per-element defaults used here
+        // (arrays whose elements share one value).
+        if (defaults) std::memcpy(a.words[v], static_cast<const char*>(defaults) + def_off[v], std::min<uint32_t>(b, 16));
+    }
+    const uint32_t blocks = std::min<uint32_t>(ceil_div(n, 256), 1024);
+    hipLaunchKernelGGL(k_fill_vars, dim3(blocks, nvars), dim3(256), 0, (hipStream_t)stream, a);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+int fgx_stream_create(void** out) {
+    hipStream_t s;
+    FGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *out = s;
+    return FGX_OK;
+}
+
+int fgx_stream_destroy(void* stream) {
+    if (stream) FGX_HIP(hipStreamDestroy((hipStream_t)stream));
+    return FGX_OK;
+}
+
+int fgx_stream_synchronize(void* stream) {
+    FGX_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return FGX_OK;
+}
+
+int fgx_device_synchronize(void) {
+    FGX_HIP(hipDeviceSynchronize());
+    return FGX_OK;
+}
+
+int fgx_event_create(void** out) {
+    hipEvent_t e;
+    FGX_HIP(hipEventCreate(&e));
+    *out = e;
+    return FGX_OK;
+}
+
+int fgx_event_destroy(void* ev) {
+    if (ev) FGX_HIP(hipEventDestroy((hipEvent_t)ev));
+    return FGX_OK;
+}
+
+int fgx_event_record(void* ev, void* stream) {
+    FGX_HIP(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return FGX_OK;
+}
+
+int fgx_stream_wait_event(void* stream, void* ev) {
+    FGX_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)ev, 0));
+    return FGX_OK;
+}
+
+int fgx_event_elapsed_ms(void* start, void* stop, float* ms) {
+    FGX_HIP(hipEventSynchronize((hipEvent_t)stop));
+    FGX_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return FGX_OK;
+}
+
+int fgx_graph_begin(void* stream) {
+    if (!stream) return fail(FGX_ERR_ARG, "fgx_graph_begin: capture needs a non-default stream");
+    FGX_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return FGX_OK;
+}
+
+int fgx_graph_end(void* stream, fgx_graph** out) {
+    auto* g = new fgx_graph();
+    hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g->graph);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(FGX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g->graph);
+        delete g;
+        return fail(FGX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return FGX_OK;
+}
+
+int fgx_graph_launch(fgx_graph* g, void* stream) {
+    if (!g) return fail(FGX_ERR_ARG, "fgx_graph_launch: NULL graph");
+    FGX_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return FGX_OK;
+}
+
+int fgx_graph_destroy(fgx_graph* g) {
+    if (!g) return FGX_OK;
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return FGX_OK;
+}
+
+int fgx_assign_ids(uint32_t* ids, uint32_t n, uint32_t next_id, void* stream) {
+    if (!n) return FGX_OK;
+    hipLaunchKernelGGL(k_assign_ids, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, ids, n, next_id);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+// MessageSpatial3D.cu:31-52 / MessageSpatial2D.cu:36-54 and numeric.h:12-20.
+static bool approx_exactly_divisible(float x, float y) {
+    const float eps = std::max(std::fabs(x), std::fabs(y)) * std::numeric_limits<float>::epsilon();
+    const float v = std::fmod(x, y);
+    return v <= eps || v > y - eps;
+}
+
+int fgx_spatial_meta_init(int dims, const float* mn, const float* mx, float radius, fgx_spatial_meta* m) {
+    if (dims != 2 && dims != 3) return fail(FGX_ERR_ARG, "spatial messages are 2-D or 3-D");
+    if (!(radius > 0.0f)) return fail(FGX_ERR_ARG, "spatial message radius must be > 0");
+    std::memset(m, 0, sizeof(*m));
+    m->dims = dims;
+    m->radius = radius;
+    m->bins = 1;
+    m->wrap_ok = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (a < dims) {
+            if (!(mx[a] > mn[a])) return fail(FGX_ERR_ARG, "spatial message max must exceed min on every axis");
+            m->min[a] = mn[a];
+            m->max[a] = mx[a];
+            m->width[a] = mx[a] - mn[a];
+            const float q = m->width[a] / radius;
+            m->grid[a] = (uint32_t)std::ceil(q);
+            const uint64_t bins = (uint64_t)m->bins * m->grid[a];
+            if (bins >= (1ull << 31)) return fail(FGX_ERR_ARG, "spatial grid too large");
+            m->bins = (uint32_t)bins;
+            m->wrap_ok = m->wrap_ok && approx_exactly_divisible(m->width[a], radius);
+        } else {
+            m->grid[a] = 1;
+        }
+    }
+    return FGX_OK;
+}
+
+int fgx_agent_sort_key_bits(int dims, const float* mn, const float* mx, float radius, uint32_t* key_bits) {
+    uint64_t bins = 1;
+    for (int a = 0; a < dims; ++a) {
+        const float w = mx[a] - mn[a];
+        bins *= w != 0.0f ? (uint64_t)std::ceil(w / radius) : 1ull;
+    }
+    if (bins >= (1ull << 31)) return fail(FGX_ERR_ARG, "agent sort grid too large");
+    *key_bits = (uint32_t)std::floor(std::log2((double)bins)) + 1u;
+    return FGX_OK;
+}
+
+}  // extern "C"

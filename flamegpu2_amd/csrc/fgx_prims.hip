@@ -1,0 +1,574 @@
+// libfgx: hot-path primitives for gfx950 (wave64).
+//
+//   message-list build : bin key + run-aggregated wave64 histogram -> single-pass
+//                        decoupled look-back scan -> PBM -> index scatter ->
+//                        stable in-bin ranking fused with the SoA payload scatter
+//   agent spatial sort : same pipeline with the unclamped sort key
+//   compaction/births  : one pass, decoupled look-back scan fused with the scatter
+//   reduction          : fixed-shape (deterministic) tree
+//
+// Nothing here allocates or synchronises with the host.
+#include "fgx_internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace fgx {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kItems = 8;
+constexpr uint32_t kTile = kBlock * kItems;   // 2048 items per look-back tile
+
+__device__ __forceinline__ uint32_t load_count(const uint32_t* d_n, uint32_t n_cap) {
+    if (!d_n) return n_cap;
+    const uint32_t n = __hip_atomic_load(d_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return n < n_cap ? n : n_cap;
+}
+
+// ================================================================ bin keys
+struct KeyArgs {
+    int32_t mode;        // 0: message bin (clamped, MessageSpatial3DDevice.cuh:646-672)
+                         // 1: agent sort key (unclamped, CUDASimulation.cu:372-408)
+    int32_t dims;
+    float min[3];
+    float radius;
+    float width[3];
+    uint32_t grid[3];
+    uint32_t key_mask;   // mode 1: (1 << key_bits) - 1
+};
+
+__device__ __forceinline__ uint32_t bin_key(const KeyArgs& k, float x, float y, float z) {
+    if (k.mode == 0) {
+        int c[3];
+        const float p[3] = {x, y, z};
+        for (int a = 0; a < 3; ++a) {
+            if (a < k.dims) {
+                const int g = (int)floorf((p[a] - k.min[a]) / k.radius);
+                const int hi = (int)k.grid[a] - 1;
+                c[a] = g < 0 ? 0 : (g > hi ? hi : g);
+            } else {
+                c[a] = 0;
+            }
+        }
+        return (uint32_t)c[2] * k.grid[0] * k.grid[1] + (uint32_t)c[1] * k.grid[0] + (uint32_t)c[0];
+    } else {
+        const int gx = (int)floorf(((x - k.min[0]) / k.width[0]) * (float)k.grid[0]);
+        const int gy = (int)floorf(((y - k.min[1]) / k.width[1]) * (float)k.grid[1]);
+        uint32_t b;
+        if (k.dims == 3) {
+            const int gz = (int)floorf(((z - k.min[2]) / k.width[2]) * (float)k.grid[2]);
+            b = (uint32_t)gz * k.grid[0] * k.grid[1] + (uint32_t)gy * k.grid[0] + (uint32_t)gx;
+        } else {
+            b = (uint32_t)gy * k.grid[0] + (uint32_t)gx;
+        }
+        return b & k.key_mask;
+    }
+}
+
+// Histogram with wave-level run aggregation: consecutive lanes with equal keys
+// (the common case, as agents are kept in bin order) issue one atomic per run.
+// sub[i] is the slot of message i inside its bin (atomic order, fixed later).
+__global__ void __launch_bounds__(kBlock) k_bin_hist(KeyArgs ka, const uint32_t* d_n, uint32_t n_cap,
+                                                     const float* __restrict__ x, const float* __restrict__ y,
+                                                     const float* __restrict__ z, uint32_t* __restrict__ key,
+                                                     uint32_t* __restrict__ sub, uint32_t* __restrict__ hist) {
+    const uint32_t n = load_count(d_n, n_cap);
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
+        const uint32_t i = base + lane;
+        const bool valid = i < n;
+        uint32_t k = 0xFFFFFFFFu;
+        if (valid) k = bin_key(ka, x[i], y[i], ka.dims == 3 ? z[i] : 0.0f);
+        const uint32_t prev = __shfl_up(k, 1, 64);
+        const bool head = valid && (lane == 0 || k != prev);
+        const uint64_t hm = __ballot(head);
+        const uint64_t vm = __ballot(valid);
+        const uint32_t nvalid = (uint32_t)__builtin_popcountll(vm);
+        const uint64_t above = lane == 63 ? 0ull : (hm & (~0ull << (lane + 1)));
+        const uint32_t end = above ? (uint32_t)__builtin_ctzll(above) : nvalid;
+        uint32_t b = 0;
+        if (head) b = atomicAdd(&hist[k], end - lane);
+        const uint64_t below = hm & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        const uint32_t hl = below ? 63u - (uint32_t)__builtin_clzll(below) : 0u;
+        b = __shfl(b, (int)hl, 64);
+        if (valid) {
+            key[i] = k;
+            sub[i] = b + (lane - hl);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_index_scatter(const uint32_t* d_n, uint32_t n_cap,
+                                                          const uint32_t* __restrict__ key, const uint32_t* __restrict__ sub,
+                                                          const uint32_t* __restrict__ start, uint32_t* __restrict__ tmp) {
+    const uint32_t n = load_count(d_n, n_cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        tmp[start[key[i]] + sub[i]] = i;
+}
+
+// For output slot s: the message e = tmp[s] sits in bin b = key[e]; its stable
+// rank in the bin is the number of bin members with a smaller input index.
+// Writes every variable of e to start[b] + rank.
+__global__ void __launch_bounds__(kBlock) k_stable_scatter(const uint32_t* d_n, uint32_t n_cap,
+                                                           const uint32_t* __restrict__ key,
+                                                           const uint32_t* __restrict__ start,
+                                                           const uint32_t* __restrict__ tmp, VarTable vt) {
+    const uint32_t n = load_count(d_n, n_cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
+        const uint32_t e = tmp[s];
+        const uint32_t b = key[e];
+        const uint32_t st = start[b];
+        const uint32_t en = start[b + 1];
+        uint32_t rank = 0;
+        for (uint32_t t = st; t < en; ++t) rank += tmp[t] < e ? 1u : 0u;
+        copy_element(vt, e, st + rank);
+    }
+}
+
+// ================================================================ single-pass exclusive scan
+// Blocked arrangement: thread t owns items [tile*2048 + t*8, +8).
+__global__ void __launch_bounds__(kBlock) k_scan_excl(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                      uint32_t n, LookbackHeader* hdr, uint64_t* status,
+                                                      uint32_t num_tiles) {
+    __shared__ uint32_t s_tile, s_epoch, s_prefix;
+    __shared__ uint32_t s_wave[kBlock / 64];
+    acquire_tile(hdr, num_tiles, &s_tile, &s_epoch);
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * kTile + threadIdx.x * kItems;
+    uint32_t v[kItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        v[j] = (base + j < n) ? in[base + j] : 0u;
+        sum += v[j];
+    }
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t incl = wave_inclusive_u32(sum);
+    if (lane == 63) s_wave[w] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0, total = 0;
+    for (uint32_t k = 0; k < kBlock / 64; ++k) {
+        if (k < w) wave_off += s_wave[k];
+        total += s_wave[k];
+    }
+    if (w == 0) {
+        const uint32_t p = lookback_wave0(status, tile, s_epoch, total, hdr);
+        if (lane == 0) s_prefix = p;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + wave_off + incl - sum;
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+    retire_tile(hdr, num_tiles);
+}
+
+// ================================================================ compaction / births
+// Striped arrangement: item (j, wave, lane) = tile*2048 + j*256 + wave*64 + lane, so loads
+// and the compacted stores of a wave are contiguous.  Index order == (j, wave, lane) order.
+struct CompactArgs {
+    const uint32_t* flags;
+    uint32_t keep;
+    const uint32_t* d_n;
+    uint32_t n_cap;
+    uint32_t num_tiles;
+    LookbackHeader* hdr;
+    uint64_t* status;
+    // compaction output
+    uint32_t* d_count_out;
+    // births (mode 1)
+    int32_t births;
+    int32_t id_var;
+    uint32_t* d_dst_count;
+    uint32_t dst_capacity;
+    uint32_t* d_next_id;
+    uint32_t* d_overflow;
+};
+
+__global__ void __launch_bounds__(kBlock) k_compact(CompactArgs a, VarTable vt) {
+    __shared__ uint32_t s_tile, s_epoch, s_prefix, s_dst0, s_id0;
+    __shared__ uint32_t s_cnt[kItems * (kBlock / 64)];
+    if (threadIdx.x == 0 && a.births) {
+        s_dst0 = __hip_atomic_load(a.d_dst_count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        s_id0 = __hip_atomic_load(a.d_next_id, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    acquire_tile(a.hdr, a.num_tiles, &s_tile, &s_epoch);
+    const uint32_t tile = s_tile;
+    const uint32_t n = load_count(a.d_n, a.n_cap);
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
+    uint64_t masks[kItems];
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+        const bool keep = i < n && ((a.flags[i] != 0u) == (a.keep != 0u));
+        masks[j] = __ballot(keep);
+        if (lane == 0) s_cnt[j * (kBlock / 64) + w] = (uint32_t)__builtin_popcountll(masks[j]);
+    }
+    __syncthreads();
+    if (w == 0) {
+        // exclusive scan of the 32 (j, wave) counts in index order, then look-back
+        const uint32_t c = lane < kItems * (kBlock / 64) ? s_cnt[lane] : 0u;
+        const uint32_t inc = wave_inclusive_u32(c);
+        const uint32_t total = __shfl(inc, 63, 64);
+        const uint32_t p = lookback_wave0(a.status, tile, s_epoch, total, a.hdr);
+        if (lane < kItems * (kBlock / 64)) s_cnt[lane] = inc - c;
+        if (lane == 0) s_prefix = p;
+        if (lane == 0 && tile == a.num_tiles - 1) {
+            const uint32_t kept = p + total;
+            if (!a.births) {
+                *a.d_count_out = kept;
+            } else {
+                uint32_t nd = s_dst0 + kept;
+                if (nd > a.dst_capacity) {
+                    nd = a.dst_capacity;
+                    if (a.d_overflow) *a.d_overflow = 1u;
+                }
+                *a.d_dst_count = nd;
+                *a.d_next_id = s_id0 + kept;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t prefix = s_prefix;
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        if (!((masks[j] >> lane) & 1ull)) continue;
+        const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+        const uint32_t r = prefix + s_cnt[j * (kBlock / 64) + w] + (uint32_t)__builtin_popcountll(masks[j] & lanes_below);
+        if (!a.births) {
+            copy_element(vt, i, r);
+        } else {
+            const uint32_t d = s_dst0 + r;
+            if (d >= a.dst_capacity) continue;
+            copy_element(vt, i, d, a.id_var);
+            if (a.id_var >= 0) reinterpret_cast<uint32_t*>(vt.out[a.id_var])[d] = s_id0 + r;
+        }
+    }
+    retire_tile(a.hdr, a.num_tiles);
+}
+
+// ================================================================ deterministic reduction
+constexpr uint32_t kReduceBlocks = 256;
+
+template <typename T, typename Acc>
+__global__ void __launch_bounds__(kBlock) k_reduce_partial(const T* __restrict__ in, const uint32_t* d_n, uint32_t n_cap,
+                                                           Acc* __restrict__ partial) {
+    __shared__ Acc s[kBlock];
+    const uint32_t n = load_count(d_n, n_cap);
+    const uint32_t stride = kReduceBlocks * kBlock;
+    Acc acc = 0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) acc += (Acc)in[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+template <typename Acc>
+__global__ void __launch_bounds__(kBlock) k_reduce_final(const Acc* __restrict__ partial, Acc* __restrict__ out) {
+    __shared__ Acc s[kBlock];
+    s[threadIdx.x] = threadIdx.x < kReduceBlocks ? partial[threadIdx.x] : (Acc)0;
+    __syncthreads();
+    for (uint32_t o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+// ================================================================ scratch layouts
+struct BuildLayout {
+    size_t hist, key, sub, tmp, hdr, status, total;
+};
+
+static BuildLayout build_layout(uint32_t n, uint32_t bins) {
+    BuildLayout L;
+    size_t off = 0;
+    L.hist = off; off += align256((size_t)(bins + 1) * 4);
+    L.key = off; off += align256((size_t)std::max(n, 1u) * 4);
+    L.sub = off; off += align256((size_t)std::max(n, 1u) * 4);
+    L.tmp = off; off += align256((size_t)std::max(n, 1u) * 4);
+    L.hdr = off; off += sizeof(LookbackHeader);
+    L.status = off; off += align256((size_t)ceil_div(bins + 1, kTile) * 8);
+    L.total = off;
+    return L;
+}
+
+static size_t lookback_bytes(uint32_t n) {
+    return sizeof(LookbackHeader) + align256((size_t)std::max(ceil_div(n, kTile), 1u) * 8);
+}
+
+static uint32_t stream_grid(const Ctx* c, uint32_t n) {
+    // Enough waves to cover HBM latency on every CU; grid-stride beyond that.
+    const uint32_t cap = (uint32_t)std::max(c ? c->cu_count : 256, 1) * 8u;
+    return std::max(1u, std::min(ceil_div(n, kBlock), cap));
+}
+
+}  // namespace fgx
+
+using namespace fgx;
+
+static int run_scan(const uint32_t* in, uint32_t* out, uint32_t n, char* scratch, void* stream) {
+    if (!n) return FGX_OK;
+    auto* hdr = reinterpret_cast<LookbackHeader*>(scratch);
+    auto* status = reinterpret_cast<uint64_t*>(scratch + sizeof(LookbackHeader));
+    const uint32_t tiles = ceil_div(n, kTile);
+    hipLaunchKernelGGL(k_scan_excl, dim3(tiles), dim3(kBlock), 0, (hipStream_t)stream, in, out, n, hdr, status, tiles);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+// Shared by the message build and the agent sort.
+static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const uint32_t* d_n, uint32_t n_cap,
+                        const float* x, const float* y, const float* z, const VarTable& vt, uint32_t* start_out,
+                        void* scratch, size_t scratch_bytes, void* stream) {
+    const BuildLayout L = build_layout(n_cap, bins);
+    if (!scratch || scratch_bytes < L.total) return fail(FGX_ERR_CAPACITY, "scratch too small for bin sort");
+    char* s = static_cast<char*>(scratch);
+    auto* hist = reinterpret_cast<uint32_t*>(s + L.hist);
+    auto* key = reinterpret_cast<uint32_t*>(s + L.key);
+    auto* sub = reinterpret_cast<uint32_t*>(s + L.sub);
+    auto* tmp = reinterpret_cast<uint32_t*>(s + L.tmp);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = fgx_fill_u32(hist, 0u, (size_t)bins + 1, stream);
+    if (rc) return rc;
+    const uint32_t g = stream_grid(ctx ? &ctx->c : nullptr, n_cap);
+    if (n_cap) {
+        hipLaunchKernelGGL(k_bin_hist, dim3(g), dim3(kBlock), 0, st, ka, d_n, n_cap, x, y, z, key, sub, hist);
+        FGX_HIP(hipGetLastError());
+    }
+    // scan hist -> start (bins+1 entries, start[bins] = n)
+    {
+        auto* hdr = reinterpret_cast<LookbackHeader*>(s + L.hdr);
+        auto* status = reinterpret_cast<uint64_t*>(s + L.status);
+        const uint32_t tiles = ceil_div(bins + 1, kTile);
+        hipLaunchKernelGGL(k_scan_excl, dim3(tiles), dim3(kBlock), 0, st, hist, start_out, bins + 1, hdr, status, tiles);
+        FGX_HIP(hipGetLastError());
+    }
+    if (n_cap) {
+        hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, sub, start_out, tmp);
+        FGX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, start_out, tmp, vt);
+        FGX_HIP(hipGetLastError());
+    }
+    return FGX_OK;
+}
+
+extern "C" {
+
+int fgx_scratch_bytes(int op, uint32_t n, uint32_t bins, size_t* bytes) {
+    if (!bytes) return fail(FGX_ERR_ARG, "fgx_scratch_bytes: bytes is NULL");
+    switch (op) {
+        case FGX_OP_SPATIAL_BUILD:
+            *bytes = build_layout(n, bins).total;
+            return FGX_OK;
+        case FGX_OP_AGENT_SORT:
+            // start table of the sort keys lives in scratch too
+            *bytes = build_layout(n, bins).total + align256((size_t)(bins + 1) * 4);
+            return FGX_OK;
+        case FGX_OP_COMPACT:
+        case FGX_OP_BIRTHS:
+        case FGX_OP_SCAN:
+            *bytes = lookback_bytes(n) + align256(kReduceBlocks * 8);
+            return FGX_OK;
+        default:
+            return fail(FGX_ERR_ARG, "fgx_scratch_bytes: unknown op");
+    }
+}
+
+int fgx_scratch_clear(void* scratch, size_t bytes, void* stream) {
+    if (!bytes) return FGX_OK;
+    if (bytes & 3u) return fail(FGX_ERR_ARG, "fgx_scratch_clear: size must be a multiple of 4");
+    return fgx_fill_u32(static_cast<uint32_t*>(scratch), 0u, bytes / 4, stream);
+}
+
+int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t* d_n, uint32_t n_cap,
+                      const float* x, const float* y, const float* z,
+                      uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                      uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream) {
+    if (!meta || !pbm) return fail(FGX_ERR_ARG, "fgx_spatial_build: NULL meta or pbm");
+    if (meta->dims != 2 && meta->dims != 3) return fail(FGX_ERR_ARG, "fgx_spatial_build: dims must be 2 or 3");
+    if (n_cap && (!x || !y || (meta->dims == 3 && !z)))
+        return fail(FGX_ERR_ARG, "fgx_spatial_build: missing location array");
+    VarTable vt;
+    int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
+    if (rc) return rc;
+    KeyArgs ka;
+    std::memset(&ka, 0, sizeof(ka));
+    ka.mode = 0;
+    ka.dims = meta->dims;
+    ka.radius = meta->radius;
+    for (int a = 0; a < 3; ++a) {
+        ka.min[a] = meta->min[a];
+        ka.width[a] = meta->width[a];
+        ka.grid[a] = meta->grid[a] ? meta->grid[a] : 1u;
+    }
+    return run_bin_sort(ctx, ka, meta->bins, d_n, n_cap, x, y, z, vt, pbm, scratch, scratch_bytes, stream);
+}
+
+int fgx_agent_sort(fgx_ctx* ctx, int dims, const float* mn, const float* mx, float radius,
+                   const uint32_t* d_n, uint32_t n_cap, const float* x, const float* y, const float* z,
+                   uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                   void* scratch, size_t scratch_bytes, void* stream) {
+    if (dims != 2 && dims != 3) return fail(FGX_ERR_ARG, "fgx_agent_sort: dims must be 2 or 3");
+    VarTable vt;
+    int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
+    if (rc) return rc;
+    uint32_t bits = 0;
+    rc = fgx_agent_sort_key_bits(dims, mn, mx, radius, &bits);
+    if (rc) return rc;
+    if (bits > 30) return fail(FGX_ERR_ARG, "fgx_agent_sort: key range too large");
+    const uint32_t bins = 1u << bits;
+    KeyArgs ka;
+    std::memset(&ka, 0, sizeof(ka));
+    ka.mode = 1;
+    ka.dims = dims;
+    ka.radius = radius;
+    ka.key_mask = bins - 1u;
+    for (int a = 0; a < 3; ++a) {
+        if (a < dims) {
+            ka.min[a] = mn[a];
+            ka.width[a] = mx[a] - mn[a];
+            ka.grid[a] = ka.width[a] != 0.0f ? (uint32_t)std::ceil(ka.width[a] / radius) : 1u;
+        } else {
+            ka.grid[a] = 1u;
+            ka.width[a] = 1.0f;
+        }
+    }
+    const BuildLayout L = build_layout(n_cap, bins);
+    if (!scratch || scratch_bytes < L.total + align256((size_t)(bins + 1) * 4))
+        return fail(FGX_ERR_CAPACITY, "fgx_agent_sort: scratch too small");
+    auto* start = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) + L.total);
+    return run_bin_sort(ctx, ka, bins, d_n, n_cap, x, y, z, vt, start, scratch, L.total, stream);
+}
+
+int fgx_exclusive_scan_u32(fgx_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
+                           void* scratch, size_t scratch_bytes, void* stream) {
+    (void)ctx;
+    if (!scratch || scratch_bytes < lookback_bytes(n)) return fail(FGX_ERR_CAPACITY, "fgx_exclusive_scan_u32: scratch too small");
+    return run_scan(in, out, n, static_cast<char*>(scratch), stream);
+}
+
+static int launch_compact(CompactArgs a, const VarTable& vt, void* scratch, size_t scratch_bytes, void* stream) {
+    if (!scratch || scratch_bytes < lookback_bytes(a.n_cap)) return fail(FGX_ERR_CAPACITY, "compaction scratch too small");
+    char* s = static_cast<char*>(scratch);
+    a.hdr = reinterpret_cast<LookbackHeader*>(s);
+    a.status = reinterpret_cast<uint64_t*>(s + sizeof(LookbackHeader));
+    a.num_tiles = std::max(ceil_div(a.n_cap, kTile), 1u);
+    hipLaunchKernelGGL(k_compact, dim3(a.num_tiles), dim3(kBlock), 0, (hipStream_t)stream, a, vt);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+int fgx_compact(fgx_ctx* ctx, const uint32_t* flags, uint32_t keep, const uint32_t* d_n, uint32_t n_cap,
+                uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                uint32_t* d_count_out, void* scratch, size_t scratch_bytes, void* stream) {
+    (void)ctx;
+    if (!d_count_out) return fail(FGX_ERR_ARG, "fgx_compact: d_count_out is NULL");
+    if (n_cap && !flags) return fail(FGX_ERR_ARG, "fgx_compact: flags is NULL");
+    VarTable vt;
+    int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
+    if (rc) return rc;
+    CompactArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.flags = flags;
+    a.keep = keep;
+    a.d_n = d_n;
+    a.n_cap = n_cap;
+    a.d_count_out = d_count_out;
+    a.births = 0;
+    a.id_var = -1;
+    return launch_compact(a, vt, scratch, scratch_bytes, stream);
+}
+
+int fgx_append_births(fgx_ctx* ctx, const uint32_t* flags, const uint32_t* d_n_parents, uint32_t n_parents_cap,
+                      uint32_t nvars, const void* const* newbuf, void* const* dst, const uint32_t* var_bytes,
+                      int32_t id_var, uint32_t* d_dst_count, uint32_t dst_capacity, uint32_t* d_next_id,
+                      uint32_t* d_overflow, void* scratch, size_t scratch_bytes, void* stream) {
+    (void)ctx;
+    if (!d_dst_count || !d_next_id) return fail(FGX_ERR_ARG, "fgx_append_births: NULL device counters");
+    if (n_parents_cap && !flags) return fail(FGX_ERR_ARG, "fgx_append_births: flags is NULL");
+    if (id_var >= (int32_t)nvars) return fail(FGX_ERR_ARG, "fgx_append_births: id_var out of range");
+    if (id_var >= 0 && var_bytes[id_var] != 4) return fail(FGX_ERR_ARG, "fgx_append_births: id variable must be 32-bit");
+    VarTable vt;
+    int rc = make_var_table(nvars, newbuf, dst, var_bytes, &vt);
+    if (rc) return rc;
+    // The ID variable is produced (next_id + rank), not copied from newbuf.
+    CompactArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.flags = flags;
+    a.keep = 1;
+    a.d_n = d_n_parents;
+    a.n_cap = n_parents_cap;
+    a.births = 1;
+    a.id_var = id_var;
+    a.d_dst_count = d_dst_count;
+    a.dst_capacity = dst_capacity;
+    a.d_next_id = d_next_id;
+    a.d_overflow = d_overflow;
+    return launch_compact(a, vt, scratch, scratch_bytes, stream);
+}
+
+int fgx_reduce_sum(fgx_ctx* ctx, int type, const void* in, const uint32_t* d_n, uint32_t n_cap,
+                   void* d_out, void* scratch, size_t scratch_bytes, void* stream) {
+    (void)ctx;
+    const size_t need = lookback_bytes(n_cap) + align256(kReduceBlocks * 8);
+    if (!scratch || scratch_bytes < need) return fail(FGX_ERR_CAPACITY, "fgx_reduce_sum: scratch too small");
+    char* part = static_cast<char*>(scratch) + lookback_bytes(n_cap);
+    hipStream_t st = (hipStream_t)stream;
+    switch (type) {
+        case 0:
+            hipLaunchKernelGGL((k_reduce_partial<float, double>), dim3(kReduceBlocks), dim3(kBlock), 0, st,
+                               static_cast<const float*>(in), d_n, n_cap, reinterpret_cast<double*>(part));
+            hipLaunchKernelGGL((k_reduce_final<double>), dim3(1), dim3(kBlock), 0, st, reinterpret_cast<double*>(part),
+                               static_cast<double*>(d_out));
+            break;
+        case 1:
+            hipLaunchKernelGGL((k_reduce_partial<double, double>), dim3(kReduceBlocks), dim3(kBlock), 0, st,
+                               static_cast<const double*>(in), d_n, n_cap, reinterpret_cast<double*>(part));
+            hipLaunchKernelGGL((k_reduce_final<double>), dim3(1), dim3(kBlock), 0, st, reinterpret_cast<double*>(part),
+                               static_cast<double*>(d_out));
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_reduce_partial<int32_t, long long>), dim3(kReduceBlocks), dim3(kBlock), 0, st,
+                               static_cast<const int32_t*>(in), d_n, n_cap, reinterpret_cast<long long*>(part));
+            hipLaunchKernelGGL((k_reduce_final<long long>), dim3(1), dim3(kBlock), 0, st,
+                               reinterpret_cast<long long*>(part), static_cast<long long*>(d_out));
+            break;
+        case 3:
+            hipLaunchKernelGGL((k_reduce_partial<uint32_t, unsigned long long>), dim3(kReduceBlocks), dim3(kBlock), 0, st,
+                               static_cast<const uint32_t*>(in), d_n, n_cap, reinterpret_cast<unsigned long long*>(part));
+            hipLaunchKernelGGL((k_reduce_final<unsigned long long>), dim3(1), dim3(kBlock), 0, st,
+                               reinterpret_cast<unsigned long long*>(part), static_cast<unsigned long long*>(d_out));
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_reduce_partial<long long, long long>), dim3(kReduceBlocks), dim3(kBlock), 0, st,
+                               static_cast<const long long*>(in), d_n, n_cap, reinterpret_cast<long long*>(part));
+            hipLaunchKernelGGL((k_reduce_final<long long>), dim3(1), dim3(kBlock), 0, st,
+                               reinterpret_cast<long long*>(part), static_cast<long long*>(d_out));
+            break;
+        case 5:
+            hipLaunchKernelGGL((k_reduce_partial<unsigned long long, unsigned long long>), dim3(kReduceBlocks), dim3(kBlock), 0, st,
+                               static_cast<const unsigned long long*>(in), d_n, n_cap,
+                               reinterpret_cast<unsigned long long*>(part));
+            hipLaunchKernelGGL((k_reduce_final<unsigned long long>), dim3(1), dim3(kBlock), 0, st,
+                               reinterpret_cast<unsigned long long*>(part), static_cast<unsigned long long*>(d_out));
+            break;
+        default:
+            return fail(FGX_ERR_ARG, "fgx_reduce_sum: unknown type");
+    }
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+"""ctypes binding of libfgx.so (include/fgx.h).
+
+Device buffers are passed as integer device addresses (torch CUDA tensors via
+.data_ptr()); streams are hipStream_t handles as integers (0 = null stream).
+Every non-zero status raises FgxError carrying fgx_last_error().
+"""
+import ctypes
+import os
+
+from . import load
+
+_c = ctypes
+vp = _c.c_void_p
+u32 = _c.c_uint32
+i32 = _c.c_int32
+sz = _c.c_size_t
+
+FGX_OP_SPATIAL_BUILD = 0
+FGX_OP_COMPACT = 1
+FGX_OP_BIRTHS = 2
+FGX_OP_AGENT_SORT = 3
+FGX_OP_SCAN = 4
+FGX_MAX_VARS = 32
+
+
+class FgxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fgx error {code}: {msg}")
+        self.code = code
+
+
+class SpatialMeta(_c.Structure):
+    _fields_ = [("dims", i32), ("min", _c.c_float * 3), ("max", _c.c_float * 3), ("radius", _c.c_float),
+                ("width", _c.c_float * 3), ("grid", u32 * 3), ("bins", u32), ("wrap_ok", i32)]
+
+
+_SIGS = {
+    "fgx_ctx_create": [_c.c_int, _c.POINTER(vp)],
+    "fgx_ctx_destroy": [vp],
+    "fgx_device_info": [vp, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _c.POINTER(sz)],
+    "fgx_malloc": [vp, sz, _c.POINTER(vp)],
+    "fgx_free": [vp, vp],
+    "fgx_memcpy_h2d": [vp, vp, sz, vp],
+    "fgx_memcpy_d2h": [vp, vp, sz, vp],
+    "fgx_memcpy_d2d": [vp, vp, sz, vp],
+    "fgx_fill_u32": [vp, u32, sz, vp],
+    "fgx_stream_synchronize": [vp],
+    "fgx_device_synchronize": [],
+    "fgx_scratch_bytes": [_c.c_int, u32, u32, _c.POINTER(sz)],
+    "fgx_scratch_clear": [vp, sz, vp],
+    "fgx_spatial_meta_init": [_c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float,
+                              _c.POINTER(SpatialMeta)],
+    "fgx_spatial_build": [vp, _c.POINTER(SpatialMeta), vp, u32, vp, vp, vp, u32, _c.POINTER(vp), _c.POINTER(vp),
+                          _c.POINTER(u32), vp, vp, sz, vp],
+    "fgx_agent_sort_key_bits": [_c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float,
+                                _c.POINTER(u32)],
+    "fgx_agent_sort": [vp, _c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float, vp, u32, vp, vp,
+                       vp, u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), vp, sz, vp],
+    "fgx_compact": [vp, vp, u32, vp, u32, u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), vp, vp, sz, vp],
+    "fgx_append_births": [vp, vp, vp, u32, u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), i32, vp, u32, vp,
+                          vp, vp, sz, vp],
+    "fgx_exclusive_scan_u32": [vp, vp, vp, u32, vp, sz, vp],
+    "fgx_assign_ids": [vp, u32, u32, vp],
+    "fgx_reduce_sum": [vp, _c.c_int, vp, vp, u32, vp, vp, sz, vp],
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = load("libfgx.so")
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _c.c_int
+        L.fgx_last_error.restype = _c.c_char_p
+        L.fgx_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise FgxError(rc, lib().fgx_last_error().decode())
+    return rc
+
+
+def header_symbols():
+    """Every function declared in include/fgx.h."""
+    import re
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    txt = open(os.path.join(here, "include", "fgx.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fgx_\w+)\s*\(", txt, flags=re.M)))
+
+
+class Context:
+    def __init__(self, device=0):
+        h = vp()
+        check(lib().fgx_ctx_create(device, _c.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().fgx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _arr(t, vals):
+    return (t * max(len(vals), 1))(*vals)
+
+
+def spatial_meta(dims, mn, mx, radius):
+    m = SpatialMeta()
+    mn = list(mn) + [0.0] * (3 - len(mn))
+    mx = list(mx) + [0.0] * (3 - len(mx))
+    check(lib().fgx_spatial_meta_init(dims, _arr(_c.c_float, mn), _arr(_c.c_float, mx), radius, _c.byref(m)))
+    return m
+
+
+def scratch_bytes(op, n, bins=0):
+    b = sz()
+    check(lib().fgx_scratch_bytes(op, n, bins, _c.byref(b)))
+    return b.value
+
+
+def spatial_build(ctx, meta, n_cap, x, y, z, vars_in, vars_out, var_bytes, pbm, scratch, scratch_len,
+                  stream=0, d_n=None):
+    nv = len(vars_in)
+    check(lib().fgx_spatial_build(ctx.h, _c.byref(meta), d_n, n_cap, x, y, z, nv, _arr(vp, vars_in),
+                                  _arr(vp, vars_out), _arr(u32, var_bytes), pbm, scratch, scratch_len, stream))
+
+
+def agent_sort(ctx, dims, mn, mx, radius, n_cap, x, y, z, vars_in, vars_out, var_bytes, scratch, scratch_len,
+               stream=0, d_n=None):
+    mn = list(mn) + [0.0] * (3 - len(mn))
+    mx = list(mx) + [0.0] * (3 - len(mx))
+    nv = len(vars_in)
+    check(lib().fgx_agent_sort(ctx.h, dims, _arr(_c.c_float, mn), _arr(_c.c_float, mx), radius, d_n, n_cap, x, y,
+                               z, nv, _arr(vp, vars_in), _arr(vp, vars_out), _arr(u32, var_bytes), scratch,
+                               scratch_len, stream))
+
+
+def agent_sort_key_bits(dims, mn, mx, radius):
+    mn = list(mn) + [0.0] * (3 - len(mn))
+    mx = list(mx) + [0.0] * (3 - len(mx))
+    b = u32()
+    check(lib().fgx_agent_sort_key_bits(dims, _arr(_c.c_float, mn), _arr(_c.c_float, mx), radius, _c.byref(b)))
+    return b.value
+
+
+def compact(ctx, flags, keep, n_cap, vars_in, vars_out, var_bytes, d_count_out, scratch, scratch_len, stream=0,
+            d_n=None):
+    nv = len(vars_in)
+    check(lib().fgx_compact(ctx.h, flags, keep, d_n, n_cap, nv, _arr(vp, vars_in), _arr(vp, vars_out),
+                            _arr(u32, var_bytes), d_count_out, scratch, scratch_len, stream))
+
+
+def append_births(ctx, flags, n_parents_cap, newbuf, dst, var_bytes, id_var, d_dst_count, dst_capacity, d_next_id,
+                  scratch, scratch_len, stream=0, d_n_parents=None, d_overflow=None):
+    nv = len(dst)
+    check(lib().fgx_append_births(ctx.h, flags, d_n_parents, n_parents_cap, nv, _arr(vp, newbuf), _arr(vp, dst),
+                                  _arr(u32, var_bytes), id_var, d_dst_count, dst_capacity, d_next_id, d_overflow,
+                                  scratch, scratch_len, stream))
+
+
+def exclusive_scan(ctx, d_in, d_out, n, scratch, scratch_len, stream=0):
+    check(lib().fgx_exclusive_scan_u32(ctx.h, d_in, d_out, n, scratch, scratch_len, stream))
+
+
+def scratch_clear(scratch, nbytes, stream=0):
+    check(lib().fgx_scratch_clear(scratch, nbytes, stream))
+
+
+def reduce_sum(ctx, type_code, d_in, n_cap, d_out, scratch, scratch_len, stream=0, d_n=None):
+    check(lib().fgx_reduce_sum(ctx.h, type_code, d_in, d_n, n_cap, d_out, scratch, scratch_len, stream))

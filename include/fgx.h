@@ -1,0 +1,190 @@
+/*
+ * fgx.h - C ABI of the MI355X (gfx950) hot-path primitives (libfgx.so).
+ *
+ * Plain pointers and sizes only: device pointers are `void*`/typed pointers
+ * into HBM, streams are `void*` (hipStream_t).  Every call is stream-ordered,
+ * allocates nothing on the hot path (scratch is sized by fgx_scratch_bytes and
+ * owned by the caller) and performs no hidden host synchronisation, so a whole
+ * simulation step can be captured into a hipGraph.  Counts that change on the
+ * device (compaction, births) stay in device memory.
+ *
+ * Return value: 0 on success, otherwise an FGX_ERR_* code; the message is
+ * available from fgx_last_error() (thread-local).  This replaces the
+ * reference's C++ exceptions (include/flamegpu/exception/FLAMEGPUException.h:16,
+ * include/flamegpu/simulation/detail/CUDAErrorChecking.cuh:18-48); the C++
+ * facade rethrows them as flamegpu::exception::* types.
+ *
+ * Calls on one context may run on several streams at once as long as each
+ * concurrent call has its own scratch buffer.  Scratch buffers must be
+ * zero-filled once (fgx_scratch_clear) before their first use; they keep
+ * themselves consistent afterwards.
+ */
+#ifndef FGX_H_
+#define FGX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FGX_OK = 0,
+    FGX_ERR_ARG = 1,       /* invalid argument (reference: exception::InvalidArgument) */
+    FGX_ERR_HIP = 2,       /* HIP runtime error (reference: exception::CUDAError) */
+    FGX_ERR_CAPACITY = 3,  /* a device buffer is too small for the requested work */
+    FGX_ERR_DEVICE = 4,    /* device-side check failed (reference: exception::DeviceError) */
+    FGX_ERR_STATE = 5      /* call made in the wrong state (e.g. graph not begun) */
+};
+
+#define FGX_MAX_VARS 32
+
+typedef struct fgx_ctx fgx_ctx;
+typedef struct fgx_graph fgx_graph;
+
+/* ---------------------------------------------------------------- context
+ * Replaces the per-simulation CUDA context / stream set up by
+ * CUDASimulation::initialiseSingletons (src/flamegpu/simulation/CUDASimulation.cu:1596-1701). */
+int fgx_ctx_create(int device, fgx_ctx** out);
+int fgx_ctx_destroy(fgx_ctx* ctx);
+const char* fgx_last_error(void);
+/* Device properties used for launch shaping. */
+int fgx_device_info(fgx_ctx* ctx, int* cu_count, int* xcd_count, size_t* hbm_bytes);
+
+/* ---------------------------------------------------------------- memory, streams, events, graphs
+ * Thin wrappers so the C++ runtime never includes HIP headers. */
+int fgx_malloc(fgx_ctx* ctx, size_t bytes, void** out);
+int fgx_free(fgx_ctx* ctx, void* p);
+int fgx_host_alloc(size_t bytes, void** out);   /* pinned host memory */
+int fgx_host_free(void* p);
+int fgx_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int fgx_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int fgx_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+/* Fill n 32-bit words with `value` using a kernel (graph-capture safe). */
+int fgx_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* stream);
+/* Fill `nvars` variable arrays of n elements with per-variable default values
+ * (element bytes var_bytes[v], default value bytes at defaults + def_off[v]);
+ * replaces CUDAScatter::broadcastInit (src/flamegpu/simulation/detail/CUDAScatter.cu:404-422). */
+int fgx_fill_vars(uint32_t nvars, void* const* vars, const uint32_t* var_bytes, const void* defaults,
+                  const uint32_t* def_off, uint32_t n, void* stream);
+int fgx_stream_create(void** out);
+int fgx_stream_destroy(void* stream);
+int fgx_stream_synchronize(void* stream);
+int fgx_device_synchronize(void);
+int fgx_event_create(void** out);
+int fgx_event_destroy(void* ev);
+int fgx_event_record(void* ev, void* stream);
+int fgx_stream_wait_event(void* stream, void* ev);
+int fgx_event_elapsed_ms(void* start, void* stop, float* ms);
+/* Stream capture of whole steps (replaces the reference's per-layer host
+ * orchestration, src/flamegpu/simulation/CUDASimulation.cu:644-1109). */
+int fgx_graph_begin(void* stream);
+int fgx_graph_end(void* stream, fgx_graph** out);
+int fgx_graph_launch(fgx_graph* g, void* stream);
+int fgx_graph_destroy(fgx_graph* g);
+
+/* ---------------------------------------------------------------- scratch sizing
+ * CUB-style two-phase sizing (replaces CubTemporaryMemory,
+ * src/flamegpu/simulation/detail/CubTemporaryMemory.cu:23-32 and the
+ * scan/compaction buffers of CUDAScanCompaction, include/flamegpu/simulation/detail/CUDAScanCompaction.h:72-138). */
+enum {
+    FGX_OP_SPATIAL_BUILD = 0,  /* n = max messages, bins = grid bins */
+    FGX_OP_COMPACT = 1,        /* n = max items */
+    FGX_OP_BIRTHS = 2,         /* n = max parents */
+    FGX_OP_AGENT_SORT = 3,     /* n = max agents, bins = 2^key_bits */
+    FGX_OP_SCAN = 4            /* n = max items */
+};
+int fgx_scratch_bytes(int op, uint32_t n, uint32_t bins, size_t* bytes);
+int fgx_scratch_clear(void* scratch, size_t bytes, void* stream);
+
+/* ---------------------------------------------------------------- spatial grid metadata
+ * MessageSpatial3D::CUDAModelHandler ctor (src/flamegpu/runtime/messaging/MessageSpatial3D.cu:31-52),
+ * 2-D twin (src/flamegpu/runtime/messaging/MessageSpatial2D.cu:36-54),
+ * MetaData (include/flamegpu/runtime/messaging/MessageSpatial3D.h:38-68). */
+typedef struct {
+    int32_t dims;          /* 2 or 3 */
+    float min[3], max[3];
+    float radius;
+    float width[3];
+    uint32_t grid[3];      /* grid[2] == 1 in 2-D */
+    uint32_t bins;
+    int32_t wrap_ok;       /* approxExactlyDivisible(width, radius) on every axis */
+} fgx_spatial_meta;
+int fgx_spatial_meta_init(int dims, const float* min, const float* max, float radius, fgx_spatial_meta* out);
+
+/* ---------------------------------------------------------------- message-list build (PBM)
+ * Replaces MessageSpatial3D/2D::CUDAModelHandler::buildIndex
+ * (src/flamegpu/runtime/messaging/MessageSpatial3D.cu:113-146,
+ *  src/flamegpu/runtime/messaging/MessageSpatial2D.cu:113-146):
+ * atomicHistogram3D (:54-72) + cub::DeviceScan::ExclusiveSum (:134) +
+ * CUDAScatter::pbm_reorder (src/flamegpu/simulation/detail/CUDAScatter.cu:276-337).
+ * Bins each message (x,y,z; z ignored in 2-D), writes pbm[bins+1] (bin start
+ * table, pbm[bins] = n) and scatters every variable into vars_out in bin
+ * order, STABLY (input order within a bin).  n = *d_n if d_n != NULL
+ * (device-resident count, must be <= n_cap), else n_cap. */
+int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t* d_n, uint32_t n_cap,
+                      const float* x, const float* y, const float* z,
+                      uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                      uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream);
+
+/* ---------------------------------------------------------------- auto spatial agent sort
+ * Replaces CUDASimulation::spatialSortAgent_async (src/flamegpu/simulation/CUDASimulation.cu:463-573):
+ * key = floor(((p-min)/width)*grid) unclamped, masked to floor(log2(bins))+1 bits
+ * (calculateSpatialHash, :372-408), stable sort by key (cub::DeviceRadixSort::SortPairs,
+ * include/flamegpu/runtime/agent/HostAgentAPI.cuh:858-924), gather of every
+ * agent variable (scatterSort_async, src/flamegpu/simulation/detail/CUDAFatAgentStateList.cu:228-243). */
+int fgx_agent_sort_key_bits(int dims, const float* min, const float* max, float radius, uint32_t* key_bits);
+int fgx_agent_sort(fgx_ctx* ctx, int dims, const float* min, const float* max, float radius,
+                   const uint32_t* d_n, uint32_t n_cap, const float* x, const float* y, const float* z,
+                   uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                   void* scratch, size_t scratch_bytes, void* stream);
+
+/* ---------------------------------------------------------------- compaction
+ * Replaces CUDAScatter::scatter + the death scan (scatter_generic,
+ * src/flamegpu/simulation/detail/CUDAScatter.cu:68-88,140-179;
+ * CUDAFatAgent::processDeath, src/flamegpu/simulation/detail/CUDAFatAgent.cu:105-137):
+ * one pass (decoupled look-back scan fused with the scatter) that keeps the
+ * items whose flag != 0 equals keep != 0, stably, over every variable.  The
+ * kept count is written to *d_count_out (device memory). */
+int fgx_compact(fgx_ctx* ctx, const uint32_t* flags, uint32_t keep, const uint32_t* d_n, uint32_t n_cap,
+                uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                uint32_t* d_count_out, void* scratch, size_t scratch_bytes, void* stream);
+
+/* ---------------------------------------------------------------- birth append
+ * Replaces CUDAAgentStateList::scatterNew (src/flamegpu/simulation/detail/CUDAAgentStateList.cu:189-250)
+ * and the device ID counter (CUDAFatAgent::getDeviceNextID/notifyDeviceBirths,
+ * src/flamegpu/simulation/detail/CUDAFatAgent.cu:320-346; DeviceAPI::AgentOut::genID,
+ * include/flamegpu/runtime/DeviceAPI.cuh:546-557).  Parents with flags[i] != 0
+ * append newbuf[*][i] after *d_dst_count entries of dst, in parent order;
+ * variable id_var (-1: none) receives *d_next_id + rank.  *d_dst_count and
+ * *d_next_id advance by the number of births.  Births beyond dst_capacity
+ * are dropped and reported through *d_overflow (set to 1) when non-NULL. */
+int fgx_append_births(fgx_ctx* ctx, const uint32_t* flags, const uint32_t* d_n_parents, uint32_t n_parents_cap,
+                      uint32_t nvars, const void* const* newbuf, void* const* dst, const uint32_t* var_bytes,
+                      int32_t id_var, uint32_t* d_dst_count, uint32_t dst_capacity, uint32_t* d_next_id,
+                      uint32_t* d_overflow, void* scratch, size_t scratch_bytes, void* stream);
+
+/* ---------------------------------------------------------------- scan
+ * Exclusive prefix sum of n u32 (cub::DeviceScan::ExclusiveSum call sites,
+ * e.g. src/flamegpu/runtime/messaging/MessageSpatial3D.cu:134). */
+int fgx_exclusive_scan_u32(fgx_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
+                           void* scratch, size_t scratch_bytes, void* stream);
+
+/* ---------------------------------------------------------------- ID assignment
+ * allocateIDs (src/flamegpu/simulation/detail/CUDAFatAgent.cu:311-319): ids[i] = next_id + i
+ * where ids[i] == 0 (ID_NOT_SET). */
+int fgx_assign_ids(uint32_t* ids, uint32_t n, uint32_t next_id, void* stream);
+
+/* ---------------------------------------------------------------- reductions (host agent API)
+ * Deterministic on-device reductions for HostAgentAPI::count/sum
+ * (include/flamegpu/runtime/agent/HostAgentAPI.cuh:174,689): fixed-shape
+ * tree so the result does not depend on scheduling.  type: 0 f32, 1 f64, 2 i32, 3 u32, 4 i64, 5 u64.
+ * Writes a double (f32/f64) or 64-bit integer to *d_out. */
+int fgx_reduce_sum(fgx_ctx* ctx, int type, const void* in, const uint32_t* d_n, uint32_t n_cap,
+                   void* d_out, void* scratch, size_t scratch_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif  /* FGX_H_ */

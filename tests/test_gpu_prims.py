@@ -1,0 +1,253 @@
+"""GPU parity of the libfgx primitives against the serial oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_util import dev, empty, host, ptr, sync, scratch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from flamegpu2_amd import fgx
+    c = fgx.Context(0)
+    yield c
+    c.close()
+
+
+def _build_case(ctx, dims, n, width, radius, seed, use_dn=False, spread=0.0):
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(seed)
+    P = rng.uniform(-spread * width, (1 + spread) * width, size=(dims, n)).astype(np.float32)
+    ids = (np.arange(n, dtype=np.uint32) * 7 + 3).astype(np.uint32)
+    extra = rng.uniform(size=n)                          # 8-byte variable
+    mn, mx = [0.0] * dims, [float(width)] * dims
+    mo = O.spatial_meta(dims, mn, mx, radius)
+    pbm_o, perm_o, outs_o = O.spatial_build(mo, P[0], P[1], P[2] if dims == 3 else None, extra=[ids, extra])
+    m = fgx.spatial_meta(dims, mn, mx, radius)
+    assert m.bins == mo.bins
+    d_in = [dev(P[a]) for a in range(dims)] + [dev(ids), dev(extra)]
+    d_out = [empty(n, np.float32) for _ in range(dims)] + [empty(n, np.uint32), empty(n, np.float64)]
+    nbytes = [4] * dims + [4, 8]
+    pbm = empty(m.bins + 1, np.uint32)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_SPATIAL_BUILD, n, m.bins)
+    s = scratch(sb)
+    d_n = None
+    cap = n
+    if use_dn:
+        cnt = dev(np.array([n], np.uint32))
+        d_n = ptr(cnt)
+        cap = n  # capacity == n; the count is read on the device
+    for _ in range(2):   # second call reuses the scratch (look-back epoch)
+        fgx.spatial_build(ctx, m, cap, ptr(d_in[0]), ptr(d_in[1]), ptr(d_in[2]) if dims == 3 else None,
+                          [ptr(t) for t in d_in], [ptr(t) for t in d_out], nbytes, ptr(pbm), ptr(s), sb, d_n=d_n)
+    sync()
+    assert np.array_equal(host(pbm, np.uint32, m.bins + 1), pbm_o)
+    for a in range(dims):
+        assert np.array_equal(host(d_out[a], np.float32, n), outs_o[a])
+    assert np.array_equal(host(d_out[dims], np.uint32, n), outs_o[dims])
+    assert np.array_equal(host(d_out[dims + 1], np.float64, n), outs_o[dims + 1])
+
+
+@pytest.mark.parametrize("dims,n,width,radius,seed", [
+    (3, 16384, float(np.float32(np.cbrt(16384.0))), 2.0, 12),     # circles config
+    (3, 1 << 20, 3.2, 0.05, 7),                                   # boids config: 64^3 bins
+    (2, 250000, 1000.0, 5.0, 3),                                  # predator-prey grid
+    (3, 1, 5.0, 1.0, 1),
+    (3, 2049, 5.0, 1.0, 2),
+])
+def test_spatial_build_bit_exact(ctx, dims, n, width, radius, seed):
+    _build_case(ctx, dims, n, width, radius, seed)
+
+
+def test_spatial_build_out_of_range_and_device_count(ctx):
+    _build_case(ctx, 3, 50000, 10.0, 1.0, 5, use_dn=True, spread=0.3)
+
+
+def test_spatial_build_empty(ctx):
+    from flamegpu2_amd import fgx
+    m = fgx.spatial_meta(3, [0, 0, 0], [4, 4, 4], 1.0)
+    pbm = dev(np.full(m.bins + 1, 7, np.uint32))
+    sb = fgx.scratch_bytes(fgx.FGX_OP_SPATIAL_BUILD, 0, m.bins)
+    s = scratch(sb)
+    fgx.spatial_build(ctx, m, 0, None, None, None, [], [], [], ptr(pbm), ptr(s), sb)
+    sync()
+    assert np.all(host(pbm, np.uint32) == 0)
+
+
+def test_spatial_build_clumped_bins(ctx):
+    # many messages in a few bins: the stable in-bin ranking must still hold
+    rng = np.random.default_rng(9)
+    n = 20000
+    P = rng.uniform(0, 0.3, size=(3, n)).astype(np.float32)   # 27 bins of a 10^3 grid
+    from flamegpu2_amd import fgx
+    mo = O.spatial_meta(3, [0] * 3, [1.0] * 3, 0.1)
+    ids = np.arange(n, dtype=np.uint32)
+    pbm_o, perm_o, outs_o = O.spatial_build(mo, P[0], P[1], P[2], extra=[ids])
+    m = fgx.spatial_meta(3, [0] * 3, [1.0] * 3, 0.1)
+    d_in = [dev(P[a]) for a in range(3)] + [dev(ids)]
+    d_out = [empty(n, np.float32) for _ in range(3)] + [empty(n, np.uint32)]
+    pbm = empty(m.bins + 1, np.uint32)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_SPATIAL_BUILD, n, m.bins)
+    s = scratch(sb)
+    fgx.spatial_build(ctx, m, n, ptr(d_in[0]), ptr(d_in[1]), ptr(d_in[2]), [ptr(t) for t in d_in],
+                      [ptr(t) for t in d_out], [4, 4, 4, 4], ptr(pbm), ptr(s), sb)
+    sync()
+    assert np.array_equal(host(pbm, np.uint32, m.bins + 1), pbm_o)
+    assert np.array_equal(host(d_out[3], np.uint32, n), outs_o[3])
+
+
+@pytest.mark.parametrize("dims,n,mn,mx,r,seed", [
+    (3, 1 << 20, [0] * 3, [3.2] * 3, 0.05, 7),
+    (3, 16384, [0] * 3, [25.398416] * 3, 2.0, 12),
+    (2, 100000, [0, 0], [1000, 1000], 5.0, 3),
+    (3, 4, [-5] * 3, [5] * 3, 0.2, 0),
+])
+def test_agent_sort_bit_exact(ctx, dims, n, mn, mx, r, seed):
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(seed)
+    lo, hi = np.float32(mn[0]), np.float32(mx[0])
+    P = rng.uniform(lo - 0.1 * (hi - lo), hi + 0.1 * (hi - lo), size=(dims, n)).astype(np.float32)
+    if n == 4:
+        P = np.stack([-np.arange(4, dtype=np.float32)] * dims)
+    keys, bits = O.agent_sort_keys(dims, mn, mx, r, P[0], P[1], P[2] if dims == 3 else None)
+    perm = O.stable_sort_perm(keys)
+    order = np.arange(n, dtype=np.uint32)
+    d_in = [dev(P[a]) for a in range(dims)] + [dev(order)]
+    d_out = [empty(n, np.float32) for _ in range(dims)] + [empty(n, np.uint32)]
+    sb = fgx.scratch_bytes(fgx.FGX_OP_AGENT_SORT, n, 1 << bits)
+    s = scratch(sb)
+    fgx.agent_sort(ctx, dims, mn, mx, r, n, ptr(d_in[0]), ptr(d_in[1]), ptr(d_in[2]) if dims == 3 else None,
+                   [ptr(t) for t in d_in], [ptr(t) for t in d_out], [4] * (dims + 1), ptr(s), sb)
+    sync()
+    assert np.array_equal(host(d_out[dims], np.uint32, n), perm)
+    if n == 4:
+        assert perm.tolist() == [3, 2, 1, 0]
+
+
+@pytest.mark.parametrize("n,p,keep", [(1, 1.0, 1), (2047, 0.5, 1), (2048, 0.9, 1), (2049, 0.1, 0),
+                                      (1_250_000, 0.7, 1), (0, 0.5, 1)])
+def test_compact_stable(ctx, n, p, keep):
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(n)
+    flags = (rng.uniform(size=n) < p).astype(np.uint32)
+    a = rng.uniform(size=n).astype(np.float32)
+    b = np.arange(n, dtype=np.uint32)
+    c = rng.uniform(size=n)
+    cnt_o, outs_o = O.compact(flags, keep, [a, b, c])
+    d_f = dev(flags)
+    d_in = [dev(a), dev(b), dev(c)]
+    d_out = [empty(n, np.float32), empty(n, np.uint32), empty(n, np.float64)]
+    d_cnt = empty(1, np.uint32)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_COMPACT, n)
+    s = scratch(sb)
+    for _ in range(3):
+        fgx.compact(ctx, ptr(d_f), keep, n, [ptr(t) for t in d_in], [ptr(t) for t in d_out], [4, 4, 8],
+                    ptr(d_cnt), ptr(s), sb)
+    sync()
+    cnt = int(host(d_cnt, np.uint32)[0])
+    assert cnt == cnt_o
+    assert np.array_equal(host(d_out[0], np.float32, cnt), outs_o[0])
+    assert np.array_equal(host(d_out[1], np.uint32, cnt), outs_o[1])
+    assert np.array_equal(host(d_out[2], np.float64, cnt), outs_o[2])
+
+
+def test_compact_device_count(ctx):
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(3)
+    cap, n = 100000, 61234
+    flags = (rng.uniform(size=cap) < 0.4).astype(np.uint32)
+    b = np.arange(cap, dtype=np.uint32)
+    cnt_o, outs_o = O.compact(flags[:n], 1, [b[:n]])
+    d_f, d_b, d_o = dev(flags), dev(b), empty(cap, np.uint32)
+    d_n = dev(np.array([n], np.uint32))
+    d_cnt = empty(1, np.uint32)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_COMPACT, cap)
+    s = scratch(sb)
+    fgx.compact(ctx, ptr(d_f), 1, cap, [ptr(d_b)], [ptr(d_o)], [4], ptr(d_cnt), ptr(s), sb, d_n=ptr(d_n))
+    sync()
+    cnt = int(host(d_cnt, np.uint32)[0])
+    assert cnt == cnt_o and np.array_equal(host(d_o, np.uint32, cnt), outs_o[0])
+
+
+@pytest.mark.parametrize("n_par,existing,p", [(5000, 1234, 0.3), (1_000_000, 500_000, 0.02), (3, 0, 1.0)])
+def test_births_bit_exact(ctx, n_par, existing, p):
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(n_par)
+    flags = (rng.uniform(size=n_par) < p).astype(np.uint32)
+    newx = rng.uniform(size=n_par).astype(np.float32)
+    cap = existing + n_par
+    dstx = np.zeros(cap, np.float32); dstx[:existing] = 7.0
+    dstid = np.zeros(cap, np.uint32); dstid[:existing] = np.arange(existing) + 1
+    ox, oid = dstx.copy(), dstid.copy()
+    c_o, nid_o = O.births(flags, [newx, None], [ox, oid], existing, 1, existing + 1)
+    d_f, d_new = dev(flags), dev(newx)
+    d_x, d_id = dev(dstx), dev(dstid)
+    d_cnt = dev(np.array([existing], np.uint32))
+    d_nid = dev(np.array([existing + 1], np.uint32))
+    sb = fgx.scratch_bytes(fgx.FGX_OP_BIRTHS, n_par)
+    s = scratch(sb)
+    fgx.append_births(ctx, ptr(d_f), n_par, [ptr(d_new), 0], [ptr(d_x), ptr(d_id)], [4, 4], 1, ptr(d_cnt), cap,
+                      ptr(d_nid), ptr(s), sb)
+    sync()
+    assert int(host(d_cnt, np.uint32)[0]) == c_o
+    assert int(host(d_nid, np.uint32)[0]) == nid_o
+    assert np.array_equal(host(d_x, np.float32, c_o), ox[:c_o])
+    assert np.array_equal(host(d_id, np.uint32, c_o), oid[:c_o])
+
+
+def test_births_overflow_flag(ctx):
+    from flamegpu2_amd import fgx
+    n_par, existing, cap = 1000, 900, 950
+    flags = np.ones(n_par, np.uint32)
+    d_f, d_new = dev(flags), dev(np.arange(n_par, dtype=np.float32))
+    d_x = empty(cap, np.float32); d_id = empty(cap, np.uint32)
+    d_cnt = dev(np.array([existing], np.uint32)); d_nid = dev(np.array([1], np.uint32))
+    d_ovf = empty(1, np.uint32)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_BIRTHS, n_par)
+    s = scratch(sb)
+    fgx.append_births(ctx, ptr(d_f), n_par, [ptr(d_new), 0], [ptr(d_x), ptr(d_id)], [4, 4], 1, ptr(d_cnt), cap,
+                      ptr(d_nid), ptr(s), sb, d_overflow=ptr(d_ovf))
+    sync()
+    assert int(host(d_cnt, np.uint32)[0]) == cap
+    assert int(host(d_ovf, np.uint32)[0]) == 1
+    assert np.array_equal(host(d_x, np.float32)[existing:cap], np.arange(cap - existing, dtype=np.float32))
+
+
+@pytest.mark.parametrize("n", [1, 2, 2047, 2048, 2049, 262145, 10_000_000])
+def test_exclusive_scan(ctx, n):
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 64, size=n).astype(np.uint32)
+    expect = np.concatenate([[0], np.cumsum(a.astype(np.uint64))[:-1]]).astype(np.uint32)
+    d_a, d_o = dev(a), empty(n, np.uint32)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_SCAN, n)
+    s = scratch(sb)
+    fgx.exclusive_scan(ctx, ptr(d_a), ptr(d_o), n, ptr(s), sb)
+    fgx.exclusive_scan(ctx, ptr(d_a), ptr(d_o), n, ptr(s), sb)
+    sync()
+    assert np.array_equal(host(d_o, np.uint32, n), expect)
+
+
+def test_reduce_sum_deterministic(ctx):
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(1)
+    n = 3_000_001
+    a = rng.uniform(size=n).astype(np.float32)
+    u = rng.integers(0, 1000, size=n).astype(np.uint32)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_SCAN, n)
+    s = scratch(sb)
+    d_a, d_u = dev(a), dev(u)
+    outs = []
+    for _ in range(2):
+        o = empty(1, np.float64)
+        fgx.reduce_sum(ctx, 0, ptr(d_a), n, ptr(o), ptr(s), sb)
+        sync()
+        outs.append(float(host(o, np.float64)[0]))
+    assert outs[0] == outs[1]
+    assert abs(outs[0] - a.astype(np.float64).sum()) < 1e-6 * n
+    o = empty(1, np.uint64)
+    fgx.reduce_sum(ctx, 3, ptr(d_u), n, ptr(o), ptr(s), sb)
+    sync()
+    assert int(host(o, np.uint64)[0]) == int(u.astype(np.uint64).sum())
