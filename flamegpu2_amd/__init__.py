@@ -20,9 +20,12 @@ def lib_path(name):
 
 
 def load(name):
-    """Load a built shared library from flamegpu2_amd/lib; fail loudly if absent."""
+    """Load a built shared library from flamegpu2_amd/lib; fail loudly if absent.
+
+    Libraries are loaded with local symbol scope: every model library exports
+    the same C entry points and must resolve them against itself."""
     p = lib_path(name)
     if not os.path.exists(p):
         raise NativeLibraryMissing(
             f"{p} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` first")
-    return ctypes.CDLL(p, mode=ctypes.RTLD_GLOBAL)
+    return ctypes.CDLL(p, mode=ctypes.RTLD_LOCAL)

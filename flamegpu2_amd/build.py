@@ -21,6 +21,8 @@ OBJ = os.path.join(PKG, "lib", "obj")
 INC_C = os.path.join(ROOT, "include")
 INC_CPP = os.path.join(PKG, "include")
 EXAMPLES = os.path.join(PKG, "examples")
+TEST_MODELS = os.path.join(ROOT, "tests", "models")
+MODEL_DIRS = [EXAMPLES, TEST_MODELS]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -65,16 +67,26 @@ def build_libfgx(jobs=8):
 
 
 def model_targets():
-    """Model libraries: every flamegpu2_amd/examples/<name>.hip with FGX_MODEL_ABI -> lib/libfgxm_<name>.so."""
-    if not os.path.isdir(EXAMPLES):
-        return []
-    return sorted(f[:-4] for f in os.listdir(EXAMPLES) if f.endswith(".hip"))
+    """Model libraries: every <dir>/<name>.hip of MODEL_DIRS, built with FGX_MODEL_ABI -> lib/libfgxm_<name>.so."""
+    names = []
+    for d in MODEL_DIRS:
+        if os.path.isdir(d):
+            names += [f[:-4] for f in os.listdir(d) if f.endswith(".hip")]
+    return sorted(names)
+
+
+def _model_src(name):
+    for d in MODEL_DIRS:
+        p = os.path.join(d, name + ".hip")
+        if os.path.exists(p):
+            return p
+    raise FileNotFoundError(name)
 
 
 def build_model(name):
-    src = os.path.join(EXAMPLES, name + ".hip")
+    src = _model_src(name)
     obj = os.path.join(OBJ, "model_" + name + ".o")
-    _compile(src, obj, extra=["-DFGX_MODEL_ABI"])
+    _compile(src, obj, extra=["-DFGX_MODEL_ABI", "-fvisibility=hidden", "-fvisibility-inlines-hidden"])
     out = os.path.join(LIB, f"libfgxm_{name}.so")
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, obj, f"-L{LIB}", "-lfgx",

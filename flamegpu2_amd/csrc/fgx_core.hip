@@ -62,6 +62,10 @@ __global__ void k_fill_vars(FillArgs a) {
     }
 }
 
+__global__ void k_add_u32(uint32_t* p, uint32_t delta) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p += delta;
+}
+
 __global__ void k_assign_ids(uint32_t* __restrict__ ids, uint32_t n, uint32_t next_id) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && ids[i] == 0u) ids[i] = next_id + i;
@@ -161,6 +165,13 @@ int fgx_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* stream) {
     if (!n) return FGX_OK;
     const uint32_t blocks = std::min<uint32_t>(ceil_div(n, 256), 4096);
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, (hipStream_t)stream, dst, value, n);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+int fgx_add_u32(uint32_t* p, uint32_t delta, void* stream) {
+    if (!p) return fail(FGX_ERR_ARG, "fgx_add_u32: NULL pointer");
+    hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, (hipStream_t)stream, p, delta);
     FGX_HIP(hipGetLastError());
     return FGX_OK;
 }

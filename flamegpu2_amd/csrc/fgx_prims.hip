@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(kBlock) k_compact(CompactArgs a, VarTable vt) 
 #pragma unroll
     for (uint32_t j = 0; j < kItems; ++j) {
         const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-        const bool keep = i < n && ((a.flags[i] != 0u) == (a.keep != 0u));
+        const bool keep = i < n && (!a.flags || ((a.flags[i] != 0u) == (a.keep != 0u)));
         masks[j] = __ballot(keep);
         if (lane == 0) s_cnt[j * (kBlock / 64) + w] = (uint32_t)__builtin_popcountll(masks[j]);
     }
@@ -497,7 +497,6 @@ int fgx_append_births(fgx_ctx* ctx, const uint32_t* flags, const uint32_t* d_n_p
                       uint32_t* d_overflow, void* scratch, size_t scratch_bytes, void* stream) {
     (void)ctx;
     if (!d_dst_count || !d_next_id) return fail(FGX_ERR_ARG, "fgx_append_births: NULL device counters");
-    if (n_parents_cap && !flags) return fail(FGX_ERR_ARG, "fgx_append_births: flags is NULL");
     if (id_var >= (int32_t)nvars) return fail(FGX_ERR_ARG, "fgx_append_births: id_var out of range");
     if (id_var >= 0 && var_bytes[id_var] != 4) return fail(FGX_ERR_ARG, "fgx_append_births: id variable must be 32-bit");
     VarTable vt;

@@ -1,0 +1,116 @@
+// Circles, brute-force messaging: examples/cpp/circles_bruteforce/src/main.cu,
+// with the same box clamp as the spatial variant.
+#include <cstdio>
+#include <random>
+
+#include "flamegpu/flamegpu.h"
+
+FLAMEGPU_AGENT_FUNCTION(output_message, flamegpu::MessageNone, flamegpu::MessageBruteForce) {
+    FLAMEGPU->message_out.setVariable<flamegpu::id_t>("id", FLAMEGPU->getID());
+    FLAMEGPU->message_out.setVariable<float>("x", FLAMEGPU->getVariable<float>("x"));
+    FLAMEGPU->message_out.setVariable<float>("y", FLAMEGPU->getVariable<float>("y"));
+    FLAMEGPU->message_out.setVariable<float>("z", FLAMEGPU->getVariable<float>("z"));
+    return flamegpu::ALIVE;
+}
+
+FLAMEGPU_AGENT_FUNCTION(move, flamegpu::MessageBruteForce, flamegpu::MessageNone) {
+    const flamegpu::id_t ID = FLAMEGPU->getID();
+    const float REPULSE_FACTOR = FLAMEGPU->environment.getProperty<float>("repulse");
+    const float RADIUS = FLAMEGPU->environment.getProperty<float>("radius");
+    const float ENV_MAX = FLAMEGPU->environment.getProperty<float>("ENV_MAX");
+    float fx = 0.0;
+    float fy = 0.0;
+    float fz = 0.0;
+    const float x1 = FLAMEGPU->getVariable<float>("x");
+    const float y1 = FLAMEGPU->getVariable<float>("y");
+    const float z1 = FLAMEGPU->getVariable<float>("z");
+    int count = 0;
+    for (const auto &message : FLAMEGPU->message_in) {
+        if (message.getVariable<flamegpu::id_t>("id") != ID) {
+            const float x2 = message.getVariable<float>("x");
+            const float y2 = message.getVariable<float>("y");
+            const float z2 = message.getVariable<float>("z");
+            float x21 = x2 - x1;
+            float y21 = y2 - y1;
+            float z21 = z2 - z1;
+            const float separation = sqrtf(x21 * x21 + y21 * y21 + z21 * z21);
+            if (separation < RADIUS && separation > 0.0f) {
+                float k = sinf((separation / RADIUS) * 3.141f * -2) * REPULSE_FACTOR;
+                x21 /= separation;
+                y21 /= separation;
+                z21 /= separation;
+                fx += k * x21;
+                fy += k * y21;
+                fz += k * z21;
+                count++;
+            }
+        }
+    }
+    fx /= count > 0 ? count : 1;
+    fy /= count > 0 ? count : 1;
+    fz /= count > 0 ? count : 1;
+    float nx = x1 + fx, ny = y1 + fy, nz = z1 + fz;
+    nx = nx < 0.0f ? 0.0f : (nx > ENV_MAX ? ENV_MAX : nx);
+    ny = ny < 0.0f ? 0.0f : (ny > ENV_MAX ? ENV_MAX : ny);
+    nz = nz < 0.0f ? 0.0f : (nz > ENV_MAX ? ENV_MAX : nz);
+    FLAMEGPU->setVariable<float>("x", nx);
+    FLAMEGPU->setVariable<float>("y", ny);
+    FLAMEGPU->setVariable<float>("z", nz);
+    FLAMEGPU->setVariable<float>("drift", sqrtf(fx * fx + fy * fy + fz * fz));
+    return flamegpu::ALIVE;
+}
+
+// params: {ENV_MAX, radius, repulse}
+flamegpu::ModelDescription fgxm_build_model(const char*, const float* p, uint32_t n) {
+    const float ENV_MAX = n > 0 ? p[0] : 25.0f;
+    flamegpu::ModelDescription model("Circles BruteForce");
+    {
+        flamegpu::MessageBruteForce::Description message = model.newMessage("location");
+        message.newVariable<flamegpu::id_t>("id");
+        message.newVariable<float>("x");
+        message.newVariable<float>("y");
+        message.newVariable<float>("z");
+    }
+    {
+        flamegpu::AgentDescription agent = model.newAgent("Circle");
+        agent.newVariable<float>("x");
+        agent.newVariable<float>("y");
+        agent.newVariable<float>("z");
+        agent.newVariable<float>("drift");
+        agent.newFunction("output_message", output_message).setMessageOutput("location");
+        agent.newFunction("move", move).setMessageInput("location");
+    }
+    {
+        flamegpu::EnvironmentDescription env = model.Environment();
+        env.newProperty("repulse", n > 2 ? p[2] : 0.05f);
+        env.newProperty("radius", n > 1 ? p[1] : 2.0f);
+        env.newProperty("ENV_MAX", ENV_MAX);
+    }
+    model.newLayer().addAgentFunction(output_message);
+    model.newLayer().addAgentFunction(move);
+    return model;
+}
+
+#ifdef FGX_MODEL_ABI
+#include "flamegpu/model_abi.h"
+#else
+int main(int argc, const char** argv) {
+    const unsigned int AGENT_COUNT = 1024;
+    const float ENV_MAX = static_cast<float>(floor(cbrt(AGENT_COUNT)));
+    const float params[3] = {ENV_MAX, 2.0f, 0.05f};
+    flamegpu::ModelDescription model = fgxm_build_model(nullptr, params, 3);
+    flamegpu::CUDASimulation cudaSimulation(model, argc, argv);
+    std::mt19937_64 rng;
+    std::uniform_real_distribution<float> dist(0.0f, ENV_MAX);
+    flamegpu::AgentVector population(model.Agent("Circle"), AGENT_COUNT);
+    for (unsigned int i = 0; i < AGENT_COUNT; i++) {
+        flamegpu::AgentVector::Agent instance = population[i];
+        instance.setVariable<float>("x", dist(rng));
+        instance.setVariable<float>("y", dist(rng));
+        instance.setVariable<float>("z", dist(rng));
+    }
+    cudaSimulation.setPopulationData(population);
+    cudaSimulation.simulate();
+    return 0;
+}
+#endif

@@ -1,0 +1,282 @@
+// Core definitions shared by the host runtime and the generated agent kernels.
+//
+// Replaces include/flamegpu/defines.h, include/flamegpu/exception/FLAMEGPUException.h
+// and the curve hash table (include/flamegpu/runtime/detail/curve/DeviceCurve.cuh):
+// a variable name is hashed at compile time (FNV-1a over the string literal) and
+// resolved against a small binding table that travels in the kernel argument
+// segment, so every lookup is a handful of scalar compares that the compiler
+// hoists out of message loops.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdarg>
+#include <stdexcept>
+#include <string>
+#include <typeindex>
+
+#include "fgx.h"
+
+namespace flamegpu {
+
+// include/flamegpu/defines.h
+typedef unsigned int id_t;
+typedef unsigned int size_type;
+constexpr id_t ID_NOT_SET = 0;
+
+// include/flamegpu/runtime/AgentFunction.cuh:19
+enum AGENT_STATUS { ALIVE = 1, DEAD = 0 };
+typedef AGENT_STATUS FLAMEGPU_AGENT_STATUS;
+// include/flamegpu/runtime/HostAPI_macros.h
+enum CONDITION_RESULT { CONTINUE = 0, EXIT = 1 };
+typedef CONDITION_RESULT FLAMEGPU_CONDITION_RESULT;
+
+namespace exception {
+// include/flamegpu/exception/FLAMEGPUException.h:60-300 (the subset raised on this path)
+class FLAMEGPUException : public std::exception {
+ public:
+    explicit FLAMEGPUException(std::string msg = "") : msg_(std::move(msg)) {}
+    const char* what() const noexcept override { return msg_.c_str(); }
+    virtual const char* exception_type() const { return "FLAMEGPUException"; }
+
+ protected:
+    std::string msg_;
+};
+#define FGX_DERIVED_EXCEPTION(name)                                                    \
+    class name : public FLAMEGPUException {                                            \
+     public:                                                                           \
+        explicit name(const std::string& m = "") : FLAMEGPUException(m) {}             \
+        const char* exception_type() const override { return #name; }                 \
+    };
+FGX_DERIVED_EXCEPTION(CUDAError)
+FGX_DERIVED_EXCEPTION(InvalidArgument)
+FGX_DERIVED_EXCEPTION(InvalidAgentVar)
+FGX_DERIVED_EXCEPTION(InvalidAgentName)
+FGX_DERIVED_EXCEPTION(InvalidAgentState)
+FGX_DERIVED_EXCEPTION(InvalidAgentFunc)
+FGX_DERIVED_EXCEPTION(InvalidFuncLayerIndx)
+FGX_DERIVED_EXCEPTION(InvalidLayerMember)
+FGX_DERIVED_EXCEPTION(InvalidMessageName)
+FGX_DERIVED_EXCEPTION(InvalidMessageType)
+FGX_DERIVED_EXCEPTION(InvalidMessageVar)
+FGX_DERIVED_EXCEPTION(InvalidEnvProperty)
+FGX_DERIVED_EXCEPTION(InvalidEnvPropertyType)
+FGX_DERIVED_EXCEPTION(InvalidDependencyGraph)
+FGX_DERIVED_EXCEPTION(InvalidOperation)
+FGX_DERIVED_EXCEPTION(InvalidVarType)
+FGX_DERIVED_EXCEPTION(InvalidCUDAdevice)
+FGX_DERIVED_EXCEPTION(DeviceError)
+FGX_DERIVED_EXCEPTION(OutOfBoundsException)
+FGX_DERIVED_EXCEPTION(UnsupportedVarType)
+FGX_DERIVED_EXCEPTION(EnsembleError)
+FGX_DERIVED_EXCEPTION(InvalidStateName)
+FGX_DERIVED_EXCEPTION(ReservedName)
+FGX_DERIVED_EXCEPTION(DuplicateEnvProperty)
+FGX_DERIVED_EXCEPTION(InvalidHostFunc)
+FGX_DERIVED_EXCEPTION(InvalidPopulationData)
+FGX_DERIVED_EXCEPTION(InvalidConfig)
+#undef FGX_DERIVED_EXCEPTION
+}  // namespace exception
+
+namespace detail {
+
+inline std::string format(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    return std::string(buf);
+}
+
+// Raise a C-ABI failure as flamegpu::exception::CUDAError (or the mapped type).
+inline void fgx_check(int rc, const char* what) {
+    if (rc == FGX_OK) return;
+    const std::string m = std::string(what) + ": " + fgx_last_error();
+    if (rc == FGX_ERR_ARG) throw exception::InvalidArgument(m);
+    if (rc == FGX_ERR_DEVICE) throw exception::DeviceError(m);
+    throw exception::CUDAError(m);
+}
+
+// FNV-1a over the characters of a name (compile-time for string literals).
+__host__ __device__ constexpr uint32_t name_hash(const char* s, unsigned n) {
+    uint32_t h = 2166136261u;
+    for (unsigned i = 0; i < n && s[i]; ++i) {
+        h ^= (uint8_t)s[i];
+        h *= 16777619u;
+    }
+    return h;
+}
+inline uint32_t name_hash(const std::string& s) { return name_hash(s.c_str(), (unsigned)s.size()); }
+
+// ------------------------------------------------------------------ kernel ABI
+// Binding tables are perfect hashes built on the host: a name with hash h lives
+// in slot (h * mul) >> (32 - log2(SLOTS)).  A device lookup is one multiply, one
+// shift and one scalar load from the kernel argument segment, free of loops, so
+// the compiler hoists it out of message loops.
+constexpr unsigned MAX_AGENT_VARS = 16;
+constexpr unsigned MAX_MSG_VARS = 16;
+constexpr unsigned MAX_ENV_PROPS = 32;
+constexpr unsigned VAR_SLOTS = 32;
+constexpr unsigned VAR_SLOT_BITS = 5;
+constexpr unsigned ENV_SLOTS = 64;
+constexpr unsigned ENV_SLOT_BITS = 6;
+
+struct VarBinding {
+    uint32_t hash;
+    uint32_t bytes;   // element bytes (type size x array length)
+    char* ptr;
+};
+
+struct EnvBinding {
+    uint32_t hash;
+    uint16_t offset;
+    uint16_t bytes;
+};
+
+// Device mirror of the spatial message metadata (MessageSpatial3D.h:38-68).
+struct SpatialDev {
+    float min[3];
+    float max[3];
+    float radius;
+    float width[3];
+    uint32_t grid[3];
+    int32_t wrap_ok;
+    const uint32_t* pbm;
+};
+
+// Device error codes (the device half of exception::DeviceError, replacing DTHROW).
+enum DeviceErrorCode : uint32_t {
+    DERR_NONE = 0,
+    DERR_UNKNOWN_AGENT_VAR = 1,
+    DERR_AGENT_VAR_TYPE = 2,
+    DERR_UNKNOWN_MSG_VAR = 3,
+    DERR_MSG_VAR_TYPE = 4,
+    DERR_WRAP_OUT_OF_BOUNDS = 5,
+    DERR_WRAP_NOT_FACTOR = 6,
+    DERR_DEATH_DISABLED = 7,
+    DERR_AGENT_OUTPUT_DISABLED = 8,
+    DERR_RESERVED_NAME = 9,
+    DERR_UNKNOWN_ENV = 10,
+    DERR_ENV_TYPE = 11,
+    DERR_AGENT_OUT_ID = 12,
+    DERR_ARRAY_INDEX = 13,
+    DERR_RANDOM_RANGE = 14,
+    DERR_MSG_OUTPUT_DISABLED = 15,
+};
+
+inline const char* device_error_text(uint32_t code) {
+    switch (code) {
+        case DERR_UNKNOWN_AGENT_VAR: return "unknown agent variable";
+        case DERR_AGENT_VAR_TYPE: return "agent variable accessed with the wrong type or length";
+        case DERR_UNKNOWN_MSG_VAR: return "unknown message variable";
+        case DERR_MSG_VAR_TYPE: return "message variable accessed with the wrong type or length";
+        case DERR_WRAP_OUT_OF_BOUNDS: return "location exceeds environment bounds, unsupported by the wrapped iterator (MessageSpatial::In::wrap())";
+        case DERR_WRAP_NOT_FACTOR: return "spatial messaging radius is not a factor of the environment dimensions, unsupported by the wrapped iterator";
+        case DERR_DEATH_DISABLED: return "agent death must be enabled per agent function when defining the model";
+        case DERR_AGENT_OUTPUT_DISABLED: return "agent output must be enabled per agent function when defining the model";
+        case DERR_RESERVED_NAME: return "variable names starting with '_' are reserved for internal use";
+        case DERR_UNKNOWN_ENV: return "unknown environment property";
+        case DERR_ENV_TYPE: return "environment property accessed with the wrong type or length";
+        case DERR_AGENT_OUT_ID: return "AgentOut::getID() is not available: new agent IDs are assigned by the ordered birth scan";
+        case DERR_ARRAY_INDEX: return "array variable index out of bounds";
+        case DERR_RANDOM_RANGE: return "invalid arguments passed to AgentRandom::uniform()";
+        case DERR_MSG_OUTPUT_DISABLED: return "message output used by a function without message output";
+        default: return "unknown device error";
+    }
+}
+
+struct FnArgs {
+    uint32_t n_cap;              // launch bound (state list capacity)
+    uint32_t stream_id;          // RNG stream of this agent function
+    const uint32_t* d_n;         // device-resident agent count
+    const uint32_t* d_step;      // device-resident step counter
+    uint64_t seed;
+    uint32_t* d_error;           // [code, detail hash, agent index]
+    uint32_t* death_flags;       // non-null when agent death is enabled
+    uint32_t* msg_out_flags;     // non-null with message output
+    uint32_t* birth_flags;       // non-null with agent output
+    const char* env;             // environment property buffer (device)
+    const uint32_t* d_msg_in_count;
+    const uint32_t* ids;         // the agent list's "_id" array
+    int32_t msg_out_optional;
+    uint32_t agent_mul, msg_in_mul, msg_out_mul, new_mul, env_mul;
+    SpatialDev in_meta;
+    VarBinding agent[VAR_SLOTS];
+    VarBinding msg_in[VAR_SLOTS];
+    VarBinding msg_out[VAR_SLOTS];
+    VarBinding new_agent[VAR_SLOTS];
+    EnvBinding env_b[ENV_SLOTS];
+};
+
+struct Found {
+    char* ptr;
+    uint32_t bytes;
+};
+
+__host__ __device__ constexpr uint32_t table_slot(uint32_t h, uint32_t mul, unsigned bits) {
+    return (h * mul) >> (32u - bits);
+}
+
+__device__ __forceinline__ Found find_binding(const VarBinding (&tab)[VAR_SLOTS], uint32_t mul, uint32_t h) {
+    const VarBinding& b = tab[table_slot(h, mul, VAR_SLOT_BITS)];
+    if (b.hash != h) return Found{nullptr, 0};
+    return Found{b.ptr, b.bytes};
+}
+
+__device__ __forceinline__ void report_error(uint32_t* d_error, uint32_t code, uint32_t detail, uint32_t index) {
+    if (!d_error) return;
+    if (atomicCAS(&d_error[0], 0u, code) == 0u) {
+        d_error[1] = detail;
+        d_error[2] = index;
+    }
+}
+
+// Host: pick a multiplier that places every key in its own slot.
+inline uint32_t perfect_hash_mul(const uint32_t* hashes, unsigned n, unsigned bits) {
+    const unsigned slots = 1u << bits;
+    for (uint32_t k = 0; k < 1000000u; ++k) {
+        const uint32_t mul = 2654435761u + 2u * k * 40503u;
+        uint64_t used[2] = {0, 0};
+        bool ok = true;
+        for (unsigned i = 0; i < n && ok; ++i) {
+            const uint32_t s = table_slot(hashes[i], mul, bits);
+            if (used[s >> 6] & (1ull << (s & 63))) ok = false;
+            used[s >> 6] |= 1ull << (s & 63);
+        }
+        if (ok) return mul;
+    }
+    (void)slots;
+    throw exception::InvalidOperation("could not build a collision-free variable table");
+}
+
+// Philox4x32-10 (Salmon et al., SC'11), used for the counter-based agent RNG.
+__host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+#if defined(__HIP_DEVICE_COMPILE__)
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+#else
+        const uint32_t hi0 = (uint32_t)(((uint64_t)0xD2511F53u * c[0]) >> 32);
+        const uint32_t hi1 = (uint32_t)(((uint64_t)0xCD9E8D57u * c[2]) >> 32);
+#endif
+        const uint32_t lo0 = 0xD2511F53u * c[0];
+        const uint32_t lo1 = 0xCD9E8D57u * c[2];
+        const uint32_t n0 = hi1 ^ c[1] ^ k0;
+        const uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+    }
+}
+
+}  // namespace detail
+}  // namespace flamegpu

@@ -1,0 +1,1715 @@
+// Host runtime: AgentVector, HostAPI, CUDASimulation, RunPlan(Vector), CUDAEnsemble.
+//
+// Same surface as include/flamegpu/simulation/{CUDASimulation,AgentVector,RunPlan,
+// RunPlanVector,CUDAEnsemble}.h and include/flamegpu/runtime/HostAPI.h.  The step
+// (src/flamegpu/simulation/CUDASimulation.cu:575-1109) is re-designed:
+//   * all device work goes through the libfgx C ABI, stream-ordered, with every
+//     agent/message count kept in device memory (no per-compaction readback);
+//   * the host only synchronises once per step, and only when a model can grow
+//     (births, state transitions), to size buffers for the step's upper bound;
+//   * functions of one layer run on their own streams (fork/join events);
+//   * whole steps are captured into a hipGraph and replayed.
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdlib>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <random>
+#include <set>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "flamegpu/runtime/device.h"
+
+namespace flamegpu {
+
+class CUDASimulation;
+
+// ------------------------------------------------------------------ AgentVector
+// include/flamegpu/simulation/AgentVector.h: host-side population, stored as
+// one contiguous array per variable.
+class AgentVector {
+ public:
+    class Agent {
+     public:
+        Agent(AgentVector* v, size_t i) : v_(v), i_(i) {}
+        template <typename T>
+        void setVariable(const std::string& name, T value) {
+            std::memcpy(v_->elem<T>(name, 1, 0, i_), &value, sizeof(T));
+        }
+        template <typename T, size_type N>
+        void setVariable(const std::string& name, const std::array<T, N>& value) {
+            std::memcpy(v_->elem<T>(name, N, 0, i_), value.data(), sizeof(T) * N);
+        }
+        template <typename T, size_type N = 0>
+        void setVariable(const std::string& name, unsigned int index, T value) {
+            const size_type len = v_->length(name);
+            if (index >= len) throw exception::OutOfBoundsException("Index out of bounds for variable '" + name + "'");
+            std::memcpy(v_->elem<T>(name, len, index, i_), &value, sizeof(T));
+        }
+        template <typename T>
+        T getVariable(const std::string& name) const {
+            T r;
+            std::memcpy(&r, v_->elem<T>(name, 1, 0, i_), sizeof(T));
+            return r;
+        }
+        template <typename T, size_type N>
+        std::array<T, N> getVariable(const std::string& name) const {
+            std::array<T, N> r;
+            std::memcpy(r.data(), v_->elem<T>(name, N, 0, i_), sizeof(T) * N);
+            return r;
+        }
+        template <typename T, size_type N = 0>
+        T getVariable(const std::string& name, unsigned int index) const {
+            const size_type len = v_->length(name);
+            if (index >= len) throw exception::OutOfBoundsException("Index out of bounds for variable '" + name + "'");
+            T r;
+            std::memcpy(&r, v_->elem<T>(name, len, index, i_), sizeof(T));
+            return r;
+        }
+        id_t getID() const { return getVariable<id_t>("_id"); }
+        size_t getIndex() const { return i_; }
+
+     private:
+        AgentVector* v_;
+        size_t i_;
+    };
+    typedef Agent CAgent;
+
+    class iterator {
+     public:
+        iterator(AgentVector* v, size_t i) : v_(v), i_(i) {}
+        Agent operator*() const { return Agent(v_, i_); }
+        iterator& operator++() {
+            ++i_;
+            return *this;
+        }
+        bool operator!=(const iterator& o) const { return i_ != o.i_; }
+        bool operator==(const iterator& o) const { return i_ == o.i_; }
+
+     private:
+        AgentVector* v_;
+        size_t i_;
+    };
+
+    explicit AgentVector(const CAgentDescription& agent, size_t count = 0) : a_(agent.data()) {
+        for (auto& kv : a_->variables) data_[kv.first];
+        resize(count);
+    }
+    Agent operator[](size_t i) { return Agent(this, i); }
+    Agent at(size_t i) {
+        if (i >= n_) throw exception::OutOfBoundsException("AgentVector index out of bounds");
+        return Agent(this, i);
+    }
+    Agent front() { return at(0); }
+    Agent back() { return at(n_ - 1); }
+    iterator begin() { return iterator(this, 0); }
+    iterator end() { return iterator(this, n_); }
+    size_t size() const { return n_; }
+    bool empty() const { return n_ == 0; }
+    void clear() { resize(0); }
+    void push_back() { resize(n_ + 1); }
+    void resize(size_t n) {
+        for (auto& kv : a_->variables) {
+            const Variable& v = kv.second;
+            auto& buf = data_[kv.first];
+            const size_t b = v.bytes();
+            buf.resize(n * b);
+            for (size_t i = n_; i < n; ++i) std::memcpy(buf.data() + i * b, v.default_value.data(), b);
+        }
+        n_ = n;
+    }
+    std::string getAgentName() const { return a_->name; }
+    const std::shared_ptr<AgentData>& agentData() const { return a_; }
+    // Raw access used by the runtime.
+    std::vector<char>& raw(const std::string& name) { return data_.at(name); }
+
+ private:
+    size_type length(const std::string& name) const {
+        auto it = a_->variables.find(name);
+        if (it == a_->variables.end())
+            throw exception::InvalidAgentVar("Agent '" + a_->name + "' does not contain variable '" + name + "'");
+        return it->second.elements;
+    }
+    template <typename T>
+    char* elem(const std::string& name, size_type n, size_type e, size_t i) {
+        auto it = a_->variables.find(name);
+        if (it == a_->variables.end())
+            throw exception::InvalidAgentVar("Agent '" + a_->name + "' does not contain variable '" + name + "'");
+        if (it->second.type != std::type_index(typeid(T)) || it->second.elements != n)
+            throw exception::InvalidVarType("Variable '" + name + "' accessed with the wrong type or length");
+        if (i >= n_) throw exception::OutOfBoundsException("AgentVector index out of bounds");
+        return data_.at(name).data() + i * it->second.bytes() + (size_t)e * sizeof(T);
+    }
+    std::shared_ptr<AgentData> a_;
+    std::map<std::string, std::vector<char>> data_;
+    size_t n_ = 0;
+};
+
+namespace detail {
+
+struct VarBuf {
+    std::string name;
+    uint32_t bytes = 0;
+    void* cur = nullptr;
+    void* alt = nullptr;
+    std::vector<char> def;
+};
+
+struct StateList {
+    std::string state;
+    std::vector<VarBuf> vars;     // name order, includes _id
+    int id_slot = -1;
+    uint32_t cap = 0;
+    uint32_t* d_count = nullptr;
+    uint32_t h_count = 0;
+    void* sort_scratch = nullptr;
+    size_t sort_scratch_bytes = 0;
+    int var_index(const std::string& n) const {
+        for (size_t i = 0; i < vars.size(); ++i)
+            if (vars[i].name == n) return (int)i;
+        return -1;
+    }
+};
+
+struct AgentRT {
+    std::shared_ptr<AgentData> d;
+    std::map<std::string, StateList> states;
+    uint32_t* d_next_id = nullptr;
+    uint32_t h_next_id = 1;
+};
+
+struct MessageRT {
+    std::shared_ptr<MessageData> d;
+    std::vector<VarBuf> vars;
+    uint32_t cap = 0;
+    uint32_t* d_count = nullptr;
+    fgx_spatial_meta meta{};
+    bool spatial = false;
+    uint32_t* pbm = nullptr;
+    void* build_scratch = nullptr;
+    size_t build_scratch_bytes = 0;
+    bool truncate = true;
+    bool pbm_dirty = true;
+    int var_index(const std::string& n) const {
+        for (size_t i = 0; i < vars.size(); ++i)
+            if (vars[i].name == n) return (int)i;
+        return -1;
+    }
+};
+
+struct FnRT {
+    std::shared_ptr<AgentFunctionData> d;
+    AgentRT* agent = nullptr;
+    std::string in_state, end_state;
+    MessageRT* min = nullptr;
+    MessageRT* mout = nullptr;
+    AgentRT* out_agent = nullptr;
+    std::string out_state;
+    int sort_dims = 0;              // 2/3 when the function triggers the auto spatial sort
+    uint32_t cap = 0;               // per-launch buffers sized for this many agents
+    uint32_t* death_flags = nullptr;
+    uint32_t* birth_flags = nullptr;
+    uint32_t* msg_flags = nullptr;
+    uint32_t* d_launch_count = nullptr;
+    std::vector<VarBuf> newbuf;     // output agent variables (cur only)
+    void* death_scratch = nullptr;
+    size_t death_scratch_bytes = 0;
+    void* births_scratch = nullptr;
+    size_t births_scratch_bytes = 0;
+    void* append_scratch = nullptr;
+    size_t append_scratch_bytes = 0;
+    void* trans_scratch = nullptr;
+    size_t trans_scratch_bytes = 0;
+    void* stream = nullptr;
+};
+
+struct KernelTime {
+    double ms = 0.0;
+    uint64_t count = 0;
+};
+
+}  // namespace detail
+
+// ------------------------------------------------------------------ host API
+class HostAgentAPI {
+ public:
+    HostAgentAPI(CUDASimulation& s, detail::AgentRT& a, detail::StateList& l) : s_(s), a_(a), l_(l) {}
+    // include/flamegpu/runtime/agent/HostAgentAPI.cuh:174
+    unsigned int count();
+    // include/flamegpu/runtime/agent/HostAgentAPI.cuh:689 (deterministic fixed-shape tree on the device)
+    template <typename InT, typename OutT = InT>
+    OutT sum(const std::string& variable);
+    template <typename T>
+    T min(const std::string& variable);
+    template <typename T>
+    T max(const std::string& variable);
+    template <typename T>
+    std::vector<T> getVariableValues(const std::string& variable);
+
+ private:
+    template <typename T>
+    const detail::VarBuf& var(const std::string& name);
+    CUDASimulation& s_;
+    detail::AgentRT& a_;
+    detail::StateList& l_;
+};
+
+class HostEnvironment {
+ public:
+    explicit HostEnvironment(CUDASimulation& s) : s_(s) {}
+    template <typename T>
+    T getProperty(const std::string& name) const;
+    template <typename T>
+    T setProperty(const std::string& name, T value);
+
+ private:
+    CUDASimulation& s_;
+};
+
+class HostRandom {
+ public:
+    explicit HostRandom(uint64_t seed) : g_(seed) {}
+    template <typename T>
+    T uniform() {
+        return std::uniform_real_distribution<T>(0, 1)(g_);
+    }
+    template <typename T>
+    T uniform(T mn, T mx) {
+        if constexpr (std::is_integral<T>::value)
+            return std::uniform_int_distribution<T>(mn, mx)(g_);
+        else
+            return std::uniform_real_distribution<T>(mn, mx)(g_);
+    }
+    template <typename T>
+    T normal() {
+        return std::normal_distribution<T>(0, 1)(g_);
+    }
+    void setSeed(uint64_t s) { g_.seed(s); }
+
+ private:
+    std::mt19937_64 g_;
+};
+
+class HostAPI {
+ public:
+    HostAPI(CUDASimulation& s, uint64_t seed) : environment(s), random(seed), s_(s) {}
+    HostAgentAPI agent(const std::string& name, const std::string& state = "");
+    unsigned int getStepCounter() const;
+    HostEnvironment environment;
+    HostRandom random;
+
+ private:
+    CUDASimulation& s_;
+};
+
+// include/flamegpu/runtime/HostAPI_macros.h
+#define FLAMEGPU_INIT_FUNCTION(funcName)                      \
+    void funcName##_impl(flamegpu::HostAPI* FLAMEGPU);        \
+    flamegpu::FLAMEGPU_INIT_FUNCTION_POINTER funcName = funcName##_impl; \
+    void funcName##_impl(flamegpu::HostAPI* FLAMEGPU)
+#define FLAMEGPU_STEP_FUNCTION(funcName)                      \
+    void funcName##_impl(flamegpu::HostAPI* FLAMEGPU);        \
+    flamegpu::FLAMEGPU_STEP_FUNCTION_POINTER funcName = funcName##_impl; \
+    void funcName##_impl(flamegpu::HostAPI* FLAMEGPU)
+#define FLAMEGPU_EXIT_FUNCTION(funcName)                      \
+    void funcName##_impl(flamegpu::HostAPI* FLAMEGPU);        \
+    flamegpu::FLAMEGPU_EXIT_FUNCTION_POINTER funcName = funcName##_impl; \
+    void funcName##_impl(flamegpu::HostAPI* FLAMEGPU)
+#define FLAMEGPU_HOST_FUNCTION(funcName)                      \
+    void funcName##_impl(flamegpu::HostAPI* FLAMEGPU);        \
+    flamegpu::FLAMEGPU_HOST_FUNCTION_POINTER funcName = funcName##_impl; \
+    void funcName##_impl(flamegpu::HostAPI* FLAMEGPU)
+#define FLAMEGPU_EXIT_CONDITION(funcName)                     \
+    flamegpu::CONDITION_RESULT funcName##_impl(flamegpu::HostAPI* FLAMEGPU); \
+    flamegpu::FLAMEGPU_EXIT_CONDITION_POINTER funcName = funcName##_impl; \
+    flamegpu::CONDITION_RESULT funcName##_impl(flamegpu::HostAPI* FLAMEGPU)
+
+// ------------------------------------------------------------------ run plans
+// include/flamegpu/simulation/RunPlan.h
+class RunPlan {
+ public:
+    explicit RunPlan(const ModelDescription& model) : m_(model.data()) {}
+    void setRandomSimulationSeed(uint64_t s) { seed_ = s; }
+    void setSteps(unsigned int s) { steps_ = s; }
+    void setOutputSubdirectory(const std::string& d) { subdir_ = d; }
+    template <typename T>
+    void setProperty(const std::string& name, T value) {
+        auto it = m_->env.props.find(name);
+        if (it == m_->env.props.end()) throw exception::InvalidEnvProperty("RunPlan: unknown environment property '" + name + "'");
+        if (it->second.type != std::type_index(typeid(T)))
+            throw exception::InvalidEnvPropertyType("RunPlan: property '" + name + "' type mismatch");
+        std::vector<char> b(sizeof(T));
+        std::memcpy(b.data(), &value, sizeof(T));
+        props_[name] = b;
+    }
+    template <typename T>
+    T getProperty(const std::string& name) const {
+        auto it = props_.find(name);
+        if (it != props_.end()) {
+            T v;
+            std::memcpy(&v, it->second.data(), sizeof(T));
+            return v;
+        }
+        return EnvironmentDescription(m_.get()).getProperty<T>(name);
+    }
+    uint64_t getRandomSimulationSeed() const { return seed_; }
+    unsigned int getSteps() const { return steps_; }
+    std::string getOutputSubdirectory() const { return subdir_; }
+    const std::map<std::string, std::vector<char>>& properties() const { return props_; }
+
+ private:
+    std::shared_ptr<ModelData> m_;
+    uint64_t seed_ = 0;
+    unsigned int steps_ = 1;
+    std::string subdir_;
+    std::map<std::string, std::vector<char>> props_;
+};
+
+// include/flamegpu/simulation/RunPlanVector.h:36-139
+class RunPlanVector : public std::vector<RunPlan> {
+ public:
+    RunPlanVector(const ModelDescription& model, unsigned int n) : std::vector<RunPlan>(n, RunPlan(model)), rng_(0) {}
+    void setRandomSimulationSeed(uint64_t initial, unsigned int step = 0) {
+        uint64_t s = initial;
+        for (auto& p : *this) {
+            p.setRandomSimulationSeed(s);
+            s += step;
+        }
+    }
+    void setSteps(unsigned int s) {
+        for (auto& p : *this) p.setSteps(s);
+    }
+    void setOutputSubdirectory(const std::string& d) {
+        for (auto& p : *this) p.setOutputSubdirectory(d);
+    }
+    template <typename T>
+    void setProperty(const std::string& name, T value) {
+        for (auto& p : *this) p.setProperty<T>(name, value);
+    }
+    template <typename T>
+    void setPropertyLerpRange(const std::string& name, T mn, T mx) {
+        const size_t n = size();
+        for (size_t i = 0; i < n; ++i) {
+            const double a = n > 1 ? (double)i / (double)(n - 1) : 0.0;
+            const double v = (double)mn * (1.0 - a) + (double)mx * a;
+            (*this)[i].template setProperty<T>(name, std::is_integral<T>::value ? (T)std::llround(v) : (T)v);
+        }
+    }
+    void setRandomPropertySeed(uint64_t s) { rng_.seed(s); }
+    template <typename T>
+    void setPropertyUniformRandom(const std::string& name, T mn, T mx) {
+        for (auto& p : *this) {
+            if constexpr (std::is_integral<T>::value)
+                p.template setProperty<T>(name, std::uniform_int_distribution<T>(mn, mx)(rng_));
+            else
+                p.template setProperty<T>(name, std::uniform_real_distribution<T>(mn, mx)(rng_));
+        }
+    }
+
+ private:
+    std::mt19937_64 rng_;
+};
+
+// ------------------------------------------------------------------ simulation
+class CUDASimulation {
+ public:
+    // include/flamegpu/simulation/Simulation.h:37-70
+    struct Config {
+        std::string input_file;
+        uint64_t random_seed = 0;
+        unsigned int steps = 1;
+        bool verbose = false;
+        bool timing = false;
+        bool truncate_log_files = true;
+    };
+    // include/flamegpu/simulation/CUDASimulation.h:100-129 (+ graph capture switch)
+    struct CUDAConfig {
+        int device_id = 0;
+        bool inLayerConcurrency = true;
+        bool graphs = true;          // capture whole steps as hipGraphs where the model allows
+        bool is_ensemble = false;
+    };
+
+    explicit CUDASimulation(const ModelDescription& model, int argc = 0, const char** argv = nullptr)
+        : m_(model.data()) {
+        env_host_ = m_->env.buffer;
+        config_.random_seed = (uint64_t)std::chrono::high_resolution_clock::now().time_since_epoch().count();
+        if (argc && argv) initialise(argc, argv);
+    }
+    CUDASimulation(const CUDASimulation&) = delete;
+    CUDASimulation& operator=(const CUDASimulation&) = delete;
+    ~CUDASimulation() { release(); }
+
+    Config& SimulationConfig() { return config_; }
+    const Config& getSimulationConfig() const { return config_; }
+    CUDAConfig& CUDAConfig() { return cuda_config_; }
+    const struct CUDAConfig& getCUDAConfig() const { return cuda_config_; }
+
+    // src/flamegpu/simulation/Simulation.cu:228-350 (the step/seed/device/timing subset)
+    void initialise(int argc, const char** argv) {
+        for (int i = 1; i < argc; ++i) {
+            const std::string a = argv[i];
+            auto next = [&](const char* what) -> std::string {
+                if (i + 1 >= argc) throw exception::InvalidArgument(std::string("missing value for ") + what);
+                return argv[++i];
+            };
+            if (a == "-s" || a == "--steps") config_.steps = (unsigned)std::stoul(next("--steps"));
+            else if (a == "-r" || a == "--random") config_.random_seed = std::stoull(next("--random"));
+            else if (a == "-d" || a == "--device") cuda_config_.device_id = std::stoi(next("--device"));
+            else if (a == "-i" || a == "--in") config_.input_file = next("--in");
+            else if (a == "-t" || a == "--timing") config_.timing = true;
+            else if (a == "-v" || a == "--verbose") config_.verbose = true;
+            else if (a == "-q" || a == "--quiet") config_.verbose = false;
+            else if (a == "--no-graphs") cuda_config_.graphs = false;
+        }
+        if (!config_.input_file.empty())
+            throw exception::InvalidArgument("state file input (-i) is not supported by this runtime");
+    }
+    void applyConfig() { ensure_init(); }
+
+    // src/flamegpu/simulation/CUDASimulation.cu:575-642
+    bool step();
+    // src/flamegpu/simulation/CUDASimulation.cu:1220-1344
+    void simulate() {
+        const auto t0 = std::chrono::steady_clock::now();
+        ensure_init();
+        run_init_functions();
+        for (unsigned int s = 0; s < config_.steps || config_.steps == 0; ++s) {
+            if (!step()) break;
+            if (config_.steps == 0 && s > 1000000000u) break;
+        }
+        run_exit_functions();
+        synchronize();
+        check_device_error();
+        elapsed_sim_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void simulate(const RunPlan& plan) {
+        config_.random_seed = plan.getRandomSimulationSeed();
+        config_.steps = plan.getSteps();
+        for (auto& kv : plan.properties()) set_env_raw(kv.first, kv.second);
+        simulate();
+    }
+    // src/flamegpu/simulation/CUDASimulation.cu:1418-1452
+    void setPopulationData(AgentVector& pop, const std::string& state = "");
+    void getPopulationData(AgentVector& pop, const std::string& state = "");
+    void synchronize() {
+        if (stream_) detail::fgx_check(fgx_stream_synchronize(stream_), "stream synchronize");
+    }
+    unsigned int getStepCounter() const { return step_; }
+    void resetStepCounter() { step_ = 0; if (d_step_) detail::fgx_check(fgx_fill_u32(d_step_, 0, 1, stream_), "reset step"); }
+    std::vector<double> getElapsedTimeSteps() const { return step_times_; }
+    double getElapsedTimeStep(unsigned int s) const { return step_times_.at(s); }
+    double getElapsedTimeSimulation() const { return elapsed_sim_; }
+    template <typename T>
+    void setEnvironmentProperty(const std::string& name, T value) {
+        const EnvProp& p = env_prop<T>(name);
+        std::memcpy(env_host_.data() + p.offset, &value, sizeof(T));
+        env_dirty_ = true;
+    }
+    template <typename T>
+    T getEnvironmentProperty(const std::string& name) const {
+        const EnvProp& p = env_prop<T>(name);
+        T v;
+        std::memcpy(&v, env_host_.data() + p.offset, sizeof(T));
+        return v;
+    }
+    void exportData(const std::string&) {}
+
+    // ---- extensions used by the benchmark and the model C ABI
+    void setKernelTiming(bool on) { timing_ = on; }
+    const std::map<std::string, detail::KernelTime>& getKernelTimes() {
+        collect_timings();
+        return ktimes_;
+    }
+    void resetKernelTimes() {
+        collect_timings();
+        ktimes_.clear();
+    }
+    uint32_t messageBins(const std::string& msg) {
+        ensure_init();
+        auto& m = msgs_.at(msg);
+        return m.spatial ? m.meta.bins : 0u;
+    }
+    void getPBM(const std::string& msg, uint32_t* host) {
+        auto& m = msgs_.at(msg);
+        if (!m.spatial) throw exception::InvalidMessageType("message '" + msg + "' has no spatial index");
+        synchronize();
+        detail::fgx_check(fgx_memcpy_d2h(host, m.pbm, (size_t)(m.meta.bins + 1) * 4, stream_), "PBM readback");
+        synchronize();
+    }
+    void* stream() { ensure_init(); return stream_; }
+    bool graphActive() const { return !graphs_.empty(); }
+    uint64_t seed() const { return config_.random_seed; }
+
+    // internal (HostAPI)
+    detail::AgentRT& agent_rt(const std::string& name) {
+        ensure_init();
+        auto it = agents_.find(name);
+        if (it == agents_.end()) throw exception::InvalidAgentName("Agent '" + name + "' was not found");
+        return it->second;
+    }
+    fgx_ctx* ctx() { return ctx_; }
+    void* reduce_scratch(size_t bytes) {
+        if (bytes > reduce_scratch_bytes_) {
+            if (reduce_scratch_) fgx_free(ctx_, reduce_scratch_);
+            detail::fgx_check(fgx_malloc(ctx_, bytes, &reduce_scratch_), "reduction scratch");
+            detail::fgx_check(fgx_scratch_clear(reduce_scratch_, bytes, stream_), "reduction scratch clear");
+            reduce_scratch_bytes_ = bytes;
+        }
+        return reduce_scratch_;
+    }
+    void* reduce_out() { return d_reduce_out_; }
+    void check_device_error();
+    void mark_env_dirty() { env_dirty_ = true; }
+    std::vector<char>& env_host() { return env_host_; }
+    const std::shared_ptr<ModelData>& model() const { return m_; }
+    void sync_counts();
+
+ private:
+    template <typename T>
+    const EnvProp& env_prop(const std::string& name) const {
+        auto it = m_->env.props.find(name);
+        if (it == m_->env.props.end()) throw exception::InvalidEnvProperty("Environment property '" + name + "' was not found");
+        if (it->second.type != std::type_index(typeid(T)))
+            throw exception::InvalidEnvPropertyType("Environment property '" + name + "' type mismatch");
+        return it->second;
+    }
+    void set_env_raw(const std::string& name, const std::vector<char>& b) {
+        auto it = m_->env.props.find(name);
+        if (it == m_->env.props.end()) throw exception::InvalidEnvProperty("Environment property '" + name + "' was not found");
+        std::memcpy(env_host_.data() + it->second.offset, b.data(), std::min(b.size(), it->second.type_size * it->second.elements));
+        env_dirty_ = true;
+    }
+
+    void ensure_init();
+    void release();
+    void alloc(void** p, size_t bytes) { detail::fgx_check(fgx_malloc(ctx_, bytes ? bytes : 4, p), "device allocation"); }
+    void free_dev(void*& p) {
+        if (p) fgx_free(ctx_, p);
+        p = nullptr;
+    }
+    void* new_scratch(int op, uint32_t n, uint32_t bins, size_t* bytes) {
+        detail::fgx_check(fgx_scratch_bytes(op, n, bins, bytes), "scratch sizing");
+        void* p = nullptr;
+        alloc(&p, *bytes);
+        detail::fgx_check(fgx_scratch_clear(p, *bytes, stream_), "scratch clear");
+        return p;
+    }
+    void grow_list(detail::StateList& l, uint32_t cap);
+    void grow_message(detail::MessageRT& m, uint32_t cap);
+    void size_function(detail::FnRT& f, uint32_t cap);
+    bool ensure_capacity();
+    bool model_can_grow() const;
+    void enqueue_step();
+    void enqueue_layer(const LayerData& layer);
+    void launch_function(detail::FnRT& f, void* st);
+    void unmap_function(detail::FnRT& f, void* st);
+    void enqueue_end_of_step();
+    void run_init_functions();
+    void run_exit_functions();
+    void upload_env();
+    void tick(const std::string& label, void* st, bool begin);
+    void collect_timings();
+
+    std::shared_ptr<ModelData> m_;
+    Config config_;
+    struct CUDAConfig cuda_config_;
+    bool initialised_ = false;
+    fgx_ctx* ctx_ = nullptr;
+    void* stream_ = nullptr;
+    std::map<std::string, detail::AgentRT> agents_;
+    std::map<std::string, detail::MessageRT> msgs_;
+    std::vector<detail::FnRT> fns_;
+    std::map<const AgentFunctionData*, size_t> fn_index_;
+    std::vector<char> env_host_;
+    char* d_env_ = nullptr;
+    bool env_dirty_ = true;
+    uint32_t* d_small_ = nullptr;    // [0..2] error, [3] step, [4] dummy counter, [8..] counts
+    uint32_t* d_error_ = nullptr;
+    uint32_t* d_step_ = nullptr;
+    uint32_t* d_dummy_ = nullptr;
+    uint32_t small_used_ = 8;
+    uint32_t* h_small_ = nullptr;    // pinned mirror
+    void* d_reduce_out_ = nullptr;
+    void* reduce_scratch_ = nullptr;
+    size_t reduce_scratch_bytes_ = 0;
+    unsigned int step_ = 0;
+    std::vector<double> step_times_;
+    double elapsed_sim_ = 0.0;
+    std::unique_ptr<HostAPI> host_api_;
+    bool init_functions_ran_ = false;
+    // graphs
+    struct GraphEntry {
+        std::vector<void*> start, end;
+        fgx_graph* g = nullptr;
+    };
+    std::vector<GraphEntry> graphs_;
+    bool graph_eligible_ = false;
+    void clear_graphs() {
+        for (auto& g : graphs_) fgx_graph_destroy(g.g);
+        graphs_.clear();
+    }
+    std::vector<void*> buffer_state() const;
+    void set_buffer_state(const std::vector<void*>& st);
+    std::vector<void*> join_events_;
+    void* fork_event_ = nullptr;
+    // kernel timing
+    bool timing_ = false;
+    struct PendingTime {
+        std::string label;
+        void* start;
+        void* stop;
+    };
+    std::vector<PendingTime> pending_;
+    std::vector<void*> free_events_;
+    std::map<std::string, void*> open_events_;
+    std::map<std::string, detail::KernelTime> ktimes_;
+    friend class HostAgentAPI;
+};
+
+// ------------------------------------------------------------------ implementation
+inline void CUDASimulation::ensure_init() {
+    if (initialised_) return;
+    detail::fgx_check(fgx_ctx_create(cuda_config_.device_id, &ctx_), "fgx_ctx_create");
+    detail::fgx_check(fgx_stream_create(&stream_), "stream create");
+    alloc((void**)&d_small_, 4096 * 4);
+    detail::fgx_check(fgx_fill_u32(d_small_, 0, 4096, stream_), "zero counters");
+    detail::fgx_check(fgx_host_alloc(4096 * 4, (void**)&h_small_), "pinned counters");
+    d_error_ = d_small_;
+    d_step_ = d_small_ + 3;
+    d_dummy_ = d_small_ + 4;
+    alloc(&d_reduce_out_, 64);
+    auto counter = [&]() {
+        if (small_used_ >= 4096) throw exception::InvalidOperation("too many agent states and message lists");
+        return d_small_ + small_used_++;
+    };
+    // agents and their state lists
+    for (auto& kv : m_->agents) {
+        detail::AgentRT a;
+        a.d = kv.second;
+        a.d_next_id = counter();
+        for (auto& s : kv.second->states) {
+            detail::StateList l;
+            l.state = s;
+            for (auto& v : kv.second->variables) {
+                detail::VarBuf b;
+                b.name = v.first;
+                b.bytes = (uint32_t)v.second.bytes();
+                b.def = v.second.default_value;
+                l.vars.push_back(b);
+            }
+            l.id_slot = l.var_index("_id");
+            l.d_count = counter();
+            a.states.emplace(s, std::move(l));
+        }
+        agents_.emplace(kv.first, std::move(a));
+    }
+    // message lists
+    for (auto& kv : m_->messages) {
+        detail::MessageRT mr;
+        mr.d = kv.second;
+        for (auto& v : kv.second->variables) {
+            detail::VarBuf b;
+            b.name = v.first;
+            b.bytes = (uint32_t)v.second.bytes();
+            b.def = v.second.default_value;
+            mr.vars.push_back(b);
+        }
+        if (mr.vars.size() > detail::MAX_MSG_VARS)
+            throw exception::InvalidMessageVar("Message '" + kv.first + "' has too many variables");
+        mr.d_count = counter();
+        mr.spatial = kv.second->kind == MsgKind::Spatial2D || kv.second->kind == MsgKind::Spatial3D;
+        if (mr.spatial) {
+            const int dims = kv.second->kind == MsgKind::Spatial3D ? 3 : 2;
+            if (!kv.second->radius_set || !kv.second->min_set || !kv.second->max_set)
+                throw exception::InvalidMessageType("Spatial message '" + kv.first + "' needs a radius, min and max");
+            detail::fgx_check(fgx_spatial_meta_init(dims, kv.second->min, kv.second->max, kv.second->radius, &mr.meta),
+                              "spatial message metadata");
+            alloc((void**)&mr.pbm, (size_t)(mr.meta.bins + 1) * 4);
+            detail::fgx_check(fgx_fill_u32(mr.pbm, 0, mr.meta.bins + 1, stream_), "PBM init");
+        }
+        msgs_.emplace(kv.first, std::move(mr));
+    }
+    // functions, in layer order
+    for (auto& layer : m_->layers) {
+        for (auto& f : layer->agent_functions) {
+            if (fn_index_.count(f.get())) continue;
+            detail::FnRT r;
+            r.d = f;
+            r.agent = &agents_.at(f->parent->name);
+            r.in_state = f->initial_state;
+            r.end_state = f->end_state;
+            if (auto mi = f->message_input.lock()) r.min = &msgs_.at(mi->name);
+            if (auto mo = f->message_output.lock()) r.mout = &msgs_.at(mo->name);
+            if (f->in_kind != MsgKind::None && !r.min)
+                throw exception::InvalidAgentFunc("Agent function '" + f->name + "' reads messages but has no message input set");
+            if (f->out_kind != MsgKind::None && !r.mout)
+                throw exception::InvalidAgentFunc("Agent function '" + f->name + "' outputs messages but has no message output set");
+            if (auto ao = f->agent_output.lock()) {
+                r.out_agent = &agents_.at(ao->name);
+                r.out_state = f->agent_output_state;
+            }
+            if (r.min && r.min->spatial && f->parent->sortPeriod != 0) {
+                const auto& vars = f->parent->variables;
+                auto isf = [&](const char* n) {
+                    auto it = vars.find(n);
+                    return it != vars.end() && it->second.type == std::type_index(typeid(float)) && it->second.elements == 1;
+                };
+                if (r.min->meta.dims == 3 && isf("x") && isf("y") && isf("z")) r.sort_dims = 3;
+                if (r.min->meta.dims == 2 && isf("x") && isf("y")) r.sort_dims = 2;
+            }
+            r.d_launch_count = counter();
+            fn_index_[f.get()] = fns_.size();
+            fns_.push_back(std::move(r));
+        }
+    }
+    // environment
+    alloc((void**)&d_env_, std::max<size_t>(env_host_.size(), 8));
+    env_dirty_ = true;
+    host_api_.reset(new HostAPI(*this, config_.random_seed));
+    graph_eligible_ = true;
+    for (auto& l : m_->layers)
+        if (!l->host_functions.empty()) graph_eligible_ = false;
+    for (auto& a : m_->agents)
+        if (a.second->sortPeriod > 1) graph_eligible_ = false;
+    for (auto& mm : m_->messages)
+        if (mm.second->persistent) graph_eligible_ = false;
+    detail::fgx_check(fgx_event_create(&fork_event_), "event create");
+    initialised_ = true;
+}
+
+inline void CUDASimulation::release() {
+    if (!initialised_) return;
+    fgx_stream_synchronize(stream_);
+    collect_timings();
+    clear_graphs();
+    for (auto& kv : agents_)
+        for (auto& s : kv.second.states) {
+            for (auto& v : s.second.vars) {
+                free_dev(v.cur);
+                free_dev(v.alt);
+            }
+            free_dev(s.second.sort_scratch);
+        }
+    for (auto& kv : msgs_) {
+        for (auto& v : kv.second.vars) {
+            free_dev(v.cur);
+            free_dev(v.alt);
+        }
+        void* p = kv.second.pbm;
+        free_dev(p);
+        free_dev(kv.second.build_scratch);
+    }
+    for (auto& f : fns_) {
+        void* p = f.death_flags;
+        free_dev(p);
+        p = f.birth_flags;
+        free_dev(p);
+        p = f.msg_flags;
+        free_dev(p);
+        for (auto& v : f.newbuf) free_dev(v.cur);
+        free_dev(f.death_scratch);
+        free_dev(f.births_scratch);
+        free_dev(f.append_scratch);
+        free_dev(f.trans_scratch);
+        if (f.stream) fgx_stream_destroy(f.stream);
+    }
+    for (auto e : join_events_) fgx_event_destroy(e);
+    if (fork_event_) fgx_event_destroy(fork_event_);
+    for (auto e : free_events_) fgx_event_destroy(e);
+    void* p = d_env_;
+    free_dev(p);
+    p = d_small_;
+    free_dev(p);
+    free_dev(d_reduce_out_);
+    free_dev(reduce_scratch_);
+    if (h_small_) fgx_host_free(h_small_);
+    fgx_stream_destroy(stream_);
+    fgx_ctx_destroy(ctx_);
+    initialised_ = false;
+}
+
+inline void CUDASimulation::grow_list(detail::StateList& l, uint32_t cap) {
+    if (cap <= l.cap) return;
+    cap = std::max<uint32_t>(cap, 256);
+    for (auto& v : l.vars) {
+        void *nc = nullptr, *na = nullptr;
+        alloc(&nc, (size_t)cap * v.bytes);
+        alloc(&na, (size_t)cap * v.bytes);
+        if (v.cur && l.cap) detail::fgx_check(fgx_memcpy_d2d(nc, v.cur, (size_t)l.cap * v.bytes, stream_), "grow copy");
+        // Keep both buffers valid before the old ones go.
+        detail::fgx_check(fgx_stream_synchronize(stream_), "grow sync");
+        free_dev(v.cur);
+        free_dev(v.alt);
+        v.cur = nc;
+        v.alt = na;
+    }
+    free_dev(l.sort_scratch);
+    l.cap = cap;
+}
+
+inline void CUDASimulation::grow_message(detail::MessageRT& m, uint32_t cap) {
+    if (cap <= m.cap) return;
+    cap = std::max<uint32_t>(cap, 256);
+    for (auto& v : m.vars) {
+        void *nc = nullptr, *na = nullptr;
+        alloc(&nc, (size_t)cap * v.bytes);
+        alloc(&na, (size_t)cap * v.bytes);
+        if (v.cur && m.cap) detail::fgx_check(fgx_memcpy_d2d(nc, v.cur, (size_t)m.cap * v.bytes, stream_), "grow copy");
+        detail::fgx_check(fgx_stream_synchronize(stream_), "grow sync");
+        free_dev(v.cur);
+        free_dev(v.alt);
+        v.cur = nc;
+        v.alt = na;
+    }
+    free_dev(m.build_scratch);
+    if (m.spatial) {
+        m.build_scratch = new_scratch(FGX_OP_SPATIAL_BUILD, cap, m.meta.bins, &m.build_scratch_bytes);
+    }
+    m.cap = cap;
+}
+
+inline void CUDASimulation::size_function(detail::FnRT& f, uint32_t cap) {
+    if (cap <= f.cap) return;
+    cap = std::max<uint32_t>(cap, 256);
+    auto realloc_u32 = [&](uint32_t*& p) {
+        void* q = p;
+        free_dev(q);
+        alloc(&q, (size_t)cap * 4);
+        p = (uint32_t*)q;
+    };
+    detail::fgx_check(fgx_stream_synchronize(stream_), "resize sync");
+    if (f.d->has_agent_death) {
+        realloc_u32(f.death_flags);
+        free_dev(f.death_scratch);
+        f.death_scratch = new_scratch(FGX_OP_COMPACT, cap, 0, &f.death_scratch_bytes);
+    }
+    if (f.mout) {
+        realloc_u32(f.msg_flags);
+        free_dev(f.append_scratch);
+        f.append_scratch = new_scratch(FGX_OP_BIRTHS, cap, 0, &f.append_scratch_bytes);
+    }
+    if (f.out_agent) {
+        realloc_u32(f.birth_flags);
+        free_dev(f.births_scratch);
+        f.births_scratch = new_scratch(FGX_OP_BIRTHS, cap, 0, &f.births_scratch_bytes);
+        for (auto& v : f.newbuf) free_dev(v.cur);
+        f.newbuf.clear();
+        for (auto& kv : f.out_agent->d->variables) {
+            detail::VarBuf b;
+            b.name = kv.first;
+            b.bytes = (uint32_t)kv.second.bytes();
+            b.def = kv.second.default_value;
+            alloc(&b.cur, (size_t)cap * b.bytes);
+            f.newbuf.push_back(b);
+        }
+    }
+    if (f.in_state != f.end_state) {
+        free_dev(f.trans_scratch);
+        f.trans_scratch = new_scratch(FGX_OP_BIRTHS, cap, 0, &f.trans_scratch_bytes);
+    }
+    f.cap = cap;
+}
+
+inline bool CUDASimulation::model_can_grow() const {
+    for (auto& f : fns_)
+        if (f.out_agent || f.in_state != f.end_state) return true;
+    return false;
+}
+
+inline void CUDASimulation::sync_counts() {
+    synchronize();
+    size_t k = 0;
+    std::vector<uint32_t*> order;
+    for (auto& kv : agents_)
+        for (auto& s : kv.second.states) order.push_back(s.second.d_count);
+    // counts live in one block: copy the whole used prefix
+    detail::fgx_check(fgx_memcpy_d2h(h_small_, d_small_, (size_t)small_used_ * 4, stream_), "count readback");
+    synchronize();
+    for (auto& kv : agents_) {
+        kv.second.h_next_id = h_small_[kv.second.d_next_id - d_small_];
+        for (auto& s : kv.second.states) s.second.h_count = h_small_[s.second.d_count - d_small_];
+    }
+    (void)k;
+    (void)order;
+    if (h_small_[0] != 0) check_device_error();
+}
+
+// Upper bounds of every list for the coming step (births append, transitions move).
+inline bool CUDASimulation::ensure_capacity() {
+    std::map<detail::StateList*, uint64_t> ub, peak;
+    for (auto& kv : agents_)
+        for (auto& s : kv.second.states) {
+            ub[&s.second] = s.second.h_count;
+            peak[&s.second] = s.second.h_count;
+        }
+    std::map<detail::MessageRT*, uint64_t> mub;
+    std::map<detail::FnRT*, uint64_t> fub;
+    for (auto& layer : m_->layers) {
+        for (auto& fd : layer->agent_functions) {
+            auto& f = fns_[fn_index_.at(fd.get())];
+            auto* in = &f.agent->states.at(f.in_state);
+            const uint64_t n = ub[in];
+            fub[&f] = std::max(fub[&f], n);
+            if (f.mout) mub[f.mout] += n;
+            if (f.out_agent) {
+                auto* dst = &f.out_agent->states.at(f.out_state);
+                ub[dst] += n;
+                peak[dst] = std::max(peak[dst], ub[dst]);
+            }
+            if (f.in_state != f.end_state) {
+                auto* dst = &f.agent->states.at(f.end_state);
+                ub[dst] += n;
+                peak[dst] = std::max(peak[dst], ub[dst]);
+                ub[in] = 0;
+            }
+        }
+    }
+    bool changed = false;
+    auto need = [](uint64_t want, uint32_t have) -> uint32_t {
+        if (want <= have) return 0;
+        uint64_t c = std::max<uint64_t>(want, (uint64_t)have + have / 2);
+        if (c > 0xFFFFFFFFull / 2) throw exception::InvalidOperation("population exceeds the 32-bit index range");
+        return (uint32_t)c;
+    };
+    for (auto& kv : peak) {
+        if (uint32_t c = need(kv.second, kv.first->cap)) {
+            grow_list(*kv.first, c);
+            changed = true;
+        }
+    }
+    for (auto& kv : msgs_) {
+        auto& m = kv.second;
+        uint64_t want = mub[&m] + (m.d->persistent ? m.cap : 0);
+        if (uint32_t c = need(std::max<uint64_t>(want, 1), m.cap)) {
+            grow_message(m, c);
+            changed = true;
+        }
+    }
+    // Per-function flags, new-agent buffers and scratch cover the whole input list
+    // capacity: launches and compactions are shaped by that capacity.
+    for (auto& f : fns_) {
+        const uint64_t list_cap = f.agent->states.at(f.in_state).cap;
+        const uint64_t want = std::max<uint64_t>(std::max<uint64_t>(fub[&f], list_cap), 1);
+        if (want > f.cap) {
+            size_function(f, (uint32_t)want);
+            changed = true;
+        }
+    }
+    // sort scratch depends on list capacity
+    for (auto& f : fns_) {
+        if (!f.sort_dims) continue;
+        auto& l = f.agent->states.at(f.in_state);
+        if (!l.sort_scratch) {
+            uint32_t bits = 0;
+            detail::fgx_check(fgx_agent_sort_key_bits(f.sort_dims, f.min->meta.min, f.min->meta.max, f.min->meta.radius, &bits),
+                              "sort key bits");
+            l.sort_scratch = new_scratch(FGX_OP_AGENT_SORT, l.cap, 1u << bits, &l.sort_scratch_bytes);
+            changed = true;
+        }
+    }
+    return changed;
+}
+
+inline void CUDASimulation::setPopulationData(AgentVector& pop, const std::string& state_in) {
+    ensure_init();
+    auto& a = agent_rt(pop.getAgentName());
+    const std::string state = state_in.empty() ? a.d->initial_state : state_in;
+    auto it = a.states.find(state);
+    if (it == a.states.end()) throw exception::InvalidAgentState("Agent '" + a.d->name + "' has no state '" + state + "'");
+    auto& l = it->second;
+    if (step_ > 0 || initialised_) sync_counts();
+    const uint32_t n = (uint32_t)pop.size();
+    // IDs: CUDAFatAgent::assignIDs (src/flamegpu/simulation/detail/CUDAFatAgent.cu:347-391)
+    auto& ids = pop.raw("_id");
+    id_t mx = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        id_t v;
+        std::memcpy(&v, ids.data() + (size_t)i * 4, 4);
+        mx = std::max(mx, v);
+    }
+    uint32_t next = std::max<uint32_t>(a.h_next_id, mx + 1);
+    for (uint32_t i = 0; i < n; ++i) {
+        id_t v;
+        std::memcpy(&v, ids.data() + (size_t)i * 4, 4);
+        if (v == ID_NOT_SET) {
+            v = next + i;
+            std::memcpy(ids.data() + (size_t)i * 4, &v, 4);
+        }
+    }
+    next += n;
+    a.h_next_id = next;
+    grow_list(l, std::max<uint32_t>(n, 1));
+    for (auto& v : l.vars)
+        detail::fgx_check(fgx_memcpy_h2d(v.cur, pop.raw(v.name).data(), (size_t)n * v.bytes, stream_), "population upload");
+    l.h_count = n;
+    detail::fgx_check(fgx_memcpy_h2d(l.d_count, &n, 4, stream_), "count upload");
+    detail::fgx_check(fgx_memcpy_h2d(a.d_next_id, &next, 4, stream_), "next id upload");
+    synchronize();
+    clear_graphs();
+}
+
+inline void CUDASimulation::getPopulationData(AgentVector& pop, const std::string& state_in) {
+    ensure_init();
+    auto& a = agent_rt(pop.getAgentName());
+    const std::string state = state_in.empty() ? a.d->initial_state : state_in;
+    auto it = a.states.find(state);
+    if (it == a.states.end()) throw exception::InvalidAgentState("Agent '" + a.d->name + "' has no state '" + state + "'");
+    sync_counts();
+    check_device_error();
+    auto& l = it->second;
+    pop.resize(l.h_count);
+    for (auto& v : l.vars)
+        if (l.h_count)
+            detail::fgx_check(fgx_memcpy_d2h(pop.raw(v.name).data(), v.cur, (size_t)l.h_count * v.bytes, stream_),
+                              "population download");
+    synchronize();
+}
+
+inline void CUDASimulation::check_device_error() {
+    synchronize();
+    uint32_t e[3] = {0, 0, 0};
+    detail::fgx_check(fgx_memcpy_d2h(e, d_error_, 12, stream_), "error readback");
+    synchronize();
+    if (e[0] != detail::DERR_NONE) {
+        detail::fgx_check(fgx_fill_u32(d_error_, 0, 3, stream_), "error reset");
+        synchronize();
+        throw exception::DeviceError(detail::format("Device error %u (%s), name hash 0x%08x, agent index %u", e[0],
+                                                    detail::device_error_text(e[0]), e[1], e[2]));
+    }
+}
+
+inline void CUDASimulation::upload_env() {
+    if (!env_dirty_) return;
+    if (!env_host_.empty())
+        detail::fgx_check(fgx_memcpy_h2d(d_env_, env_host_.data(), env_host_.size(), stream_), "environment upload");
+    synchronize();
+    env_dirty_ = false;
+}
+
+inline void CUDASimulation::tick(const std::string& label, void* st, bool begin) {
+    if (!timing_) return;
+    if (begin) {
+        void* e = nullptr;
+        if (!free_events_.empty()) {
+            e = free_events_.back();
+            free_events_.pop_back();
+        } else {
+            detail::fgx_check(fgx_event_create(&e), "event create");
+        }
+        fgx_event_record(e, st);
+        open_events_[label] = e;
+    } else {
+        void* e = nullptr;
+        if (!free_events_.empty()) {
+            e = free_events_.back();
+            free_events_.pop_back();
+        } else {
+            detail::fgx_check(fgx_event_create(&e), "event create");
+        }
+        fgx_event_record(e, st);
+        pending_.push_back({label, open_events_.at(label), e});
+        open_events_.erase(label);
+    }
+}
+
+inline void CUDASimulation::collect_timings() {
+    if (pending_.empty()) return;
+    for (auto& p : pending_) {
+        float ms = 0.0f;
+        fgx_event_elapsed_ms(p.start, p.stop, &ms);
+        auto& k = ktimes_[p.label];
+        k.ms += ms;
+        k.count++;
+        free_events_.push_back(p.start);
+        free_events_.push_back(p.stop);
+    }
+    pending_.clear();
+}
+
+inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
+    auto& l = f.agent->states.at(f.in_state);
+    if (f.cap < l.cap)
+        throw exception::InvalidOperation("function '" + f.d->name + "' buffers are smaller than its agent list");
+    detail::FnArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.n_cap = l.cap;
+    a.stream_id = f.d->stream_id;
+    a.d_n = l.d_count;
+    a.d_step = d_step_;
+    a.seed = config_.random_seed;
+    a.d_error = d_error_;
+    a.env = d_env_;
+    a.ids = l.id_slot >= 0 ? (const uint32_t*)l.vars[l.id_slot].cur : nullptr;
+    auto fill_table = [](detail::VarBinding* tab, uint32_t* mul, const std::vector<detail::VarBuf>& vars, bool use_alt) {
+        std::vector<uint32_t> hs;
+        for (auto& v : vars) hs.push_back(detail::name_hash(v.name));
+        *mul = detail::perfect_hash_mul(hs.data(), (unsigned)hs.size(), detail::VAR_SLOT_BITS);
+        for (size_t i = 0; i < vars.size(); ++i) {
+            const uint32_t s = detail::table_slot(hs[i], *mul, detail::VAR_SLOT_BITS);
+            tab[s] = {hs[i], vars[i].bytes, (char*)(use_alt ? vars[i].alt : vars[i].cur)};
+        }
+    };
+    fill_table(a.agent, &a.agent_mul, l.vars, false);
+    if (f.min) {
+        fill_table(a.msg_in, &a.msg_in_mul, f.min->vars, false);
+        a.d_msg_in_count = f.min->d_count;
+        if (f.min->spatial) {
+            const auto& m = f.min->meta;
+            for (int k = 0; k < 3; ++k) {
+                a.in_meta.min[k] = m.min[k];
+                a.in_meta.max[k] = m.max[k];
+                a.in_meta.width[k] = m.width[k];
+                a.in_meta.grid[k] = m.grid[k];
+            }
+            a.in_meta.radius = m.radius;
+            a.in_meta.wrap_ok = m.wrap_ok;
+            a.in_meta.pbm = f.min->pbm;
+        }
+    }
+    if (f.mout) {
+        fill_table(a.msg_out, &a.msg_out_mul, f.mout->vars, true);
+        a.msg_out_flags = f.msg_flags;
+        a.msg_out_optional = f.d->message_output_optional ? 1 : 0;
+    }
+    if (f.out_agent) {
+        fill_table(a.new_agent, &a.new_mul, f.newbuf, false);
+        std::vector<void*> ptrs;
+        std::vector<uint32_t> bytes, offs;
+        std::vector<char> defs;
+        for (size_t i = 0; i < f.newbuf.size(); ++i) {
+            ptrs.push_back(f.newbuf[i].cur);
+            bytes.push_back(f.newbuf[i].bytes);
+            offs.push_back((uint32_t)defs.size());
+            defs.insert(defs.end(), f.newbuf[i].def.begin(), f.newbuf[i].def.end());
+        }
+        a.birth_flags = f.birth_flags;
+        // broadcastInit: new agents start from the variable defaults.
+        detail::fgx_check(fgx_fill_vars((uint32_t)ptrs.size(), ptrs.data(), bytes.data(), defs.data(), offs.data(), l.cap, st),
+                          "new-agent defaults");
+    }
+    if (f.d->has_agent_death) a.death_flags = f.death_flags;
+    {
+        std::vector<uint32_t> hs;
+        for (auto& kv : m_->env.props) hs.push_back(detail::name_hash(kv.first));
+        a.env_mul = detail::perfect_hash_mul(hs.data(), (unsigned)hs.size(), detail::ENV_SLOT_BITS);
+        size_t i = 0;
+        for (auto& kv : m_->env.props) {
+            const uint32_t s = detail::table_slot(hs[i], a.env_mul, detail::ENV_SLOT_BITS);
+            a.env_b[s] = {hs[i], (uint16_t)kv.second.offset, (uint16_t)(kv.second.type_size * kv.second.elements)};
+            ++i;
+        }
+    }
+    detail::fgx_check(fgx_memcpy_d2d(f.d_launch_count, l.d_count, 4, st), "launch count");
+    const uint32_t block = 256;
+    const uint32_t grid = std::max<uint32_t>(1, (l.cap + block - 1) / block);
+    const std::string label = "fn:" + f.agent->d->name + "::" + f.d->name;
+    tick(label, st, true);
+    f.d->launch(a, grid, block, st);
+    tick(label, st, false);
+    if (hipGetLastError() != hipSuccess) throw exception::CUDAError("agent function launch failed: " + f.d->name);
+}
+
+inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
+    auto& l = f.agent->states.at(f.in_state);
+    // 1. message output: swap (first mandatory output this step) or append
+    if (f.mout) {
+        auto& m = *f.mout;
+        tick("msgout:" + f.d->name, st, true);
+        if (m.truncate && !f.d->message_output_optional) {
+            for (auto& v : m.vars) std::swap(v.cur, v.alt);
+            detail::fgx_check(fgx_memcpy_d2d(m.d_count, f.d_launch_count, 4, st), "message count");
+        } else {
+            if (m.truncate) detail::fgx_check(fgx_fill_u32(m.d_count, 0, 1, st), "message truncate");
+            std::vector<const void*> src;
+            std::vector<void*> dst;
+            std::vector<uint32_t> by;
+            for (auto& v : m.vars) {
+                src.push_back(v.alt);
+                dst.push_back(v.cur);
+                by.push_back(v.bytes);
+            }
+            detail::fgx_check(fgx_append_births(ctx_, f.msg_flags, f.d_launch_count, l.cap, (uint32_t)m.vars.size(),
+                                                src.data(), dst.data(), by.data(), -1, m.d_count, m.cap, d_dummy_,
+                                                nullptr, f.append_scratch, f.append_scratch_bytes, st),
+                              "message append");
+        }
+        tick("msgout:" + f.d->name, st, false);
+        m.truncate = false;
+        m.pbm_dirty = true;
+    }
+    // 2. death (CUDAFatAgent::processDeath): stable compaction of every variable
+    if (f.d->has_agent_death) {
+        std::vector<const void*> src;
+        std::vector<void*> dst;
+        std::vector<uint32_t> by;
+        for (auto& v : l.vars) {
+            src.push_back(v.cur);
+            dst.push_back(v.alt);
+            by.push_back(v.bytes);
+        }
+        tick("death:" + f.d->name, st, true);
+        detail::fgx_check(fgx_compact(ctx_, f.death_flags, 1, l.d_count, l.cap, (uint32_t)l.vars.size(), src.data(),
+                                      dst.data(), by.data(), l.d_count, f.death_scratch, f.death_scratch_bytes, st),
+                          "death compaction");
+        tick("death:" + f.d->name, st, false);
+        for (auto& v : l.vars) std::swap(v.cur, v.alt);
+    }
+    // 3. state transition (CUDAFatAgent::transitionState): append all to the end state
+    if (f.in_state != f.end_state) {
+        auto& dl = f.agent->states.at(f.end_state);
+        std::vector<const void*> src;
+        std::vector<void*> dst;
+        std::vector<uint32_t> by;
+        for (size_t i = 0; i < l.vars.size(); ++i) {
+            src.push_back(l.vars[i].cur);
+            dst.push_back(dl.vars[i].cur);
+            by.push_back(l.vars[i].bytes);
+        }
+        detail::fgx_check(fgx_append_births(ctx_, nullptr, l.d_count, l.cap, (uint32_t)l.vars.size(), src.data(), dst.data(),
+                                            by.data(), -1, dl.d_count, dl.cap, d_dummy_, nullptr, f.trans_scratch,
+                                            f.trans_scratch_bytes, st),
+                          "state transition");
+        detail::fgx_check(fgx_fill_u32(l.d_count, 0, 1, st), "state transition count");
+    }
+    // 4. births (CUDAAgentStateList::scatterNew): appended after the survivors, ordered IDs
+    if (f.out_agent) {
+        auto& dl = f.out_agent->states.at(f.out_state);
+        std::vector<const void*> src;
+        std::vector<void*> dst;
+        std::vector<uint32_t> by;
+        for (size_t i = 0; i < dl.vars.size(); ++i) {
+            src.push_back(f.newbuf[i].cur);
+            dst.push_back(dl.vars[i].cur);
+            by.push_back(dl.vars[i].bytes);
+        }
+        tick("births:" + f.d->name, st, true);
+        detail::fgx_check(fgx_append_births(ctx_, f.birth_flags, f.d_launch_count, f.cap, (uint32_t)dl.vars.size(), src.data(),
+                                            dst.data(), by.data(), dl.id_slot, dl.d_count, dl.cap, f.out_agent->d_next_id,
+                                            d_error_ + 2, f.births_scratch, f.births_scratch_bytes, st),
+                          "agent births");
+        tick("births:" + f.d->name, st, false);
+    }
+}
+
+inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
+    if (!layer.host_functions.empty()) {
+        synchronize();
+        for (auto fn : layer.host_functions) fn(host_api_.get());
+        upload_env();
+        return;
+    }
+    // auto spatial sort (CUDASimulation.cu:671-682) and lazy index builds, on the main stream
+    for (auto& fd : layer.agent_functions) {
+        auto& f = fns_[fn_index_.at(fd.get())];
+        const unsigned period = f.d->parent->sortPeriod;
+        if (f.sort_dims && period && (step_ % period == 0)) {
+            auto& l = f.agent->states.at(f.in_state);
+            const int ix = l.var_index("x"), iy = l.var_index("y"), iz = l.var_index("z");
+            std::vector<const void*> src;
+            std::vector<void*> dst;
+            std::vector<uint32_t> by;
+            for (auto& v : l.vars) {
+                src.push_back(v.cur);
+                dst.push_back(v.alt);
+                by.push_back(v.bytes);
+            }
+            tick("sort:" + f.agent->d->name, stream_, true);
+            detail::fgx_check(fgx_agent_sort(ctx_, f.sort_dims, f.min->meta.min, f.min->meta.max, f.min->meta.radius, l.d_count,
+                                             l.cap, (const float*)l.vars[ix].cur, (const float*)l.vars[iy].cur,
+                                             f.sort_dims == 3 ? (const float*)l.vars[iz].cur : nullptr,
+                                             (uint32_t)l.vars.size(), src.data(), dst.data(), by.data(), l.sort_scratch,
+                                             l.sort_scratch_bytes, stream_),
+                              "agent spatial sort");
+            tick("sort:" + f.agent->d->name, stream_, false);
+            for (auto& v : l.vars) std::swap(v.cur, v.alt);
+        }
+    }
+    for (auto& fd : layer.agent_functions) {
+        auto& f = fns_[fn_index_.at(fd.get())];
+        if (f.min && f.min->spatial && f.min->pbm_dirty) {
+            auto& m = *f.min;
+            const int ix = m.var_index("x"), iy = m.var_index("y"), iz = m.var_index("z");
+            std::vector<const void*> src;
+            std::vector<void*> dst;
+            std::vector<uint32_t> by;
+            for (auto& v : m.vars) {
+                src.push_back(v.cur);
+                dst.push_back(v.alt);
+                by.push_back(v.bytes);
+            }
+            tick("build:" + m.d->name, stream_, true);
+            detail::fgx_check(fgx_spatial_build(ctx_, &m.meta, m.d_count, m.cap, (const float*)m.vars[ix].cur,
+                                                (const float*)m.vars[iy].cur,
+                                                m.meta.dims == 3 ? (const float*)m.vars[iz].cur : nullptr,
+                                                (uint32_t)m.vars.size(), src.data(), dst.data(), by.data(), m.pbm,
+                                                m.build_scratch, m.build_scratch_bytes, stream_),
+                              "message index build");
+            tick("build:" + m.d->name, stream_, false);
+            for (auto& v : m.vars) std::swap(v.cur, v.alt);
+            m.pbm_dirty = false;
+        }
+    }
+    const bool concurrent = cuda_config_.inLayerConcurrency && layer.agent_functions.size() > 1 && !timing_;
+    if (!concurrent) {
+        for (auto& fd : layer.agent_functions) {
+            auto& f = fns_[fn_index_.at(fd.get())];
+            launch_function(f, stream_);
+            unmap_function(f, stream_);
+        }
+        return;
+    }
+    // fork: each function of the layer on its own stream, joined back into the main stream
+    detail::fgx_check(fgx_event_record(fork_event_, stream_), "fork");
+    size_t k = 0;
+    for (auto& fd : layer.agent_functions) {
+        auto& f = fns_[fn_index_.at(fd.get())];
+        if (!f.stream) detail::fgx_check(fgx_stream_create(&f.stream), "stream create");
+        detail::fgx_check(fgx_stream_wait_event(f.stream, fork_event_), "fork wait");
+        launch_function(f, f.stream);
+        unmap_function(f, f.stream);
+        while (join_events_.size() <= k) {
+            void* e = nullptr;
+            detail::fgx_check(fgx_event_create(&e), "event create");
+            join_events_.push_back(e);
+        }
+        detail::fgx_check(fgx_event_record(join_events_[k], f.stream), "join");
+        ++k;
+    }
+    for (size_t j = 0; j < k; ++j) detail::fgx_check(fgx_stream_wait_event(stream_, join_events_[j]), "join wait");
+}
+
+inline void CUDASimulation::enqueue_end_of_step() {
+    // CUDASimulation.cu:619-624: non-persistent message lists are emptied
+    for (auto& kv : msgs_) {
+        auto& m = kv.second;
+        if (!m.d->persistent) {
+            detail::fgx_check(fgx_fill_u32(m.d_count, 0, 1, stream_), "message reset");
+            m.truncate = true;
+            m.pbm_dirty = true;
+        }
+    }
+    detail::fgx_check(fgx_add_u32(d_step_, 1, stream_), "step counter");
+}
+
+inline void CUDASimulation::enqueue_step() {
+    for (auto& layer : m_->layers) enqueue_layer(*layer);
+    enqueue_end_of_step();
+}
+
+inline std::vector<void*> CUDASimulation::buffer_state() const {
+    std::vector<void*> st;
+    for (auto& kv : agents_)
+        for (auto& s : kv.second.states)
+            for (auto& v : s.second.vars) {
+                st.push_back(v.cur);
+                st.push_back(v.alt);
+            }
+    for (auto& kv : msgs_)
+        for (auto& v : kv.second.vars) {
+            st.push_back(v.cur);
+            st.push_back(v.alt);
+        }
+    return st;
+}
+
+inline void CUDASimulation::set_buffer_state(const std::vector<void*>& st) {
+    size_t k = 0;
+    for (auto& kv : agents_)
+        for (auto& s : kv.second.states)
+            for (auto& v : s.second.vars) {
+                v.cur = st[k++];
+                v.alt = st[k++];
+            }
+    for (auto& kv : msgs_)
+        for (auto& v : kv.second.vars) {
+            v.cur = st[k++];
+            v.alt = st[k++];
+        }
+}
+
+inline bool CUDASimulation::step() {
+    const auto t0 = std::chrono::steady_clock::now();
+    ensure_init();
+    if (!init_functions_ran_) run_init_functions();
+    bool changed = false;
+    if (model_can_grow() || step_ == 0) {
+        sync_counts();
+        changed = ensure_capacity();
+    }
+    upload_env();
+    const bool use_graph = cuda_config_.graphs && graph_eligible_ && !timing_;
+    if (use_graph) {
+        if (changed) clear_graphs();
+        // One single-step graph per double-buffer arrangement at step start; a step
+        // swaps buffers, so a model alternates between a small number of arrangements.
+        const auto start = buffer_state();
+        GraphEntry* hit = nullptr;
+        for (auto& g : graphs_)
+            if (g.start == start) hit = &g;
+        if (!hit && graphs_.size() < 4) {
+            GraphEntry e;
+            e.start = start;
+            detail::fgx_check(fgx_graph_begin(stream_), "graph begin");
+            enqueue_step();
+            detail::fgx_check(fgx_graph_end(stream_, &e.g), "graph end");
+            e.end = buffer_state();
+            set_buffer_state(start);   // capture recorded the work; nothing has run yet
+            graphs_.push_back(std::move(e));
+            hit = &graphs_.back();
+        }
+        if (hit) {
+            detail::fgx_check(fgx_graph_launch(hit->g, stream_), "graph launch");
+            set_buffer_state(hit->end);
+        } else {
+            enqueue_step();
+        }
+    } else {
+        enqueue_step();
+    }
+    ++step_;
+    for (auto fn : m_->step_functions) {
+        synchronize();
+        fn(host_api_.get());
+        upload_env();
+    }
+    bool cont = true;
+    for (auto fn : m_->exit_conditions) {
+        synchronize();
+        if (fn(host_api_.get()) == EXIT) cont = false;
+    }
+    if (config_.timing) synchronize();
+    step_times_.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    return cont;
+}
+
+inline void CUDASimulation::run_init_functions() {
+    if (init_functions_ran_) return;
+    init_functions_ran_ = true;
+    ensure_init();
+    for (auto fn : m_->init_functions) {
+        synchronize();
+        fn(host_api_.get());
+    }
+    upload_env();
+}
+
+inline void CUDASimulation::run_exit_functions() {
+    for (auto fn : m_->exit_functions) {
+        synchronize();
+        fn(host_api_.get());
+    }
+}
+
+// ------------------------------------------------------------------ HostAPI implementation
+inline HostAgentAPI HostAPI::agent(const std::string& name, const std::string& state) {
+    auto& a = s_.agent_rt(name);
+    const std::string st = state.empty() ? a.d->initial_state : state;
+    auto it = a.states.find(st);
+    if (it == a.states.end()) throw exception::InvalidAgentState("Agent '" + name + "' has no state '" + st + "'");
+    return HostAgentAPI(s_, a, it->second);
+}
+inline unsigned int HostAPI::getStepCounter() const { return s_.getStepCounter(); }
+
+inline unsigned int HostAgentAPI::count() {
+    s_.sync_counts();
+    return l_.h_count;
+}
+
+template <typename T>
+inline const detail::VarBuf& HostAgentAPI::var(const std::string& name) {
+    auto vit = a_.d->variables.find(name);
+    if (vit == a_.d->variables.end())
+        throw exception::InvalidAgentVar("Agent '" + a_.d->name + "' does not contain variable '" + name + "'");
+    if (vit->second.type != std::type_index(typeid(T)) || vit->second.elements != 1)
+        throw exception::InvalidVarType("Variable '" + name + "' is not of the requested type");
+    return l_.vars[l_.var_index(name)];
+}
+
+template <typename InT, typename OutT>
+inline OutT HostAgentAPI::sum(const std::string& name) {
+    const auto& v = var<InT>(name);
+    s_.sync_counts();
+    int type;
+    if (std::is_same<InT, float>::value) type = 0;
+    else if (std::is_same<InT, double>::value) type = 1;
+    else if (std::is_same<InT, int32_t>::value) type = 2;
+    else if (std::is_same<InT, uint32_t>::value) type = 3;
+    else if (std::is_same<InT, int64_t>::value || std::is_same<InT, long long>::value) type = 4;
+    else if (std::is_same<InT, uint64_t>::value || std::is_same<InT, unsigned long long>::value) type = 5;
+    else {
+        auto vals = getVariableValues<InT>(name);
+        OutT r = 0;
+        for (auto x : vals) r += (OutT)x;
+        return r;
+    }
+    size_t sb = 0;
+    detail::fgx_check(fgx_scratch_bytes(FGX_OP_SCAN, std::max<uint32_t>(l_.cap, 1), 0, &sb), "reduction scratch");
+    void* scratch = s_.reduce_scratch(sb);
+    detail::fgx_check(fgx_reduce_sum(s_.ctx(), type, v.cur, l_.d_count, l_.cap, s_.reduce_out(), scratch, sb, s_.stream()),
+                      "reduction");
+    double dres = 0;
+    long long ires = 0;
+    if (type <= 1) {
+        detail::fgx_check(fgx_memcpy_d2h(&dres, s_.reduce_out(), 8, s_.stream()), "reduction readback");
+        s_.synchronize();
+        return (OutT)dres;
+    }
+    detail::fgx_check(fgx_memcpy_d2h(&ires, s_.reduce_out(), 8, s_.stream()), "reduction readback");
+    s_.synchronize();
+    if (type == 5 || type == 3) return (OutT)(unsigned long long)ires;
+    return (OutT)ires;
+}
+
+template <typename T>
+inline std::vector<T> HostAgentAPI::getVariableValues(const std::string& name) {
+    const auto& v = var<T>(name);
+    s_.sync_counts();
+    std::vector<T> out(l_.h_count);
+    if (l_.h_count)
+        detail::fgx_check(fgx_memcpy_d2h(out.data(), v.cur, (size_t)l_.h_count * sizeof(T), s_.stream()), "variable readback");
+    s_.synchronize();
+    return out;
+}
+
+template <typename T>
+inline T HostAgentAPI::min(const std::string& name) {
+    auto v = getVariableValues<T>(name);
+    return v.empty() ? std::numeric_limits<T>::max() : *std::min_element(v.begin(), v.end());
+}
+template <typename T>
+inline T HostAgentAPI::max(const std::string& name) {
+    auto v = getVariableValues<T>(name);
+    return v.empty() ? std::numeric_limits<T>::lowest() : *std::max_element(v.begin(), v.end());
+}
+
+template <typename T>
+inline T HostEnvironment::getProperty(const std::string& name) const {
+    return s_.getEnvironmentProperty<T>(name);
+}
+template <typename T>
+inline T HostEnvironment::setProperty(const std::string& name, T value) {
+    const T old = s_.getEnvironmentProperty<T>(name);
+    s_.setEnvironmentProperty<T>(name, value);
+    return old;
+}
+
+// ------------------------------------------------------------------ ensembles
+struct RunLog {
+    unsigned int plan_index = 0;
+    uint64_t seed = 0;
+    unsigned int steps = 0;
+    double wall_seconds = 0.0;
+    std::map<std::string, unsigned int> final_counts;
+    std::string error;
+};
+
+// include/flamegpu/simulation/CUDAEnsemble.h:40-129: independent simulations on
+// `concurrent_runs` host threads sharing the device (the plain per-simulation path).
+class CUDAEnsemble {
+ public:
+    struct EnsembleConfig {
+        std::string out_directory;
+        std::string out_format = "json";
+        unsigned int concurrent_runs = 4;
+        std::set<int> devices;
+        bool quiet = false;
+        bool verbose = false;
+        bool timing = false;
+        enum ErrorLevel { Off = 0, Slow = 1, Fast = 2 };
+        ErrorLevel error_level = Slow;
+        bool truncate_log_files = false;
+    };
+    explicit CUDAEnsemble(const ModelDescription& model, int argc = 0, const char** argv = nullptr) : model_(model) {
+        for (int i = 1; i < argc; ++i) {
+            const std::string a = argv[i];
+            if ((a == "-c" || a == "--concurrent") && i + 1 < argc) config_.concurrent_runs = (unsigned)std::stoul(argv[++i]);
+            else if (a == "-q" || a == "--quiet") config_.quiet = true;
+            else if (a == "-t" || a == "--timing") config_.timing = true;
+        }
+    }
+    EnsembleConfig& Config() { return config_; }
+    const EnsembleConfig& getConfig() const { return config_; }
+    // Population initialiser called for every run before it starts.
+    void setPopulationInit(std::function<void(CUDASimulation&, const RunPlan&, unsigned int)> f) { init_ = std::move(f); }
+    unsigned int simulate(const RunPlanVector& plans) {
+        logs_.clear();
+        logs_.resize(plans.size());
+        std::atomic<unsigned> next{0}, errors{0};
+        std::atomic<bool> stop{false};
+        const auto t0 = std::chrono::steady_clock::now();
+        auto worker = [&]() {
+            while (!stop) {
+                const unsigned i = next++;
+                if (i >= plans.size()) break;
+                RunLog& lg = logs_[i];
+                lg.plan_index = i;
+                lg.seed = plans[i].getRandomSimulationSeed();
+                lg.steps = plans[i].getSteps();
+                const auto r0 = std::chrono::steady_clock::now();
+                try {
+                    CUDASimulation sim(model_);
+                    sim.CUDAConfig().is_ensemble = true;
+                    sim.SimulationConfig().random_seed = plans[i].getRandomSimulationSeed();
+                    sim.SimulationConfig().steps = plans[i].getSteps();
+                    for (auto& kv : plans[i].properties()) sim_set_env(sim, kv.first, kv.second);
+                    if (init_) init_(sim, plans[i], i);
+                    sim.simulate();
+                    for (auto& a : model_.data()->agents) lg.final_counts[a.first] = sim_count(sim, a.first);
+                } catch (const std::exception& e) {
+                    lg.error = e.what();
+                    ++errors;
+                    if (config_.error_level == EnsembleConfig::Fast) stop = true;
+                }
+                lg.wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - r0).count();
+            }
+        };
+        std::vector<std::thread> th;
+        const unsigned nt = std::max(1u, std::min<unsigned>(config_.concurrent_runs, (unsigned)plans.size()));
+        for (unsigned t = 0; t < nt; ++t) th.emplace_back(worker);
+        for (auto& t : th) t.join();
+        elapsed_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (errors && config_.error_level != EnsembleConfig::Off) {
+            throw exception::EnsembleError(detail::format("%u/%zu ensemble runs failed: %s", errors.load(), plans.size(),
+                                                          first_error().c_str()));
+        }
+        return errors;
+    }
+    const std::vector<RunLog>& getLogs() const { return logs_; }
+    double getEnsembleElapsedTime() const { return elapsed_; }
+
+ private:
+    std::string first_error() const {
+        for (auto& l : logs_)
+            if (!l.error.empty()) return l.error;
+        return "";
+    }
+    static void sim_set_env(CUDASimulation& s, const std::string& name, const std::vector<char>& b) {
+        auto& props = s.model()->env.props;
+        auto it = props.find(name);
+        if (it == props.end()) throw exception::InvalidEnvProperty("unknown property " + name);
+        std::memcpy(s.env_host().data() + it->second.offset, b.data(), b.size());
+        s.mark_env_dirty();
+    }
+    static unsigned int sim_count(CUDASimulation& s, const std::string& agent) {
+        HostAPI api(s, 0);
+        return api.agent(agent).count();
+    }
+    ModelDescription model_;
+    EnsembleConfig config_;
+    std::vector<RunLog> logs_;
+    std::function<void(CUDASimulation&, const RunPlan&, unsigned int)> init_;
+    double elapsed_ = 0.0;
+};
+
+namespace util {
+inline void cleanup() { fgx_device_synchronize(); }
+}  // namespace util
+
+}  // namespace flamegpu

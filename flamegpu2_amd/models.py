@@ -1,0 +1,164 @@
+"""Python driver for model libraries (include/fgx_model.h) via ctypes.
+
+    sim = Simulation("boids_spatial3D", params, seed=7)
+    sim.set_agents("Boid", [x, y, z, fx, fy, fz])
+    sim.step(10)
+    arrays, ids = sim.get_agents("Boid")
+
+The model runs entirely in the library (C++ runtime over libfgx.so); this
+module only moves host arrays across the C ABI.
+"""
+import ctypes
+
+import numpy as np
+
+from . import load
+from .fgx import FgxError
+
+_DT = {0: np.float32, 1: np.float64, 2: np.int32, 3: np.uint32, 4: np.int64, 5: np.uint64,
+       6: np.int8, 7: np.uint8, 8: np.int16, 9: np.uint16}
+_libs = {}
+
+
+def _lib(model):
+    if model in _libs:
+        return _libs[model]
+    L = load(f"libfgxm_{model}.so")
+    c = ctypes
+    vp, u32p = c.c_void_p, c.POINTER(c.c_uint32)
+    sigs = {
+        "fgxm_create": [c.c_char_p, c.POINTER(c.c_float), c.c_uint32, c.c_uint64, c.c_int, c.POINTER(vp)],
+        "fgxm_destroy": [vp],
+        "fgxm_var_count": [vp, c.c_char_p, u32p],
+        "fgxm_var_info": [vp, c.c_char_p, c.c_uint32, c.POINTER(c.c_char_p), c.POINTER(c.c_int32), u32p, u32p],
+        "fgxm_set_agents": [vp, c.c_char_p, c.c_char_p, c.c_uint32, c.POINTER(vp), u32p],
+        "fgxm_count": [vp, c.c_char_p, c.c_char_p, u32p],
+        "fgxm_get_agents": [vp, c.c_char_p, c.c_char_p, c.POINTER(vp), u32p],
+        "fgxm_step": [vp, c.c_uint32],
+        "fgxm_set_option": [vp, c.c_char_p, c.c_int],
+        "fgxm_sync": [vp],
+        "fgxm_get_pbm": [vp, c.c_char_p, u32p, c.c_uint32, u32p],
+        "fgxm_kernel_times": [vp, c.c_char_p, c.c_uint32, c.c_int],
+        "fgxm_timed_steps": [vp, c.c_uint32, c.POINTER(c.c_double)],
+        "fgxm_step_counter": [vp, u32p],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = c.c_int
+    L.fgxm_last_error.restype = c.c_char_p
+    L.fgxm_last_error.argtypes = []
+    _libs[model] = L
+    return L
+
+
+def _b(s):
+    return None if s is None else s.encode()
+
+
+class Simulation:
+    def __init__(self, model, params=(), seed=0, variant=None, device=0):
+        self.L = _lib(model)
+        self.model = model
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        h = ctypes.c_void_p()
+        self._chk(self.L.fgxm_create(_b(variant), p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(p), seed,
+                                     device, ctypes.byref(h)))
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise FgxError(rc, self.L.fgxm_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fgxm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def var_info(self, agent):
+        n = ctypes.c_uint32()
+        self._chk(self.L.fgxm_var_count(self.h, _b(agent), ctypes.byref(n)))
+        out = []
+        for v in range(n.value):
+            name = ctypes.c_char_p()
+            t, e, b = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_uint32()
+            self._chk(self.L.fgxm_var_info(self.h, _b(agent), v, ctypes.byref(name), ctypes.byref(t), ctypes.byref(e),
+                                           ctypes.byref(b)))
+            out.append((name.value.decode(), _DT.get(t.value, np.uint8), e.value, b.value))
+        return out
+
+    def set_agents(self, agent, arrays, ids=None, state=None):
+        info = self.var_info(agent)
+        assert len(arrays) == len(info), f"{agent} has {len(info)} variables"
+        n = len(arrays[0])
+        keep = []
+        for a, (name, dt, e, b) in zip(arrays, info):
+            a = np.ascontiguousarray(a, dtype=dt)
+            assert a.nbytes == n * b, f"variable {name}: expected {n * b} bytes"
+            keep.append(a)
+        vp = (ctypes.c_void_p * max(len(keep), 1))(*[a.ctypes.data for a in keep])
+        idp = None
+        if ids is not None:
+            ida = np.ascontiguousarray(ids, dtype=np.uint32)
+            keep.append(ida)
+            idp = ida.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        self._chk(self.L.fgxm_set_agents(self.h, _b(agent), _b(state), n, vp, idp))
+
+    def count(self, agent, state=None):
+        n = ctypes.c_uint32()
+        self._chk(self.L.fgxm_count(self.h, _b(agent), _b(state), ctypes.byref(n)))
+        return n.value
+
+    def get_agents(self, agent, state=None):
+        info = self.var_info(agent)
+        n = self.count(agent, state)
+        outs = [np.zeros(n * e, dtype=dt) for (_, dt, e, _) in info]
+        ids = np.zeros(n, dtype=np.uint32)
+        vp = (ctypes.c_void_p * max(len(outs), 1))(*[o.ctypes.data for o in outs])
+        self._chk(self.L.fgxm_get_agents(self.h, _b(agent), _b(state), vp,
+                                         ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return outs, ids
+
+    def step(self, n=1):
+        self._chk(self.L.fgxm_step(self.h, n))
+
+    def set_option(self, key, value):
+        self._chk(self.L.fgxm_set_option(self.h, _b(key), int(value)))
+
+    def sync(self):
+        self._chk(self.L.fgxm_sync(self.h))
+
+    def pbm(self, msg, max_bins=1 << 26):
+        bins = ctypes.c_uint32()
+        # first call with capacity 0 only reports the bin count
+        rc = self.L.fgxm_get_pbm(self.h, _b(msg), None, 0, ctypes.byref(bins))
+        out = np.zeros(bins.value + 1, dtype=np.uint32)
+        self._chk(self.L.fgxm_get_pbm(self.h, _b(msg), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                      len(out), ctypes.byref(bins)))
+        del rc
+        return out
+
+    def kernel_times(self, reset=False):
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._chk(self.L.fgxm_kernel_times(self.h, buf, len(buf), 1 if reset else 0))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            label, ms, cnt = line.rsplit(" ", 2)
+            out[label] = (float(ms), int(cnt))
+        return out
+
+    def timed_steps(self, n):
+        ms = ctypes.c_double()
+        self._chk(self.L.fgxm_timed_steps(self.h, n, ctypes.byref(ms)))
+        return ms.value
+
+    def step_counter(self):
+        n = ctypes.c_uint32()
+        self._chk(self.L.fgxm_step_counter(self.h, ctypes.byref(n)))
+        return n.value

@@ -63,6 +63,8 @@ int fgx_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int fgx_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 /* Fill n 32-bit words with `value` using a kernel (graph-capture safe). */
 int fgx_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* stream);
+/* *p += delta on the device (step counters inside captured graphs). */
+int fgx_add_u32(uint32_t* p, uint32_t delta, void* stream);
 /* Fill `nvars` variable arrays of n elements with per-variable default values
  * (element bytes var_bytes[v], default value bytes at defaults + def_off[v]);
  * replaces CUDAScatter::broadcastInit (src/flamegpu/simulation/detail/CUDAScatter.cu:404-422). */
@@ -155,7 +157,9 @@ int fgx_compact(fgx_ctx* ctx, const uint32_t* flags, uint32_t keep, const uint32
  * Replaces CUDAAgentStateList::scatterNew (src/flamegpu/simulation/detail/CUDAAgentStateList.cu:189-250)
  * and the device ID counter (CUDAFatAgent::getDeviceNextID/notifyDeviceBirths,
  * src/flamegpu/simulation/detail/CUDAFatAgent.cu:320-346; DeviceAPI::AgentOut::genID,
- * include/flamegpu/runtime/DeviceAPI.cuh:546-557).  Parents with flags[i] != 0
+ * include/flamegpu/runtime/DeviceAPI.cuh:546-557).  flags == NULL appends every
+ * parent (state transition append, CUDAFatAgent::transitionState,
+ * src/flamegpu/simulation/detail/CUDAFatAgent.cu:139-184).  Parents with flags[i] != 0
  * append newbuf[*][i] after *d_dst_count entries of dst, in parent order;
  * variable id_var (-1: none) receives *d_next_id + rank.  *d_dst_count and
  * *d_next_id advance by the number of births.  Births beyond dst_capacity
