@@ -37,7 +37,7 @@ def boids(n=1 << 20, seed=7, width=3.2, radius=0.05, speed=0.05):
     norm = np.sqrt((vel.astype(np.float64) ** 2).sum(axis=0))
     norm[norm == 0] = 1.0
     vel = (vel / norm * speed).astype(np.float32)
-    params = np.array([0.0, w, radius, 0.015, 1.0, 1.0, 0.005, 0.01, 0.125, speed * 0.5, speed],
+    params = np.array([0.0, w, radius, 0.015, 1.0, 0.15, 0.005, 0.01, 0.125, speed * 0.5, speed],
                       dtype=np.float32)
     return params, [pos[0].copy(), pos[1].copy(), pos[2].copy(), vel[0].copy(), vel[1].copy(), vel[2].copy()]
 
