@@ -126,8 +126,29 @@ constexpr unsigned ENV_SLOT_BITS = 6;
 
 struct VarBinding {
     uint32_t hash;
-    uint32_t bytes;   // element bytes (type size x array length)
+    uint16_t bytes;   // element bytes (type size x array length)
+    uint16_t lds;     // input messages: staged array index in LDS, NOT_STAGED otherwise
     char* ptr;
+};
+constexpr uint16_t NOT_STAGED = 0xFFFF;
+
+// LDS staging of spatial message neighbourhoods (see agent_function_kernel_staged).
+constexpr unsigned STAGE_MAX_GROUPS = 4;
+constexpr unsigned STAGE_MAX_VARS = 4;
+constexpr unsigned STAGE_MAX_ENTRIES = 768;   // staged virtual bins per block (3 per thread)
+struct StageGroup {
+    int cy, cz;        // the group's own row
+    int xlo, xhi;      // x extent of its agents' cells
+    int xbase;         // first staged (virtual) x
+    int width;         // staged virtual bins per neighbour row
+    uint32_t ebase;    // first entry of the group
+};
+struct StageHeader {
+    uint32_t ngroups;
+    uint32_t nentries;
+    uint32_t ok;
+    uint32_t total;
+    StageGroup g[STAGE_MAX_GROUPS];
 };
 
 struct EnvBinding {
@@ -203,6 +224,13 @@ struct FnArgs {
     const uint32_t* ids;         // the agent list's "_id" array
     int32_t msg_out_optional;
     uint32_t agent_mul, msg_in_mul, msg_out_mul, new_mul, env_mul;
+    // neighbourhood staging (spatial message input)
+    int32_t stage_ok;                 // agent x/y(/z) present and staged vars chosen
+    uint32_t stage_nvars;             // staged 4-byte message variables
+    uint32_t stage_max_entries;       // virtual bins per block
+    uint32_t stage_cap;               // staged messages per block
+    const float* stage_pos[3];        // agent x, y, z (the assumed search origin)
+    const uint32_t* stage_src[STAGE_MAX_VARS];
     SpatialDev in_meta;
     VarBinding agent[VAR_SLOTS];
     VarBinding msg_in[VAR_SLOTS];
@@ -223,7 +251,7 @@ __host__ __device__ constexpr uint32_t table_slot(uint32_t h, uint32_t mul, unsi
 __device__ __forceinline__ Found find_binding(const VarBinding (&tab)[VAR_SLOTS], uint32_t mul, uint32_t h) {
     const VarBinding& b = tab[table_slot(h, mul, VAR_SLOT_BITS)];
     if (b.hash != h) return Found{nullptr, 0};
-    return Found{b.ptr, b.bytes};
+    return Found{b.ptr, (uint32_t)b.bytes};
 }
 
 __device__ __forceinline__ void report_error(uint32_t* d_error, uint32_t code, uint32_t detail, uint32_t index) {

@@ -52,7 +52,7 @@ typedef FLAMEGPU_HOST_FUNCTION_POINTER FLAMEGPU_EXIT_FUNCTION_POINTER;
 typedef CONDITION_RESULT (*FLAMEGPU_EXIT_CONDITION_POINTER)(HostAPI*);
 
 namespace detail {
-typedef void (*LaunchFn)(const FnArgs& args, uint32_t grid, uint32_t block, void* stream);
+typedef void (*LaunchFn)(const FnArgs& args, uint32_t grid, uint32_t block, uint32_t smem, void* stream);
 template <typename T>
 Variable make_variable(const std::string& name, size_type elements, const T* defaults) {
     static_assert(std::is_arithmetic<T>::value, "agent/message variables must be arithmetic types");
@@ -334,7 +334,7 @@ typedef AgentFunctionDescription CAgentFunctionDescription;
 
 namespace detail {
 template <typename F>
-void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, void* stream);
+void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, uint32_t smem, void* stream);
 }
 
 // ------------------------------------------------------------------ agents

@@ -160,6 +160,7 @@ FGXM_EXPORT int fgxm_set_option(fgxm_sim* s, const char* key, int value) {
         if (k == "graphs") s->sim->CUDAConfig().graphs = value != 0;
         else if (k == "concurrency") s->sim->CUDAConfig().inLayerConcurrency = value != 0;
         else if (k == "kernel_timing") s->sim->setKernelTiming(value != 0);
+        else if (k == "staging") s->sim->setNeighbourhoodStaging(value != 0);
         else throw flamegpu::exception::InvalidArgument("unknown option " + k);
     });
 }

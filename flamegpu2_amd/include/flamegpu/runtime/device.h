@@ -63,6 +63,15 @@ __device__ __forceinline__ uint32_t load_count(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ uint32_t wave_inclusive(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    return v;
+}
+
 }  // namespace detail
 
 // ------------------------------------------------------------------ RNG
@@ -326,35 +335,83 @@ __device__ __forceinline__ uint32_t cell_hash(const SpatialDev& m, int x, int y,
     const uint32_t ux = (uint32_t)(x < 0 ? 0 : (x >= hx ? hx : x));
     return (uint32_t)z * m.grid[0] * m.grid[1] + (uint32_t)y * m.grid[0] + ux;
 }
-// getVirtualX (MessageSpatial3DDevice.cuh:373-397).
+// getVirtualX (MessageSpatial3DDevice.cuh:373-397): x2 - (x21/|x21|)*width.  On the
+// taken branch |x21| > width/2 > 0, so x21/|x21| is exactly +-1 and the product is
+// exactly +-width: copysignf gives the identical value without a division.
 __device__ __forceinline__ float virtual_coord(float x2, float x1, float width) {
     const float x21 = x2 - x1;
-    return fabsf(x21) > width / 2.0f ? x2 - (x21 / fabsf(x21) * width) : x2;
+    return fabsf(x21) > width / 2.0f ? x2 - copysignf(width, x21) : x2;
+}
+// (c + d) mod g for |d| <= 1 and 0 <= c < g, without an integer division.
+__device__ __forceinline__ int wrap_cell(int c, int d, int g) {
+    int a = c + d;
+    a = a < 0 ? a + g : a;
+    return a >= g ? a - g : a;
 }
 }  // namespace detail
 
 // Spatial message with its search origin, shared by the 2-D and 3-D filters.
+// Two sources for the same visit order: the block's LDS copy of the neighbourhood
+// (agent_function_kernel_staged) when the search origin's cell is covered by it,
+// otherwise the message list in HBM.
 template <int DIMS, bool WRAP>
 class SpatialFilter {
  public:
-    __device__ __forceinline__ SpatialFilter(const detail::FnArgs& a, float x, float y, float z) : r_{a}, m_(a.in_meta) {
+    __device__ __forceinline__ SpatialFilter(const detail::FnArgs& a, float x, float y, float z, const uint32_t* smem,
+                                             const detail::StageHeader* sh)
+        : a_(a), m_(a.in_meta) {
         loc_[0] = x;
         loc_[1] = y;
         loc_[2] = z;
         cell_[0] = detail::grid_pos(m_, 0, x);
         cell_[1] = detail::grid_pos(m_, 1, y);
         cell_[2] = DIMS == 3 ? detail::grid_pos(m_, 2, z) : 0;
+        // Staged reads only when every active lane of the wave is covered, so the
+        // choice between LDS and HBM is a scalar branch, never both paths.
+        int covered = 0;
+        if (smem) {
+            for (uint32_t k = 0; k < sh->ngroups; ++k) {
+                const detail::StageGroup& g = sh->g[k];
+                if (g.cy == cell_[1] && g.cz == cell_[2] && cell_[0] >= g.xlo && cell_[0] <= g.xhi) {
+                    covered = 1;
+                    ebase_ = g.ebase;
+                    xbase_ = g.xbase;
+                    width_ = (uint32_t)g.width;
+                }
+            }
+        }
+        staged_ = __builtin_amdgcn_readfirstlane(__all(covered) ? 1 : 0) != 0;
+        smem_ = smem;
     }
     class Message {
      public:
-        __device__ __forceinline__ Message(const SpatialFilter& f, uint32_t i) : f_(f), i_(i) {}
+        __device__ __forceinline__ Message(const SpatialFilter& f, uint32_t g, uint32_t l) : f_(f), g_(g), l_(l) {}
         template <typename T, unsigned N>
         __device__ __forceinline__ T getVariable(const char (&name)[N]) const {
-            return f_.r_.template get<T>(name, i_);
+            const uint32_t h = detail::name_hash(name, N);
+            const detail::VarBinding& b = f_.a_.msg_in[detail::table_slot(h, f_.a_.msg_in_mul, detail::VAR_SLOT_BITS)];
+            if (b.hash != h) {
+                detail::report_error(f_.a_.d_error, detail::DERR_UNKNOWN_MSG_VAR, h, g_);
+                return T();
+            }
+            if (b.bytes != sizeof(T)) {
+                detail::report_error(f_.a_.d_error, detail::DERR_MSG_VAR_TYPE, h, g_);
+                return T();
+            }
+            if constexpr (sizeof(T) == 4) {
+                if (f_.staged_ && b.lds != detail::NOT_STAGED) {
+                    const uint32_t w = f_.smem_[2 * (f_.a_.stage_max_entries + 1) + (uint32_t)b.lds * f_.a_.stage_cap + l_];
+                    T v;
+                    __builtin_memcpy(&v, &w, 4);
+                    return v;
+                }
+            }
+            return reinterpret_cast<const T*>(b.ptr)[g_];
         }
         template <typename T, size_type L, unsigned N>
         __device__ __forceinline__ T getVariable(const char (&name)[N], unsigned int e) const {
-            return f_.r_.template get<T, L>(name, i_, e);
+            return detail::load_elem<T>(f_.a_.msg_in, f_.a_.msg_in_mul, name, g_, L, e, f_.a_.d_error,
+                                        detail::DERR_UNKNOWN_MSG_VAR, detail::DERR_MSG_VAR_TYPE);
         }
         __device__ __forceinline__ float getVirtualX() const { return getVirtualX(f_.loc_[0]); }
         __device__ __forceinline__ float getVirtualY() const { return getVirtualY(f_.loc_[1]); }
@@ -368,11 +425,11 @@ class SpatialFilter {
         __device__ __forceinline__ float getVirtualZ(float z1) const {
             return detail::virtual_coord(getVariable<float>("z"), z1, f_.m_.width[2]);
         }
-        __device__ __forceinline__ size_type getIndex() const { return i_; }
+        __device__ __forceinline__ size_type getIndex() const { return g_; }
 
      private:
         const SpatialFilter& f_;
-        uint32_t i_;
+        uint32_t g_, l_;
     };
     // Visit order of the reference iterators:
     //  Filter 3-D (MessageSpatial3DDevice.cuh:52-59,693-719): 9 x-strips, dy outer, dz inner;
@@ -381,26 +438,12 @@ class SpatialFilter {
     static constexpr int kUnits = WRAP ? (DIMS == 3 ? 27 : 9) : (DIMS == 3 ? 9 : 3);
     class iterator {
      public:
-        __device__ __forceinline__ iterator(const SpatialFilter& f, int unit) : f_(f), unit_(unit), cur_(0), end_(0) {}
+        __device__ __forceinline__ iterator(const SpatialFilter& f, int unit) : f_(f), unit_(unit), g_(0), l_(0), lend_(0) {}
         __device__ __forceinline__ void advance_unit() {
             const detail::SpatialDev& m = f_.m_;
             while (++unit_ < kUnits) {
-                uint32_t s, e;
-                if (!WRAP) {
-                    int dy, dz;
-                    if (DIMS == 3) {
-                        dy = unit_ / 3 - 1;
-                        dz = unit_ % 3 - 1;
-                    } else {
-                        dy = unit_ - 1;
-                        dz = 0;
-                    }
-                    const int ay = f_.cell_[1] + dy, az = f_.cell_[2] + dz;
-                    if (ay < 0 || ay >= (int)m.grid[1] || az < 0 || az >= (int)m.grid[2]) continue;
-                    s = detail::cell_hash(m, f_.cell_[0] - 1, ay, az);
-                    e = detail::cell_hash(m, f_.cell_[0] + 1, ay, az) + 1;
-                } else {
-                    int dx, dy, dz;
+                int dx, dy, dz;
+                if (WRAP) {
                     if (DIMS == 3) {
                         dx = unit_ / 9 - 1;
                         dy = (unit_ / 3) % 3 - 1;
@@ -410,28 +453,61 @@ class SpatialFilter {
                         dy = unit_ % 3 - 1;
                         dz = 0;
                     }
-                    const int ax = (f_.cell_[0] + dx + (int)m.grid[0]) % (int)m.grid[0];
-                    const int ay = (f_.cell_[1] + dy + (int)m.grid[1]) % (int)m.grid[1];
-                    const int az = (f_.cell_[2] + dz + (int)m.grid[2]) % (int)m.grid[2];
-                    s = detail::cell_hash(m, ax, ay, az);
-                    e = s + 1;
+                } else {
+                    dx = 0;
+                    if (DIMS == 3) {
+                        dy = unit_ / 3 - 1;
+                        dz = unit_ % 3 - 1;
+                    } else {
+                        dy = unit_ - 1;
+                        dz = 0;
+                    }
+                    const int ay = f_.cell_[1] + dy, az = f_.cell_[2] + dz;
+                    if (ay < 0 || ay >= (int)m.grid[1] || az < 0 || az >= (int)m.grid[2]) continue;
                 }
-                cur_ = m.pbm[s];
-                end_ = m.pbm[e];
-                if (cur_ < end_) return;
+                const int hx = (int)m.grid[0] - 1;
+                const int xs = WRAP ? f_.cell_[0] + dx : (f_.cell_[0] - 1 < 0 ? 0 : f_.cell_[0] - 1);
+                const int xe = WRAP ? xs : (f_.cell_[0] + 1 > hx ? hx : f_.cell_[0] + 1);
+                if (f_.staged_) {
+                    const uint32_t j = DIMS == 3 ? (uint32_t)((dy + 1) * 3 + (dz + 1)) : (uint32_t)(dy + 1);
+                    const uint32_t row = f_.ebase_ + j * f_.width_;
+                    const uint32_t es = row + (uint32_t)(xs - f_.xbase_);
+                    const uint32_t ee = row + (uint32_t)(xe - f_.xbase_);
+                    l_ = f_.smem_[es];
+                    lend_ = f_.smem_[ee + 1];
+                    g_ = f_.smem_[f_.a_.stage_max_entries + 1 + es];
+                } else {
+                    uint32_t s, e;
+                    if (WRAP) {
+                        const int ax = detail::wrap_cell(f_.cell_[0], dx, (int)m.grid[0]);
+                        const int ay = detail::wrap_cell(f_.cell_[1], dy, (int)m.grid[1]);
+                        const int az = detail::wrap_cell(f_.cell_[2], dz, (int)m.grid[2]);
+                        s = detail::cell_hash(m, ax, ay, az);
+                        e = s + 1;
+                    } else {
+                        const int ay = f_.cell_[1] + dy, az = f_.cell_[2] + dz;
+                        s = detail::cell_hash(m, xs, ay, az);
+                        e = detail::cell_hash(m, xe, ay, az) + 1;
+                    }
+                    g_ = m.pbm[s];
+                    l_ = g_;
+                    lend_ = m.pbm[e];
+                }
+                if (l_ < lend_) return;
             }
         }
         __device__ __forceinline__ iterator& operator++() {
-            if (++cur_ >= end_) advance_unit();
+            ++g_;
+            if (++l_ >= lend_) advance_unit();
             return *this;
         }
         __device__ __forceinline__ bool operator!=(const iterator&) const { return unit_ < kUnits; }
-        __device__ __forceinline__ Message operator*() const { return Message(f_, cur_); }
+        __device__ __forceinline__ Message operator*() const { return Message(f_, g_, l_); }
 
      private:
         const SpatialFilter& f_;
         int unit_;
-        uint32_t cur_, end_;
+        uint32_t g_, l_, lend_;
     };
     __device__ __forceinline__ iterator begin() const {
         iterator it(*this, -1);
@@ -441,54 +517,73 @@ class SpatialFilter {
     __device__ __forceinline__ iterator end() const { return iterator(*this, kUnits); }
 
  private:
-    detail::MsgReader r_;
+    const detail::FnArgs& a_;
     const detail::SpatialDev& m_;
     float loc_[3];
     int cell_[3];
+    const uint32_t* smem_;
+    bool staged_;
+    uint32_t ebase_ = 0;
+    int xbase_ = 0;
+    uint32_t width_ = 0;
 };
 
 // include/flamegpu/runtime/messaging/MessageSpatial3D/MessageSpatial3DDevice.cuh:503-558
 class MessageSpatial3D::In {
  public:
+    static constexpr int dims = 3;
     typedef SpatialFilter<3, false> Filter;
     typedef SpatialFilter<3, true> WrapFilter;
     __device__ __forceinline__ explicit In(const detail::FnArgs& a) : a_(a) {}
-    __device__ __forceinline__ Filter operator()(float x, float y, float z) const { return Filter(a_, x, y, z); }
+    __device__ __forceinline__ void enable_stage(const uint32_t* smem, const detail::StageHeader* sh) {
+        smem_ = smem;
+        sh_ = sh;
+    }
+    __device__ __forceinline__ Filter operator()(float x, float y, float z) const { return Filter(a_, x, y, z, smem_, sh_); }
     __device__ __forceinline__ WrapFilter wrap(float x, float y, float z) const {
         const detail::SpatialDev& m = a_.in_meta;
         if (x > m.max[0] || y > m.max[1] || z > m.max[2] || x < m.min[0] || y < m.min[1] || z < m.min[2]) {
             detail::report_error(a_.d_error, detail::DERR_WRAP_OUT_OF_BOUNDS, 0, 0);
-            return WrapFilter(a_, m.min[0], m.min[1], m.min[2]);
+            return WrapFilter(a_, m.min[0], m.min[1], m.min[2], nullptr, nullptr);
         }
         if (!m.wrap_ok) detail::report_error(a_.d_error, detail::DERR_WRAP_NOT_FACTOR, 0, 0);
-        return WrapFilter(a_, x, y, z);
+        return WrapFilter(a_, x, y, z, smem_, sh_);
     }
     __device__ __forceinline__ float radius() const { return a_.in_meta.radius; }
 
  private:
     const detail::FnArgs& a_;
+    const uint32_t* smem_ = nullptr;
+    const detail::StageHeader* sh_ = nullptr;
 };
 
 // include/flamegpu/runtime/messaging/MessageSpatial2D/MessageSpatial2DDevice.cuh
 class MessageSpatial2D::In {
  public:
+    static constexpr int dims = 2;
     typedef SpatialFilter<2, false> Filter;
     typedef SpatialFilter<2, true> WrapFilter;
     __device__ __forceinline__ explicit In(const detail::FnArgs& a) : a_(a) {}
-    __device__ __forceinline__ Filter operator()(float x, float y) const { return Filter(a_, x, y, 0.0f); }
+    __device__ __forceinline__ void enable_stage(const uint32_t* smem, const detail::StageHeader* sh) {
+        smem_ = smem;
+        sh_ = sh;
+    }
+    __device__ __forceinline__ Filter operator()(float x, float y) const { return Filter(a_, x, y, 0.0f, smem_, sh_); }
     __device__ __forceinline__ WrapFilter wrap(float x, float y) const {
         const detail::SpatialDev& m = a_.in_meta;
         if (x > m.max[0] || y > m.max[1] || x < m.min[0] || y < m.min[1]) {
             detail::report_error(a_.d_error, detail::DERR_WRAP_OUT_OF_BOUNDS, 0, 0);
-            return WrapFilter(a_, m.min[0], m.min[1], 0.0f);
+            return WrapFilter(a_, m.min[0], m.min[1], 0.0f, nullptr, nullptr);
         }
         if (!m.wrap_ok) detail::report_error(a_.d_error, detail::DERR_WRAP_NOT_FACTOR, 0, 0);
-        return WrapFilter(a_, x, y, 0.0f);
+        return WrapFilter(a_, x, y, 0.0f, smem_, sh_);
     }
     __device__ __forceinline__ float radius() const { return a_.in_meta.radius; }
 
  private:
     const detail::FnArgs& a_;
+    const uint32_t* smem_ = nullptr;
+    const detail::StageHeader* sh_ = nullptr;
 };
 
 class MessageSpatial2D::Out : public MessageBruteForce::Out {
@@ -605,7 +700,7 @@ class DeviceAPI {
     uint32_t idx_;
 };
 
-// ------------------------------------------------------------------ the agent-function kernel
+// ------------------------------------------------------------------ the agent-function kernels
 // include/flamegpu/runtime/AgentFunction.cuh:60-120: one thread per agent of the
 // state list; the count is read from device memory so launches can be replayed
 // from a captured graph while populations change.
@@ -625,9 +720,191 @@ __global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
         report_error(a.d_error, DERR_DEATH_DISABLED, 0, i);
 }
 
+// Stages, for the block's 256 bin-sorted agents, every message of the bins their
+// search neighbourhoods can touch: agents are grouped by their own cell row
+// (cy, cz); group g needs the 9 (3 in 2-D) rows around it over the virtual x range
+// [xlo-1, xhi+1] (wrapped onto the grid, which also covers the clamped strips of
+// the non-wrapping iterator).  LDS holds, per virtual bin, its first staged slot
+// and its first global message index, then the staged variables as SoA arrays.
+// Returns false (block-uniform) when the block does not fit; its agents then read HBM.
+template <int DIMS>
+__device__ __forceinline__ bool stage_block(const FnArgs& a, uint32_t* smem, StageHeader& sh, uint32_t i, bool active) {
+    __shared__ int s_row[256];
+    __shared__ uint32_t s_w[4];
+    const SpatialDev& m = a.in_meta;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint32_t maxe = a.stage_max_entries;
+    uint32_t* lstart = smem;
+    uint32_t* gstart = smem + maxe + 1;
+    uint32_t* vars = smem + 2 * (maxe + 1);
+    const int gx = (int)m.grid[0], gy = (int)m.grid[1], gz = (int)m.grid[2];
+    int cx = 0, cy = 0, cz = 0, row = -1;
+    if (active) {
+        cx = grid_pos(m, 0, a.stage_pos[0][i]);
+        cy = grid_pos(m, 1, a.stage_pos[1][i]);
+        cz = DIMS == 3 ? grid_pos(m, 2, a.stage_pos[2][i]) : 0;
+        row = cz * gy + cy;
+    }
+    s_row[tid] = row;
+    __syncthreads();
+    const bool head = active && (tid == 0 || s_row[tid - 1] != row);
+    const uint64_t hm = __ballot(head);
+    if (lane == 0) s_w[w] = (uint32_t)__builtin_popcountll(hm);
+    __syncthreads();
+    uint32_t before = 0, groups = 0;
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k < w) before += s_w[k];
+        groups += s_w[k];
+    }
+    if (groups > STAGE_MAX_GROUPS) return false;
+    const int gidx = (int)(before + (uint32_t)__builtin_popcountll(hm & ((2ull << lane) - 1ull))) - 1;
+    if (head) {
+        sh.g[gidx].cy = cy;
+        sh.g[gidx].cz = cz;
+        sh.g[gidx].xlo = 0x7FFFFFFF;
+        sh.g[gidx].xhi = -1;
+    }
+    __syncthreads();
+    if (active) {
+        atomicMin(&sh.g[gidx].xlo, cx);
+        atomicMax(&sh.g[gidx].xhi, cx);
+    }
+    __syncthreads();
+    constexpr uint32_t kRows = DIMS == 3 ? 9u : 3u;
+    if (tid == 0) {
+        uint32_t e = 0;
+        for (uint32_t k = 0; k < groups; ++k) {
+            StageGroup& g = sh.g[k];
+            g.xbase = g.xlo - 1;
+            g.width = g.xhi - g.xlo + 3;
+            g.ebase = e;
+            e += kRows * (uint32_t)g.width;
+        }
+        sh.ngroups = groups;
+        sh.nentries = e;
+        sh.ok = e <= maxe ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!sh.ok) return false;
+    const uint32_t E = sh.nentries;
+    // bin table: entry e = (group, neighbour row, virtual x); all loads issued before use
+    uint32_t local = 0;
+    uint32_t cnt[3], gs[3];
+#pragma unroll
+    for (uint32_t q = 0; q < 3; ++q) {
+        const uint32_t e = tid + q * 256u;
+        cnt[q] = 0;
+        gs[q] = 0;
+        if (e < E) {
+            uint32_t k = 0;
+            while (k + 1 < groups && e >= sh.g[k + 1].ebase) ++k;
+            const StageGroup& g = sh.g[k];
+            const uint32_t rel = e - g.ebase;
+            const int j = (int)(rel / (uint32_t)g.width);
+            const int v = (int)(rel % (uint32_t)g.width);
+            const int dy = DIMS == 3 ? j / 3 - 1 : j - 1;
+            const int dz = DIMS == 3 ? j % 3 - 1 : 0;
+            const int ay = wrap_cell(g.cy, dy, gy);
+            const int az = DIMS == 3 ? wrap_cell(g.cz, dz, gz) : 0;
+            int x = g.xbase + v;
+            x = x < 0 ? x + gx : (x >= gx ? x - gx : x);
+            const uint32_t h = (uint32_t)az * m.grid[0] * m.grid[1] + (uint32_t)ay * m.grid[0] + (uint32_t)x;
+            gs[q] = m.pbm[h];
+            cnt[q] = m.pbm[h + 1];
+        }
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < 3; ++q) {
+        const uint32_t e = tid + q * 256u;
+        if (e < E) {
+            cnt[q] -= gs[q];
+            gstart[e] = gs[q];
+            lstart[e] = cnt[q];
+        }
+    }
+    __syncthreads();
+    // exclusive scan of the bin counts: thread t owns entries [3t, 3t+3)
+    uint32_t mine[3];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 3; ++q) {
+        const uint32_t e = tid * 3u + q;
+        mine[q] = e < E ? lstart[e] : 0u;
+        sum += mine[q];
+    }
+    const uint32_t inc = wave_inclusive(sum);
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, total = 0;
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k < w) off += s_w[k];
+        total += s_w[k];
+    }
+    uint32_t run = off + inc - sum;
+#pragma unroll
+    for (uint32_t q = 0; q < 3; ++q) {
+        const uint32_t e = tid * 3u + q;
+        if (e < E) lstart[e] = run;
+        run += mine[q];
+    }
+    if (tid == 0) lstart[E] = total;
+    if (total > a.stage_cap) return false;
+    __syncthreads();
+    // copy: staged slot l -> its bin (binary search over the bin starts) -> global index;
+    // consecutive slots map to consecutive messages, so the loads coalesce.
+    const uint32_t cap = a.stage_cap, nv = a.stage_nvars;
+    for (uint32_t l = tid; l < total; l += 256u) {
+        uint32_t lo = 0, hi = E;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (lstart[mid] <= l) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t g = gstart[lo] + (l - lstart[lo]);
+        uint32_t val[STAGE_MAX_VARS];
+#pragma unroll
+        for (uint32_t v = 0; v < STAGE_MAX_VARS; ++v) val[v] = v < nv ? a.stage_src[v][g] : 0u;
+#pragma unroll
+        for (uint32_t v = 0; v < STAGE_MAX_VARS; ++v)
+            if (v < nv) vars[v * cap + l] = val[v];
+    }
+    __syncthreads();
+    return true;
+}
+
+// Spatial-input agent functions: neighbourhood staged in LDS, then the user function.
 template <typename F>
-void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, void* stream) {
-    hipLaunchKernelGGL(agent_function_kernel<F>, dim3(grid), dim3(block), 0, (hipStream_t)stream, a);
+__global__ void __launch_bounds__(256) agent_function_kernel_staged(const FnArgs a) {
+    extern __shared__ uint32_t fgx_stage_smem[];
+    __shared__ StageHeader sh;
+    const uint32_t n = a.d_n ? load_count(a.d_n) : a.n_cap;
+    const uint32_t b0 = blockIdx.x * blockDim.x;
+    if (b0 >= n) return;
+    const uint32_t i = b0 + threadIdx.x;
+    const bool active = i < n;
+    bool staged = false;
+    if (a.stage_ok) staged = stage_block<F::MsgIn::In::dims>(a, fgx_stage_smem, sh, i, active);
+    if (!active) return;
+    if (a.msg_out_flags) a.msg_out_flags[i] = a.msg_out_optional ? 0u : 1u;
+    if (a.birth_flags) a.birth_flags[i] = 0u;
+    DeviceAPI<typename F::MsgIn, typename F::MsgOut> api(a, i);
+    if (staged) api.message_in.enable_stage(fgx_stage_smem, &sh);
+    const AGENT_STATUS s = F()(&api);
+    if (a.death_flags)
+        a.death_flags[i] = (uint32_t)s;
+    else if (s == DEAD)
+        report_error(a.d_error, DERR_DEATH_DISABLED, 0, i);
+}
+
+template <typename F>
+void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, uint32_t smem, void* stream) {
+    constexpr MsgKind k = F::MsgIn::kind;
+    if constexpr (k == MsgKind::Spatial2D || k == MsgKind::Spatial3D) {
+        hipLaunchKernelGGL(agent_function_kernel_staged<F>, dim3(grid), dim3(block), smem, (hipStream_t)stream, a);
+    } else {
+        (void)smem;
+        hipLaunchKernelGGL(agent_function_kernel<F>, dim3(grid), dim3(block), 0, (hipStream_t)stream, a);
+    }
 }
 }  // namespace detail
 

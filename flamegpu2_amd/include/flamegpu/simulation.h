@@ -523,6 +523,12 @@ class CUDASimulation {
 
     // ---- extensions used by the benchmark and the model C ABI
     void setKernelTiming(bool on) { timing_ = on; }
+    // Stage spatial message neighbourhoods in LDS before iterating them (default off: on
+    // MI355X the L2-resident HBM path is faster for the bundled models); off reads HBM directly.
+    void setNeighbourhoodStaging(bool on) {
+        stage_enabled_ = on;
+        clear_graphs();
+    }
     const std::map<std::string, detail::KernelTime>& getKernelTimes() {
         collect_timings();
         return ktimes_;
@@ -609,6 +615,10 @@ class CUDASimulation {
     void enqueue_step();
     void enqueue_layer(const LayerData& layer);
     void launch_function(detail::FnRT& f, void* st);
+    void stage_neighbourhood(detail::FnRT& f, detail::StateList& l, detail::FnArgs& a);
+    static uint32_t stage_smem_bytes(const detail::FnArgs& a) {
+        return a.stage_ok ? (2u * (a.stage_max_entries + 1u) + a.stage_nvars * a.stage_cap) * 4u : 0u;
+    }
     void unmap_function(detail::FnRT& f, void* st);
     void enqueue_end_of_step();
     void run_init_functions();
@@ -661,6 +671,7 @@ class CUDASimulation {
     void* fork_event_ = nullptr;
     // kernel timing
     bool timing_ = false;
+    bool stage_enabled_ = false;
     struct PendingTime {
         std::string label;
         void* start;
@@ -1153,7 +1164,7 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
         *mul = detail::perfect_hash_mul(hs.data(), (unsigned)hs.size(), detail::VAR_SLOT_BITS);
         for (size_t i = 0; i < vars.size(); ++i) {
             const uint32_t s = detail::table_slot(hs[i], *mul, detail::VAR_SLOT_BITS);
-            tab[s] = {hs[i], vars[i].bytes, (char*)(use_alt ? vars[i].alt : vars[i].cur)};
+            tab[s] = {hs[i], (uint16_t)vars[i].bytes, detail::NOT_STAGED, (char*)(use_alt ? vars[i].alt : vars[i].cur)};
         }
     };
     fill_table(a.agent, &a.agent_mul, l.vars, false);
@@ -1171,6 +1182,7 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
             a.in_meta.radius = m.radius;
             a.in_meta.wrap_ok = m.wrap_ok;
             a.in_meta.pbm = f.min->pbm;
+            stage_neighbourhood(f, l, a);
         }
     }
     if (f.mout) {
@@ -1211,9 +1223,50 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
     const uint32_t grid = std::max<uint32_t>(1, (l.cap + block - 1) / block);
     const std::string label = "fn:" + f.agent->d->name + "::" + f.d->name;
     tick(label, st, true);
-    f.d->launch(a, grid, block, st);
+    f.d->launch(a, grid, block, stage_smem_bytes(a), st);
     tick(label, st, false);
     if (hipGetLastError() != hipSuccess) throw exception::CUDAError("agent function launch failed: " + f.d->name);
+}
+
+
+// LDS staging plan for a spatial-input function: the agent's own x/y(/z) is the
+// assumed search origin, and up to four 4-byte message variables are staged
+// (x, y, z first, then "id", then the rest in name order).  The budget keeps
+// three 256-agent blocks resident per CU (52 KB of dynamic LDS each).
+inline void CUDASimulation::stage_neighbourhood(detail::FnRT& f, detail::StateList& l, detail::FnArgs& a) {
+    a.stage_ok = 0;
+    if (!stage_enabled_) return;
+    const int dims = f.min->meta.dims;
+    const char* axes[3] = {"x", "y", "z"};
+    for (int k = 0; k < dims; ++k) {
+        const int vi = l.var_index(axes[k]);
+        if (vi < 0) return;
+        const auto& var = f.agent->d->variables.at(axes[k]);
+        if (var.type != std::type_index(typeid(float)) || var.elements != 1) return;
+        a.stage_pos[k] = (const float*)l.vars[vi].cur;
+    }
+    std::vector<int> pick;
+    for (int k = 0; k < dims; ++k) pick.push_back(f.min->var_index(axes[k]));
+    auto eligible = [&](int vi) {
+        if (vi < 0 || std::find(pick.begin(), pick.end(), vi) != pick.end()) return false;
+        const auto& var = f.min->d->variables.at(f.min->vars[vi].name);
+        return var.elements == 1 && var.type_size == 4;
+    };
+    if (eligible(f.min->var_index("id"))) pick.push_back(f.min->var_index("id"));
+    for (int vi = 0; vi < (int)f.min->vars.size() && pick.size() < detail::STAGE_MAX_VARS; ++vi)
+        if (eligible(vi)) pick.push_back(vi);
+    const uint32_t budget_words = 13312;   // 52 KB
+    a.stage_max_entries = detail::STAGE_MAX_ENTRIES;
+    a.stage_nvars = (uint32_t)pick.size();
+    a.stage_cap = (budget_words - 2u * (a.stage_max_entries + 1u)) / a.stage_nvars;
+    for (size_t k = 0; k < pick.size(); ++k) {
+        const auto& v = f.min->vars[pick[k]];
+        a.stage_src[k] = (const uint32_t*)v.cur;
+        const uint32_t h = detail::name_hash(v.name);
+        detail::VarBinding& b = a.msg_in[detail::table_slot(h, a.msg_in_mul, detail::VAR_SLOT_BITS)];
+        b.lds = (uint16_t)k;
+    }
+    a.stage_ok = 1;
 }
 
 inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {

@@ -45,7 +45,8 @@ int fgxm_get_agents(fgxm_sim* sim, const char* agent, const char* state, void* c
 int fgxm_step(fgxm_sim* sim, uint32_t nsteps);
 /* options: "graphs" (capture steps as hipGraphs, default 1), "concurrency"
  * (in-layer streams, default 1), "kernel_timing" (HIP events around every
- * launch, eager steps only, default 0). */
+ * launch, eager steps only, default 0), "staging" (LDS staging of spatial
+ * message neighbourhoods, default 0). */
 int fgxm_set_option(fgxm_sim* sim, const char* key, int value);
 int fgxm_sync(fgxm_sim* sim);
 /* Bin start table of a spatial message list as last built. */

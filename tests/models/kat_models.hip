@@ -1,0 +1,417 @@
+// Known-answer test models, restated from the reference's own GoogleTest suite and
+// written against the drop-in C++ API.  One library, one model per `variant`:
+//   mandatory3d / optional3d / optionalnone3d / mandatory2d
+//       tests/test_cases/runtime/messaging/test_spatial_3d.cu:72-447, test_spatial_2d.cu:66-320
+//   wrapped3d / wrapped2d (params: x, y, z offset)
+//       test_spatial_3d.cu:855-975, test_spatial_2d.cu:884-955
+//   bounds3d            test_spatial_3d.cu:1100-1189
+//   sort3d (params: sort period)
+//       tests/test_cases/runtime/agent/detail/test_spatial_agent_sort.cu:23-115
+//   birth_mandatory / birth_optional / birth_mandatory_death / birth_optional_death
+//       tests/test_cases/runtime/agent/test_device_agent_creation.cu:23-300
+//   transition          tests/test_cases/runtime/agent/detail/test_agent_state_transition.cu:67-174
+//   bruteforce          tests/test_cases/runtime/messaging/test_brute_force.cu:73-200
+//   random              tests/test_cases/runtime/random/test_agent_random.cu:28-185
+//   death_disabled / wrap_out_of_bounds   device errors (test_spatial_3d.cu:976-1039)
+#include <string>
+
+#include "flamegpu/flamegpu.h"
+
+using flamegpu::ALIVE;
+using flamegpu::DEAD;
+using flamegpu::MessageBruteForce;
+using flamegpu::MessageNone;
+using flamegpu::MessageSpatial2D;
+using flamegpu::MessageSpatial3D;
+
+// ---------------------------------------------------------------- spatial 3-D
+FLAMEGPU_AGENT_FUNCTION(out_mandatory3D, MessageNone, MessageSpatial3D) {
+    FLAMEGPU->message_out.setVariable<flamegpu::id_t>("id", FLAMEGPU->getID());
+    FLAMEGPU->message_out.setLocation(FLAMEGPU->getVariable<float>("x"), FLAMEGPU->getVariable<float>("y"),
+                                      FLAMEGPU->getVariable<float>("z"));
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(out_optional3D, MessageNone, MessageSpatial3D) {
+    if (FLAMEGPU->getVariable<int>("do_output")) {
+        FLAMEGPU->message_out.setVariable<flamegpu::id_t>("id", FLAMEGPU->getID());
+        FLAMEGPU->message_out.setLocation(FLAMEGPU->getVariable<float>("x"), FLAMEGPU->getVariable<float>("y"),
+                                          FLAMEGPU->getVariable<float>("z"));
+    }
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(out_optional3DNone, MessageNone, MessageSpatial3D) { return ALIVE; }
+FLAMEGPU_AGENT_FUNCTION(in3D, MessageSpatial3D, MessageNone) {
+    const float x1 = FLAMEGPU->getVariable<float>("x");
+    const float y1 = FLAMEGPU->getVariable<float>("y");
+    const float z1 = FLAMEGPU->getVariable<float>("z");
+    unsigned int count = 0;
+    unsigned int badCount = 0;
+    int myBin[3] = {static_cast<int>(x1), static_cast<int>(y1), static_cast<int>(z1)};
+    for (const auto& message : FLAMEGPU->message_in(x1, y1, z1)) {
+        int messageBin[3] = {static_cast<int>(message.getVariable<float>("x")),
+                             static_cast<int>(message.getVariable<float>("y")),
+                             static_cast<int>(message.getVariable<float>("z"))};
+        bool isBad = false;
+        for (unsigned int i = 0; i < 3; ++i) {
+            int binDiff = myBin[i] - messageBin[i];
+            if (binDiff > 1 || binDiff < -1) isBad = true;
+        }
+        count++;
+        badCount = isBad ? badCount + 1 : badCount;
+    }
+    FLAMEGPU->setVariable<unsigned int>("count", count);
+    FLAMEGPU->setVariable<unsigned int>("badCount", badCount);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(inWrapped3D, MessageSpatial3D, MessageNone) {
+    const float x1 = FLAMEGPU->getVariable<float>("x");
+    const float y1 = FLAMEGPU->getVariable<float>("y");
+    const float z1 = FLAMEGPU->getVariable<float>("z");
+    const flamegpu::id_t ID = FLAMEGPU->getID();
+    unsigned int count = 0;
+    unsigned int badCount = 0;
+    float xSum = 0, ySum = 0, zSum = 0;
+    for (const auto& message : FLAMEGPU->message_in.wrap(x1, y1, z1)) {
+        const float x2 = message.getVirtualX(x1);
+        const float y2 = message.getVirtualY(y1);
+        const float z2 = message.getVirtualZ(z1);
+        float x21 = x2 - x1;
+        float y21 = y2 - y1;
+        float z21 = z2 - z1;
+        const float distance = sqrtf(x21 * x21 + y21 * y21 + z21 * z21);
+        if (distance > FLAMEGPU->message_in.radius() || (fabsf(x21) != 2.0f && x2 != x1) ||
+            (fabsf(y21) != 2.0f && y2 != y1) || (fabsf(z21) != 2.0f && z2 != z1)) {
+            badCount++;
+        } else {
+            count++;
+            if (message.getVariable<flamegpu::id_t>("id") != ID) {
+                xSum += x21;
+                ySum += y21;
+                zSum += z21;
+            }
+        }
+    }
+    FLAMEGPU->setVariable<unsigned int>("count", count);
+    FLAMEGPU->setVariable<unsigned int>("badCount", badCount);
+    FLAMEGPU->setVariable<float>("result_x", xSum);
+    FLAMEGPU->setVariable<float>("result_y", ySum);
+    FLAMEGPU->setVariable<float>("result_z", zSum);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(in_count3D, MessageSpatial3D, MessageNone) {
+    unsigned int count = 0;
+    for (const auto& message : FLAMEGPU->message_in(FLAMEGPU->getVariable<float>("x"), FLAMEGPU->getVariable<float>("y"),
+                                                    FLAMEGPU->getVariable<float>("z"))) {
+        (void)message;
+        ++count;
+    }
+    FLAMEGPU->setVariable<unsigned int>("count", count);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(in_wrap_any3D, MessageSpatial3D, MessageNone) {
+    unsigned int count = 0;
+    for (const auto& message : FLAMEGPU->message_in.wrap(FLAMEGPU->getVariable<float>("x"),
+                                                         FLAMEGPU->getVariable<float>("y"),
+                                                         FLAMEGPU->getVariable<float>("z"))) {
+        (void)message;
+        ++count;
+    }
+    FLAMEGPU->setVariable<unsigned int>("count", count);
+    return ALIVE;
+}
+
+// ---------------------------------------------------------------- spatial 2-D
+FLAMEGPU_AGENT_FUNCTION(out_mandatory2D, MessageNone, MessageSpatial2D) {
+    FLAMEGPU->message_out.setVariable<flamegpu::id_t>("id", FLAMEGPU->getID());
+    FLAMEGPU->message_out.setLocation(FLAMEGPU->getVariable<float>("x"), FLAMEGPU->getVariable<float>("y"));
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(in2D, MessageSpatial2D, MessageNone) {
+    const float x1 = FLAMEGPU->getVariable<float>("x");
+    const float y1 = FLAMEGPU->getVariable<float>("y");
+    unsigned int count = 0;
+    unsigned int badCount = 0;
+    int myBin[2] = {static_cast<int>(x1), static_cast<int>(y1)};
+    for (const auto& message : FLAMEGPU->message_in(x1, y1)) {
+        int messageBin[2] = {static_cast<int>(message.getVariable<float>("x")),
+                             static_cast<int>(message.getVariable<float>("y"))};
+        bool isBad = false;
+        for (unsigned int i = 0; i < 2; ++i) {
+            int binDiff = myBin[i] - messageBin[i];
+            if (binDiff > 1 || binDiff < -1) isBad = true;
+        }
+        count++;
+        badCount = isBad ? badCount + 1 : badCount;
+    }
+    FLAMEGPU->setVariable<unsigned int>("count", count);
+    FLAMEGPU->setVariable<unsigned int>("badCount", badCount);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(inWrapped2D, MessageSpatial2D, MessageNone) {
+    const float x1 = FLAMEGPU->getVariable<float>("x");
+    const float y1 = FLAMEGPU->getVariable<float>("y");
+    const flamegpu::id_t ID = FLAMEGPU->getID();
+    unsigned int count = 0;
+    unsigned int badCount = 0;
+    float xSum = 0, ySum = 0;
+    for (const auto& message : FLAMEGPU->message_in.wrap(x1, y1)) {
+        const float x2 = message.getVirtualX(x1);
+        const float y2 = message.getVirtualY(y1);
+        float x21 = x2 - x1;
+        float y21 = y2 - y1;
+        const float distance = sqrtf(x21 * x21 + y21 * y21);
+        if (distance > FLAMEGPU->message_in.radius() || (fabsf(x21) != 2.0f && x2 != x1) ||
+            (fabsf(y21) != 2.0f && y2 != y1)) {
+            badCount++;
+        } else {
+            count++;
+            if (message.getVariable<flamegpu::id_t>("id") != ID) {
+                xSum += x21;
+                ySum += y21;
+            }
+        }
+    }
+    FLAMEGPU->setVariable<unsigned int>("count", count);
+    FLAMEGPU->setVariable<unsigned int>("badCount", badCount);
+    FLAMEGPU->setVariable<float>("result_x", xSum);
+    FLAMEGPU->setVariable<float>("result_y", ySum);
+    return ALIVE;
+}
+
+// ---------------------------------------------------------------- sort, births, transitions, brute force
+FLAMEGPU_AGENT_FUNCTION(dummySpatialFunc_3D, MessageSpatial3D, MessageNone) { return ALIVE; }
+
+FLAMEGPU_AGENT_FUNCTION(MandatoryOutput, MessageNone, MessageNone) {
+    unsigned int id = FLAMEGPU->getVariable<unsigned int>("id") + 1;
+    FLAMEGPU->agent_out.setVariable<float>("x", id + 12.0f);
+    FLAMEGPU->agent_out.setVariable<unsigned int>("id", id);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(OptionalOutput, MessageNone, MessageNone) {
+    unsigned int id = FLAMEGPU->getVariable<unsigned int>("id") + 1;
+    if (threadIdx.x % 2 == 0) {
+        FLAMEGPU->agent_out.setVariable<float>("x", id + 12.0f);
+        FLAMEGPU->agent_out.setVariable<unsigned int>("id", id);
+    }
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(MandatoryOutputWithDeath, MessageNone, MessageNone) {
+    unsigned int id = FLAMEGPU->getVariable<unsigned int>("id") + 1;
+    FLAMEGPU->agent_out.setVariable<float>("x", id + 12.0f);
+    FLAMEGPU->agent_out.setVariable<unsigned int>("id", id);
+    return DEAD;
+}
+FLAMEGPU_AGENT_FUNCTION(OptionalOutputWithDeath, MessageNone, MessageNone) {
+    unsigned int id = FLAMEGPU->getVariable<unsigned int>("id") + 1;
+    if (threadIdx.x % 2 == 0) {
+        FLAMEGPU->agent_out.setVariable<float>("x", id + 12.0f);
+        FLAMEGPU->agent_out.setVariable<unsigned int>("id", id);
+    } else {
+        return DEAD;
+    }
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(AddOne, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<unsigned int>("x", FLAMEGPU->getVariable<unsigned int>("x") + 1);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(out_bf, MessageNone, MessageBruteForce) {
+    FLAMEGPU->message_out.setVariable<int>("x", FLAMEGPU->getVariable<int>("x"));
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(in_bf, MessageBruteForce, MessageNone) {
+    int sum = 0;
+    int product = 1;
+    for (const auto& m : FLAMEGPU->message_in) {
+        const int x = m.getVariable<int>("x");
+        sum += x;
+        product *= x;
+        product = product > 1000000 ? 1 : product;
+    }
+    FLAMEGPU->setVariable<int>("sum", sum);
+    FLAMEGPU->setVariable<int>("product", product);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(draw_random, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<float>("a", FLAMEGPU->random.uniform<float>());
+    FLAMEGPU->setVariable<float>("b", FLAMEGPU->random.uniform<float>());
+    FLAMEGPU->setVariable<int>("c", FLAMEGPU->random.uniform<int>(1, 6));
+    FLAMEGPU->setVariable<float>("d", FLAMEGPU->random.normal<float>());
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(die_without_permission, MessageNone, MessageNone) { return DEAD; }
+
+flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* p, uint32_t np) {
+    const std::string v = variant_c ? variant_c : "";
+    flamegpu::ModelDescription model("kat_" + v);
+    auto param = [&](uint32_t i, float d) { return i < np ? p[i] : d; };
+    if (v == "mandatory3d" || v == "optional3d" || v == "optionalnone3d") {
+        auto message = model.newMessage<MessageSpatial3D>("location");
+        message.setMin(0, 0, 0);
+        message.setMax(5, 5, 5);
+        message.setRadius(1);
+        message.newVariable<flamegpu::id_t>("id");
+        auto agent = model.newAgent("agent");
+        agent.newVariable<float>("x");
+        agent.newVariable<float>("y");
+        agent.newVariable<float>("z");
+        agent.newVariable<int>("do_output");
+        agent.newVariable<unsigned int>("count");
+        agent.newVariable<unsigned int>("badCount");
+        flamegpu::AgentFunctionDescription out = v == "mandatory3d" ? agent.newFunction("out", out_mandatory3D)
+                                               : v == "optional3d" ? agent.newFunction("out", out_optional3D)
+                                                                   : agent.newFunction("out", out_optional3DNone);
+        out.setMessageOutput("location");
+        out.setMessageOutputOptional(v != "mandatory3d");
+        auto in = agent.newFunction("in", in3D);
+        in.setMessageInput("location");
+        model.newLayer().addAgentFunction(out);
+        model.newLayer().addAgentFunction(in);
+    } else if (v == "mandatory2d") {
+        auto message = model.newMessage<MessageSpatial2D>("location");
+        message.setMin(0, 0);
+        message.setMax(5, 5);
+        message.setRadius(1);
+        message.newVariable<flamegpu::id_t>("id");
+        auto agent = model.newAgent("agent");
+        agent.newVariable<float>("x");
+        agent.newVariable<float>("y");
+        agent.newVariable<unsigned int>("count");
+        agent.newVariable<unsigned int>("badCount");
+        auto out = agent.newFunction("out", out_mandatory2D);
+        out.setMessageOutput("location");
+        auto in = agent.newFunction("in", in2D);
+        in.setMessageInput("location");
+        model.newLayer().addAgentFunction(out);
+        model.newLayer().addAgentFunction(in);
+    } else if (v == "wrapped3d") {
+        const float ox = param(0, 0), oy = param(1, 0), oz = param(2, 0);
+        auto message = model.newMessage<MessageSpatial3D>("location");
+        message.setMin(0 + ox, 0 + oy, 0 + oz);
+        message.setMax(70 + ox, 70 + oy, 70 + oz);
+        message.setRadius(3.5);
+        message.newVariable<flamegpu::id_t>("id");
+        auto agent = model.newAgent("agent");
+        agent.newVariable<float>("x");
+        agent.newVariable<float>("y");
+        agent.newVariable<float>("z");
+        agent.newVariable<float>("result_x");
+        agent.newVariable<float>("result_y");
+        agent.newVariable<float>("result_z");
+        agent.newVariable<unsigned int>("count");
+        agent.newVariable<unsigned int>("badCount");
+        auto out = agent.newFunction("out", out_mandatory3D);
+        out.setMessageOutput("location");
+        auto in = agent.newFunction("in", inWrapped3D);
+        in.setMessageInput("location");
+        model.newLayer().addAgentFunction(out);
+        model.newLayer().addAgentFunction(in);
+    } else if (v == "wrapped2d") {
+        const float ox = param(0, 0), oy = param(1, 0);
+        auto message = model.newMessage<MessageSpatial2D>("location");
+        message.setMin(0 + ox, 0 + oy);
+        message.setMax(70 + ox, 70 + oy);
+        message.setRadius(3.5);
+        message.newVariable<flamegpu::id_t>("id");
+        auto agent = model.newAgent("agent");
+        agent.newVariable<float>("x");
+        agent.newVariable<float>("y");
+        agent.newVariable<float>("result_x");
+        agent.newVariable<float>("result_y");
+        agent.newVariable<unsigned int>("count");
+        agent.newVariable<unsigned int>("badCount");
+        auto out = agent.newFunction("out", out_mandatory2D);
+        out.setMessageOutput("location");
+        auto in = agent.newFunction("in", inWrapped2D);
+        in.setMessageInput("location");
+        model.newLayer().addAgentFunction(out);
+        model.newLayer().addAgentFunction(in);
+    } else if (v == "bounds3d" || v == "wrap_out_of_bounds") {
+        auto message = model.newMessage<MessageSpatial3D>("location");
+        if (v == "bounds3d") {
+            message.setMin(0, 0, 0);
+            message.setMax(50.1f, 50.1f, 50.1f);
+            message.setRadius(10);
+        } else {
+            message.setMin(0, 0, 0);
+            message.setMax(10, 10, 10);
+            message.setRadius(1);
+        }
+        message.newVariable<flamegpu::id_t>("id");
+        auto agent = model.newAgent("agent");
+        agent.newVariable<float>("x");
+        agent.newVariable<float>("y");
+        agent.newVariable<float>("z");
+        agent.newVariable<unsigned int>("count", 0);
+        auto out = agent.newFunction("out", out_mandatory3D);
+        out.setMessageOutput(message);
+        auto in = v == "bounds3d" ? agent.newFunction("in", in_count3D) : agent.newFunction("in", in_wrap_any3D);
+        in.setMessageInput(message);
+        model.newLayer().addAgentFunction(out);
+        model.newLayer().addAgentFunction(in);
+    } else if (v == "sort3d") {
+        auto agent = model.newAgent("agent");
+        agent.newVariable<int>("initial_order");
+        agent.newVariable<float>("x");
+        agent.newVariable<float>("y");
+        agent.newVariable<float>("z");
+        agent.setSortPeriod((unsigned int)param(0, 1));
+        auto loc = model.newMessage<MessageSpatial3D>("location");
+        loc.setMin(-5, -5, -5);
+        loc.setMax(5, 5, 5);
+        loc.setRadius(0.2f);
+        auto f = agent.newFunction("dummySpatialFunc", dummySpatialFunc_3D);
+        f.setMessageInput("location");
+        model.newLayer().addAgentFunction(f);
+    } else if (v.rfind("birth_", 0) == 0) {
+        auto agent = model.newAgent("agent");
+        agent.newVariable<float>("x");
+        agent.newVariable<unsigned int>("id");
+        flamegpu::AgentFunctionDescription f =
+            v == "birth_mandatory"         ? agent.newFunction("output", MandatoryOutput)
+            : v == "birth_optional"        ? agent.newFunction("output", OptionalOutput)
+            : v == "birth_mandatory_death" ? agent.newFunction("output", MandatoryOutputWithDeath)
+                                           : agent.newFunction("output", OptionalOutputWithDeath);
+        f.setAgentOutput(agent);
+        f.setAllowAgentDeath(v == "birth_mandatory_death" || v == "birth_optional_death");
+        model.newLayer().addAgentFunction(f);
+    } else if (v == "transition") {
+        auto agent = model.newAgent("agent");
+        agent.newState("a");
+        agent.newState("b");
+        agent.newVariable<unsigned int>("x");
+        auto f = agent.newFunction("add_one", AddOne);
+        f.setInitialState("a");
+        f.setEndState("b");
+        model.newLayer().addAgentFunction(f);
+    } else if (v == "bruteforce") {
+        auto message = model.newMessage("msg");
+        message.newVariable<int>("x");
+        auto agent = model.newAgent("agent");
+        agent.newVariable<int>("x");
+        agent.newVariable<int>("sum");
+        agent.newVariable<int>("product");
+        auto out = agent.newFunction("out", out_bf);
+        out.setMessageOutput("msg");
+        auto in = agent.newFunction("in", in_bf);
+        in.setMessageInput("msg");
+        model.newLayer().addAgentFunction(out);
+        model.newLayer().addAgentFunction(in);
+    } else if (v == "random") {
+        auto agent = model.newAgent("agent");
+        agent.newVariable<float>("a");
+        agent.newVariable<float>("b");
+        agent.newVariable<int>("c");
+        agent.newVariable<float>("d");
+        model.newLayer().addAgentFunction(agent.newFunction("draw", draw_random));
+    } else if (v == "death_disabled") {
+        auto agent = model.newAgent("agent");
+        agent.newVariable<float>("x");
+        model.newLayer().addAgentFunction(agent.newFunction("die", die_without_permission));
+    } else {
+        throw flamegpu::exception::InvalidArgument("unknown test model variant '" + v + "'");
+    }
+    return model;
+}
+
+#include "flamegpu/model_abi.h"

@@ -1,0 +1,202 @@
+"""The reference's own known-answer tests, restated through the drop-in C++ API
+(tests/models/kat_models.hip) and run on the GPU.  Expected values are computed
+on the host exactly as the reference tests compute them."""
+import numpy as np
+import pytest
+
+from flamegpu2_amd.fgx import FgxError
+from flamegpu2_amd.models import Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(variant, params=(), seed=1):
+    return Simulation("kat_models", params, seed=seed, variant=variant)
+
+
+def _expected_counts(P, dims, grid, mask=None):
+    """test_spatial_3d.cu:127-171: per bin, messages in the clamped Moore neighbourhood."""
+    cells = np.floor(P).astype(np.int64).clip(0, grid - 1)
+    occ = np.zeros((grid,) * dims, dtype=np.int64)
+    src = cells if mask is None else cells[:, mask]
+    np.add.at(occ, tuple(src), 1)
+    pad = np.pad(occ, 1)
+    nb = np.zeros_like(occ)
+    for off in np.ndindex(*([3] * dims)):
+        sl = tuple(slice(o, o + grid) for o in off)
+        nb += pad[sl]
+    return nb[tuple(cells)]
+
+
+@pytest.mark.parametrize("variant", ["mandatory3d", "optional3d", "optionalnone3d"])
+def test_spatial3d_message_counts(variant):
+    rng = np.random.default_rng(11)
+    n = 2049
+    P = rng.uniform(0, 5, size=(3, n)).astype(np.float32)
+    do_output = (rng.uniform(size=n) < 0.8).astype(np.int32)
+    s = _sim(variant)
+    z = np.zeros(n, np.uint32)
+    s.set_agents("agent", [P[0], P[1], P[2], do_output, z, z])
+    s.step(1)
+    (x, y, zz, dout, count, bad), ids = s.get_agents("agent")
+    Q = np.stack([x, y, zz])
+    if variant == "mandatory3d":
+        expect = _expected_counts(Q, 3, 5)
+    elif variant == "optional3d":
+        expect = _expected_counts(Q, 3, 5, mask=dout == 1)
+    else:
+        expect = np.zeros(n, np.int64)
+    assert np.array_equal(count.astype(np.int64), expect)
+    assert np.all(bad == 0)
+
+
+def test_spatial2d_message_counts():
+    rng = np.random.default_rng(12)
+    n = 2049
+    P = rng.uniform(0, 5, size=(2, n)).astype(np.float32)
+    s = _sim("mandatory2d")
+    z = np.zeros(n, np.uint32)
+    s.set_agents("agent", [P[0], P[1], z, z])
+    s.step(1)
+    (x, y, count, bad), _ = s.get_agents("agent")
+    assert np.array_equal(count.astype(np.int64), _expected_counts(np.stack([x, y]), 2, 5))
+    assert np.all(bad == 0)
+
+
+@pytest.mark.parametrize("offset", [(0.0, 0.0, 0.0), (141.0, -540.0, 200.0), (-1401.5, 5640.3, -2008.8)])
+def test_spatial3d_wrapped(offset):
+    off = np.array(offset, np.float32)
+    i, j, k = np.meshgrid(np.arange(35), np.arange(35), np.arange(35), indexing="ij")
+    P = [(i.ravel() * np.float32(2.0)).astype(np.float32) + off[0],
+         (j.ravel() * np.float32(2.0)).astype(np.float32) + off[1],
+         (k.ravel() * np.float32(2.0)).astype(np.float32) + off[2]]
+    n = len(P[0])
+    s = _sim("wrapped3d", list(offset))
+    zf, zu = np.zeros(n, np.float32), np.zeros(n, np.uint32)
+    s.set_agents("agent", P + [zf, zf, zf, zu, zu])
+    s.step(1)
+    (x, y, z, rx, ry, rz, count, bad), _ = s.get_agents("agent")
+    assert np.all(rx == 0) and np.all(ry == 0) and np.all(rz == 0)
+    assert np.all(bad <= 189)
+    assert np.all(count == 27)
+
+
+@pytest.mark.parametrize("offset", [(0.0, 0.0), (141.0, -540.0), (-1401.5, 5640.3)])
+def test_spatial2d_wrapped(offset):
+    off = np.array(offset, np.float32)
+    i, j = np.meshgrid(np.arange(35), np.arange(35), indexing="ij")
+    P = [(i.ravel() * np.float32(2.0)).astype(np.float32) + off[0],
+         (j.ravel() * np.float32(2.0)).astype(np.float32) + off[1]]
+    n = len(P[0])
+    s = _sim("wrapped2d", list(offset))
+    zf, zu = np.zeros(n, np.float32), np.zeros(n, np.uint32)
+    s.set_agents("agent", P + [zf, zf, zu, zu])
+    s.step(1)
+    (x, y, rx, ry, count, bad), _ = s.get_agents("agent")
+    assert np.all(rx == 0) and np.all(ry == 0)
+    assert np.all(count == 9)
+
+
+def test_spatial3d_bounds_not_factor_radius():
+    h = np.float32(50.1) / np.float32(2)
+    pts = np.array([[h, 0.0, h], [h, 18.0, h], [0.0, h, 0.5], [18.0, h, 0.5],
+                    [h, 50.0, 50.1], [h, 50.0, np.float32(50.1) - np.float32(10.11)]], dtype=np.float32).T
+    s = _sim("bounds3d")
+    s.set_agents("agent", [pts[0], pts[1], pts[2], np.zeros(6, np.uint32)])
+    s.step(1)
+    (_, _, _, count), ids = s.get_agents("agent")
+    for c, i in zip(count, ids):
+        assert c == (2 if i < 5 else 1)
+
+
+@pytest.mark.parametrize("period,expect", [(1, [3, 2, 1, 0]), (0, [0, 1, 2, 3])])
+def test_auto_spatial_sort(period, expect):
+    x = -np.arange(4, dtype=np.float32)
+    s = _sim("sort3d", [float(period)])
+    s.set_agents("agent", [np.arange(4, dtype=np.int32), x, x, x])
+    s.step(1)
+    (order, _, _, _), _ = s.get_agents("agent")
+    assert order.tolist() == expect
+
+
+@pytest.mark.parametrize("variant,n_total,n1,n12", [
+    ("birth_mandatory", 2048, 1024, 1024),
+    ("birth_optional", 1536, 1024, 512),
+    ("birth_mandatory_death", 1024, 0, 1024),
+    ("birth_optional_death", 1024, 512, 512),
+])
+def test_device_agent_creation(variant, n_total, n1, n12):
+    n = 1024
+    s = _sim(variant)
+    s.set_agents("agent", [np.arange(n, dtype=np.float32) + 1.0, np.arange(n, dtype=np.uint32)])
+    s.step(1)
+    (x, idv), ids = s.get_agents("agent")
+    assert len(x) == n_total
+    val = x - idv.astype(np.float32)
+    assert int((val == 1.0).sum()) == n1
+    assert int((val == 12.0).sum()) == n12
+    # _id: unique; births numbered after the parents, in parent order
+    assert len(set(ids.tolist())) == len(ids)
+    born = ids[val == 12.0]
+    assert np.array_equal(born, np.sort(born)) and born.min() == n + 1
+
+
+def test_state_transition():
+    n = 1000
+    s = _sim("transition")
+    s.set_agents("agent", [np.arange(n, dtype=np.uint32)], state="a")
+    s.step(1)
+    assert s.count("agent", "a") == 0 and s.count("agent", "b") == n
+    (x,), _ = s.get_agents("agent", "b")
+    assert np.array_equal(x, np.arange(n, dtype=np.uint32) + 1)
+
+
+def test_bruteforce_sum():
+    n = 2000
+    rng = np.random.default_rng(4)
+    x = rng.integers(1, 10, size=n).astype(np.int32)
+    s = _sim("bruteforce")
+    z = np.zeros(n, np.int32)
+    s.set_agents("agent", [x, z, z])
+    s.step(1)
+    (_, ssum, _), _ = s.get_agents("agent")
+    assert np.all(ssum == int(x.sum()))
+
+
+def test_agent_random_distinct_and_in_range():
+    n = 4096
+    s = _sim("random", seed=1234)
+    z = np.zeros(n, np.float32)
+    s.set_agents("agent", [z, z, np.zeros(n, np.int32), z])
+    s.step(1)
+    (a, b, c, d), ids = s.get_agents("agent")
+    assert np.all((a >= 0) & (a < 1)) and np.all((b >= 0) & (b < 1))
+    assert len(np.unique(a)) > 0.99 * n and not np.array_equal(a, b)
+    assert np.all((c >= 1) & (c <= 6))
+    assert abs(float(d.mean())) < 0.1 and abs(float(d.std()) - 1.0) < 0.1
+    # counter-based: same seed -> same draws; different seed -> different draws
+    s2 = _sim("random", seed=1234)
+    s2.set_agents("agent", [z, z, np.zeros(n, np.int32), z])
+    s2.step(1)
+    (a2, _, _, _), _ = s2.get_agents("agent")
+    assert np.array_equal(a, a2)
+    s3 = _sim("random", seed=99)
+    s3.set_agents("agent", [z, z, np.zeros(n, np.int32), z])
+    s3.step(1)
+    (a3, _, _, _), _ = s3.get_agents("agent")
+    assert not np.array_equal(a, a3)
+
+
+def test_device_error_death_not_enabled():
+    s = _sim("death_disabled")
+    s.set_agents("agent", [np.zeros(10, np.float32)])
+    with pytest.raises(FgxError, match="DeviceError"):
+        s.step(1)
+
+
+def test_device_error_wrap_out_of_bounds():
+    s = _sim("wrap_out_of_bounds")
+    x = np.array([5.0, 11.0], np.float32)
+    s.set_agents("agent", [x, x, x, np.zeros(2, np.uint32)])
+    with pytest.raises(FgxError, match="DeviceError"):
+        s.step(1)

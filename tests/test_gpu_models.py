@@ -15,9 +15,10 @@ from flamegpu2_amd.models import Simulation
 pytestmark = pytest.mark.gpu
 
 
-def _run_pair(model, oracle_model, params, pops, agent_names, steps, seed, graphs=True):
+def _run_pair(model, oracle_model, params, pops, agent_names, steps, seed, graphs=True, staging=0):
     sim = Simulation(model, params, seed=seed)
     sim.set_option("graphs", graphs)
+    sim.set_option("staging", staging)
     ref = O.Sim(oracle_model, params, seed)
     for t, (name, arrs) in enumerate(zip(agent_names, pops)):
         sim.set_agents(name, arrs)
@@ -68,17 +69,20 @@ def test_circles_long_run_statistics():
     assert abs(float(got[3].mean()) - float(exp[3].mean())) <= 0.01 * max(float(exp[3].mean()), 1e-6) + 1e-6
 
 
-@pytest.mark.parametrize("n,steps,graphs", [(32768, 5, True), (32768, 5, False), (1 << 20, 1, True)])
-def test_boids_bit_exact(n, steps, graphs):
+@pytest.mark.parametrize("n,steps,graphs,staging", [(32768, 5, True, 0), (32768, 5, False, 0), (1 << 20, 1, True, 0),
+                                                    (32768, 5, True, 1), (1 << 20, 1, True, 1)])
+def test_boids_bit_exact(n, steps, graphs, staging):
     params, arrs = workloads.boids(n, seed=7)
-    sim, ref = _run_pair("boids_spatial3D", 1, params, [arrs], ["Boid"], steps, 7, graphs=graphs)
+    sim, ref = _run_pair("boids_spatial3D", 1, params, [arrs], ["Boid"], steps, 7, graphs=graphs, staging=staging)
     assert np.array_equal(sim.pbm("location"), ref.pbm(0))
     _compare(sim, ref, 0, "Boid", exact=True)
 
 
-def test_predprey_small_bit_exact_traces():
+@pytest.mark.parametrize("staging", [0, 1])
+def test_predprey_small_bit_exact_traces(staging):
     params, prey, pred = workloads.predprey(40000, 10000, seed=3, width=200.0)
     sim = Simulation("predator_prey", params, seed=3)
+    sim.set_option("staging", staging)
     ref = O.Sim(2, params, 3)
     sim.set_agents("Prey", prey)
     sim.set_agents("Predator", pred)
