@@ -124,15 +124,15 @@ constexpr unsigned VAR_SLOT_BITS = 5;
 constexpr unsigned ENV_SLOTS = 64;
 constexpr unsigned ENV_SLOT_BITS = 6;
 
-struct VarBinding {
+struct alignas(16) VarBinding {
     uint32_t hash;
     uint16_t bytes;   // element bytes (type size x array length)
-    uint16_t lds;     // input messages: staged array index in LDS, NOT_STAGED otherwise
+    uint16_t reserved;
     char* ptr;
 };
-constexpr uint16_t NOT_STAGED = 0xFFFF;
 
 // LDS staging of spatial message neighbourhoods (see agent_function_kernel_staged).
+// Staged variables, in order: x, y (, z), then "id" when the message has a 4-byte id.
 constexpr unsigned STAGE_MAX_GROUPS = 4;
 constexpr unsigned STAGE_MAX_VARS = 4;
 constexpr unsigned STAGE_MAX_ENTRIES = 768;   // staged virtual bins per block (3 per thread)
@@ -151,7 +151,7 @@ struct StageHeader {
     StageGroup g[STAGE_MAX_GROUPS];
 };
 
-struct EnvBinding {
+struct alignas(8) EnvBinding {
     uint32_t hash;
     uint16_t offset;
     uint16_t bytes;
@@ -211,6 +211,8 @@ inline const char* device_error_text(uint32_t code) {
 
 struct FnArgs {
     uint32_t n_cap;              // launch bound (state list capacity)
+    const char* zero_page;       // 64 zero bytes: reads through an unresolved binding land here
+    char* sink_page;             // 64 bytes: writes through an unresolved binding land here
     uint32_t stream_id;          // RNG stream of this agent function
     const uint32_t* d_n;         // device-resident agent count
     const uint32_t* d_step;      // device-resident step counter
@@ -226,7 +228,7 @@ struct FnArgs {
     uint32_t agent_mul, msg_in_mul, msg_out_mul, new_mul, env_mul;
     // neighbourhood staging (spatial message input)
     int32_t stage_ok;                 // agent x/y(/z) present and staged vars chosen
-    uint32_t stage_nvars;             // staged 4-byte message variables
+    uint32_t stage_nvars;             // staged 4-byte message variables (dims, + 1 with "id")
     uint32_t stage_max_entries;       // virtual bins per block
     uint32_t stage_cap;               // staged messages per block
     const float* stage_pos[3];        // agent x, y, z (the assumed search origin)
@@ -237,6 +239,37 @@ struct FnArgs {
     VarBinding msg_out[VAR_SLOTS];
     VarBinding new_agent[VAR_SLOTS];
     EnvBinding env_b[ENV_SLOTS];
+};
+
+// One simulation of a batch in a segmented launch.  The batch's agent and message
+// lists are the concatenation of its simulations' lists; a simulation owns the
+// agents [agent_off, agent_off + n) and, for its message input, the bins of
+// meta.pbm (which points at the simulation's part of the batch's bin table and
+// holds message indices into the concatenated message list).
+struct SegDev {
+    uint32_t agent_off;
+    uint32_t n;
+    uint32_t tile_off;           // first block of the simulation in this launch
+    uint32_t pad0;
+    uint64_t seed;
+    const uint32_t* d_step;      // the simulation's step counter
+    const char* env;             // the simulation's environment properties
+    uint32_t* d_error;           // the simulation's error words [code, hash, index]
+    SpatialDev meta;
+};
+
+__device__ __forceinline__ void report_error(uint32_t* d_error, uint32_t code, uint32_t detail, uint32_t index);
+
+// Device error reporting for one agent function call.  Accessors test their
+// (wave-uniform) binding checks with a scalar branch around the report, whose
+// global atomics keep the compiler from turning it into selects: the checks cost
+// a compare and a branch, and no per-lane state is carried through message loops.
+struct ErrState {
+    uint32_t* d_error;
+    uint32_t index;
+    __device__ __forceinline__ void note(bool bad, uint32_t c, uint32_t h) const {
+        if (bad) report_error(d_error, c, h, index);
+    }
 };
 
 struct Found {

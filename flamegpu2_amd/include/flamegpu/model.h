@@ -53,6 +53,8 @@ typedef CONDITION_RESULT (*FLAMEGPU_EXIT_CONDITION_POINTER)(HostAPI*);
 
 namespace detail {
 typedef void (*LaunchFn)(const FnArgs& args, uint32_t grid, uint32_t block, uint32_t smem, void* stream);
+typedef void (*LaunchSegFn)(const FnArgs& args, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t grid,
+                            uint32_t block, void* stream);
 template <typename T>
 Variable make_variable(const std::string& name, size_type elements, const T* defaults) {
     static_assert(std::is_arithmetic<T>::value, "agent/message variables must be arithmetic types");
@@ -83,6 +85,7 @@ struct AgentFunctionData {
     std::string name;
     AgentData* parent = nullptr;
     detail::LaunchFn launch = nullptr;
+    detail::LaunchSegFn launch_seg = nullptr;   // batched (segmented) launch
     MsgKind in_kind = MsgKind::None, out_kind = MsgKind::None;
     std::weak_ptr<MessageData> message_input, message_output;
     bool message_output_optional = false;
@@ -335,6 +338,9 @@ typedef AgentFunctionDescription CAgentFunctionDescription;
 namespace detail {
 template <typename F>
 void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, uint32_t smem, void* stream);
+template <typename F>
+void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t grid,
+                               uint32_t block, void* stream);
 }
 
 // ------------------------------------------------------------------ agents
@@ -379,6 +385,7 @@ class AgentDescription {
         f->name = name;
         f->parent = d_.get();
         f->launch = &detail::launch_agent_function<F>;
+        f->launch_seg = &detail::launch_agent_function_seg<F>;
         f->in_kind = F::MsgIn::kind;
         f->out_kind = F::MsgOut::kind;
         f->initial_state = d_->initial_state;

@@ -1,15 +1,22 @@
 // Device runtime: DeviceAPI, message input iterators / output writers, AgentRandom,
-// DeviceEnvironment and the agent-function kernel.
+// DeviceEnvironment and the agent-function kernels.
 //
 // Same surface as include/flamegpu/runtime/DeviceAPI.cuh:163-557,
 // include/flamegpu/runtime/messaging/*/Message*Device.cuh and
 // include/flamegpu/runtime/random/AgentRandom.cuh; the variable tables are
 // compile-time hashed names resolved against kernel-argument bindings
 // (flamegpu/detail/core.h) instead of the curve shared-memory hash table.
+//
+// A binding is resolved to (base pointer, index mask); an unknown name or a wrong
+// type reports a device error and selects the zero page (reads) or the sink page
+// (writes) with a zero mask, so execution continues without a data-dependent
+// branch.  The checks depend only on kernel arguments and compile-time name
+// hashes: inside message loops they are loop-invariant scalar work.
 #pragma once
 
 #include <cmath>
 #include <limits>
+#include <type_traits>
 
 #include "flamegpu/model.h"
 
@@ -17,46 +24,100 @@ namespace flamegpu {
 
 namespace detail {
 
+// Per-block LDS copy of the spatial neighbourhood (agent_function_kernel_staged).
+struct StageView {
+    const uint32_t* lstart;   // per virtual bin entry: first staged slot (exclusive scan)
+    const uint32_t* gstart;   // per virtual bin entry: first global message index
+    const uint32_t* vars;     // staged variables, SoA of `cap` slots: x, y (, z) (, id)
+    uint32_t cap;
+    uint32_t has_id;
+    const StageHeader* sh;
+};
+
+// Everything one agent-function call needs besides its DeviceAPI: the launch
+// arguments, the agent's list index, the values that differ between the
+// simulations of a batched launch, and the call's error state.
+struct ThreadCtx {
+    const FnArgs& a;
+    ErrState err;               // where this call reports device errors
+    const char* zero_page;      // copies of a.zero_page / a.sink_page held in registers
+    char* sink_page;
+    uint32_t idx;
+    const SpatialDev& meta;     // message-input spatial metadata
+    const char* env;            // environment property buffer
+    const uint32_t* d_step;     // step counter
+    uint64_t seed;
+    const StageView* stage;     // null outside the staged kernel
+};
+
+// Global-memory pointers (address space 1), so loads through pointers rebuilt from
+// integers stay global loads instead of flat loads.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+struct Bound {
+    uint64_t addr;   // base address: the bound array, or the zero/sink page
+    uint32_t mask;   // all ones when resolved, zero when addr is the zero/sink page
+    template <typename T>
+    __device__ __forceinline__ gptr<T> as() const {
+        return (gptr<T>)addr;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ Bound resolve(const VarBinding (&tab)[VAR_SLOTS], uint32_t mul, uint32_t h, uint32_t len,
+                                         const ThreadCtx& c, uint32_t err_unknown, uint32_t err_type, bool write) {
+    // one 16-byte load of the whole binding, then selects on the values (a select
+    // of addresses would chain two dependent scalar loads)
+    const uint4 raw = *reinterpret_cast<const uint4*>(&tab[table_slot(h, mul, VAR_SLOT_BITS)]);
+    const bool known = raw.x == h;
+    const bool typed = (raw.y & 0xFFFFu) == (uint32_t)sizeof(T) * len;
+    c.err.note(!known, err_unknown, h);
+    c.err.note(known && !typed, err_type, h);
+    const bool ok = known && typed;
+    const uint64_t bound = ((uint64_t)raw.w << 32) | raw.z;
+    const uint64_t fallback = write ? (uint64_t)c.sink_page : (uint64_t)c.zero_page;
+    return Bound{ok ? bound : fallback, ok ? 0xFFFFFFFFu : 0u};
+}
+
+template <typename T, unsigned N>
+__device__ __forceinline__ T load_var(const VarBinding (&tab)[VAR_SLOTS], uint32_t mul, const char (&name)[N], uint32_t idx,
+                                      const ThreadCtx& c, uint32_t err_unknown, uint32_t err_type) {
+    const uint32_t h = name_hash(name, N);
+    const Bound b = resolve<T>(tab, mul, h, 1, c, err_unknown, err_type, false);
+    return b.as<const T>()[idx & b.mask];
+}
+
 template <typename T, unsigned N>
 __device__ __forceinline__ T load_elem(const VarBinding (&tab)[VAR_SLOTS], uint32_t mul, const char (&name)[N], uint32_t idx,
-                                       uint32_t len, uint32_t elem, uint32_t* d_error, uint32_t err_unknown,
+                                       uint32_t len, uint32_t elem, const ThreadCtx& c, uint32_t err_unknown,
                                        uint32_t err_type) {
     const uint32_t h = name_hash(name, N);
-    const Found f = find_binding(tab, mul, h);
-    if (!f.ptr) {
-        report_error(d_error, err_unknown, h, idx);
-        return T();
-    }
-    if (f.bytes != sizeof(T) * len) {
-        report_error(d_error, err_type, h, idx);
-        return T();
-    }
-    if (elem >= len) {
-        report_error(d_error, DERR_ARRAY_INDEX, h, idx);
-        return T();
-    }
-    return reinterpret_cast<const T*>(f.ptr)[(size_t)idx * len + elem];
+    const Bound b = resolve<T>(tab, mul, h, len, c, err_unknown, err_type, false);
+    const bool in_range = elem < len;
+    c.err.note(!in_range, DERR_ARRAY_INDEX, h);
+    const uint64_t m = (b.mask && in_range) ? ~0ull : 0ull;
+    return b.as<const T>()[((uint64_t)idx * len + elem) & m];
+}
+
+template <typename T, unsigned N>
+__device__ __forceinline__ void store_var(const VarBinding (&tab)[VAR_SLOTS], uint32_t mul, const char (&name)[N], uint32_t idx,
+                                          T v, const ThreadCtx& c, uint32_t err_unknown, uint32_t err_type) {
+    const uint32_t h = name_hash(name, N);
+    const Bound b = resolve<T>(tab, mul, h, 1, c, err_unknown, err_type, true);
+    b.as<T>()[idx & b.mask] = v;
 }
 
 template <typename T, unsigned N>
 __device__ __forceinline__ void store_elem(const VarBinding (&tab)[VAR_SLOTS], uint32_t mul, const char (&name)[N], uint32_t idx,
-                                           uint32_t len, uint32_t elem, T v, uint32_t* d_error, uint32_t err_unknown,
+                                           uint32_t len, uint32_t elem, T v, const ThreadCtx& c, uint32_t err_unknown,
                                            uint32_t err_type) {
     const uint32_t h = name_hash(name, N);
-    const Found f = find_binding(tab, mul, h);
-    if (!f.ptr) {
-        report_error(d_error, err_unknown, h, idx);
-        return;
-    }
-    if (f.bytes != sizeof(T) * len) {
-        report_error(d_error, err_type, h, idx);
-        return;
-    }
-    if (elem >= len) {
-        report_error(d_error, DERR_ARRAY_INDEX, h, idx);
-        return;
-    }
-    reinterpret_cast<T*>(f.ptr)[(size_t)idx * len + elem] = v;
+    const Bound b = resolve<T>(tab, mul, h, len, c, err_unknown, err_type, true);
+    const bool in_range = elem < len;
+    c.err.note(!in_range, DERR_ARRAY_INDEX, h);
+    const uint64_t m = (b.mask && in_range) ? ~0ull : 0ull;
+    b.as<T>()[((uint64_t)idx * len + elem) & m] = v;
 }
 
 __device__ __forceinline__ uint32_t load_count(const uint32_t* p) {
@@ -81,7 +142,7 @@ __device__ __forceinline__ uint32_t wave_inclusive(uint32_t v) {
 // population order or co-scheduled work, and resume exactly.
 class AgentRandom {
  public:
-    __device__ __forceinline__ AgentRandom(const detail::FnArgs& a, uint32_t idx) : a_(a), idx_(idx) {}
+    __device__ __forceinline__ explicit AgentRandom(const detail::ThreadCtx& c) : c_(c) {}
     template <typename T>
     __device__ __forceinline__ T uniform() const;
     template <typename T>
@@ -95,8 +156,9 @@ class AgentRandom {
     __device__ __forceinline__ uint32_t next_u32() const {
         const uint32_t blk = counter_ >> 2;
         if (blk != cached_) {
-            uint32_t c[4] = {blk, a_.stream_id, a_.d_step ? *a_.d_step : 0u, a_.ids ? a_.ids[idx_] : idx_};
-            detail::philox4x32_10(c, (uint32_t)a_.seed, (uint32_t)(a_.seed >> 32));
+            const detail::FnArgs& a = c_.a;
+            uint32_t c[4] = {blk, a.stream_id, c_.d_step ? *c_.d_step : 0u, a.ids ? a.ids[c_.idx] : c_.idx};
+            detail::philox4x32_10(c, (uint32_t)c_.seed, (uint32_t)(c_.seed >> 32));
             r0_ = c[0];
             r1_ = c[1];
             r2_ = c[2];
@@ -108,8 +170,7 @@ class AgentRandom {
     }
 
  private:
-    const detail::FnArgs& a_;
-    uint32_t idx_;
+    const detail::ThreadCtx& c_;
     mutable uint32_t counter_ = 0;
     mutable uint32_t cached_ = 0xFFFFFFFFu;
     mutable uint32_t r0_ = 0, r1_ = 0, r2_ = 0, r3_ = 0;
@@ -133,7 +194,7 @@ __device__ __forceinline__ double AgentRandom::uniform<double>() const {
 template <typename T>
 __device__ __forceinline__ T AgentRandom::uniform(T min, T max) const {
     static_assert(std::is_integral<T>::value, "AgentRandom::uniform(min, max) is for integer types");
-    if (min > max) detail::report_error(a_.d_error, detail::DERR_RANDOM_RANGE, 0, idx_);
+    c_.err.note(min > max, detail::DERR_RANDOM_RANGE, 0);
     return static_cast<T>(min + (1 + max - min) * uniform<float>());
 }
 // Box-Muller on two 24-bit uniforms mapped to (0, 1].
@@ -178,7 +239,7 @@ __device__ __forceinline__ unsigned int AgentRandom::poisson(double mean) const 
 // include/flamegpu/runtime/environment/DeviceEnvironment.cuh
 class DeviceEnvironment {
  public:
-    __device__ __forceinline__ explicit DeviceEnvironment(const detail::FnArgs& a) : a_(a) {}
+    __device__ __forceinline__ explicit DeviceEnvironment(const detail::ThreadCtx& c) : c_(c) {}
     template <typename T, unsigned N>
     __device__ __forceinline__ T getProperty(const char (&name)[N]) const {
         return get<T>(name, 1, 0);
@@ -190,137 +251,107 @@ class DeviceEnvironment {
     template <unsigned N>
     __device__ __forceinline__ bool containsProperty(const char (&name)[N]) const {
         const uint32_t h = detail::name_hash(name, N);
-        return a_.env_b[detail::table_slot(h, a_.env_mul, detail::ENV_SLOT_BITS)].hash == h;
+        return c_.a.env_b[detail::table_slot(h, c_.a.env_mul, detail::ENV_SLOT_BITS)].hash == h;
     }
 
  private:
     template <typename T, unsigned N>
     __device__ __forceinline__ T get(const char (&name)[N], uint32_t len, uint32_t i) const {
         const uint32_t h = detail::name_hash(name, N);
-        const detail::EnvBinding& b = a_.env_b[detail::table_slot(h, a_.env_mul, detail::ENV_SLOT_BITS)];
-        if (b.hash != h) {
-            detail::report_error(a_.d_error, detail::DERR_UNKNOWN_ENV, h, 0);
-            return T();
-        }
-        if (b.bytes != sizeof(T) * len || i >= len) {
-            detail::report_error(a_.d_error, detail::DERR_ENV_TYPE, h, 0);
-            return T();
-        }
-        return reinterpret_cast<const T*>(a_.env + b.offset)[i];
+        const uint2 raw = *reinterpret_cast<const uint2*>(&c_.a.env_b[detail::table_slot(h, c_.a.env_mul, detail::ENV_SLOT_BITS)]);
+        const bool known = raw.x == h;
+        const bool typed = (raw.y >> 16) == (uint32_t)sizeof(T) * len && i < len;
+        c_.err.note(!known, detail::DERR_UNKNOWN_ENV, h);
+        c_.err.note(known && !typed, detail::DERR_ENV_TYPE, h);
+        const bool ok = known && typed;
+        const T* p = reinterpret_cast<const T*>(ok ? c_.env + (raw.y & 0xFFFFu) : c_.zero_page);
+        return p[ok ? i : 0u];
     }
-    const detail::FnArgs& a_;
+    const detail::ThreadCtx& c_;
 };
 
 // ------------------------------------------------------------------ message inputs
 class MessageNone::In {
  public:
-    __device__ __forceinline__ explicit In(const detail::FnArgs&) {}
+    __device__ __forceinline__ explicit In(const detail::ThreadCtx&) {}
 };
 class MessageNone::Out {
  public:
-    __device__ __forceinline__ Out(const detail::FnArgs&, uint32_t) {}
+    __device__ __forceinline__ explicit Out(const detail::ThreadCtx&) {}
 };
-
-namespace detail {
-// Shared access to the bound input message list.
-struct MsgReader {
-    const FnArgs& a;
-    template <typename T, unsigned N>
-    __device__ __forceinline__ T get(const char (&name)[N], uint32_t idx) const {
-        const uint32_t h = name_hash(name, N);
-        const Found f = find_binding(a.msg_in, a.msg_in_mul, h);
-        if (!f.ptr) {
-            report_error(a.d_error, DERR_UNKNOWN_MSG_VAR, h, idx);
-            return T();
-        }
-        if (f.bytes != sizeof(T)) {
-            report_error(a.d_error, DERR_MSG_VAR_TYPE, h, idx);
-            return T();
-        }
-        return reinterpret_cast<const T*>(f.ptr)[idx];
-    }
-    template <typename T, size_type L, unsigned N>
-    __device__ __forceinline__ T get(const char (&name)[N], uint32_t idx, uint32_t i) const {
-        return load_elem<T>(a.msg_in, a.msg_in_mul, name, idx, L, i, a.d_error, DERR_UNKNOWN_MSG_VAR,
-                                          DERR_MSG_VAR_TYPE);
-    }
-};
-}  // namespace detail
 
 // include/flamegpu/runtime/messaging/MessageBruteForce/MessageBruteForceDevice.cuh:19-172
 class MessageBruteForce::In {
  public:
-    __device__ __forceinline__ explicit In(const detail::FnArgs& a) : r_{a}, len_(a.d_msg_in_count ? detail::load_count(a.d_msg_in_count) : 0u) {}
+    __device__ __forceinline__ explicit In(const detail::ThreadCtx& c)
+        : c_(c), len_(c.a.d_msg_in_count ? detail::load_count(c.a.d_msg_in_count) : 0u) {}
     class Message {
      public:
-        __device__ __forceinline__ Message(const detail::MsgReader& r, uint32_t i) : r_(r), i_(i) {}
+        __device__ __forceinline__ Message(const detail::ThreadCtx& c, uint32_t i) : c_(c), i_(i) {}
         template <typename T, unsigned N>
         __device__ __forceinline__ T getVariable(const char (&name)[N]) const {
-            return r_.template get<T>(name, i_);
+            return detail::load_var<T>(c_.a.msg_in, c_.a.msg_in_mul, name, i_, c_, detail::DERR_UNKNOWN_MSG_VAR,
+                                       detail::DERR_MSG_VAR_TYPE);
         }
         template <typename T, size_type L, unsigned N>
         __device__ __forceinline__ T getVariable(const char (&name)[N], unsigned int e) const {
-            return r_.template get<T, L>(name, i_, e);
+            return detail::load_elem<T>(c_.a.msg_in, c_.a.msg_in_mul, name, i_, L, e, c_, detail::DERR_UNKNOWN_MSG_VAR,
+                                        detail::DERR_MSG_VAR_TYPE);
         }
         __device__ __forceinline__ size_type getIndex() const { return i_; }
 
      private:
-        const detail::MsgReader& r_;
+        const detail::ThreadCtx& c_;
         uint32_t i_;
     };
     class iterator {
      public:
-        __device__ __forceinline__ iterator(const detail::MsgReader& r, uint32_t i) : r_(r), i_(i) {}
+        __device__ __forceinline__ iterator(const detail::ThreadCtx& c, uint32_t i) : c_(c), i_(i) {}
         __device__ __forceinline__ iterator& operator++() {
             ++i_;
             return *this;
         }
         __device__ __forceinline__ bool operator!=(const iterator& o) const { return i_ != o.i_; }
         __device__ __forceinline__ bool operator==(const iterator& o) const { return i_ == o.i_; }
-        __device__ __forceinline__ Message operator*() const { return Message(r_, i_); }
+        __device__ __forceinline__ Message operator*() const { return Message(c_, i_); }
 
      private:
-        const detail::MsgReader& r_;
+        const detail::ThreadCtx& c_;
         uint32_t i_;
     };
-    __device__ __forceinline__ iterator begin() const { return iterator(r_, 0); }
-    __device__ __forceinline__ iterator end() const { return iterator(r_, len_); }
+    __device__ __forceinline__ iterator begin() const { return iterator(c_, 0); }
+    __device__ __forceinline__ iterator end() const { return iterator(c_, len_); }
     __device__ __forceinline__ size_type size() const { return len_; }
-    __device__ __forceinline__ Message at(size_type i) const { return Message(r_, i); }
+    __device__ __forceinline__ Message at(size_type i) const { return Message(c_, i); }
 
  private:
-    detail::MsgReader r_;
+    const detail::ThreadCtx& c_;
     uint32_t len_;
 };
 
 // include/flamegpu/runtime/messaging/MessageBruteForce/MessageBruteForceDevice.cuh:259-294
 class MessageBruteForce::Out {
  public:
-    __device__ __forceinline__ Out(const detail::FnArgs& a, uint32_t idx) : a_(a), idx_(idx) {}
+    __device__ __forceinline__ explicit Out(const detail::ThreadCtx& c) : c_(c) {}
     template <typename T, unsigned N>
     __device__ __forceinline__ void setVariable(const char (&name)[N], T value) const {
-        if (!a_.msg_out_flags) {
-            detail::report_error(a_.d_error, detail::DERR_MSG_OUTPUT_DISABLED, 0, idx_);
-            return;
-        }
-        detail::store_elem<T>(a_.msg_out, a_.msg_out_mul, name, idx_, 1, 0, value, a_.d_error,
-                                                    detail::DERR_UNKNOWN_MSG_VAR, detail::DERR_MSG_VAR_TYPE);
-        if (a_.msg_out_optional) a_.msg_out_flags[idx_] = 1u;
+        const bool on = c_.a.msg_out_flags != nullptr;
+        c_.err.note(!on, detail::DERR_MSG_OUTPUT_DISABLED, 0);
+        detail::store_var<T>(c_.a.msg_out, c_.a.msg_out_mul, name, c_.idx, value, c_, detail::DERR_UNKNOWN_MSG_VAR,
+                             detail::DERR_MSG_VAR_TYPE);
+        if (on && c_.a.msg_out_optional) c_.a.msg_out_flags[c_.idx] = 1u;
     }
     template <typename T, size_type L, unsigned N>
     __device__ __forceinline__ void setVariable(const char (&name)[N], unsigned int e, T value) const {
-        if (!a_.msg_out_flags) {
-            detail::report_error(a_.d_error, detail::DERR_MSG_OUTPUT_DISABLED, 0, idx_);
-            return;
-        }
-        detail::store_elem<T>(a_.msg_out, a_.msg_out_mul, name, idx_, L, e, value, a_.d_error,
-                                                    detail::DERR_UNKNOWN_MSG_VAR, detail::DERR_MSG_VAR_TYPE);
-        if (a_.msg_out_optional) a_.msg_out_flags[idx_] = 1u;
+        const bool on = c_.a.msg_out_flags != nullptr;
+        c_.err.note(!on, detail::DERR_MSG_OUTPUT_DISABLED, 0);
+        detail::store_elem<T>(c_.a.msg_out, c_.a.msg_out_mul, name, c_.idx, L, e, value, c_, detail::DERR_UNKNOWN_MSG_VAR,
+                              detail::DERR_MSG_VAR_TYPE);
+        if (on && c_.a.msg_out_optional) c_.a.msg_out_flags[c_.idx] = 1u;
     }
 
  protected:
-    const detail::FnArgs& a_;
-    uint32_t idx_;
+    const detail::ThreadCtx& c_;
 };
 
 namespace detail {
@@ -329,11 +360,6 @@ __device__ __forceinline__ int grid_pos(const SpatialDev& m, int a, float p) {
     const int g = (int)floorf((p - m.min[a]) / m.radius);
     const int hi = (int)m.grid[a] - 1;
     return g < 0 ? 0 : (g > hi ? hi : g);
-}
-__device__ __forceinline__ uint32_t cell_hash(const SpatialDev& m, int x, int y, int z) {
-    const int hx = (int)m.grid[0] - 1;
-    const uint32_t ux = (uint32_t)(x < 0 ? 0 : (x >= hx ? hx : x));
-    return (uint32_t)z * m.grid[0] * m.grid[1] + (uint32_t)y * m.grid[0] + ux;
 }
 // getVirtualX (MessageSpatial3DDevice.cuh:373-397): x2 - (x21/|x21|)*width.  On the
 // taken branch |x21| > width/2 > 0, so x21/|x21| is exactly +-1 and the product is
@@ -348,184 +374,307 @@ __device__ __forceinline__ int wrap_cell(int c, int d, int g) {
     a = a < 0 ? a + g : a;
     return a >= g ? a - g : a;
 }
+constexpr uint32_t kHashX = name_hash("x", 2);
+constexpr uint32_t kHashY = name_hash("y", 2);
+constexpr uint32_t kHashZ = name_hash("z", 2);
+constexpr uint32_t kHashId = name_hash("id", 3);
 }  // namespace detail
 
-// Spatial message with its search origin, shared by the 2-D and 3-D filters.
-// Two sources for the same visit order: the block's LDS copy of the neighbourhood
-// (agent_function_kernel_staged) when the search origin's cell is covered by it,
-// otherwise the message list in HBM.
+// Spatial message iteration shared by the 2-D and 3-D filters.
+//
+// Visit order of the reference iterators:
+//  Filter 3-D (MessageSpatial3DDevice.cuh:52-59,693-719): 9 x-strips, dy outer, dz inner;
+//  WrapFilter 3-D (:264-279,727-749): 27 cells, dx outer, dy, dz inner, modulo grid;
+//  Filter 2-D (MessageSpatial2DDevice.cuh:646-672): 3 strips; WrapFilter 2-D (:679-698): 9 cells.
+// Units (strips or cells) are numbered in visit order; a unit's neighbour offsets
+// follow from its number by constant divisions, and its bins from the search
+// origin's cell with selects only (no per-lane branches).
+//
+// The iterator keeps one candidate of look-ahead: while the body runs on message k,
+// the loads of message k+1's x, y (, z) and id are in flight, and the bin-table
+// range of the unit after the current one is loaded when the current one starts.
+// Messages read through their position / id use those registers; other variables
+// are loaded at access time.
+//
+// Source: the message list in HBM (bin table pbm), or, in the staged kernel when
+// every lane of the wave is covered by the block's LDS copy, the LDS tables (units
+// then name virtual bin entries and the look-ahead walks LDS slots).
 template <int DIMS, bool WRAP>
 class SpatialFilter {
  public:
-    __device__ __forceinline__ SpatialFilter(const detail::FnArgs& a, float x, float y, float z, const uint32_t* smem,
-                                             const detail::StageHeader* sh)
-        : a_(a), m_(a.in_meta) {
+    static constexpr int kUnits = WRAP ? (DIMS == 3 ? 27 : 9) : (DIMS == 3 ? 9 : 3);
+
+    __device__ __forceinline__ SpatialFilter(const detail::ThreadCtx& c, float x, float y, float z, bool allow_stage)
+        : c_(c) {
+        const detail::SpatialDev& m = c.meta;
         loc_[0] = x;
         loc_[1] = y;
         loc_[2] = z;
-        cell_[0] = detail::grid_pos(m_, 0, x);
-        cell_[1] = detail::grid_pos(m_, 1, y);
-        cell_[2] = DIMS == 3 ? detail::grid_pos(m_, 2, z) : 0;
-        // Staged reads only when every active lane of the wave is covered, so the
-        // choice between LDS and HBM is a scalar branch, never both paths.
-        int covered = 0;
-        if (smem) {
+        width_[0] = m.width[0];
+        width_[1] = m.width[1];
+        width_[2] = m.width[2];
+        const int cx = detail::grid_pos(m, 0, x);
+        const int cy = detail::grid_pos(m, 1, y);
+        const int cz = DIMS == 3 ? detail::grid_pos(m, 2, z) : 0;
+        // hot variables in HBM
+        const detail::VarBinding& bx = c.a.msg_in[detail::table_slot(detail::kHashX, c.a.msg_in_mul, detail::VAR_SLOT_BITS)];
+        const detail::VarBinding& by = c.a.msg_in[detail::table_slot(detail::kHashY, c.a.msg_in_mul, detail::VAR_SLOT_BITS)];
+        const detail::VarBinding& bz = c.a.msg_in[detail::table_slot(detail::kHashZ, c.a.msg_in_mul, detail::VAR_SLOT_BITS)];
+        const detail::VarBinding& bi = c.a.msg_in[detail::table_slot(detail::kHashId, c.a.msg_in_mul, detail::VAR_SLOT_BITS)];
+        px_ = reinterpret_cast<const float*>(bx.ptr);
+        py_ = reinterpret_cast<const float*>(by.ptr);
+        pz_ = DIMS == 3 ? reinterpret_cast<const float*>(bz.ptr) : px_;
+        const bool id_known = bi.hash == detail::kHashId;
+        const bool id_ok = id_known && bi.bytes == 4;
+        id_err_ = id_ok ? 0u : (id_known ? (uint32_t)detail::DERR_MSG_VAR_TYPE : (uint32_t)detail::DERR_UNKNOWN_MSG_VAR);
+        pid_ = reinterpret_cast<const uint32_t*>(id_ok ? bi.ptr : c.zero_page);
+        id_mask_ = id_ok ? 0xFFFFFFFFu : 0u;
+        pbm_ = m.pbm;
+        sv_ = nullptr;
+        // LDS source when the whole wave is covered by the block's staged neighbourhood
+        if (allow_stage && c.stage) {
+            const detail::StageHeader* sh = c.stage->sh;
+            int covered = 0, ebase = 0, xbase = 0, width = 0;
             for (uint32_t k = 0; k < sh->ngroups; ++k) {
                 const detail::StageGroup& g = sh->g[k];
-                if (g.cy == cell_[1] && g.cz == cell_[2] && cell_[0] >= g.xlo && cell_[0] <= g.xhi) {
+                if (g.cy == cy && g.cz == cz && cx >= g.xlo && cx <= g.xhi) {
                     covered = 1;
-                    ebase_ = g.ebase;
-                    xbase_ = g.xbase;
-                    width_ = (uint32_t)g.width;
+                    ebase = (int)g.ebase;
+                    xbase = g.xbase;
+                    width = g.width;
                 }
             }
+            if (__builtin_amdgcn_readfirstlane(__all(covered) ? 1 : 0)) {
+                sv_ = c.stage;
+                plan(m, cx, cy, cz, ebase, xbase, width);
+                return;
+            }
         }
-        staged_ = __builtin_amdgcn_readfirstlane(__all(covered) ? 1 : 0) != 0;
-        smem_ = smem;
+        plan(m, cx, cy, cz, 0, 0, 0);
     }
+
     class Message {
      public:
-        __device__ __forceinline__ Message(const SpatialFilter& f, uint32_t g, uint32_t l) : f_(f), g_(g), l_(l) {}
+        __device__ __forceinline__ Message(const SpatialFilter& f, uint32_t g, float x, float y, float z, uint32_t id)
+            : f_(f), g_(g), x_(x), y_(y), z_(z), id_(id) {}
         template <typename T, unsigned N>
         __device__ __forceinline__ T getVariable(const char (&name)[N]) const {
             const uint32_t h = detail::name_hash(name, N);
-            const detail::VarBinding& b = f_.a_.msg_in[detail::table_slot(h, f_.a_.msg_in_mul, detail::VAR_SLOT_BITS)];
-            if (b.hash != h) {
-                detail::report_error(f_.a_.d_error, detail::DERR_UNKNOWN_MSG_VAR, h, g_);
-                return T();
-            }
-            if (b.bytes != sizeof(T)) {
-                detail::report_error(f_.a_.d_error, detail::DERR_MSG_VAR_TYPE, h, g_);
-                return T();
+            if (h == detail::kHashX || h == detail::kHashY || (DIMS == 3 && h == detail::kHashZ)) {
+                if constexpr (std::is_same<T, float>::value) {
+                    return h == detail::kHashX ? x_ : (h == detail::kHashY ? y_ : z_);
+                } else {
+                    f_.c_.err.note(true, detail::DERR_MSG_VAR_TYPE, h);
+                    return T();
+                }
             }
             if constexpr (sizeof(T) == 4) {
-                if (f_.staged_ && b.lds != detail::NOT_STAGED) {
-                    const uint32_t w = f_.smem_[2 * (f_.a_.stage_max_entries + 1) + (uint32_t)b.lds * f_.a_.stage_cap + l_];
+                if (h == detail::kHashId) {
+                    f_.c_.err.note(f_.id_err_ != 0u, f_.id_err_, h);
                     T v;
-                    __builtin_memcpy(&v, &w, 4);
+                    __builtin_memcpy(&v, &id_, 4);
                     return v;
                 }
             }
-            return reinterpret_cast<const T*>(b.ptr)[g_];
+            return detail::load_var<T>(f_.c_.a.msg_in, f_.c_.a.msg_in_mul, name, g_, f_.c_, detail::DERR_UNKNOWN_MSG_VAR,
+                                       detail::DERR_MSG_VAR_TYPE);
         }
         template <typename T, size_type L, unsigned N>
         __device__ __forceinline__ T getVariable(const char (&name)[N], unsigned int e) const {
-            return detail::load_elem<T>(f_.a_.msg_in, f_.a_.msg_in_mul, name, g_, L, e, f_.a_.d_error,
+            return detail::load_elem<T>(f_.c_.a.msg_in, f_.c_.a.msg_in_mul, name, g_, L, e, f_.c_,
                                         detail::DERR_UNKNOWN_MSG_VAR, detail::DERR_MSG_VAR_TYPE);
         }
         __device__ __forceinline__ float getVirtualX() const { return getVirtualX(f_.loc_[0]); }
         __device__ __forceinline__ float getVirtualY() const { return getVirtualY(f_.loc_[1]); }
         __device__ __forceinline__ float getVirtualZ() const { return getVirtualZ(f_.loc_[2]); }
-        __device__ __forceinline__ float getVirtualX(float x1) const {
-            return detail::virtual_coord(getVariable<float>("x"), x1, f_.m_.width[0]);
-        }
-        __device__ __forceinline__ float getVirtualY(float y1) const {
-            return detail::virtual_coord(getVariable<float>("y"), y1, f_.m_.width[1]);
-        }
-        __device__ __forceinline__ float getVirtualZ(float z1) const {
-            return detail::virtual_coord(getVariable<float>("z"), z1, f_.m_.width[2]);
-        }
+        __device__ __forceinline__ float getVirtualX(float x1) const { return detail::virtual_coord(x_, x1, f_.width_[0]); }
+        __device__ __forceinline__ float getVirtualY(float y1) const { return detail::virtual_coord(y_, y1, f_.width_[1]); }
+        __device__ __forceinline__ float getVirtualZ(float z1) const { return detail::virtual_coord(z_, z1, f_.width_[2]); }
         __device__ __forceinline__ size_type getIndex() const { return g_; }
 
      private:
         const SpatialFilter& f_;
-        uint32_t g_, l_;
+        uint32_t g_;
+        float x_, y_, z_;
+        uint32_t id_;
     };
-    // Visit order of the reference iterators:
-    //  Filter 3-D (MessageSpatial3DDevice.cuh:52-59,693-719): 9 x-strips, dy outer, dz inner;
-    //  WrapFilter 3-D (:264-279,727-749): 27 cells, dx outer, dy, dz inner, modulo grid;
-    //  Filter 2-D (MessageSpatial2DDevice.cuh:646-672): 3 strips; WrapFilter 2-D (:679-698): 9 cells.
-    static constexpr int kUnits = WRAP ? (DIMS == 3 ? 27 : 9) : (DIMS == 3 ? 9 : 3);
+
     class iterator {
      public:
-        __device__ __forceinline__ iterator(const SpatialFilter& f, int unit) : f_(f), unit_(unit), g_(0), l_(0), lend_(0) {}
-        __device__ __forceinline__ void advance_unit() {
-            const detail::SpatialDev& m = f_.m_;
-            while (++unit_ < kUnits) {
-                int dx, dy, dz;
-                if (WRAP) {
-                    if (DIMS == 3) {
-                        dx = unit_ / 9 - 1;
-                        dy = (unit_ / 3) % 3 - 1;
-                        dz = unit_ % 3 - 1;
-                    } else {
-                        dx = unit_ / 3 - 1;
-                        dy = unit_ % 3 - 1;
-                        dz = 0;
-                    }
-                } else {
-                    dx = 0;
-                    if (DIMS == 3) {
-                        dy = unit_ / 3 - 1;
-                        dz = unit_ % 3 - 1;
-                    } else {
-                        dy = unit_ - 1;
-                        dz = 0;
-                    }
-                    const int ay = f_.cell_[1] + dy, az = f_.cell_[2] + dz;
-                    if (ay < 0 || ay >= (int)m.grid[1] || az < 0 || az >= (int)m.grid[2]) continue;
-                }
-                const int hx = (int)m.grid[0] - 1;
-                const int xs = WRAP ? f_.cell_[0] + dx : (f_.cell_[0] - 1 < 0 ? 0 : f_.cell_[0] - 1);
-                const int xe = WRAP ? xs : (f_.cell_[0] + 1 > hx ? hx : f_.cell_[0] + 1);
-                if (f_.staged_) {
-                    const uint32_t j = DIMS == 3 ? (uint32_t)((dy + 1) * 3 + (dz + 1)) : (uint32_t)(dy + 1);
-                    const uint32_t row = f_.ebase_ + j * f_.width_;
-                    const uint32_t es = row + (uint32_t)(xs - f_.xbase_);
-                    const uint32_t ee = row + (uint32_t)(xe - f_.xbase_);
-                    l_ = f_.smem_[es];
-                    lend_ = f_.smem_[ee + 1];
-                    g_ = f_.smem_[f_.a_.stage_max_entries + 1 + es];
-                } else {
-                    uint32_t s, e;
-                    if (WRAP) {
-                        const int ax = detail::wrap_cell(f_.cell_[0], dx, (int)m.grid[0]);
-                        const int ay = detail::wrap_cell(f_.cell_[1], dy, (int)m.grid[1]);
-                        const int az = detail::wrap_cell(f_.cell_[2], dz, (int)m.grid[2]);
-                        s = detail::cell_hash(m, ax, ay, az);
-                        e = s + 1;
-                    } else {
-                        const int ay = f_.cell_[1] + dy, az = f_.cell_[2] + dz;
-                        s = detail::cell_hash(m, xs, ay, az);
-                        e = detail::cell_hash(m, xe, ay, az) + 1;
-                    }
-                    g_ = m.pbm[s];
-                    l_ = g_;
-                    lend_ = m.pbm[e];
-                }
-                if (l_ < lend_) return;
-            }
+        // end sentinel
+        __device__ __forceinline__ explicit iterator(const SpatialFilter& f) : f_(f), valid_(false) {}
+        // first message
+        __device__ __forceinline__ iterator(const SpatialFilter& f, int) : f_(f) {
+            pu_ = 0;
+            f_.load_unit(0, ps_, pe_, po_);
+            next_unit();
+            fetch();
+            shift();
+            step();
         }
         __device__ __forceinline__ iterator& operator++() {
-            ++g_;
-            if (++l_ >= lend_) advance_unit();
+            shift();
+            step();
             return *this;
         }
-        __device__ __forceinline__ bool operator!=(const iterator&) const { return unit_ < kUnits; }
-        __device__ __forceinline__ Message operator*() const { return Message(f_, g_, l_); }
+        __device__ __forceinline__ bool operator!=(const iterator&) const { return valid_; }
+        __device__ __forceinline__ Message operator*() const { return Message(f_, g_, x_, y_, z_, id_); }
 
      private:
+        // Move the look-ahead to the first message of the next non-empty unit.
+        __device__ __forceinline__ void next_unit() {
+            more_ = false;
+#pragma clang loop unroll(disable)
+            while (pu_ < kUnits) {
+                l_ = ps_;
+                e_ = pe_;
+                o_ = po_;
+                ++pu_;
+                if (pu_ < kUnits) f_.load_unit(pu_, ps_, pe_, po_);
+                if (l_ < e_) {
+                    more_ = true;
+                    return;
+                }
+            }
+        }
+        __device__ __forceinline__ void step() {
+            ++l_;
+            if (l_ >= e_) next_unit();
+            fetch();
+        }
+        // Unconditional (index 0 past the last message) so the loads issue without a branch.
+        __device__ __forceinline__ void fetch() { f_.fetch(more_ ? l_ : 0u, nx_, ny_, nz_, nid_); }
+        __device__ __forceinline__ void shift() {
+            valid_ = more_;
+            g_ = l_ + o_;
+            x_ = nx_;
+            y_ = ny_;
+            z_ = nz_;
+            id_ = nid_;
+        }
+
         const SpatialFilter& f_;
-        int unit_;
-        uint32_t g_, l_, lend_;
+        bool valid_ = false, more_ = false;
+        // current message
+        uint32_t g_ = 0;
+        float x_ = 0.f, y_ = 0.f, z_ = 0.f;
+        uint32_t id_ = 0;
+        // look-ahead: source index l_ in unit [.., e_), global index l_ + o_, its values
+        uint32_t l_ = 0, e_ = 0, o_ = 0;
+        float nx_ = 0.f, ny_ = 0.f, nz_ = 0.f;
+        uint32_t nid_ = 0;
+        // the unit after the look-ahead's: its number and loaded range
+        int pu_ = 0;
+        uint32_t ps_ = 0, pe_ = 0, po_ = 0;
     };
-    __device__ __forceinline__ iterator begin() const {
-        iterator it(*this, -1);
-        it.advance_unit();
-        return it;
-    }
-    __device__ __forceinline__ iterator end() const { return iterator(*this, kUnits); }
+    __device__ __forceinline__ iterator begin() const { return iterator(*this, 0); }
+    __device__ __forceinline__ iterator end() const { return iterator(*this); }
 
  private:
-    const detail::FnArgs& a_;
-    const detail::SpatialDev& m_;
+    // Per-lane search origin.  HBM: bins of the grid.  LDS (ebase/xbase/width of the
+    // covering group): virtual bin entries, rows of `width` entries for the 9 (3)
+    // neighbour rows, x counted from xbase.
+    __device__ __forceinline__ void plan(const detail::SpatialDev& m, int cx, int cy, int cz, int ebase, int xbase,
+                                         int width) {
+        gx_ = (int)m.grid[0];
+        gy_ = (int)m.grid[1];
+        gz_ = (int)m.grid[2];
+        cx_ = cx;
+        cy_ = cy;
+        cz_ = cz;
+        ebase_ = ebase;
+        xbase_ = xbase;
+        ewidth_ = width;
+        if (!WRAP) {
+            xs_ = cx - 1 < 0 ? 0 : cx - 1;
+            const int xe = cx + 1 > gx_ - 1 ? gx_ - 1 : cx + 1;
+            span_ = xe - xs_;
+        } else {
+            xs_ = 0;
+            span_ = 0;
+        }
+    }
+    // Range [s, e) of unit u in the source's index space; o = global - source index.
+    __device__ __forceinline__ void load_unit(int u, uint32_t& s, uint32_t& e, uint32_t& o) const {
+        int dx = 0, dy, dz = 0;
+        if (WRAP && DIMS == 3) {
+            dx = u / 9;
+            const int r = u - 9 * dx;
+            dy = r / 3;
+            dz = r - 3 * dy;
+        } else if (WRAP || DIMS == 3) {
+            const int q = u / 3;
+            if (WRAP) {
+                dx = q;
+                dy = u - 3 * q;
+            } else {
+                dy = q;
+                dz = u - 3 * q;
+            }
+        } else {
+            dy = u;
+        }
+        // offsets are 0..2 here (neighbour offset + 1)
+        const int ay = cy_ + dy - 1;
+        const int az = cz_ + dz - 1;
+        bool ok = true;
+        if (!WRAP) ok = ay >= 0 && ay < gy_ && (DIMS == 2 || (az >= 0 && az < gz_));
+        uint32_t first;
+        if (sv_) {
+            const int row = DIMS == 3 ? 3 * dy + dz : dy;
+            const int vx = WRAP ? cx_ + dx - 1 : xs_;
+            first = ok ? (uint32_t)(ebase_ + row * ewidth_ + (vx - xbase_)) : 0u;
+            s = sv_->lstart[first];
+            const uint32_t t = sv_->lstart[first + (uint32_t)span_ + 1u];
+            o = sv_->gstart[first] - s;
+            e = ok ? t : s;
+        } else {
+            const int x = WRAP ? detail::wrap_cell(cx_, dx - 1, gx_) : xs_;
+            const int y = WRAP ? detail::wrap_cell(cy_, dy - 1, gy_) : ay;
+            const int z = DIMS == 3 ? (WRAP ? detail::wrap_cell(cz_, dz - 1, gz_) : az) : 0;
+            first = ok ? (uint32_t)((z * gy_ + y) * gx_ + x) : 0u;
+            uint32_t t;
+            if (WRAP) {
+                uint32_t pair[2];
+                __builtin_memcpy(pair, pbm_ + first, 8);
+                s = pair[0];
+                t = pair[1];
+            } else {
+                s = pbm_[first];
+                t = pbm_[first + (uint32_t)span_ + 1u];
+            }
+            o = 0;
+            e = ok ? t : s;
+        }
+    }
+    __device__ __forceinline__ void fetch(uint32_t l, float& x, float& y, float& z, uint32_t& id) const {
+        if (sv_) {
+            const uint32_t cap = sv_->cap;
+            const float* v = reinterpret_cast<const float*>(sv_->vars);
+            x = v[l];
+            y = v[cap + l];
+            z = DIMS == 3 ? v[2 * cap + l] : 0.0f;
+            id = sv_->has_id ? sv_->vars[(uint32_t)DIMS * cap + l] : 0u;
+        } else {
+            x = px_[l];
+            y = py_[l];
+            z = DIMS == 3 ? pz_[l] : 0.0f;
+            id = pid_[l & id_mask_];
+        }
+    }
+
+    const detail::ThreadCtx& c_;
+    const detail::StageView* sv_;
     float loc_[3];
-    int cell_[3];
-    const uint32_t* smem_;
-    bool staged_;
-    uint32_t ebase_ = 0;
-    int xbase_ = 0;
-    uint32_t width_ = 0;
+    float width_[3];
+    int gx_, gy_, gz_;              // grid
+    int cx_, cy_, cz_;              // search origin's cell
+    int xs_, span_;                 // non-wrapping strips: first x, last x - first x
+    int ebase_, xbase_, ewidth_;    // LDS source: covering group
+    const uint32_t* pbm_;
+    const float *px_, *py_, *pz_;
+    const uint32_t* pid_;
+    uint32_t id_mask_, id_err_;
 };
 
 // include/flamegpu/runtime/messaging/MessageSpatial3D/MessageSpatial3DDevice.cuh:503-558
@@ -534,27 +683,19 @@ class MessageSpatial3D::In {
     static constexpr int dims = 3;
     typedef SpatialFilter<3, false> Filter;
     typedef SpatialFilter<3, true> WrapFilter;
-    __device__ __forceinline__ explicit In(const detail::FnArgs& a) : a_(a) {}
-    __device__ __forceinline__ void enable_stage(const uint32_t* smem, const detail::StageHeader* sh) {
-        smem_ = smem;
-        sh_ = sh;
-    }
-    __device__ __forceinline__ Filter operator()(float x, float y, float z) const { return Filter(a_, x, y, z, smem_, sh_); }
+    __device__ __forceinline__ explicit In(const detail::ThreadCtx& c) : c_(c) {}
+    __device__ __forceinline__ Filter operator()(float x, float y, float z) const { return Filter(c_, x, y, z, true); }
     __device__ __forceinline__ WrapFilter wrap(float x, float y, float z) const {
-        const detail::SpatialDev& m = a_.in_meta;
-        if (x > m.max[0] || y > m.max[1] || z > m.max[2] || x < m.min[0] || y < m.min[1] || z < m.min[2]) {
-            detail::report_error(a_.d_error, detail::DERR_WRAP_OUT_OF_BOUNDS, 0, 0);
-            return WrapFilter(a_, m.min[0], m.min[1], m.min[2], nullptr, nullptr);
-        }
-        if (!m.wrap_ok) detail::report_error(a_.d_error, detail::DERR_WRAP_NOT_FACTOR, 0, 0);
-        return WrapFilter(a_, x, y, z, smem_, sh_);
+        const detail::SpatialDev& m = c_.meta;
+        const bool oob = x > m.max[0] || y > m.max[1] || z > m.max[2] || x < m.min[0] || y < m.min[1] || z < m.min[2];
+        c_.err.note(oob, detail::DERR_WRAP_OUT_OF_BOUNDS, 0);
+        c_.err.note(!m.wrap_ok, detail::DERR_WRAP_NOT_FACTOR, 0);
+        return WrapFilter(c_, oob ? m.min[0] : x, oob ? m.min[1] : y, oob ? m.min[2] : z, !oob);
     }
-    __device__ __forceinline__ float radius() const { return a_.in_meta.radius; }
+    __device__ __forceinline__ float radius() const { return c_.meta.radius; }
 
  private:
-    const detail::FnArgs& a_;
-    const uint32_t* smem_ = nullptr;
-    const detail::StageHeader* sh_ = nullptr;
+    const detail::ThreadCtx& c_;
 };
 
 // include/flamegpu/runtime/messaging/MessageSpatial2D/MessageSpatial2DDevice.cuh
@@ -563,32 +704,24 @@ class MessageSpatial2D::In {
     static constexpr int dims = 2;
     typedef SpatialFilter<2, false> Filter;
     typedef SpatialFilter<2, true> WrapFilter;
-    __device__ __forceinline__ explicit In(const detail::FnArgs& a) : a_(a) {}
-    __device__ __forceinline__ void enable_stage(const uint32_t* smem, const detail::StageHeader* sh) {
-        smem_ = smem;
-        sh_ = sh;
-    }
-    __device__ __forceinline__ Filter operator()(float x, float y) const { return Filter(a_, x, y, 0.0f, smem_, sh_); }
+    __device__ __forceinline__ explicit In(const detail::ThreadCtx& c) : c_(c) {}
+    __device__ __forceinline__ Filter operator()(float x, float y) const { return Filter(c_, x, y, 0.0f, true); }
     __device__ __forceinline__ WrapFilter wrap(float x, float y) const {
-        const detail::SpatialDev& m = a_.in_meta;
-        if (x > m.max[0] || y > m.max[1] || x < m.min[0] || y < m.min[1]) {
-            detail::report_error(a_.d_error, detail::DERR_WRAP_OUT_OF_BOUNDS, 0, 0);
-            return WrapFilter(a_, m.min[0], m.min[1], 0.0f, nullptr, nullptr);
-        }
-        if (!m.wrap_ok) detail::report_error(a_.d_error, detail::DERR_WRAP_NOT_FACTOR, 0, 0);
-        return WrapFilter(a_, x, y, 0.0f, smem_, sh_);
+        const detail::SpatialDev& m = c_.meta;
+        const bool oob = x > m.max[0] || y > m.max[1] || x < m.min[0] || y < m.min[1];
+        c_.err.note(oob, detail::DERR_WRAP_OUT_OF_BOUNDS, 0);
+        c_.err.note(!m.wrap_ok, detail::DERR_WRAP_NOT_FACTOR, 0);
+        return WrapFilter(c_, oob ? m.min[0] : x, oob ? m.min[1] : y, 0.0f, !oob);
     }
-    __device__ __forceinline__ float radius() const { return a_.in_meta.radius; }
+    __device__ __forceinline__ float radius() const { return c_.meta.radius; }
 
  private:
-    const detail::FnArgs& a_;
-    const uint32_t* smem_ = nullptr;
-    const detail::StageHeader* sh_ = nullptr;
+    const detail::ThreadCtx& c_;
 };
 
 class MessageSpatial2D::Out : public MessageBruteForce::Out {
  public:
-    __device__ __forceinline__ Out(const detail::FnArgs& a, uint32_t idx) : MessageBruteForce::Out(a, idx) {}
+    __device__ __forceinline__ explicit Out(const detail::ThreadCtx& c) : MessageBruteForce::Out(c) {}
     __device__ __forceinline__ void setLocation(float x, float y) const {
         setVariable<float>("x", x);
         setVariable<float>("y", y);
@@ -598,7 +731,7 @@ class MessageSpatial2D::Out : public MessageBruteForce::Out {
 // MessageSpatial3DDevice.cuh:674-684
 class MessageSpatial3D::Out : public MessageBruteForce::Out {
  public:
-    __device__ __forceinline__ Out(const detail::FnArgs& a, uint32_t idx) : MessageBruteForce::Out(a, idx) {}
+    __device__ __forceinline__ explicit Out(const detail::ThreadCtx& c) : MessageBruteForce::Out(c) {}
     __device__ __forceinline__ void setLocation(float x, float y, float z) const {
         setVariable<float>("x", x);
         setVariable<float>("y", y);
@@ -613,81 +746,77 @@ class DeviceAPI {
     // include/flamegpu/runtime/DeviceAPI.cuh:187-260 (AgentOut)
     class AgentOut {
      public:
-        __device__ __forceinline__ AgentOut(const detail::FnArgs& a, uint32_t idx) : a_(a), idx_(idx) {}
+        __device__ __forceinline__ explicit AgentOut(const detail::ThreadCtx& c) : c_(c) {}
+        // (name[0] is a constant once the literal is inlined: the reserved-name test costs nothing)
         template <typename T, unsigned N>
         __device__ __forceinline__ void setVariable(const char (&name)[N], T value) const {
-            if (!a_.birth_flags) {
-                detail::report_error(a_.d_error, detail::DERR_AGENT_OUTPUT_DISABLED, 0, idx_);
-                return;
-            }
             if (name[0] == '_') {
-                detail::report_error(a_.d_error, detail::DERR_RESERVED_NAME, 0, idx_);
+                c_.err.note(true, detail::DERR_RESERVED_NAME, 0);
                 return;
             }
-            detail::store_elem<T>(a_.new_agent, a_.new_mul, name, idx_, 1, 0, value, a_.d_error,
-                                                          detail::DERR_UNKNOWN_AGENT_VAR, detail::DERR_AGENT_VAR_TYPE);
-            a_.birth_flags[idx_] = 1u;
+            const bool on = c_.a.birth_flags != nullptr;
+            c_.err.note(!on, detail::DERR_AGENT_OUTPUT_DISABLED, 0);
+            detail::store_var<T>(c_.a.new_agent, c_.a.new_mul, name, c_.idx, value, c_, detail::DERR_UNKNOWN_AGENT_VAR,
+                                 detail::DERR_AGENT_VAR_TYPE);
+            if (on) c_.a.birth_flags[c_.idx] = 1u;
         }
         template <typename T, size_type L, unsigned N>
         __device__ __forceinline__ void setVariable(const char (&name)[N], unsigned int e, T value) const {
-            if (!a_.birth_flags) {
-                detail::report_error(a_.d_error, detail::DERR_AGENT_OUTPUT_DISABLED, 0, idx_);
-                return;
-            }
             if (name[0] == '_') {
-                detail::report_error(a_.d_error, detail::DERR_RESERVED_NAME, 0, idx_);
+                c_.err.note(true, detail::DERR_RESERVED_NAME, 0);
                 return;
             }
-            detail::store_elem<T>(a_.new_agent, a_.new_mul, name, idx_, L, e, value, a_.d_error,
-                                                          detail::DERR_UNKNOWN_AGENT_VAR, detail::DERR_AGENT_VAR_TYPE);
-            a_.birth_flags[idx_] = 1u;
+            const bool on = c_.a.birth_flags != nullptr;
+            c_.err.note(!on, detail::DERR_AGENT_OUTPUT_DISABLED, 0);
+            detail::store_elem<T>(c_.a.new_agent, c_.a.new_mul, name, c_.idx, L, e, value, c_,
+                                  detail::DERR_UNKNOWN_AGENT_VAR, detail::DERR_AGENT_VAR_TYPE);
+            if (on) c_.a.birth_flags[c_.idx] = 1u;
         }
         // New-agent IDs are nextID + the parent's rank among this launch's births, which is
         // only known after the launch; the ID can be read from the new agent next step.
         __device__ __forceinline__ id_t getID() const {
-            detail::report_error(a_.d_error, detail::DERR_AGENT_OUT_ID, 0, idx_);
+            c_.err.note(true, detail::DERR_AGENT_OUT_ID, 0);
             return ID_NOT_SET;
         }
 
      private:
-        const detail::FnArgs& a_;
-        uint32_t idx_;
+        const detail::ThreadCtx& c_;
     };
 
-    __device__ __forceinline__ DeviceAPI(const detail::FnArgs& a, uint32_t idx)
-        : message_in(a), message_out(a, idx), agent_out(a, idx), random(a, idx), environment(a), a_(a), idx_(idx) {}
+    __device__ __forceinline__ explicit DeviceAPI(const detail::ThreadCtx& c)
+        : message_in(c), message_out(c), agent_out(c), random(c), environment(c), c_(c) {}
 
     template <typename T, unsigned N>
     __device__ __forceinline__ T getVariable(const char (&name)[N]) const {
-        return detail::load_elem<T>(a_.agent, a_.agent_mul, name, idx_, 1, 0, a_.d_error,
-                                                            detail::DERR_UNKNOWN_AGENT_VAR, detail::DERR_AGENT_VAR_TYPE);
+        return detail::load_var<T>(c_.a.agent, c_.a.agent_mul, name, c_.idx, c_, detail::DERR_UNKNOWN_AGENT_VAR,
+                                   detail::DERR_AGENT_VAR_TYPE);
     }
     template <typename T, size_type L, unsigned N>
     __device__ __forceinline__ T getVariable(const char (&name)[N], unsigned int e) const {
-        return detail::load_elem<T>(a_.agent, a_.agent_mul, name, idx_, L, e, a_.d_error,
-                                                            detail::DERR_UNKNOWN_AGENT_VAR, detail::DERR_AGENT_VAR_TYPE);
+        return detail::load_elem<T>(c_.a.agent, c_.a.agent_mul, name, c_.idx, L, e, c_, detail::DERR_UNKNOWN_AGENT_VAR,
+                                    detail::DERR_AGENT_VAR_TYPE);
     }
     template <typename T, unsigned N>
     __device__ __forceinline__ void setVariable(const char (&name)[N], T value) const {
         if (name[0] == '_') {
-            detail::report_error(a_.d_error, detail::DERR_RESERVED_NAME, 0, idx_);
+            c_.err.note(true, detail::DERR_RESERVED_NAME, 0);
             return;
         }
-        detail::store_elem<T>(a_.agent, a_.agent_mul, name, idx_, 1, 0, value, a_.d_error,
-                                                      detail::DERR_UNKNOWN_AGENT_VAR, detail::DERR_AGENT_VAR_TYPE);
+        detail::store_var<T>(c_.a.agent, c_.a.agent_mul, name, c_.idx, value, c_, detail::DERR_UNKNOWN_AGENT_VAR,
+                             detail::DERR_AGENT_VAR_TYPE);
     }
     template <typename T, size_type L, unsigned N>
     __device__ __forceinline__ void setVariable(const char (&name)[N], unsigned int e, T value) const {
         if (name[0] == '_') {
-            detail::report_error(a_.d_error, detail::DERR_RESERVED_NAME, 0, idx_);
+            c_.err.note(true, detail::DERR_RESERVED_NAME, 0);
             return;
         }
-        detail::store_elem<T>(a_.agent, a_.agent_mul, name, idx_, L, e, value, a_.d_error,
-                                                      detail::DERR_UNKNOWN_AGENT_VAR, detail::DERR_AGENT_VAR_TYPE);
+        detail::store_elem<T>(c_.a.agent, c_.a.agent_mul, name, c_.idx, L, e, value, c_, detail::DERR_UNKNOWN_AGENT_VAR,
+                              detail::DERR_AGENT_VAR_TYPE);
     }
-    __device__ __forceinline__ id_t getID() const { return a_.ids ? a_.ids[idx_] : ID_NOT_SET; }
-    __device__ __forceinline__ unsigned int getStepCounter() const { return a_.d_step ? *a_.d_step : 0u; }
-    __device__ __forceinline__ unsigned int getThreadIndex() const { return idx_; }
+    __device__ __forceinline__ id_t getID() const { return c_.a.ids ? c_.a.ids[c_.idx] : ID_NOT_SET; }
+    __device__ __forceinline__ unsigned int getStepCounter() const { return c_.d_step ? *c_.d_step : 0u; }
+    __device__ __forceinline__ unsigned int getThreadIndex() const { return c_.idx; }
 
     typename MsgIn::In message_in;
     typename MsgOut::Out message_out;
@@ -696,8 +825,7 @@ class DeviceAPI {
     DeviceEnvironment environment;
 
  private:
-    const detail::FnArgs& a_;
-    uint32_t idx_;
+    const detail::ThreadCtx& c_;
 };
 
 // ------------------------------------------------------------------ the agent-function kernels
@@ -705,19 +833,53 @@ class DeviceAPI {
 // state list; the count is read from device memory so launches can be replayed
 // from a captured graph while populations change.
 namespace detail {
+
+template <typename F>
+__device__ __forceinline__ void run_agent(const ThreadCtx& c) {
+    const FnArgs& a = c.a;
+    const uint32_t i = c.idx;
+    if (a.msg_out_flags) a.msg_out_flags[i] = a.msg_out_optional ? 0u : 1u;
+    if (a.birth_flags) a.birth_flags[i] = 0u;
+    DeviceAPI<typename F::MsgIn, typename F::MsgOut> api(c);
+    const AGENT_STATUS s = F()(&api);
+    if (a.death_flags)
+        a.death_flags[i] = (uint32_t)s;
+    else
+        c.err.note(s == DEAD, DERR_DEATH_DISABLED, 0);
+}
+
 template <typename F>
 __global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
     const uint32_t n = a.d_n ? load_count(a.d_n) : a.n_cap;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (a.msg_out_flags) a.msg_out_flags[i] = a.msg_out_optional ? 0u : 1u;
-    if (a.birth_flags) a.birth_flags[i] = 0u;
-    DeviceAPI<typename F::MsgIn, typename F::MsgOut> api(a, i);
-    const AGENT_STATUS s = F()(&api);
-    if (a.death_flags)
-        a.death_flags[i] = (uint32_t)s;
-    else if (s == DEAD)
-        report_error(a.d_error, DERR_DEATH_DISABLED, 0, i);
+    const ThreadCtx c{a, ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step, a.seed, nullptr};
+    run_agent<F>(c);
+}
+
+// Batched launch: the simulations of a batch share one launch; block b belongs to
+// the simulation s with segs[s].tile_off <= b < segs[s+1].tile_off (tiles of
+// blockDim agents) and runs with that simulation's metadata, environment, step,
+// seed and error words.  A grid smaller than the tile count loops over tiles.
+template <typename F>
+__global__ void __launch_bounds__(1024) agent_function_kernel_seg(const FnArgs a, const SegDev* __restrict__ segs,
+                                                                  uint32_t nsegs, uint32_t ntiles) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint32_t lo = 0, hi = nsegs;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (segs[mid].tile_off <= t) lo = mid;
+            else hi = mid;
+        }
+        const SegDev& s = segs[lo];
+        const uint32_t local = (t - s.tile_off) * blockDim.x + threadIdx.x;
+        if (local < s.n) {
+            const uint32_t i = s.agent_off + local;
+            const ThreadCtx c{a, ErrState{s.d_error, i}, a.zero_page, a.sink_page, i, s.meta, s.env, s.d_step, s.seed,
+                              nullptr};
+            run_agent<F>(c);
+        }
+    }
 }
 
 // Stages, for the block's 256 bin-sorted agents, every message of the bins their
@@ -788,7 +950,6 @@ __device__ __forceinline__ bool stage_block(const FnArgs& a, uint32_t* smem, Sta
     if (!sh.ok) return false;
     const uint32_t E = sh.nentries;
     // bin table: entry e = (group, neighbour row, virtual x); all loads issued before use
-    uint32_t local = 0;
     uint32_t cnt[3], gs[3];
 #pragma unroll
     for (uint32_t q = 0; q < 3; ++q) {
@@ -872,7 +1033,8 @@ __device__ __forceinline__ bool stage_block(const FnArgs& a, uint32_t* smem, Sta
     return true;
 }
 
-// Spatial-input agent functions: neighbourhood staged in LDS, then the user function.
+// Spatial-input agent functions with the neighbourhood staged in LDS, then the user
+// function (waves whose lanes are all covered read LDS, the others HBM).
 template <typename F>
 __global__ void __launch_bounds__(256) agent_function_kernel_staged(const FnArgs a) {
     extern __shared__ uint32_t fgx_stage_smem[];
@@ -882,29 +1044,34 @@ __global__ void __launch_bounds__(256) agent_function_kernel_staged(const FnArgs
     if (b0 >= n) return;
     const uint32_t i = b0 + threadIdx.x;
     const bool active = i < n;
-    bool staged = false;
-    if (a.stage_ok) staged = stage_block<F::MsgIn::In::dims>(a, fgx_stage_smem, sh, i, active);
+    const bool staged = stage_block<F::MsgIn::In::dims>(a, fgx_stage_smem, sh, i, active);
     if (!active) return;
-    if (a.msg_out_flags) a.msg_out_flags[i] = a.msg_out_optional ? 0u : 1u;
-    if (a.birth_flags) a.birth_flags[i] = 0u;
-    DeviceAPI<typename F::MsgIn, typename F::MsgOut> api(a, i);
-    if (staged) api.message_in.enable_stage(fgx_stage_smem, &sh);
-    const AGENT_STATUS s = F()(&api);
-    if (a.death_flags)
-        a.death_flags[i] = (uint32_t)s;
-    else if (s == DEAD)
-        report_error(a.d_error, DERR_DEATH_DISABLED, 0, i);
+    const uint32_t maxe = a.stage_max_entries;
+    const StageView sv{fgx_stage_smem, fgx_stage_smem + maxe + 1, fgx_stage_smem + 2 * (maxe + 1), a.stage_cap,
+                       a.stage_nvars > (uint32_t)F::MsgIn::In::dims ? 1u : 0u, &sh};
+    const ThreadCtx c{a,         ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step,
+                      a.seed,    staged ? &sv : nullptr};
+    run_agent<F>(c);
 }
 
 template <typename F>
 void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, uint32_t smem, void* stream) {
     constexpr MsgKind k = F::MsgIn::kind;
     if constexpr (k == MsgKind::Spatial2D || k == MsgKind::Spatial3D) {
-        hipLaunchKernelGGL(agent_function_kernel_staged<F>, dim3(grid), dim3(block), smem, (hipStream_t)stream, a);
-    } else {
-        (void)smem;
-        hipLaunchKernelGGL(agent_function_kernel<F>, dim3(grid), dim3(block), 0, (hipStream_t)stream, a);
+        if (a.stage_ok) {
+            hipLaunchKernelGGL(agent_function_kernel_staged<F>, dim3(grid), dim3(block), smem, (hipStream_t)stream, a);
+            return;
+        }
     }
+    (void)smem;
+    hipLaunchKernelGGL(agent_function_kernel<F>, dim3(grid), dim3(block), 0, (hipStream_t)stream, a);
+}
+
+template <typename F>
+void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t grid,
+                               uint32_t block, void* stream) {
+    hipLaunchKernelGGL(agent_function_kernel_seg<F>, dim3(grid), dim3(block), 0, (hipStream_t)stream, a, segs, nsegs,
+                       ntiles);
 }
 }  // namespace detail
 

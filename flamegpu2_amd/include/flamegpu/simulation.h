@@ -647,6 +647,7 @@ class CUDASimulation {
     uint32_t small_used_ = 8;
     uint32_t* h_small_ = nullptr;    // pinned mirror
     void* d_reduce_out_ = nullptr;
+    void* d_pages_ = nullptr;        // [0, 128): zero page, [128, 256): sink page
     void* reduce_scratch_ = nullptr;
     size_t reduce_scratch_bytes_ = 0;
     unsigned int step_ = 0;
@@ -696,6 +697,8 @@ inline void CUDASimulation::ensure_init() {
     d_step_ = d_small_ + 3;
     d_dummy_ = d_small_ + 4;
     alloc(&d_reduce_out_, 64);
+    alloc(&d_pages_, 256);
+    detail::fgx_check(fgx_fill_u32((uint32_t*)d_pages_, 0, 64, stream_), "zero page");
     auto counter = [&]() {
         if (small_used_ >= 4096) throw exception::InvalidOperation("too many agent states and message lists");
         return d_small_ + small_used_++;
@@ -839,6 +842,7 @@ inline void CUDASimulation::release() {
     p = d_small_;
     free_dev(p);
     free_dev(d_reduce_out_);
+    free_dev(d_pages_);
     free_dev(reduce_scratch_);
     if (h_small_) fgx_host_free(h_small_);
     fgx_stream_destroy(stream_);
@@ -1151,6 +1155,8 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
     detail::FnArgs a;
     std::memset(&a, 0, sizeof(a));
     a.n_cap = l.cap;
+    a.zero_page = (const char*)d_pages_;
+    a.sink_page = (char*)d_pages_ + 128;
     a.stream_id = f.d->stream_id;
     a.d_n = l.d_count;
     a.d_step = d_step_;
@@ -1164,7 +1170,7 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
         *mul = detail::perfect_hash_mul(hs.data(), (unsigned)hs.size(), detail::VAR_SLOT_BITS);
         for (size_t i = 0; i < vars.size(); ++i) {
             const uint32_t s = detail::table_slot(hs[i], *mul, detail::VAR_SLOT_BITS);
-            tab[s] = {hs[i], (uint16_t)vars[i].bytes, detail::NOT_STAGED, (char*)(use_alt ? vars[i].alt : vars[i].cur)};
+            tab[s] = {hs[i], (uint16_t)vars[i].bytes, 0, (char*)(use_alt ? vars[i].alt : vars[i].cur)};
         }
     };
     fill_table(a.agent, &a.agent_mul, l.vars, false);
@@ -1245,27 +1251,20 @@ inline void CUDASimulation::stage_neighbourhood(detail::FnRT& f, detail::StateLi
         if (var.type != std::type_index(typeid(float)) || var.elements != 1) return;
         a.stage_pos[k] = (const float*)l.vars[vi].cur;
     }
+    // staged: the message position, then "id" when it is a 4-byte scalar (the variables
+    // the iterator reads ahead of the loop body)
     std::vector<int> pick;
     for (int k = 0; k < dims; ++k) pick.push_back(f.min->var_index(axes[k]));
-    auto eligible = [&](int vi) {
-        if (vi < 0 || std::find(pick.begin(), pick.end(), vi) != pick.end()) return false;
-        const auto& var = f.min->d->variables.at(f.min->vars[vi].name);
-        return var.elements == 1 && var.type_size == 4;
-    };
-    if (eligible(f.min->var_index("id"))) pick.push_back(f.min->var_index("id"));
-    for (int vi = 0; vi < (int)f.min->vars.size() && pick.size() < detail::STAGE_MAX_VARS; ++vi)
-        if (eligible(vi)) pick.push_back(vi);
+    const int vid = f.min->var_index("id");
+    if (vid >= 0) {
+        const auto& var = f.min->d->variables.at("id");
+        if (var.elements == 1 && var.type_size == 4) pick.push_back(vid);
+    }
     const uint32_t budget_words = 13312;   // 52 KB
     a.stage_max_entries = detail::STAGE_MAX_ENTRIES;
     a.stage_nvars = (uint32_t)pick.size();
     a.stage_cap = (budget_words - 2u * (a.stage_max_entries + 1u)) / a.stage_nvars;
-    for (size_t k = 0; k < pick.size(); ++k) {
-        const auto& v = f.min->vars[pick[k]];
-        a.stage_src[k] = (const uint32_t*)v.cur;
-        const uint32_t h = detail::name_hash(v.name);
-        detail::VarBinding& b = a.msg_in[detail::table_slot(h, a.msg_in_mul, detail::VAR_SLOT_BITS)];
-        b.lds = (uint16_t)k;
-    }
+    for (size_t k = 0; k < pick.size(); ++k) a.stage_src[k] = (const uint32_t*)f.min->vars[pick[k]].cur;
     a.stage_ok = 1;
 }
 

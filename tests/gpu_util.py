@@ -1,37 +1,72 @@
-"""Helpers for GPU tests: torch provides device memory; the library does the work."""
+"""Helpers for GPU tests: device buffers through libfgx's own allocator and copies.
+
+Everything runs on the one HIP runtime libfgx links (torch ships a second one;
+two runtimes in one process can leave the device to whichever starts first).
+"""
+import ctypes
+
 import numpy as np
 
-_torch = None
+from flamegpu2_amd import fgx
+
+_ctx = None
 
 
-def torch():
-    global _torch
-    if _torch is None:
-        import torch as t
-        assert t.cuda.is_available(), "GPU tests need a GPU"
-        _torch = t
-    return _torch
+def ctx():
+    global _ctx
+    if _ctx is None:
+        _ctx = fgx.Context(0)
+    return _ctx
+
+
+class DeviceArray:
+    """A device allocation holding `n` elements of `dtype`."""
+
+    def __init__(self, n, dtype):
+        self.dtype = np.dtype(dtype)
+        self.n = max(int(n), 1)
+        self.nbytes = self.n * self.dtype.itemsize
+        self.alloc_bytes = (self.nbytes + 3) // 4 * 4
+        p = ctypes.c_void_p()
+        fgx.check(fgx.lib().fgx_malloc(ctx().h, self.alloc_bytes, ctypes.byref(p)))
+        self.addr = p.value
+
+    def data_ptr(self):
+        return self.addr
+
+    def numel(self):
+        return self.n
+
+    def __del__(self):
+        try:
+            if self.addr:
+                fgx.lib().fgx_free(ctx().h, ctypes.c_void_p(self.addr))
+                self.addr = None
+        except Exception:
+            pass
 
 
 def dev(a):
-    t = torch()
     a = np.ascontiguousarray(a)
-    if a.dtype == np.uint32:
-        return t.from_numpy(a.view(np.int32)).cuda()
-    if a.dtype == np.uint64:
-        return t.from_numpy(a.view(np.int64)).cuda()
-    return t.from_numpy(a).cuda()
+    d = DeviceArray(len(a), a.dtype)
+    if a.size:
+        fgx.check(fgx.lib().fgx_memcpy_h2d(ctypes.c_void_p(d.addr), a.ctypes.data_as(ctypes.c_void_p), a.nbytes, None))
+    sync()
+    return d
 
 
 def empty(n, dtype):
-    t = torch()
-    tmap = {np.float32: t.float32, np.float64: t.float64, np.uint32: t.int32, np.int32: t.int32,
-            np.uint8: t.uint8, np.uint64: t.int64, np.int64: t.int64}
-    return t.zeros(max(n, 1), dtype=tmap[np.dtype(dtype).type], device="cuda")
+    d = DeviceArray(n, dtype)
+    fgx.check(fgx.lib().fgx_fill_u32(ctypes.c_void_p(d.addr), 0, d.alloc_bytes // 4, None))
+    sync()
+    return d
 
 
 def host(t, dtype=None, n=None):
-    a = t.detach().cpu().numpy()
+    sync()
+    a = np.empty(t.n, dtype=t.dtype)
+    fgx.check(fgx.lib().fgx_memcpy_d2h(a.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(t.addr), t.nbytes, None))
+    sync()
     if dtype is not None:
         a = a.view(dtype)
     if n is not None:
@@ -40,15 +75,15 @@ def host(t, dtype=None, n=None):
 
 
 def ptr(t):
-    return t.data_ptr()
+    return t.addr
 
 
 def sync():
-    torch().cuda.synchronize()
+    fgx.check(fgx.lib().fgx_device_synchronize())
 
 
 def scratch(nbytes):
-    from flamegpu2_amd import fgx
-    s = torch().zeros(max(nbytes // 4, 64), dtype=torch().int32, device="cuda")
-    fgx.scratch_clear(s.data_ptr(), s.numel() * 4)
+    s = DeviceArray(max(nbytes // 4, 64), np.uint32)
+    fgx.scratch_clear(s.addr, s.nbytes)
+    sync()
     return s
