@@ -65,6 +65,95 @@ def cpu_baseline(params, arrs, steps):
                       f"{dt:.1f} s"}
 
 
+def ensemble_plans(count=256, seed=2024, n_lo=1024, n_hi=262144, r_lo=1.0, r_hi=3.0):
+    """The batched-ensemble workload: circles simulations with log-uniform populations,
+    uniform radius, density 1 (width cbrt(N)), one seed each."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        r = float(rng.uniform(r_lo, r_hi))
+        out.append((n, r, 1000 + k))
+    return out
+
+
+def bench_batched(args):
+    """Batched ensemble vs the same simulations run one after another (plain launches)."""
+    from flamegpu2_amd import autotune, workloads
+    from flamegpu2_amd.models import Batch, Simulation
+    plans = ensemble_plans(args.ensemble)
+    steps = args.ensemble_steps
+    made = [workloads.circles(n, seed=s, radius=r) for (n, r, s) in plans]
+    agents = sum(n for (n, _, _) in plans)
+    total = float(agents) * steps
+    # batched: all simulations in one batch; the tuner's trial steps are part of the run
+    b = Batch("circles_spatial3D", made[0][0])
+    hs = [b.add(p, s, steps, {"Circle": a}) for (p, a), (_, _, s) in zip(made, plans)]
+    t0 = time.perf_counter()
+    b.step(1)
+    b.sync()
+    tuned, timings = None, {}
+    done = 1
+    if args.autotune and steps - done > autotune.steps_needed(2) + 2:
+        tuned, timings = autotune.autotune(b, steps=2)
+        done += autotune.steps_needed(2)
+    b.step(steps - done)
+    for h in hs:
+        b.get_agents(h, "Circle")
+    wall_b = time.perf_counter() - t0
+    print(f"bench: batched {wall_b:.2f} s", file=sys.stderr, flush=True)
+    # steady-state step time of the tuned batch (separate short run, same simulations)
+    b2 = Batch("circles_spatial3D", made[0][0])
+    if tuned:
+        b2.set_tuning(*tuned)
+    for (p, a), (_, _, s) in zip(made, plans):
+        b2.add(p, s, 8, {"Circle": a})
+    ms_step_b = b2.timed_steps(4) / 4
+    b2.close()
+    b.close()
+    # plain: each simulation with its own launches, one after another
+    t0 = time.perf_counter()
+    gpu_ms = 0.0
+    for (p, a), (_, _, s) in zip(made, plans):
+        sim = Simulation("circles_spatial3D", p, seed=s)
+        sim.set_agents("Circle", a)
+        gpu_ms += sim.timed_steps(steps)
+        sim.get_agents("Circle")
+        sim.close()
+    wall_p = time.perf_counter() - t0
+    print(f"bench: plain {wall_p:.2f} s", file=sys.stderr, flush=True)
+    return {
+        "workload": f"{len(plans)} circles simulations, N log-uniform [1024, 262144] (seeded), r ~ U[1, 3], "
+                    f"density 1, {steps} steps each",
+        "agents": agents,
+        "batched_agent_steps_per_s": total / wall_b,
+        "plain_agent_steps_per_s": total / wall_p,
+        "speedup": wall_p / wall_b,
+        "plain_stepping_only_agent_steps_per_s": total / (gpu_ms / 1e3),
+        "speedup_vs_plain_stepping_only": (gpu_ms / 1e3) / wall_b,
+        "batched_wall_s": wall_b,
+        "plain_wall_s": wall_p,
+        "batched_ms_per_step_steady": ms_step_b,
+        "tuning": {"block": tuned[0], "agents_per_thread": tuned[1], "blocks_per_cu": tuned[2]} if tuned else None,
+        "tuning_ms_per_step": {f"{k[0]}x{k[1]}/{k[2]}": round(v, 4) for k, v in timings.items()},
+    }
+
+
+def bench_serving(args):
+    from flamegpu2_amd import serving
+    reqs = serving.poisson_requests(args.serve_rate, args.serve_seconds, seed=99)
+    srv = serving.Server(max_chunk=16)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, progress=lambda t: print("bench: " + t, file=sys.stderr, flush=True))
+    wall = time.perf_counter() - t0
+    out = serving.summarize(res, wall)
+    out["host_phase_s"] = {k: round(v, 3) for k, v in srv.phase.items()}
+    out["workload"] = (f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s, circles/boids 50/50, "
+                       f"N log-uniform [2^10, 2^20], steps U[10, 1000], random seeds; continuous batching")
+    out["offered_agent_steps_per_s"] = sum(r.n * r.steps for r in reqs) / args.serve_seconds
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +163,13 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--no-batched", dest="batched", action="store_false")
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--no-serving", dest="serving", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (single-GPU build).", file=sys.stderr)
@@ -154,7 +250,15 @@ def main():
         },
         "kernels_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(kt.items())},
     }
+    sim.close()
+    if args.batched:
+        print("bench: batched ensemble", file=sys.stderr, flush=True)
+        out["batched"] = bench_batched(args)
+    if args.serving:
+        print("bench: serving", file=sys.stderr, flush=True)
+        out["serving"] = bench_serving(args)
     if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(params, arrs, args.cpu_steps)
     print(json.dumps(out))
 

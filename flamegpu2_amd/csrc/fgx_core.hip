@@ -66,6 +66,30 @@ __global__ void k_add_u32(uint32_t* p, uint32_t delta) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p += delta;
 }
 
+__global__ void k_add_u32_n(uint32_t* __restrict__ p, uint32_t n, uint32_t delta) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] += delta;
+}
+
+// One thread per destination element d = dst_off[0] + j: its range is the last one
+// with dst_off <= d (binary search; destination ranges are contiguous and sorted),
+// then every variable's element is copied.
+__global__ void k_copy_ranges(VarTable vt, const uint32_t* __restrict__ ranges, uint32_t nranges, uint32_t n_total) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t base = ranges[1];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_total; j += stride) {
+        const uint32_t d = base + j;
+        uint32_t lo = 0, hi = nranges;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (ranges[3 * mid + 1] <= d) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t k = d - ranges[3 * lo + 1];
+        if (k < ranges[3 * lo + 2]) copy_element(vt, ranges[3 * lo] + k, d);
+    }
+}
+
 __global__ void k_assign_ids(uint32_t* __restrict__ ids, uint32_t n, uint32_t next_id) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && ids[i] == 0u) ids[i] = next_id + i;
@@ -283,6 +307,86 @@ int fgx_graph_destroy(fgx_graph* g) {
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
     delete g;
+    return FGX_OK;
+}
+
+int fgx_add_u32_n(uint32_t* p, uint32_t n, uint32_t delta, void* stream) {
+    if (!n) return FGX_OK;
+    if (!p) return fail(FGX_ERR_ARG, "fgx_add_u32_n: NULL pointer");
+    hipLaunchKernelGGL(k_add_u32_n, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, p, n, delta);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+int fgx_copy_ranges(uint32_t nvars, const void* const* src, void* const* dst, const uint32_t* var_bytes,
+                    const uint32_t* d_ranges, uint32_t nranges, uint32_t n_total, void* stream) {
+    if (!n_total || !nranges) return FGX_OK;
+    if (!d_ranges) return fail(FGX_ERR_ARG, "fgx_copy_ranges: NULL ranges");
+    VarTable vt;
+    int rc = make_var_table(nvars, src, dst, var_bytes, &vt);
+    if (rc) return rc;
+    const uint32_t blocks = std::min<uint32_t>(ceil_div(n_total, 256), 8192);
+    hipLaunchKernelGGL(k_copy_ranges, dim3(blocks), dim3(256), 0, (hipStream_t)stream, vt, d_ranges, nranges, n_total);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+static_assert(sizeof(fgx_seg_key) == 80, "fgx_seg_key layout is part of the C ABI");
+
+int fgx_seg_key_init(int kind, int dims, const float* mn, const float* mx, float radius, fgx_seg_key* out) {
+    if (!out || !mn || !mx) return fail(FGX_ERR_ARG, "fgx_seg_key_init: NULL argument");
+    if (kind != FGX_SEG_MESSAGE_BINS && kind != FGX_SEG_AGENT_SORT) return fail(FGX_ERR_ARG, "fgx_seg_key_init: unknown kind");
+    if (dims != 2 && dims != 3) return fail(FGX_ERR_ARG, "fgx_seg_key_init: dims must be 2 or 3");
+    out->dims = dims;
+    out->radius = radius;
+    if (kind == FGX_SEG_MESSAGE_BINS) {
+        fgx_spatial_meta m;
+        int rc = fgx_spatial_meta_init(dims, mn, mx, radius, &m);
+        if (rc) return rc;
+        for (int a = 0; a < 3; ++a) {
+            out->min[a] = m.min[a];
+            out->width[a] = m.width[a];
+            out->grid[a] = m.grid[a] ? m.grid[a] : 1u;
+        }
+        out->keys = m.bins;
+        out->key_mask = 0xFFFFFFFFu;
+    } else {
+        uint32_t bits = 0;
+        int rc = fgx_agent_sort_key_bits(dims, mn, mx, radius, &bits);
+        if (rc) return rc;
+        if (bits > 30) return fail(FGX_ERR_ARG, "fgx_seg_key_init: key range too large");
+        for (int a = 0; a < 3; ++a) {
+            if (a < dims) {
+                out->min[a] = mn[a];
+                out->width[a] = mx[a] - mn[a];
+                out->grid[a] = out->width[a] != 0.0f ? (uint32_t)std::ceil(out->width[a] / radius) : 1u;
+            } else {
+                out->min[a] = 0.0f;
+                out->width[a] = 1.0f;
+                out->grid[a] = 1u;
+            }
+        }
+        out->keys = 1u << bits;
+        out->key_mask = out->keys - 1u;
+    }
+    return FGX_OK;
+}
+
+int fgx_segments_pack(fgx_seg_key* segs, uint32_t nsegs, uint32_t* n_total, uint32_t* keys_total, uint32_t* tiles_total) {
+    if (nsegs && !segs) return fail(FGX_ERR_ARG, "fgx_segments_pack: NULL segments");
+    uint64_t n = 0, k = 0, t = 0;
+    for (uint32_t s = 0; s < nsegs; ++s) {
+        segs[s].elem_off = (uint32_t)n;
+        segs[s].key_off = (uint32_t)k;
+        segs[s].tile_off = (uint32_t)t;
+        n += segs[s].n;
+        k += segs[s].keys;
+        t += ceil_div(segs[s].n, 2048u);
+        if (n >= (1ull << 31) || k >= (1ull << 31)) return fail(FGX_ERR_ARG, "fgx_segments_pack: batch too large");
+    }
+    if (n_total) *n_total = (uint32_t)n;
+    if (keys_total) *keys_total = (uint32_t)k;
+    if (tiles_total) *tiles_total = (uint32_t)t;
     return FGX_OK;
 }
 

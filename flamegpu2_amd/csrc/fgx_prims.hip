@@ -100,6 +100,61 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist(KeyArgs ka, const uint32_t*
     }
 }
 
+// Segmented variant: block t is tile t of the packed batch; its simulation s (last
+// with tile_off <= t, a scalar binary search) gives the geometry and the key offset.
+// Tiles never straddle simulations, so run aggregation stays within one simulation.
+__global__ void __launch_bounds__(kBlock) k_bin_hist_seg(int32_t mode, const fgx_seg_key* __restrict__ segs,
+                                                         uint32_t nsegs, const float* __restrict__ x,
+                                                         const float* __restrict__ y, const float* __restrict__ z,
+                                                         uint32_t* __restrict__ key, uint32_t* __restrict__ sub,
+                                                         uint32_t* __restrict__ hist) {
+    const uint32_t t = blockIdx.x;
+    uint32_t lo = 0, hi = nsegs;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (segs[mid].tile_off <= t) lo = mid;
+        else hi = mid;
+    }
+    const fgx_seg_key& sg = segs[lo];
+    KeyArgs ka;
+    ka.mode = mode;
+    ka.dims = sg.dims;
+    ka.radius = sg.radius;
+    ka.key_mask = sg.key_mask;
+    for (int a = 0; a < 3; ++a) {
+        ka.min[a] = sg.min[a];
+        ka.width[a] = sg.width[a];
+        ka.grid[a] = sg.grid[a];
+    }
+    const uint32_t first = sg.elem_off + (t - sg.tile_off) * kTile;
+    const uint32_t end = sg.elem_off + sg.n;
+    const uint32_t koff = sg.key_off;
+    const uint32_t lane = lane_id();
+    for (uint32_t c = threadIdx.x >> 6; c < kTile / 64u; c += kBlock / 64u) {
+        const uint32_t i = first + c * 64u + lane;
+        const bool valid = i < end;
+        if (__ballot(valid) == 0ull) break;
+        uint32_t k = 0xFFFFFFFFu;
+        if (valid) k = koff + bin_key(ka, x[i], y[i], ka.dims == 3 ? z[i] : 0.0f);
+        const uint32_t prev = __shfl_up(k, 1, 64);
+        const bool head = valid && (lane == 0 || k != prev);
+        const uint64_t hm = __ballot(head);
+        const uint64_t vm = __ballot(valid);
+        const uint32_t nvalid = (uint32_t)__builtin_popcountll(vm);
+        const uint64_t above = lane == 63 ? 0ull : (hm & (~0ull << (lane + 1)));
+        const uint32_t run_end = above ? (uint32_t)__builtin_ctzll(above) : nvalid;
+        uint32_t b = 0;
+        if (head) b = atomicAdd(&hist[k], run_end - lane);
+        const uint64_t below = hm & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        const uint32_t hl = below ? 63u - (uint32_t)__builtin_clzll(below) : 0u;
+        b = __shfl(b, (int)hl, 64);
+        if (valid) {
+            key[i] = k;
+            sub[i] = b + (lane - hl);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_index_scatter(const uint32_t* d_n, uint32_t n_cap,
                                                           const uint32_t* __restrict__ key, const uint32_t* __restrict__ sub,
                                                           const uint32_t* __restrict__ start, uint32_t* __restrict__ tmp) {
@@ -364,7 +419,74 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
     return FGX_OK;
 }
 
+// Segmented: the keys come from k_bin_hist_seg; scan and scatters run over the packed batch.
+static int run_bin_sort_seg(fgx_ctx* ctx, int32_t mode, const fgx_seg_key* d_segs, uint32_t nsegs, uint32_t tiles,
+                            uint32_t n_total, uint32_t keys_total, const float* x, const float* y, const float* z,
+                            const VarTable& vt, uint32_t* start_out, void* scratch, size_t scratch_bytes, void* stream) {
+    const BuildLayout L = build_layout(n_total, keys_total);
+    if (!scratch || scratch_bytes < L.total) return fail(FGX_ERR_CAPACITY, "scratch too small for segmented bin sort");
+    char* s = static_cast<char*>(scratch);
+    auto* hist = reinterpret_cast<uint32_t*>(s + L.hist);
+    auto* key = reinterpret_cast<uint32_t*>(s + L.key);
+    auto* sub = reinterpret_cast<uint32_t*>(s + L.sub);
+    auto* tmp = reinterpret_cast<uint32_t*>(s + L.tmp);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = fgx_fill_u32(hist, 0u, (size_t)keys_total + 1, stream);
+    if (rc) return rc;
+    if (n_total && tiles) {
+        hipLaunchKernelGGL(k_bin_hist_seg, dim3(tiles), dim3(kBlock), 0, st, mode, d_segs, nsegs, x, y, z, key, sub, hist);
+        FGX_HIP(hipGetLastError());
+    }
+    {
+        auto* hdr = reinterpret_cast<LookbackHeader*>(s + L.hdr);
+        auto* status = reinterpret_cast<uint64_t*>(s + L.status);
+        const uint32_t st_tiles = ceil_div(keys_total + 1, kTile);
+        hipLaunchKernelGGL(k_scan_excl, dim3(st_tiles), dim3(kBlock), 0, st, hist, start_out, keys_total + 1, hdr, status,
+                           st_tiles);
+        FGX_HIP(hipGetLastError());
+    }
+    if (n_total) {
+        const uint32_t g = stream_grid(ctx ? &ctx->c : nullptr, n_total);
+        hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, nullptr, n_total, key, sub, start_out, tmp);
+        FGX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, nullptr, n_total, key, start_out, tmp, vt);
+        FGX_HIP(hipGetLastError());
+    }
+    return FGX_OK;
+}
+
 extern "C" {
+
+int fgx_spatial_build_seg(fgx_ctx* ctx, const fgx_seg_key* d_segs, uint32_t nsegs, uint32_t tiles_total,
+                          uint32_t n_total, uint32_t keys_total, const float* x, const float* y, const float* z,
+                          uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                          uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream) {
+    if (!pbm) return fail(FGX_ERR_ARG, "fgx_spatial_build_seg: NULL pbm");
+    if (nsegs && !d_segs) return fail(FGX_ERR_ARG, "fgx_spatial_build_seg: NULL segment table");
+    if (n_total && (!x || !y)) return fail(FGX_ERR_ARG, "fgx_spatial_build_seg: missing location array");
+    VarTable vt;
+    int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
+    if (rc) return rc;
+    return run_bin_sort_seg(ctx, 0, d_segs, nsegs, tiles_total, n_total, keys_total, x, y, z, vt, pbm, scratch,
+                            scratch_bytes, stream);
+}
+
+int fgx_agent_sort_seg(fgx_ctx* ctx, const fgx_seg_key* d_segs, uint32_t nsegs, uint32_t tiles_total,
+                       uint32_t n_total, uint32_t keys_total, const float* x, const float* y, const float* z,
+                       uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                       void* scratch, size_t scratch_bytes, void* stream) {
+    if (nsegs && !d_segs) return fail(FGX_ERR_ARG, "fgx_agent_sort_seg: NULL segment table");
+    if (n_total && (!x || !y)) return fail(FGX_ERR_ARG, "fgx_agent_sort_seg: missing location array");
+    VarTable vt;
+    int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
+    if (rc) return rc;
+    const BuildLayout L = build_layout(n_total, keys_total);
+    if (!scratch || scratch_bytes < L.total + align256((size_t)(keys_total + 1) * 4))
+        return fail(FGX_ERR_CAPACITY, "fgx_agent_sort_seg: scratch too small");
+    auto* start = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) + L.total);
+    return run_bin_sort_seg(ctx, 1, d_segs, nsegs, tiles_total, n_total, keys_total, x, y, z, vt, start, scratch,
+                            L.total, stream);
+}
 
 int fgx_scratch_bytes(int op, uint32_t n, uint32_t bins, size_t* bytes) {
     if (!bytes) return fail(FGX_ERR_ARG, "fgx_scratch_bytes: bytes is NULL");

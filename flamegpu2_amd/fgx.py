@@ -34,6 +34,15 @@ class SpatialMeta(_c.Structure):
                 ("width", _c.c_float * 3), ("grid", u32 * 3), ("bins", u32), ("wrap_ok", i32)]
 
 
+class SegKey(_c.Structure):
+    _fields_ = [("elem_off", u32), ("n", u32), ("key_off", u32), ("keys", u32), ("tile_off", u32), ("dims", i32),
+                ("min", _c.c_float * 3), ("radius", _c.c_float), ("width", _c.c_float * 3), ("grid", u32 * 3),
+                ("key_mask", u32), ("pad", u32 * 3)]
+
+
+FGX_SEG_MESSAGE_BINS = 0
+FGX_SEG_AGENT_SORT = 1
+
 _SIGS = {
     "fgx_ctx_create": [_c.c_int, _c.POINTER(vp)],
     "fgx_ctx_destroy": [vp],
@@ -62,6 +71,15 @@ _SIGS = {
     "fgx_exclusive_scan_u32": [vp, vp, vp, u32, vp, sz, vp],
     "fgx_assign_ids": [vp, u32, u32, vp],
     "fgx_reduce_sum": [vp, _c.c_int, vp, vp, u32, vp, vp, sz, vp],
+    "fgx_seg_key_init": [_c.c_int, _c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float,
+                         _c.POINTER(SegKey)],
+    "fgx_segments_pack": [_c.POINTER(SegKey), u32, _c.POINTER(u32), _c.POINTER(u32), _c.POINTER(u32)],
+    "fgx_spatial_build_seg": [vp, vp, u32, u32, u32, u32, vp, vp, vp, u32, _c.POINTER(vp), _c.POINTER(vp),
+                              _c.POINTER(u32), vp, vp, sz, vp],
+    "fgx_agent_sort_seg": [vp, vp, u32, u32, u32, u32, vp, vp, vp, u32, _c.POINTER(vp), _c.POINTER(vp),
+                           _c.POINTER(u32), vp, sz, vp],
+    "fgx_add_u32_n": [vp, u32, u32, vp],
+    "fgx_copy_ranges": [u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), vp, u32, u32, vp],
 }
 
 _lib = None
@@ -181,3 +199,43 @@ def scratch_clear(scratch, nbytes, stream=0):
 
 def reduce_sum(ctx, type_code, d_in, n_cap, d_out, scratch, scratch_len, stream=0, d_n=None):
     check(lib().fgx_reduce_sum(ctx.h, type_code, d_in, d_n, n_cap, d_out, scratch, scratch_len, stream))
+
+
+def seg_key(kind, dims, mn, mx, radius, n=0):
+    k = SegKey()
+    mn = list(mn) + [0.0] * (3 - len(mn))
+    mx = list(mx) + [0.0] * (3 - len(mx))
+    check(lib().fgx_seg_key_init(kind, dims, _arr(_c.c_float, mn), _arr(_c.c_float, mx), radius, _c.byref(k)))
+    k.n = n
+    return k
+
+
+def segments_pack(keys):
+    """Fills the offsets of a list of SegKey in place; returns (n_total, keys_total, tiles_total, ctypes array)."""
+    arr = (SegKey * max(len(keys), 1))(*keys)
+    n, kt, t = u32(), u32(), u32()
+    check(lib().fgx_segments_pack(arr, len(keys), _c.byref(n), _c.byref(kt), _c.byref(t)))
+    return n.value, kt.value, t.value, arr
+
+
+def spatial_build_seg(ctx, d_segs, nsegs, tiles, n_total, keys_total, x, y, z, vars_in, vars_out, var_bytes, pbm,
+                      scratch, scratch_len, stream=0):
+    nv = len(vars_in)
+    check(lib().fgx_spatial_build_seg(ctx.h, d_segs, nsegs, tiles, n_total, keys_total, x, y, z, nv, _arr(vp, vars_in),
+                                      _arr(vp, vars_out), _arr(u32, var_bytes), pbm, scratch, scratch_len, stream))
+
+
+def agent_sort_seg(ctx, d_segs, nsegs, tiles, n_total, keys_total, x, y, z, vars_in, vars_out, var_bytes, scratch,
+                   scratch_len, stream=0):
+    nv = len(vars_in)
+    check(lib().fgx_agent_sort_seg(ctx.h, d_segs, nsegs, tiles, n_total, keys_total, x, y, z, nv, _arr(vp, vars_in),
+                                   _arr(vp, vars_out), _arr(u32, var_bytes), scratch, scratch_len, stream))
+
+
+def add_u32_n(p, n, delta, stream=0):
+    check(lib().fgx_add_u32_n(p, n, delta, stream))
+
+
+def copy_ranges(src, dst, var_bytes, d_ranges, nranges, n_total, stream=0):
+    check(lib().fgx_copy_ranges(len(src), _arr(vp, src), _arr(vp, dst), _arr(u32, var_bytes), d_ranges, nranges, n_total,
+                                stream))

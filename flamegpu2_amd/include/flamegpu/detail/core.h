@@ -249,7 +249,9 @@ struct FnArgs {
 struct SegDev {
     uint32_t agent_off;
     uint32_t n;
-    uint32_t tile_off;           // first block of the simulation in this launch
+    uint32_t tile_off;           // first tile (blockDim x agents-per-thread agents) of the simulation
+    uint32_t msg_off;            // brute-force input: the simulation's messages [msg_off, msg_off + msg_n)
+    uint32_t msg_n;
     uint32_t pad0;
     uint64_t seed;
     const uint32_t* d_step;      // the simulation's step counter

@@ -5,3 +5,4 @@
 #include "flamegpu/model.h"
 #include "flamegpu/runtime/device.h"
 #include "flamegpu/simulation.h"
+#include "flamegpu/batch.h"

@@ -53,8 +53,8 @@ typedef CONDITION_RESULT (*FLAMEGPU_EXIT_CONDITION_POINTER)(HostAPI*);
 
 namespace detail {
 typedef void (*LaunchFn)(const FnArgs& args, uint32_t grid, uint32_t block, uint32_t smem, void* stream);
-typedef void (*LaunchSegFn)(const FnArgs& args, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t grid,
-                            uint32_t block, void* stream);
+typedef void (*LaunchSegFn)(const FnArgs& args, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t apt,
+                            uint32_t grid, uint32_t block, void* stream);
 template <typename T>
 Variable make_variable(const std::string& name, size_type elements, const T* defaults) {
     static_assert(std::is_arithmetic<T>::value, "agent/message variables must be arithmetic types");
@@ -339,8 +339,8 @@ namespace detail {
 template <typename F>
 void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, uint32_t smem, void* stream);
 template <typename F>
-void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t grid,
-                               uint32_t block, void* stream);
+void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t apt,
+                               uint32_t grid, uint32_t block, void* stream);
 }
 
 // ------------------------------------------------------------------ agents

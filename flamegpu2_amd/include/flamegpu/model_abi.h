@@ -2,7 +2,9 @@
 // Include once, after defining fgxm_build_model().
 #pragma once
 
+#include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -53,6 +55,32 @@ inline std::shared_ptr<flamegpu::AgentData> agent(fgxm_sim* s, const char* name)
 }
 }  // namespace fgxm_detail
 
+namespace fgxm_detail {
+inline void var_info(const std::shared_ptr<flamegpu::AgentData>& a, uint32_t v, const char** name, int32_t* type,
+                     uint32_t* elements, uint32_t* bytes) {
+    {
+        if (v >= a->declared.size()) throw flamegpu::exception::OutOfBoundsException("variable index out of range");
+        const auto& var = a->variables.at(a->declared[v]);
+        *name = var.name.c_str();
+        *bytes = (uint32_t)var.bytes();
+        *elements = var.elements;
+        const std::type_index t = var.type;
+        int32_t code = -1;
+        if (t == typeid(float)) code = 0;
+        else if (t == typeid(double)) code = 1;
+        else if (t == typeid(int32_t)) code = 2;
+        else if (t == typeid(uint32_t)) code = 3;
+        else if (t == typeid(int64_t) || t == typeid(long long)) code = 4;
+        else if (t == typeid(uint64_t) || t == typeid(unsigned long long)) code = 5;
+        else if (t == typeid(int8_t)) code = 6;
+        else if (t == typeid(uint8_t)) code = 7;
+        else if (t == typeid(int16_t)) code = 8;
+        else if (t == typeid(uint16_t)) code = 9;
+        *type = code;
+    }
+}
+}  // namespace fgxm_detail
+
 #define FGXM_EXPORT __attribute__((visibility("default")))
 
 extern "C" {
@@ -90,28 +118,9 @@ FGXM_EXPORT int fgxm_var_count(fgxm_sim* s, const char* agent, uint32_t* n) {
 
 FGXM_EXPORT int fgxm_var_info(fgxm_sim* s, const char* agent, uint32_t v, const char** name, int32_t* type, uint32_t* elements,
                   uint32_t* bytes) {
-    return fgxm_detail::guard([&] {
-        auto a = fgxm_detail::agent(s, agent);
-        if (v >= a->declared.size()) throw flamegpu::exception::OutOfBoundsException("variable index out of range");
-        const auto& var = a->variables.at(a->declared[v]);
-        *name = var.name.c_str();
-        *bytes = (uint32_t)var.bytes();
-        *elements = var.elements;
-        const std::type_index t = var.type;
-        int32_t code = -1;
-        if (t == typeid(float)) code = 0;
-        else if (t == typeid(double)) code = 1;
-        else if (t == typeid(int32_t)) code = 2;
-        else if (t == typeid(uint32_t)) code = 3;
-        else if (t == typeid(int64_t) || t == typeid(long long)) code = 4;
-        else if (t == typeid(uint64_t) || t == typeid(unsigned long long)) code = 5;
-        else if (t == typeid(int8_t)) code = 6;
-        else if (t == typeid(uint8_t)) code = 7;
-        else if (t == typeid(int16_t)) code = 8;
-        else if (t == typeid(uint16_t)) code = 9;
-        *type = code;
-    });
+    return fgxm_detail::guard([&] { fgxm_detail::var_info(fgxm_detail::agent(s, agent), v, name, type, elements, bytes); });
 }
+
 
 FGXM_EXPORT int fgxm_set_agents(fgxm_sim* s, const char* agent, const char* state, uint32_t n, const void* const* vars,
                     const uint32_t* ids) {
@@ -213,6 +222,244 @@ FGXM_EXPORT int fgxm_timed_steps(fgxm_sim* s, uint32_t steps, double* ms) {
 
 FGXM_EXPORT int fgxm_step_counter(fgxm_sim* s, uint32_t* out) {
     return fgxm_detail::guard([&] { *out = s->sim->getStepCounter(); });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+struct fgxm_batch {
+    std::string variant;
+    std::unique_ptr<flamegpu::ModelDescription> model;
+    std::unique_ptr<flamegpu::BatchSimulation> batch;
+    std::map<int32_t, flamegpu::BatchSimulation::Plan> staged;   // added, not yet committed
+    std::map<int32_t, int> live;                                 // ABI handle -> batch handle
+    int32_t next = 1;
+    void* ev0 = nullptr;
+    void* ev1 = nullptr;
+};
+
+namespace fgxm_detail {
+// A simulation's plan from its params: the environment buffer and spatial message
+// geometry of the model built from them; the structure must match the batch's model.
+inline flamegpu::BatchSimulation::Plan plan_from_params(fgxm_batch* b, const float* params, uint32_t nparams) {
+    flamegpu::ModelDescription m = fgxm_build_model(b->variant.empty() ? nullptr : b->variant.c_str(), params, nparams);
+    const auto& d = *m.data();
+    const auto& base = *b->model->data();
+    if (d.env.buffer.size() != base.env.buffer.size() || d.agents.size() != base.agents.size() ||
+        d.messages.size() != base.messages.size())
+        throw flamegpu::exception::InvalidArgument("batch simulation params build a model of a different structure");
+    flamegpu::BatchSimulation::Plan p;
+    p.env = d.env.buffer;
+    for (auto& kv : d.messages) {
+        flamegpu::SpatialBounds sb;
+        for (int a = 0; a < 3; ++a) {
+            sb.min[a] = kv.second->min[a];
+            sb.max[a] = kv.second->max[a];
+        }
+        sb.radius = kv.second->radius;
+        p.messages[kv.first] = sb;
+    }
+    return p;
+}
+inline void commit(fgxm_batch* b) {
+    for (auto& kv : b->staged) b->live[kv.first] = b->batch->add(kv.second);
+    b->staged.clear();
+}
+inline int handle(fgxm_batch* b, int32_t h) {
+    commit(b);
+    auto it = b->live.find(h);
+    if (it == b->live.end()) throw flamegpu::exception::InvalidArgument("unknown batch simulation handle");
+    return it->second;
+}
+inline std::shared_ptr<flamegpu::AgentData> batch_agent(fgxm_batch* b, const char* name) {
+    auto& agents = b->model->data()->agents;
+    auto it = agents.find(name ? name : "");
+    if (it == agents.end()) throw flamegpu::exception::InvalidAgentName(std::string("unknown agent ") + (name ? name : "(null)"));
+    return it->second;
+}
+}  // namespace fgxm_detail
+
+extern "C" {
+
+FGXM_EXPORT int fgxm_batch_create(const char* variant, const float* params, uint32_t nparams, int device, fgxm_batch** out) {
+    return fgxm_detail::guard([&] {
+        auto* b = new fgxm_batch();
+        try {
+            b->variant = variant ? variant : "";
+            b->model.reset(new flamegpu::ModelDescription(fgxm_build_model(variant, params, nparams)));
+            b->batch.reset(new flamegpu::BatchSimulation(*b->model, device));
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_destroy(fgxm_batch* b) {
+    return fgxm_detail::guard([&] {
+        if (b->ev0) fgx_event_destroy(b->ev0);
+        if (b->ev1) fgx_event_destroy(b->ev1);
+        delete b;
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_var_count(fgxm_batch* b, const char* agent, uint32_t* n) {
+    return fgxm_detail::guard([&] { *n = (uint32_t)fgxm_detail::batch_agent(b, agent)->declared.size(); });
+}
+
+FGXM_EXPORT int fgxm_batch_var_info(fgxm_batch* b, const char* agent, uint32_t v, const char** name, int32_t* type,
+                                    uint32_t* elements, uint32_t* bytes) {
+    return fgxm_detail::guard([&] { fgxm_detail::var_info(fgxm_detail::batch_agent(b, agent), v, name, type, elements, bytes); });
+}
+
+FGXM_EXPORT int fgxm_batch_add(fgxm_batch* b, const float* params, uint32_t nparams, uint64_t seed, uint32_t steps,
+                               int32_t* handle) {
+    return fgxm_detail::guard([&] {
+        auto p = fgxm_detail::plan_from_params(b, params, nparams);
+        p.seed = seed;
+        p.steps = steps;
+        const int32_t h = b->next++;
+        b->staged[h] = std::move(p);
+        *handle = h;
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_set_agents(fgxm_batch* b, int32_t h, const char* agent, uint32_t n, const void* const* vars,
+                                      const uint32_t* ids) {
+    return fgxm_detail::guard([&] {
+        auto it = b->staged.find(h);
+        if (it == b->staged.end())
+            throw flamegpu::exception::InvalidOperation("populations can only be set before the simulation joins the batch");
+        auto a = fgxm_detail::batch_agent(b, agent);
+        auto pop = std::make_shared<flamegpu::AgentVector>(flamegpu::AgentDescription(a), n);
+        for (size_t v = 0; v < a->declared.size(); ++v) {
+            auto& raw = pop->raw(a->declared[v]);
+            std::memcpy(raw.data(), vars[v], raw.size());
+        }
+        if (ids) std::memcpy(pop->raw("_id").data(), ids, (size_t)n * 4);
+        it->second.populations[a->name] = pop;
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_estimate_bytes(fgxm_batch* b, int32_t h, uint64_t* bytes) {
+    return fgxm_detail::guard([&] {
+        auto it = b->staged.find(h);
+        if (it == b->staged.end()) throw flamegpu::exception::InvalidArgument("estimate needs a staged simulation");
+        *bytes = b->batch->estimateBytes(it->second);
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_step(fgxm_batch* b, uint32_t nsteps) {
+    return fgxm_detail::guard([&] {
+        fgxm_detail::commit(b);
+        for (uint32_t i = 0; i < nsteps; ++i) b->batch->step();
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_finished(fgxm_batch* b, int32_t* handles, uint32_t capacity, uint32_t* count) {
+    return fgxm_detail::guard([&] {
+        const auto f = b->batch->finished();
+        uint32_t k = 0;
+        for (auto& kv : b->live)
+            if (std::find(f.begin(), f.end(), kv.second) != f.end()) {
+                if (k < capacity && handles) handles[k] = kv.first;
+                ++k;
+            }
+        *count = k;
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_steps_done(fgxm_batch* b, int32_t h, uint32_t* steps) {
+    return fgxm_detail::guard([&] {
+        if (b->staged.count(h)) {
+            *steps = 0;
+            return;
+        }
+        *steps = b->batch->stepsDone(fgxm_detail::handle(b, h));
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_count(fgxm_batch* b, int32_t h, const char* agent, uint32_t* n) {
+    return fgxm_detail::guard([&] {
+        auto a = fgxm_detail::batch_agent(b, agent);
+        flamegpu::AgentVector pop{flamegpu::AgentDescription(a)};
+        b->batch->getPopulationData(fgxm_detail::handle(b, h), pop);
+        *n = (uint32_t)pop.size();
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_get_agents(fgxm_batch* b, int32_t h, const char* agent, void* const* vars, uint32_t* ids) {
+    return fgxm_detail::guard([&] {
+        auto a = fgxm_detail::batch_agent(b, agent);
+        flamegpu::AgentVector pop{flamegpu::AgentDescription(a)};
+        b->batch->getPopulationData(fgxm_detail::handle(b, h), pop);
+        for (size_t v = 0; v < a->declared.size(); ++v) {
+            auto& raw = pop.raw(a->declared[v]);
+            if (vars[v]) std::memcpy(vars[v], raw.data(), raw.size());
+        }
+        if (ids) std::memcpy(ids, pop.raw("_id").data(), pop.size() * 4);
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_remove(fgxm_batch* b, int32_t h) {
+    return fgxm_detail::guard([&] {
+        if (b->staged.erase(h)) return;
+        b->batch->remove(fgxm_detail::handle(b, h));
+        b->live.erase(h);
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_set_tuning(fgxm_batch* b, uint32_t block, uint32_t agents_per_thread, uint32_t blocks_per_cu) {
+    return fgxm_detail::guard([&] {
+        flamegpu::BatchSimulation::Tuning t;
+        t.block = block;
+        t.agents_per_thread = agents_per_thread;
+        t.blocks_per_cu = blocks_per_cu;
+        b->batch->setTuning(t);
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_set_option(fgxm_batch* b, const char* key, int value) {
+    return fgxm_detail::guard([&] {
+        const std::string k = key ? key : "";
+        if (k == "graphs") b->batch->setGraphs(value != 0);
+        else throw flamegpu::exception::InvalidArgument("unknown batch option " + k);
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_info(fgxm_batch* b, uint32_t* resident, uint32_t* staged, uint64_t* resident_agents,
+                                uint64_t* resident_bytes) {
+    return fgxm_detail::guard([&] {
+        if (resident) *resident = b->batch->residentCount();
+        if (staged) *staged = (uint32_t)b->staged.size() + b->batch->pendingCount();
+        if (resident_agents) *resident_agents = b->batch->residentAgents();
+        if (resident_bytes) *resident_bytes = b->batch->residentBytes();
+    });
+}
+
+FGXM_EXPORT int fgxm_batch_sync(fgxm_batch* b) {
+    return fgxm_detail::guard([&] { b->batch->synchronize(); });
+}
+
+FGXM_EXPORT int fgxm_batch_timed_steps(fgxm_batch* b, uint32_t steps, double* ms) {
+    return fgxm_detail::guard([&] {
+        fgxm_detail::commit(b);
+        if (!b->ev0) flamegpu::detail::fgx_check(fgx_event_create(&b->ev0), "event");
+        if (!b->ev1) flamegpu::detail::fgx_check(fgx_event_create(&b->ev1), "event");
+        // two untimed steps first: joiners and leavers repack, and each of the two
+        // buffer arrangements a step alternates between is captured as a graph
+        b->batch->step();
+        b->batch->step();
+        void* st = b->batch->stream();
+        b->batch->synchronize();
+        flamegpu::detail::fgx_check(fgx_event_record(b->ev0, st), "event record");
+        for (uint32_t i = 0; i < steps; ++i) b->batch->step();
+        flamegpu::detail::fgx_check(fgx_event_record(b->ev1, st), "event record");
+        float f = 0.0f;
+        flamegpu::detail::fgx_check(fgx_event_elapsed_ms(b->ev0, b->ev1, &f), "event elapsed");
+        *ms = f;
+    });
 }
 
 }  // extern "C"

@@ -48,6 +48,8 @@ struct ThreadCtx {
     const uint32_t* d_step;     // step counter
     uint64_t seed;
     const StageView* stage;     // null outside the staged kernel
+    uint32_t msg_off;           // brute-force input range [msg_off, msg_off + msg_n)
+    uint32_t msg_n;
 };
 
 // Global-memory pointers (address space 1), so loads through pointers rebuilt from
@@ -283,8 +285,7 @@ class MessageNone::Out {
 // include/flamegpu/runtime/messaging/MessageBruteForce/MessageBruteForceDevice.cuh:19-172
 class MessageBruteForce::In {
  public:
-    __device__ __forceinline__ explicit In(const detail::ThreadCtx& c)
-        : c_(c), len_(c.a.d_msg_in_count ? detail::load_count(c.a.d_msg_in_count) : 0u) {}
+    __device__ __forceinline__ explicit In(const detail::ThreadCtx& c) : c_(c), off_(c.msg_off), len_(c.msg_n) {}
     class Message {
      public:
         __device__ __forceinline__ Message(const detail::ThreadCtx& c, uint32_t i) : c_(c), i_(i) {}
@@ -298,7 +299,7 @@ class MessageBruteForce::In {
             return detail::load_elem<T>(c_.a.msg_in, c_.a.msg_in_mul, name, i_, L, e, c_, detail::DERR_UNKNOWN_MSG_VAR,
                                         detail::DERR_MSG_VAR_TYPE);
         }
-        __device__ __forceinline__ size_type getIndex() const { return i_; }
+        __device__ __forceinline__ size_type getIndex() const { return i_ - c_.msg_off; }
 
      private:
         const detail::ThreadCtx& c_;
@@ -319,14 +320,14 @@ class MessageBruteForce::In {
         const detail::ThreadCtx& c_;
         uint32_t i_;
     };
-    __device__ __forceinline__ iterator begin() const { return iterator(c_, 0); }
-    __device__ __forceinline__ iterator end() const { return iterator(c_, len_); }
+    __device__ __forceinline__ iterator begin() const { return iterator(c_, off_); }
+    __device__ __forceinline__ iterator end() const { return iterator(c_, off_ + len_); }
     __device__ __forceinline__ size_type size() const { return len_; }
-    __device__ __forceinline__ Message at(size_type i) const { return Message(c_, i); }
+    __device__ __forceinline__ Message at(size_type i) const { return Message(c_, off_ + i); }
 
  private:
     const detail::ThreadCtx& c_;
-    uint32_t len_;
+    uint32_t off_, len_;
 };
 
 // include/flamegpu/runtime/messaging/MessageBruteForce/MessageBruteForceDevice.cuh:259-294
@@ -853,17 +854,21 @@ __global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
     const uint32_t n = a.d_n ? load_count(a.d_n) : a.n_cap;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const ThreadCtx c{a, ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step, a.seed, nullptr};
+    uint32_t mn = 0u;
+    if constexpr (F::MsgIn::kind == MsgKind::BruteForce) mn = a.d_msg_in_count ? load_count(a.d_msg_in_count) : 0u;
+    const ThreadCtx c{a, ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step, a.seed, nullptr,
+                      0u, mn};
     run_agent<F>(c);
 }
 
-// Batched launch: the simulations of a batch share one launch; block b belongs to
-// the simulation s with segs[s].tile_off <= b < segs[s+1].tile_off (tiles of
-// blockDim agents) and runs with that simulation's metadata, environment, step,
-// seed and error words.  A grid smaller than the tile count loops over tiles.
+// Batched launch: the simulations of a batch share one launch.  Tile t (blockDim x
+// apt agents) belongs to the simulation s with segs[s].tile_off <= t <
+// segs[s+1].tile_off and runs with that simulation's metadata, environment, step,
+// seed and error words; each thread runs apt agents, blockDim apart.  A grid
+// smaller than the tile count loops over tiles (persistent blocks).
 template <typename F>
 __global__ void __launch_bounds__(1024) agent_function_kernel_seg(const FnArgs a, const SegDev* __restrict__ segs,
-                                                                  uint32_t nsegs, uint32_t ntiles) {
+                                                                  uint32_t nsegs, uint32_t ntiles, uint32_t apt) {
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         uint32_t lo = 0, hi = nsegs;
         while (hi - lo > 1u) {
@@ -872,12 +877,15 @@ __global__ void __launch_bounds__(1024) agent_function_kernel_seg(const FnArgs a
             else hi = mid;
         }
         const SegDev& s = segs[lo];
-        const uint32_t local = (t - s.tile_off) * blockDim.x + threadIdx.x;
-        if (local < s.n) {
-            const uint32_t i = s.agent_off + local;
-            const ThreadCtx c{a, ErrState{s.d_error, i}, a.zero_page, a.sink_page, i, s.meta, s.env, s.d_step, s.seed,
-                              nullptr};
-            run_agent<F>(c);
+        const uint32_t base = (t - s.tile_off) * blockDim.x * apt + threadIdx.x;
+        for (uint32_t k = 0; k < apt; ++k) {
+            const uint32_t local = base + k * blockDim.x;
+            if (local < s.n) {
+                const uint32_t i = s.agent_off + local;
+                const ThreadCtx c{a,        ErrState{s.d_error, i}, a.zero_page, a.sink_page, i, s.meta, s.env, s.d_step,
+                                  s.seed,   nullptr,                s.msg_off,   s.msg_n};
+                run_agent<F>(c);
+            }
         }
     }
 }
@@ -1050,7 +1058,7 @@ __global__ void __launch_bounds__(256) agent_function_kernel_staged(const FnArgs
     const StageView sv{fgx_stage_smem, fgx_stage_smem + maxe + 1, fgx_stage_smem + 2 * (maxe + 1), a.stage_cap,
                        a.stage_nvars > (uint32_t)F::MsgIn::In::dims ? 1u : 0u, &sh};
     const ThreadCtx c{a,         ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step,
-                      a.seed,    staged ? &sv : nullptr};
+                      a.seed,    staged ? &sv : nullptr, 0u, 0u};
     run_agent<F>(c);
 }
 
@@ -1068,10 +1076,10 @@ void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, uint3
 }
 
 template <typename F>
-void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t grid,
-                               uint32_t block, void* stream) {
+void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t apt,
+                               uint32_t grid, uint32_t block, void* stream) {
     hipLaunchKernelGGL(agent_function_kernel_seg<F>, dim3(grid), dim3(block), 0, (hipStream_t)stream, a, segs, nsegs,
-                       ntiles);
+                       ntiles, apt);
 }
 }  // namespace detail
 

@@ -41,6 +41,24 @@ def _lib(model):
         "fgxm_kernel_times": [vp, c.c_char_p, c.c_uint32, c.c_int],
         "fgxm_timed_steps": [vp, c.c_uint32, c.POINTER(c.c_double)],
         "fgxm_step_counter": [vp, u32p],
+        "fgxm_batch_create": [c.c_char_p, c.POINTER(c.c_float), c.c_uint32, c.c_int, c.POINTER(vp)],
+        "fgxm_batch_destroy": [vp],
+        "fgxm_batch_var_count": [vp, c.c_char_p, u32p],
+        "fgxm_batch_var_info": [vp, c.c_char_p, c.c_uint32, c.POINTER(c.c_char_p), c.POINTER(c.c_int32), u32p, u32p],
+        "fgxm_batch_add": [vp, c.POINTER(c.c_float), c.c_uint32, c.c_uint64, c.c_uint32, c.POINTER(c.c_int32)],
+        "fgxm_batch_set_agents": [vp, c.c_int32, c.c_char_p, c.c_uint32, c.POINTER(vp), u32p],
+        "fgxm_batch_estimate_bytes": [vp, c.c_int32, c.POINTER(c.c_uint64)],
+        "fgxm_batch_step": [vp, c.c_uint32],
+        "fgxm_batch_finished": [vp, c.POINTER(c.c_int32), c.c_uint32, u32p],
+        "fgxm_batch_steps_done": [vp, c.c_int32, u32p],
+        "fgxm_batch_count": [vp, c.c_int32, c.c_char_p, u32p],
+        "fgxm_batch_get_agents": [vp, c.c_int32, c.c_char_p, c.POINTER(vp), u32p],
+        "fgxm_batch_remove": [vp, c.c_int32],
+        "fgxm_batch_set_tuning": [vp, c.c_uint32, c.c_uint32, c.c_uint32],
+        "fgxm_batch_set_option": [vp, c.c_char_p, c.c_int],
+        "fgxm_batch_info": [vp, u32p, u32p, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64)],
+        "fgxm_batch_sync": [vp],
+        "fgxm_batch_timed_steps": [vp, c.c_uint32, c.POINTER(c.c_double)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -162,3 +180,118 @@ class Simulation:
         n = ctypes.c_uint32()
         self._chk(self.L.fgxm_step_counter(self.h, ctypes.byref(n)))
         return n.value
+
+
+class Batch:
+    """Many simulations of one model library co-scheduled in shared launches
+    (include/fgx_model.h, fgxm_batch_*).  `params` of each simulation set its
+    environment and spatial message geometry, as for Simulation."""
+
+    def __init__(self, model, params=(), variant=None, device=0):
+        self.L = _lib(model)
+        self.model = model
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        h = ctypes.c_void_p()
+        self._chk(self.L.fgxm_batch_create(_b(variant), p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(p), device,
+                                           ctypes.byref(h)))
+        self.h = h
+        self._info = {}
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise FgxError(rc, self.L.fgxm_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fgxm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def var_info(self, agent):
+        if agent in self._info:
+            return self._info[agent]
+        n = ctypes.c_uint32()
+        self._chk(self.L.fgxm_batch_var_count(self.h, _b(agent), ctypes.byref(n)))
+        out = []
+        for v in range(n.value):
+            name = ctypes.c_char_p()
+            t, e, b = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_uint32()
+            self._chk(self.L.fgxm_batch_var_info(self.h, _b(agent), v, ctypes.byref(name), ctypes.byref(t),
+                                                 ctypes.byref(e), ctypes.byref(b)))
+            out.append((name.value.decode(), _DT.get(t.value, np.uint8), e.value, b.value))
+        self._info[agent] = out
+        return out
+
+    def add(self, params, seed, steps, populations):
+        """populations: {agent: [arrays in declaration order]}; returns the handle."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        h = ctypes.c_int32()
+        self._chk(self.L.fgxm_batch_add(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(p), int(seed),
+                                        int(steps), ctypes.byref(h)))
+        for agent, arrays in populations.items():
+            info = self.var_info(agent)
+            assert len(arrays) == len(info), f"{agent} has {len(info)} variables"
+            n = len(arrays[0])
+            keep = [np.ascontiguousarray(a, dtype=dt) for a, (_, dt, _, _) in zip(arrays, info)]
+            vp = (ctypes.c_void_p * max(len(keep), 1))(*[a.ctypes.data for a in keep])
+            self._chk(self.L.fgxm_batch_set_agents(self.h, h.value, _b(agent), n, vp, None))
+        return h.value
+
+    def estimate_bytes(self, handle):
+        b = ctypes.c_uint64()
+        self._chk(self.L.fgxm_batch_estimate_bytes(self.h, handle, ctypes.byref(b)))
+        return b.value
+
+    def step(self, n=1):
+        self._chk(self.L.fgxm_batch_step(self.h, n))
+
+    def finished(self):
+        cnt = ctypes.c_uint32()
+        self._chk(self.L.fgxm_batch_finished(self.h, None, 0, ctypes.byref(cnt)))
+        buf = (ctypes.c_int32 * max(cnt.value, 1))()
+        self._chk(self.L.fgxm_batch_finished(self.h, buf, cnt.value, ctypes.byref(cnt)))
+        return [buf[i] for i in range(cnt.value)]
+
+    def steps_done(self, handle):
+        n = ctypes.c_uint32()
+        self._chk(self.L.fgxm_batch_steps_done(self.h, handle, ctypes.byref(n)))
+        return n.value
+
+    def get_agents(self, handle, agent):
+        info = self.var_info(agent)
+        n = ctypes.c_uint32()
+        self._chk(self.L.fgxm_batch_count(self.h, handle, _b(agent), ctypes.byref(n)))
+        outs = [np.zeros(n.value * e, dtype=dt) for (_, dt, e, _) in info]
+        ids = np.zeros(n.value, dtype=np.uint32)
+        vp = (ctypes.c_void_p * max(len(outs), 1))(*[o.ctypes.data for o in outs])
+        self._chk(self.L.fgxm_batch_get_agents(self.h, handle, _b(agent), vp,
+                                               ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return outs, ids
+
+    def remove(self, handle):
+        self._chk(self.L.fgxm_batch_remove(self.h, handle))
+
+    def set_tuning(self, block=256, agents_per_thread=1, blocks_per_cu=0):
+        self._chk(self.L.fgxm_batch_set_tuning(self.h, block, agents_per_thread, blocks_per_cu))
+
+    def set_option(self, key, value):
+        self._chk(self.L.fgxm_batch_set_option(self.h, _b(key), int(value)))
+
+    def info(self):
+        r, s = ctypes.c_uint32(), ctypes.c_uint32()
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        self._chk(self.L.fgxm_batch_info(self.h, ctypes.byref(r), ctypes.byref(s), ctypes.byref(a), ctypes.byref(b)))
+        return {"resident": r.value, "staged": s.value, "resident_agents": a.value, "resident_bytes": b.value}
+
+    def sync(self):
+        self._chk(self.L.fgxm_batch_sync(self.h))
+
+    def timed_steps(self, n):
+        ms = ctypes.c_double()
+        self._chk(self.L.fgxm_batch_timed_steps(self.h, n, ctypes.byref(ms)))
+        return ms.value

@@ -1,0 +1,172 @@
+"""Serving loop: simulation runs arrive over time and are served by continuous batching.
+
+Each request is one simulation run (model, population size, step count, seed).
+Requests are admitted in arrival order while the HBM they need (agent, message,
+bin-table and scratch bytes, from the batch's own estimate) fits the budget;
+an admitted run joins its model's batch at the next step and shares every launch
+with the runs already resident, so runs that arrive while others are on the GPU
+overlap with them.  The batches of different models run on their own streams.
+A run leaves its batch once its steps are done and its final population has been
+read back; that read-back is its completion time.
+
+Results do not depend on co-scheduling: a run's final state equals the same run
+alone (ordered IDs, Philox keyed by seed / agent ID / step, stable orders).
+"""
+import collections
+import time
+
+import numpy as np
+
+from . import workloads
+from .models import Batch
+
+
+def circles_request(n, seed, radius=2.0):
+    params, arrs = workloads.circles(n, seed=seed, radius=radius)
+    return params, {"Circle": arrs}
+
+
+def boids_request(n, seed, radius=0.05, density=32768.0):
+    # the 1M-agent density (2^20 agents in a 3.2-wide cube), width a whole number of bins
+    bins = max(3, int(round((n / density) ** (1.0 / 3.0) / radius)))
+    params, arrs = workloads.boids(n, seed=seed, width=bins * radius, radius=radius)
+    return params, {"Boid": arrs}
+
+
+MODELS = {
+    "circles": ("circles_spatial3D", "Circle", circles_request),
+    "boids": ("boids_spatial3D", "Boid", boids_request),
+}
+
+Request = collections.namedtuple("Request", "id arrival model n steps seed")
+
+
+def poisson_requests(rate, seconds, seed, n_range=(1 << 10, 1 << 20), steps_range=(10, 1000), models=("circles", "boids")):
+    """Poisson arrivals at `rate` per second over `seconds`; model chosen uniformly,
+    population log-uniform in n_range, steps uniform in steps_range, random seed."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        n = int(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        out.append(Request(len(out), t, models[int(rng.integers(len(models)))], n,
+                           int(rng.integers(steps_range[0], steps_range[1] + 1)), int(rng.integers(1 << 31))))
+    return out
+
+
+class Server:
+    def __init__(self, hbm_budget=160 << 30, max_chunk=16, tuning=None, device=0):
+        self.budget = hbm_budget
+        self.max_chunk = max_chunk
+        self.tuning = tuning or {}
+        self.device = device
+        self.batches = {}
+
+    def _batch(self, model):
+        if model not in self.batches:
+            lib, agent, make = MODELS[model]
+            params, _ = make(1024, 0)
+            b = Batch(lib, params, device=self.device)
+            if model in self.tuning:
+                b.set_tuning(*self.tuning[model])
+            self.batches[model] = b
+        return self.batches[model]
+
+    def run(self, requests, collect=False, clock=time.perf_counter, progress=None):
+        """Serves `requests` (sorted by arrival, seconds from the start).  Returns one
+        record per request: dict(id, model, n, steps, arrival, admitted, completed,
+        latency[, agents, ids]).  `progress(text)` is called every ~5 s of serving."""
+        self.phase = collections.Counter()
+        last_report = clock()
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        queue = collections.deque()
+        active = {}        # (model, handle) -> [request, bytes, admitted, steps issued]
+        used = 0
+        results = []
+        made = {}          # request id -> (params, populations) while it waits for HBM
+        nxt = 0
+        t0 = clock()
+        while nxt < len(reqs) or queue or active:
+            now = clock() - t0
+            while nxt < len(reqs) and reqs[nxt].arrival <= now:
+                queue.append(reqs[nxt])
+                nxt += 1
+            # admission in arrival order while the HBM budget allows
+            while queue:
+                r = queue[0]
+                lib, agent, make = MODELS[r.model]
+                b = self._batch(r.model)
+                tp = clock()
+                if r.id not in made:
+                    made[r.id] = make(r.n, r.seed)
+                params, pops = made[r.id]
+                h = b.add(params, r.seed, r.steps, pops)
+                need = b.estimate_bytes(h)
+                self.phase["admit_s"] += clock() - tp
+                if used + need > self.budget and active:
+                    b.remove(h)
+                    break
+                del made[r.id]
+                used += need
+                active[(r.model, h)] = [r, need, clock() - t0, 0]
+                queue.popleft()
+            if not active:
+                if nxt < len(reqs):
+                    time.sleep(max(0.0, reqs[nxt].arrival - (clock() - t0)))
+                continue
+            # one chunk of steps per batch: up to the first completion in it
+            for model, b in self.batches.items():
+                mine = [v for (m, _), v in active.items() if m == model]
+                if not mine:
+                    continue
+                k = min(min(v[0].steps - v[3] for v in mine), self.max_chunk)
+                if k <= 0:
+                    continue
+                tp = clock()
+                b.step(k)
+                self.phase["step_call_s"] += clock() - tp
+                self.phase["steps"] += k
+                for v in mine:
+                    v[3] += k
+            # completions: read back, then leave the batch
+            for model, b in self.batches.items():
+                for h in b.finished():
+                    key = (model, h)
+                    if key not in active:
+                        continue
+                    tp = clock()
+                    r, need, admitted, _ = active.pop(key)
+                    rec = {"id": r.id, "model": r.model, "n": r.n, "steps": r.steps, "arrival": r.arrival,
+                           "admitted": admitted}
+                    if collect:
+                        arrays, ids = b.get_agents(h, MODELS[model][1])
+                        rec["agents"], rec["ids"] = arrays, ids
+                    else:
+                        b.sync()
+                    rec["completed"] = clock() - t0
+                    rec["latency"] = rec["completed"] - r.arrival
+                    b.remove(h)
+                    used -= need
+                    results.append(rec)
+                    self.phase["complete_s"] += clock() - tp
+            if progress and clock() - last_report > 5.0:
+                last_report = clock()
+                progress(f"serving t={clock() - t0:.1f}s arrived={nxt}/{len(reqs)} queued={len(queue)} "
+                         f"active={len(active)} done={len(results)} {dict(self.phase)}")
+        return results
+
+
+def summarize(results, wall):
+    lat = np.array([r["latency"] for r in results]) if results else np.zeros(1)
+    agent_steps = float(sum(r["n"] * r["steps"] for r in results))
+    return {
+        "requests": len(results),
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "mean_latency_s": float(lat.mean()),
+        "agent_steps": agent_steps,
+        "agent_steps_per_s": agent_steps / wall if wall > 0 else 0.0,
+        "wall_s": wall,
+    }

@@ -188,6 +188,60 @@ int fgx_assign_ids(uint32_t* ids, uint32_t n, uint32_t next_id, void* stream);
 int fgx_reduce_sum(fgx_ctx* ctx, int type, const void* in, const uint32_t* d_n, uint32_t n_cap,
                    void* d_out, void* scratch, size_t scratch_bytes, void* stream);
 
+/* ---------------------------------------------------------------- batched (segmented) primitives
+ * A batch runs many independent simulations in the same launches.  Their lists
+ * are packed one after another (simulation s owns elements [elem_off, elem_off + n)),
+ * and their bin tables likewise (keys [key_off, key_off + keys)).  The reference has
+ * no batched path: CUDAEnsemble runs each simulation with its own launches
+ * (src/flamegpu/simulation/CUDAEnsemble.cu, src/flamegpu/simulation/detail/SimRunner.cu:51-90);
+ * these entries replace that per-simulation launch sequence for the message-list
+ * build and the agent sort.  Each simulation's results equal a separate run of
+ * fgx_spatial_build / fgx_agent_sort on its own list. */
+typedef struct {
+    uint32_t elem_off;     /* first element of the simulation (filled by fgx_segments_pack) */
+    uint32_t n;            /* elements */
+    uint32_t key_off;      /* first key / bin of the simulation (filled by fgx_segments_pack) */
+    uint32_t keys;         /* keys (sort) or bins (build) of the simulation (set by fgx_seg_key_init) */
+    uint32_t tile_off;     /* first 2048-element tile (filled by fgx_segments_pack) */
+    int32_t dims;
+    float min[3];
+    float radius;
+    float width[3];
+    uint32_t grid[3];
+    uint32_t key_mask;     /* agent sort: keys - 1 */
+    uint32_t pad[3];
+} fgx_seg_key;             /* 80 bytes */
+
+enum { FGX_SEG_MESSAGE_BINS = 0, FGX_SEG_AGENT_SORT = 1 };
+
+/* Geometry of one simulation's keys: message bins (as fgx_spatial_meta_init) or the
+ * agent sort key (as fgx_agent_sort).  Leaves the offsets and n untouched. */
+int fgx_seg_key_init(int kind, int dims, const float* min, const float* max, float radius, fgx_seg_key* out);
+/* Packs the simulations in order: fills elem_off, key_off and tile_off from n and keys,
+ * and returns the totals (elements, keys, 2048-element tiles). */
+int fgx_segments_pack(fgx_seg_key* segs, uint32_t nsegs, uint32_t* n_total, uint32_t* keys_total, uint32_t* tiles_total);
+/* Segmented message-list build: d_segs (device copy of the packed table) with kind
+ * FGX_SEG_MESSAGE_BINS; pbm holds keys_total + 1 entries of absolute message indices.
+ * Scratch: fgx_scratch_bytes(FGX_OP_SPATIAL_BUILD, n_total, keys_total). */
+int fgx_spatial_build_seg(fgx_ctx* ctx, const fgx_seg_key* d_segs, uint32_t nsegs, uint32_t tiles_total,
+                          uint32_t n_total, uint32_t keys_total, const float* x, const float* y, const float* z,
+                          uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                          uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream);
+/* Segmented agent sort (kind FGX_SEG_AGENT_SORT).  Scratch:
+ * fgx_scratch_bytes(FGX_OP_AGENT_SORT, n_total, keys_total). */
+int fgx_agent_sort_seg(fgx_ctx* ctx, const fgx_seg_key* d_segs, uint32_t nsegs, uint32_t tiles_total,
+                       uint32_t n_total, uint32_t keys_total, const float* x, const float* y, const float* z,
+                       uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                       void* scratch, size_t scratch_bytes, void* stream);
+/* p[i] += delta for i < n (per-simulation step counters of a batch). */
+int fgx_add_u32_n(uint32_t* p, uint32_t n, uint32_t delta, void* stream);
+/* Copies element ranges of several variables in one launch: range r moves
+ * d_ranges[3r+2] elements from src offset d_ranges[3r] to dst offset d_ranges[3r+1].
+ * Destination ranges must be sorted by offset and not overlap; n_total is the
+ * sum of the range lengths (repacking a batch when simulations join or leave). */
+int fgx_copy_ranges(uint32_t nvars, const void* const* src, void* const* dst, const uint32_t* var_bytes,
+                    const uint32_t* d_ranges, uint32_t nranges, uint32_t n_total, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,46 @@ int fgxm_kernel_times(fgxm_sim* sim, char* buf, uint32_t buflen, int reset);
 int fgxm_timed_steps(fgxm_sim* sim, uint32_t steps, double* ms);
 int fgxm_step_counter(fgxm_sim* sim, uint32_t* out);
 
+/* ---------------------------------------------------------------- batches
+ * Many simulations of this model co-scheduled in shared segmented launches
+ * (flamegpu::BatchSimulation).  The reference has no batched path (CUDAEnsemble
+ * launches each simulation separately, src/flamegpu/simulation/CUDAEnsemble.cu);
+ * this is the plan-level equivalent of CUDAEnsemble::simulate with RunPlan seeds,
+ * step counts and properties, where each simulation's `params` (as for fgxm_create)
+ * give its environment and spatial message geometry.  Simulations are staged by
+ * fgxm_batch_add + fgxm_batch_set_agents and join at the next fgxm_batch_step;
+ * finished ones stay readable until fgxm_batch_remove. */
+typedef struct fgxm_batch fgxm_batch;
+int fgxm_batch_create(const char* variant, const float* params, uint32_t nparams, int device, fgxm_batch** out);
+int fgxm_batch_destroy(fgxm_batch* b);
+/* As fgxm_var_count / fgxm_var_info, for the batch's model. */
+int fgxm_batch_var_count(fgxm_batch* b, const char* agent, uint32_t* nvars);
+int fgxm_batch_var_info(fgxm_batch* b, const char* agent, uint32_t v, const char** name, int32_t* type,
+                        uint32_t* elements, uint32_t* bytes);
+int fgxm_batch_add(fgxm_batch* b, const float* params, uint32_t nparams, uint64_t seed, uint32_t steps, int32_t* handle);
+int fgxm_batch_set_agents(fgxm_batch* b, int32_t handle, const char* agent, uint32_t n, const void* const* vars,
+                          const uint32_t* ids);
+/* HBM bytes the staged simulation will occupy once resident. */
+int fgxm_batch_estimate_bytes(fgxm_batch* b, int32_t handle, uint64_t* bytes);
+int fgxm_batch_step(fgxm_batch* b, uint32_t nsteps);
+/* Handles whose steps are done. */
+int fgxm_batch_finished(fgxm_batch* b, int32_t* handles, uint32_t capacity, uint32_t* count);
+int fgxm_batch_steps_done(fgxm_batch* b, int32_t handle, uint32_t* steps);
+int fgxm_batch_count(fgxm_batch* b, int32_t handle, const char* agent, uint32_t* n);
+int fgxm_batch_get_agents(fgxm_batch* b, int32_t handle, const char* agent, void* const* vars, uint32_t* ids);
+int fgxm_batch_remove(fgxm_batch* b, int32_t handle);
+/* Launch shape of the batched agent functions: block size (multiple of 64, <= 1024),
+ * agents per thread, persistent blocks per CU (0: one block per tile). */
+int fgxm_batch_set_tuning(fgxm_batch* b, uint32_t block, uint32_t agents_per_thread, uint32_t blocks_per_cu);
+/* options: "graphs" (capture batched steps as hipGraphs, default 1). */
+int fgxm_batch_set_option(fgxm_batch* b, const char* key, int value);
+int fgxm_batch_info(fgxm_batch* b, uint32_t* resident, uint32_t* staged, uint64_t* resident_agents,
+                    uint64_t* resident_bytes);
+int fgxm_batch_sync(fgxm_batch* b);
+/* Runs two untimed steps (repack, graph capture), then `steps` batched steps
+ * between two HIP events on the batch stream. */
+int fgxm_batch_timed_steps(fgxm_batch* b, uint32_t steps, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,3 +72,31 @@ def test_scratch_sizes_monotone():
         assert 0 < a <= b and a % 256 == 0 and b % 256 == 0
     with pytest.raises(fgx.FgxError):
         fgx.scratch_bytes(99, 1, 1)
+
+
+def test_seg_key_geometry_matches_single_simulation_entries():
+    for dims, mn, mx, r in [(3, [0, 0, 0], [25.398416] * 3, 2.0), (2, [0, 0], [1000, 1000], 5.0),
+                            (3, [0, 0, 0], [50.1] * 3, 10.0)]:
+        k = fgx.seg_key(fgx.FGX_SEG_MESSAGE_BINS, dims, mn, mx, r)
+        m = fgx.spatial_meta(dims, mn, mx, r)
+        assert k.keys == m.bins
+        assert [k.grid[a] for a in range(3)] == [max(m.grid[a], 1) for a in range(3)]
+        assert [k.width[a] for a in range(dims)] == [m.width[a] for a in range(dims)]
+        s = fgx.seg_key(fgx.FGX_SEG_AGENT_SORT, dims, mn, mx, r)
+        bits = fgx.agent_sort_key_bits(dims, mn, mx, r)
+        assert s.keys == 1 << bits and s.key_mask == (1 << bits) - 1
+
+
+def test_segments_pack_offsets():
+    ns = [5, 0, 2048, 2049, 1]
+    keys = []
+    for i, n in enumerate(ns):
+        k = fgx.seg_key(fgx.FGX_SEG_MESSAGE_BINS, 3, [0, 0, 0], [4.0 + i] * 3, 1.0, n)
+        keys.append(k)
+    n_total, keys_total, tiles, arr = fgx.segments_pack(keys)
+    assert n_total == sum(ns)
+    assert keys_total == sum(k.keys for k in keys)
+    assert tiles == sum((n + 2047) // 2048 for n in ns)
+    assert [arr[i].elem_off for i in range(len(ns))] == list(np.cumsum([0] + ns[:-1]))
+    assert [arr[i].key_off for i in range(len(ns))] == list(np.cumsum([0] + [k.keys for k in keys[:-1]]))
+    assert [arr[i].tile_off for i in range(len(ns))] == list(np.cumsum([0] + [(n + 2047) // 2048 for n in ns[:-1]]))

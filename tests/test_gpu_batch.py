@@ -1,0 +1,247 @@
+"""Batched simulations on the GPU: segmented primitives against per-simulation oracle
+runs, and whole batched models against the same simulations run alone (the
+simulation result must not depend on what shares the batch with it).
+
+Bars: bin tables, sort orders, IDs and FP32 state bit-exact against the oracle
+(boids, correctly rounded arithmetic only) and against the unbatched GPU run of
+the same simulation (circles, which calls sinf)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from flamegpu2_amd import workloads
+from flamegpu2_amd.models import Batch, Simulation
+from gpu_util import dev, empty, host, ptr, sync, scratch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from flamegpu2_amd import fgx
+    c = fgx.Context(0)
+    yield c
+    c.close()
+
+
+def _segments(seed):
+    """Simulations of uneven size and geometry, including an empty one and one spanning tiles."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for n, w, r in [(1000, 10.0, 1.0), (0, 5.0, 1.0), (5000, 20.0, 2.5), (2048, 7.0, 0.7), (3, 3.0, 1.0),
+                    (20000, 25.0, 3.0)]:
+        P = rng.uniform(-0.05 * w, 1.05 * w, size=(3, n)).astype(np.float32)
+        cases.append((n, w, r, P))
+    return cases
+
+
+def test_spatial_build_seg_matches_per_simulation_builds(ctx):
+    from flamegpu2_amd import fgx
+    cases = _segments(4)
+    keys = [fgx.seg_key(fgx.FGX_SEG_MESSAGE_BINS, 3, [0] * 3, [w] * 3, r, n) for (n, w, r, _) in cases]
+    n_total, keys_total, tiles, arr = fgx.segments_pack(keys)
+    X = np.concatenate([c[3] for c in cases], axis=1) if n_total else np.zeros((3, 0), np.float32)
+    gid = np.arange(n_total, dtype=np.uint32)
+    d_in = [dev(X[a]) for a in range(3)] + [dev(gid)]
+    d_out = [empty(n_total, np.float32) for _ in range(3)] + [empty(n_total, np.uint32)]
+    pbm = empty(keys_total + 1, np.uint32)
+    d_segs = dev(np.frombuffer(bytes(arr), dtype=np.uint8)[: ctypes.sizeof(fgx.SegKey) * len(cases)].copy())
+    sb = fgx.scratch_bytes(fgx.FGX_OP_SPATIAL_BUILD, n_total, keys_total)
+    s = scratch(sb)
+    for _ in range(2):
+        fgx.spatial_build_seg(ctx, ptr(d_segs), len(cases), tiles, n_total, keys_total, ptr(d_in[0]), ptr(d_in[1]),
+                              ptr(d_in[2]), [ptr(t) for t in d_in], [ptr(t) for t in d_out], [4, 4, 4, 4], ptr(pbm),
+                              ptr(s), sb)
+    sync()
+    got_pbm = host(pbm, np.uint32, keys_total + 1)
+    got_ids = host(d_out[3], np.uint32, n_total)
+    for i, (n, w, r, P) in enumerate(cases):
+        mo = O.spatial_meta(3, [0] * 3, [w] * 3, r)
+        local = np.arange(n, dtype=np.uint32)
+        pbm_o, _, outs_o = O.spatial_build(mo, P[0], P[1], P[2], extra=[local])
+        k0, e0 = arr[i].key_off, arr[i].elem_off
+        assert np.array_equal(got_pbm[k0:k0 + mo.bins + 1], pbm_o + e0), f"simulation {i}: bin table"
+        assert np.array_equal(got_ids[e0:e0 + n], outs_o[3] + e0), f"simulation {i}: message order"
+    assert got_pbm[keys_total] == n_total
+
+
+def test_agent_sort_seg_matches_per_simulation_sorts(ctx):
+    from flamegpu2_amd import fgx
+    cases = _segments(5)
+    keys = [fgx.seg_key(fgx.FGX_SEG_AGENT_SORT, 3, [0] * 3, [w] * 3, r, n) for (n, w, r, _) in cases]
+    n_total, keys_total, tiles, arr = fgx.segments_pack(keys)
+    X = np.concatenate([c[3] for c in cases], axis=1)
+    gid = np.arange(n_total, dtype=np.uint32)
+    d_in = [dev(X[a]) for a in range(3)] + [dev(gid)]
+    d_out = [empty(n_total, np.float32) for _ in range(3)] + [empty(n_total, np.uint32)]
+    d_segs = dev(np.frombuffer(bytes(arr), dtype=np.uint8)[: ctypes.sizeof(fgx.SegKey) * len(cases)].copy())
+    sb = fgx.scratch_bytes(fgx.FGX_OP_AGENT_SORT, n_total, keys_total)
+    s = scratch(sb)
+    fgx.agent_sort_seg(ctx, ptr(d_segs), len(cases), tiles, n_total, keys_total, ptr(d_in[0]), ptr(d_in[1]),
+                       ptr(d_in[2]), [ptr(t) for t in d_in], [ptr(t) for t in d_out], [4, 4, 4, 4], ptr(s), sb)
+    sync()
+    got = host(d_out[3], np.uint32, n_total)
+    for i, (n, w, r, P) in enumerate(cases):
+        k, _ = O.agent_sort_keys(3, [0] * 3, [w] * 3, r, P[0], P[1], P[2])
+        perm = O.stable_sort_perm(k)
+        e0 = arr[i].elem_off
+        assert np.array_equal(got[e0:e0 + n], perm.astype(np.uint32) + e0), f"simulation {i}: sort order"
+
+
+def test_copy_ranges_and_add_n(ctx):
+    from flamegpu2_amd import fgx
+    src = np.arange(100, dtype=np.uint32)
+    src8 = np.arange(100, dtype=np.float64) * 0.5
+    ranges = np.array([10, 0, 5, 40, 5, 7, 90, 12, 10], dtype=np.uint32)   # contiguous destinations
+    d_src, d_src8 = dev(src), dev(src8)
+    d_dst, d_dst8 = empty(22, np.uint32), empty(22, np.float64)
+    d_r = dev(ranges)
+    fgx.copy_ranges([ptr(d_src), ptr(d_src8)], [ptr(d_dst), ptr(d_dst8)], [4, 8], ptr(d_r), 3, 22)
+    steps = dev(np.array([0, 5, 7], np.uint32))
+    fgx.add_u32_n(ptr(steps), 3, 1)
+    sync()
+    exp = np.concatenate([src[10:15], src[40:47], src[90:100]])
+    assert np.array_equal(host(d_dst, np.uint32), exp)
+    assert np.array_equal(host(d_dst8, np.float64), exp * 0.5)
+    assert host(steps, np.uint32).tolist() == [1, 6, 8]
+
+
+def _boids_sims():
+    out = []
+    for k, (n, seed) in enumerate([(3000, 11), (500, 12), (12000, 13), (1, 14), (7000, 15)]):
+        params, arrs = workloads.boids(n, seed=seed, width=1.0 + 0.2 * k, radius=0.05)
+        out.append((params, arrs, 100 + k))
+    return out
+
+
+def test_batched_boids_bit_exact_against_oracle():
+    sims = _boids_sims()
+    b = Batch("boids_spatial3D", sims[0][0])
+    hs = [b.add(p, seed, 4, {"Boid": a}) for (p, a, seed) in sims]
+    b.step(4)
+    assert sorted(b.finished()) == sorted(hs)
+    for h, (p, a, seed) in zip(hs, sims):
+        ref = O.Sim(1, p, seed)
+        ref.set_agents(0, a)
+        ref.step(4)
+        got, gid = b.get_agents(h, "Boid")
+        exp, eid = ref.get_agents(0, [np.float32] * 6)
+        assert np.array_equal(gid, eid)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e)
+
+
+def _circles_sims():
+    out = []
+    rng = np.random.default_rng(3)
+    for k in range(6):
+        n = int(np.exp(rng.uniform(np.log(1024), np.log(30000))))
+        r = float(rng.uniform(1.0, 3.0))
+        params, arrs = workloads.circles(n, seed=200 + k, radius=r)
+        out.append((params, arrs, 300 + k))
+    return out
+
+
+def _alone(model, agent, params, arrs, seed, steps):
+    s = Simulation(model, params, seed=seed)
+    s.set_agents(agent, arrs)
+    s.step(steps)
+    return s.get_agents(agent)
+
+
+@pytest.mark.parametrize("tuning", [(256, 1, 0), (64, 1, 0), (1024, 4, 0), (128, 2, 2)])
+def test_batched_circles_same_as_alone(tuning):
+    sims = _circles_sims()
+    b = Batch("circles_spatial3D", sims[0][0])
+    b.set_tuning(*tuning)
+    hs = [b.add(p, seed, 6, {"Circle": a}) for (p, a, seed) in sims]
+    b.step(6)
+    for h, (p, a, seed) in zip(hs, sims):
+        got, gid = b.get_agents(h, "Circle")
+        exp, eid = _alone("circles_spatial3D", "Circle", p, a, seed, 6)
+        assert np.array_equal(gid, eid)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e), f"tuning {tuning}: simulation {h} differs from its unbatched run"
+
+
+def test_batched_join_and_leave_same_as_alone():
+    sims = _circles_sims()
+    b = Batch("circles_spatial3D", sims[0][0])
+    plan = {}
+    # three join now with 3, 5 and 8 steps; three more join after two steps
+    for (p, a, seed), steps in zip(sims[:3], [3, 5, 8]):
+        plan[b.add(p, seed, steps, {"Circle": a})] = (p, a, seed, steps)
+    b.step(2)
+    for (p, a, seed), steps in zip(sims[3:], [4, 1, 6]):
+        plan[b.add(p, seed, steps, {"Circle": a})] = (p, a, seed, steps)
+    results = {}
+    for _ in range(20):
+        for h in b.finished():
+            results[h] = b.get_agents(h, "Circle")
+            b.remove(h)
+        if len(results) == len(plan):
+            break
+        b.step(1)
+    assert len(results) == len(plan)
+    for h, (p, a, seed, steps) in plan.items():
+        got, gid = results[h]
+        exp, eid = _alone("circles_spatial3D", "Circle", p, a, seed, steps)
+        assert np.array_equal(gid, eid)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e), f"simulation {h} ({steps} steps) differs from its unbatched run"
+    info = b.info()
+    assert info["resident"] == 0
+
+
+def test_batched_graph_and_eager_agree():
+    sims = _circles_sims()[:3]
+    out = []
+    for graphs in (True, False):
+        b = Batch("circles_spatial3D", sims[0][0])
+        b.set_option("graphs", graphs)
+        hs = [b.add(p, seed, 5, {"Circle": a}) for (p, a, seed) in sims]
+        b.step(5)
+        out.append([b.get_agents(h, "Circle") for h in hs])
+    for (ga, ia), (gb, ib) in zip(*out):
+        assert np.array_equal(ia, ib)
+        for x, y in zip(ga, gb):
+            assert np.array_equal(x, y)
+
+
+def test_batch_rejects_growing_models():
+    from flamegpu2_amd.fgx import FgxError
+    params, prey, pred = workloads.predprey(100, 10, seed=1, width=100.0)
+    with pytest.raises(FgxError):
+        Batch("predator_prey", params)
+
+
+def test_serving_results_same_alone_and_in_the_mix():
+    from flamegpu2_amd import serving
+    reqs = [serving.Request(0, 0.00, "circles", 3000, 12, 11), serving.Request(1, 0.00, "boids", 5000, 9, 12),
+            serving.Request(2, 0.05, "circles", 1500, 4, 13), serving.Request(3, 0.08, "boids", 800, 15, 14),
+            serving.Request(4, 0.10, "circles", 9000, 7, 15), serving.Request(5, 0.12, "boids", 20000, 5, 16)]
+    srv = serving.Server(max_chunk=3)
+    res = srv.run(reqs, collect=True)
+    assert sorted(r["id"] for r in res) == list(range(len(reqs)))
+    for rec in res:
+        r = reqs[rec["id"]]
+        lib, agent, make = serving.MODELS[r.model]
+        params, pops = make(r.n, r.seed)
+        exp, eid = _alone(lib, agent, params, pops[agent], r.seed, r.steps)
+        assert np.array_equal(rec["ids"], eid), f"request {r.id}"
+        for g, e in zip(rec["agents"], exp):
+            assert np.array_equal(g, e), f"request {r.id} differs from its run alone"
+        assert rec["latency"] >= 0.0
+
+
+def test_serving_hbm_budget_admits_in_order():
+    from flamegpu2_amd import serving
+    reqs = [serving.Request(i, 0.0, "circles", 4000, 6, 100 + i) for i in range(5)]
+    srv = serving.Server(hbm_budget=1, max_chunk=2)   # one run at a time
+    res = srv.run(reqs)
+    order = sorted(res, key=lambda r: r["completed"])
+    assert [r["id"] for r in order] == list(range(5))
+    for a, b in zip(order, order[1:]):
+        assert b["admitted"] >= a["completed"] - 1e-3
