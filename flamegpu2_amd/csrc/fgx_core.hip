@@ -66,6 +66,13 @@ __global__ void k_add_u32(uint32_t* p, uint32_t delta) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p += delta;
 }
 
+__global__ void k_sub_u32(uint32_t* dst, const uint32_t* a, const uint32_t* b) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint32_t x = *a, y = *b;
+        *dst = x > y ? x - y : 0u;
+    }
+}
+
 __global__ void k_add_u32_n(uint32_t* __restrict__ p, uint32_t n, uint32_t delta) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] += delta;
@@ -307,6 +314,13 @@ int fgx_graph_destroy(fgx_graph* g) {
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
     delete g;
+    return FGX_OK;
+}
+
+int fgx_sub_u32(uint32_t* dst, const uint32_t* a, const uint32_t* b, void* stream) {
+    if (!dst || !a || !b) return fail(FGX_ERR_ARG, "fgx_sub_u32: NULL pointer");
+    hipLaunchKernelGGL(k_sub_u32, dim3(1), dim3(64), 0, (hipStream_t)stream, dst, a, b);
+    FGX_HIP(hipGetLastError());
     return FGX_OK;
 }
 

@@ -68,10 +68,13 @@ __device__ __forceinline__ uint32_t bin_key(const KeyArgs& k, float x, float y, 
 // Histogram with wave-level run aggregation: consecutive lanes with equal keys
 // (the common case, as agents are kept in bin order) issue one atomic per run.
 // sub[i] is the slot of message i inside its bin (atomic order, fixed later).
+// *unsorted is set when some key is smaller than its predecessor's: when it stays 0
+// the input is already in bin order and the scatters reduce to a copy.
 __global__ void __launch_bounds__(kBlock) k_bin_hist(KeyArgs ka, const uint32_t* d_n, uint32_t n_cap,
                                                      const float* __restrict__ x, const float* __restrict__ y,
                                                      const float* __restrict__ z, uint32_t* __restrict__ key,
-                                                     uint32_t* __restrict__ sub, uint32_t* __restrict__ hist) {
+                                                     uint32_t* __restrict__ sub, uint32_t* __restrict__ hist,
+                                                     uint32_t* __restrict__ unsorted) {
     const uint32_t n = load_count(d_n, n_cap);
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -82,6 +85,12 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist(KeyArgs ka, const uint32_t*
         uint32_t k = 0xFFFFFFFFu;
         if (valid) k = bin_key(ka, x[i], y[i], ka.dims == 3 ? z[i] : 0.0f);
         const uint32_t prev = __shfl_up(k, 1, 64);
+        {
+            uint32_t before = prev;
+            if (lane == 0 && base > 0) before = bin_key(ka, x[base - 1], y[base - 1], ka.dims == 3 ? z[base - 1] : 0.0f);
+            const bool down = valid && (lane > 0 || base > 0) && k < before;
+            if (__ballot(down) != 0ull && lane == 0) atomicOr(unsorted, 1u);
+        }
         const bool head = valid && (lane == 0 || k != prev);
         const uint64_t hm = __ballot(head);
         const uint64_t vm = __ballot(valid);
@@ -107,7 +116,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist_seg(int32_t mode, const fgx
                                                          uint32_t nsegs, const float* __restrict__ x,
                                                          const float* __restrict__ y, const float* __restrict__ z,
                                                          uint32_t* __restrict__ key, uint32_t* __restrict__ sub,
-                                                         uint32_t* __restrict__ hist) {
+                                                         uint32_t* __restrict__ hist, uint32_t* __restrict__ unsorted) {
     const uint32_t t = blockIdx.x;
     uint32_t lo = 0, hi = nsegs;
     while (hi - lo > 1u) {
@@ -137,6 +146,16 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist_seg(int32_t mode, const fgx
         uint32_t k = 0xFFFFFFFFu;
         if (valid) k = koff + bin_key(ka, x[i], y[i], ka.dims == 3 ? z[i] : 0.0f);
         const uint32_t prev = __shfl_up(k, 1, 64);
+        {
+            // the predecessor of a simulation's first element belongs to an earlier one,
+            // whose keys are all smaller
+            const uint32_t c0 = first + c * 64u;
+            const bool has_pred = c0 > sg.elem_off;
+            uint32_t before = prev;
+            if (lane == 0 && has_pred) before = koff + bin_key(ka, x[c0 - 1], y[c0 - 1], ka.dims == 3 ? z[c0 - 1] : 0.0f);
+            const bool down = valid && (lane > 0 || has_pred) && k < before;
+            if (__ballot(down) != 0ull && lane == 0) atomicOr(unsorted, 1u);
+        }
         const bool head = valid && (lane == 0 || k != prev);
         const uint64_t hm = __ballot(head);
         const uint64_t vm = __ballot(valid);
@@ -157,7 +176,9 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist_seg(int32_t mode, const fgx
 
 __global__ void __launch_bounds__(kBlock) k_index_scatter(const uint32_t* d_n, uint32_t n_cap,
                                                           const uint32_t* __restrict__ key, const uint32_t* __restrict__ sub,
-                                                          const uint32_t* __restrict__ start, uint32_t* __restrict__ tmp) {
+                                                          const uint32_t* __restrict__ start, uint32_t* __restrict__ tmp,
+                                                          const uint32_t* __restrict__ unsorted) {
+    if (!*unsorted) return;   // in bin order already: the permutation is the identity
     const uint32_t n = load_count(d_n, n_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -170,9 +191,14 @@ __global__ void __launch_bounds__(kBlock) k_index_scatter(const uint32_t* d_n, u
 __global__ void __launch_bounds__(kBlock) k_stable_scatter(const uint32_t* d_n, uint32_t n_cap,
                                                            const uint32_t* __restrict__ key,
                                                            const uint32_t* __restrict__ start,
-                                                           const uint32_t* __restrict__ tmp, VarTable vt) {
+                                                           const uint32_t* __restrict__ tmp, VarTable vt,
+                                                           const uint32_t* __restrict__ unsorted) {
     const uint32_t n = load_count(d_n, n_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
+    if (!*unsorted) {
+        for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) copy_element(vt, s, s);
+        return;
+    }
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
         const uint32_t e = tmp[s];
         const uint32_t b = key[e];
@@ -242,27 +268,38 @@ struct CompactArgs {
     int32_t id_var;
     uint32_t* d_dst_count;
     uint32_t dst_capacity;
-    uint32_t* d_next_id;
+    uint32_t* d_next_id;       // may be NULL (no IDs)
     uint32_t* d_overflow;
+    // item ranges (device-resident, NULL = 0): items are [*d_src_off, n); item k has
+    // flag k - *d_keep_prefix, and the first *d_keep_prefix items are kept unconditionally
+    const uint32_t* d_src_off;
+    const uint32_t* d_keep_prefix;
 };
 
 __global__ void __launch_bounds__(kBlock) k_compact(CompactArgs a, VarTable vt) {
-    __shared__ uint32_t s_tile, s_epoch, s_prefix, s_dst0, s_id0;
+    __shared__ uint32_t s_tile, s_epoch, s_prefix, s_dst0, s_id0, s_off, s_pre;
     __shared__ uint32_t s_cnt[kItems * (kBlock / 64)];
-    if (threadIdx.x == 0 && a.births) {
-        s_dst0 = __hip_atomic_load(a.d_dst_count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        s_id0 = __hip_atomic_load(a.d_next_id, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+        if (a.births) {
+            s_dst0 = __hip_atomic_load(a.d_dst_count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            s_id0 = a.d_next_id ? __hip_atomic_load(a.d_next_id, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        }
+        s_off = a.d_src_off ? __hip_atomic_load(a.d_src_off, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        s_pre = a.d_keep_prefix ? __hip_atomic_load(a.d_keep_prefix, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0u;
     }
-    acquire_tile(a.hdr, a.num_tiles, &s_tile, &s_epoch);
+    acquire_tile(a.hdr, a.num_tiles, &s_tile, &s_epoch);   // (its barrier publishes the values above)
     const uint32_t tile = s_tile;
-    const uint32_t n = load_count(a.d_n, a.n_cap);
+    const uint32_t n_all = load_count(a.d_n, a.n_cap);
+    const uint32_t off = s_off < n_all ? s_off : n_all;
+    const uint32_t n = n_all - off;   // items k in [0, n), source element off + k
+    const uint32_t pre = s_pre;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint64_t lanes_below = (1ull << lane) - 1ull;
     uint64_t masks[kItems];
 #pragma unroll
     for (uint32_t j = 0; j < kItems; ++j) {
         const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-        const bool keep = i < n && (!a.flags || ((a.flags[i] != 0u) == (a.keep != 0u)));
+        const bool keep = i < n && (i < pre || !a.flags || ((a.flags[i - pre] != 0u) == (a.keep != 0u)));
         masks[j] = __ballot(keep);
         if (lane == 0) s_cnt[j * (kBlock / 64) + w] = (uint32_t)__builtin_popcountll(masks[j]);
     }
@@ -286,7 +323,7 @@ __global__ void __launch_bounds__(kBlock) k_compact(CompactArgs a, VarTable vt) 
                     if (a.d_overflow) *a.d_overflow = 1u;
                 }
                 *a.d_dst_count = nd;
-                *a.d_next_id = s_id0 + kept;
+                if (a.d_next_id) *a.d_next_id = s_id0 + kept;
             }
         }
     }
@@ -298,11 +335,11 @@ __global__ void __launch_bounds__(kBlock) k_compact(CompactArgs a, VarTable vt) 
         const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
         const uint32_t r = prefix + s_cnt[j * (kBlock / 64) + w] + (uint32_t)__builtin_popcountll(masks[j] & lanes_below);
         if (!a.births) {
-            copy_element(vt, i, r);
+            copy_element(vt, off + i, r);
         } else {
             const uint32_t d = s_dst0 + r;
             if (d >= a.dst_capacity) continue;
-            copy_element(vt, i, d, a.id_var);
+            copy_element(vt, off + i, d, a.id_var);
             if (a.id_var >= 0) reinterpret_cast<uint32_t*>(vt.out[a.id_var])[d] = s_id0 + r;
         }
     }
@@ -349,7 +386,7 @@ struct BuildLayout {
 static BuildLayout build_layout(uint32_t n, uint32_t bins) {
     BuildLayout L;
     size_t off = 0;
-    L.hist = off; off += align256((size_t)(bins + 1) * 4);
+    L.hist = off; off += align256((size_t)(bins + 2) * 4);   // counts [0, bins], "unsorted" flag at [bins + 1]
     L.key = off; off += align256((size_t)std::max(n, 1u) * 4);
     L.sub = off; off += align256((size_t)std::max(n, 1u) * 4);
     L.tmp = off; off += align256((size_t)std::max(n, 1u) * 4);
@@ -395,11 +432,12 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
     auto* sub = reinterpret_cast<uint32_t*>(s + L.sub);
     auto* tmp = reinterpret_cast<uint32_t*>(s + L.tmp);
     hipStream_t st = (hipStream_t)stream;
-    int rc = fgx_fill_u32(hist, 0u, (size_t)bins + 1, stream);
+    uint32_t* unsorted = hist + bins + 1;
+    int rc = fgx_fill_u32(hist, 0u, (size_t)bins + 2, stream);
     if (rc) return rc;
     const uint32_t g = stream_grid(ctx ? &ctx->c : nullptr, n_cap);
     if (n_cap) {
-        hipLaunchKernelGGL(k_bin_hist, dim3(g), dim3(kBlock), 0, st, ka, d_n, n_cap, x, y, z, key, sub, hist);
+        hipLaunchKernelGGL(k_bin_hist, dim3(g), dim3(kBlock), 0, st, ka, d_n, n_cap, x, y, z, key, sub, hist, unsorted);
         FGX_HIP(hipGetLastError());
     }
     // scan hist -> start (bins+1 entries, start[bins] = n)
@@ -411,9 +449,9 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
         FGX_HIP(hipGetLastError());
     }
     if (n_cap) {
-        hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, sub, start_out, tmp);
+        hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, sub, start_out, tmp, unsorted);
         FGX_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, start_out, tmp, vt);
+        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, start_out, tmp, vt, unsorted);
         FGX_HIP(hipGetLastError());
     }
     return FGX_OK;
@@ -431,10 +469,12 @@ static int run_bin_sort_seg(fgx_ctx* ctx, int32_t mode, const fgx_seg_key* d_seg
     auto* sub = reinterpret_cast<uint32_t*>(s + L.sub);
     auto* tmp = reinterpret_cast<uint32_t*>(s + L.tmp);
     hipStream_t st = (hipStream_t)stream;
-    int rc = fgx_fill_u32(hist, 0u, (size_t)keys_total + 1, stream);
+    uint32_t* unsorted = hist + keys_total + 1;
+    int rc = fgx_fill_u32(hist, 0u, (size_t)keys_total + 2, stream);
     if (rc) return rc;
     if (n_total && tiles) {
-        hipLaunchKernelGGL(k_bin_hist_seg, dim3(tiles), dim3(kBlock), 0, st, mode, d_segs, nsegs, x, y, z, key, sub, hist);
+        hipLaunchKernelGGL(k_bin_hist_seg, dim3(tiles), dim3(kBlock), 0, st, mode, d_segs, nsegs, x, y, z, key, sub, hist,
+                           unsorted);
         FGX_HIP(hipGetLastError());
     }
     {
@@ -447,9 +487,11 @@ static int run_bin_sort_seg(fgx_ctx* ctx, int32_t mode, const fgx_seg_key* d_seg
     }
     if (n_total) {
         const uint32_t g = stream_grid(ctx ? &ctx->c : nullptr, n_total);
-        hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, nullptr, n_total, key, sub, start_out, tmp);
+        hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, nullptr, n_total, key, sub, start_out, tmp,
+                           unsorted);
         FGX_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, nullptr, n_total, key, start_out, tmp, vt);
+        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, nullptr, n_total, key, start_out, tmp, vt,
+                           unsorted);
         FGX_HIP(hipGetLastError());
     }
     return FGX_OK;
@@ -637,6 +679,85 @@ int fgx_append_births(fgx_ctx* ctx, const uint32_t* flags, const uint32_t* d_n_p
     a.dst_capacity = dst_capacity;
     a.d_next_id = d_next_id;
     a.d_overflow = d_overflow;
+    return launch_compact(a, vt, scratch, scratch_bytes, stream);
+}
+
+int fgx_partition(fgx_ctx* ctx, const uint32_t* flags, const uint32_t* d_n, uint32_t n_cap, uint32_t nvars,
+                  const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes, uint32_t* d_first_count,
+                  uint32_t* d_work, void* scratch, size_t scratch_bytes, void* stream) {
+    (void)ctx;
+    if (!d_first_count || !d_work) return fail(FGX_ERR_ARG, "fgx_partition: NULL device counters");
+    if (n_cap && !flags) return fail(FGX_ERR_ARG, "fgx_partition: flags is NULL");
+    VarTable vt;
+    int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
+    if (rc) return rc;
+    // pass 1: flagged items to the front, their count to d_first_count
+    CompactArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.flags = flags;
+    a.keep = 1;
+    a.d_n = d_n;
+    a.n_cap = n_cap;
+    a.d_count_out = d_first_count;
+    a.id_var = -1;
+    rc = launch_compact(a, vt, scratch, scratch_bytes, stream);
+    if (rc) return rc;
+    // pass 2: the others appended after them (same scratch: the epoch advanced)
+    rc = fgx_memcpy_d2d(d_work, d_first_count, 4, stream);
+    if (rc) return rc;
+    CompactArgs b;
+    std::memset(&b, 0, sizeof(b));
+    b.flags = flags;
+    b.keep = 0;
+    b.d_n = d_n;
+    b.n_cap = n_cap;
+    b.births = 1;
+    b.id_var = -1;
+    b.d_dst_count = d_work;
+    b.dst_capacity = 0xFFFFFFFFu;
+    return launch_compact(b, vt, scratch, scratch_bytes, stream);
+}
+
+int fgx_compact_prefix(fgx_ctx* ctx, const uint32_t* flags, uint32_t keep, const uint32_t* d_prefix, const uint32_t* d_n,
+                       uint32_t n_cap, uint32_t nvars, const void* const* vars_in, void* const* vars_out,
+                       const uint32_t* var_bytes, uint32_t* d_count_out, void* scratch, size_t scratch_bytes, void* stream) {
+    (void)ctx;
+    if (!d_count_out || !d_prefix) return fail(FGX_ERR_ARG, "fgx_compact_prefix: NULL device counters");
+    if (n_cap && !flags) return fail(FGX_ERR_ARG, "fgx_compact_prefix: flags is NULL");
+    VarTable vt;
+    int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
+    if (rc) return rc;
+    CompactArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.flags = flags;
+    a.keep = keep;
+    a.d_n = d_n;
+    a.n_cap = n_cap;
+    a.d_count_out = d_count_out;
+    a.id_var = -1;
+    a.d_keep_prefix = d_prefix;
+    return launch_compact(a, vt, scratch, scratch_bytes, stream);
+}
+
+int fgx_append_range(fgx_ctx* ctx, const uint32_t* d_src_off, const uint32_t* d_n, uint32_t n_cap, uint32_t nvars,
+                     const void* const* src, void* const* dst, const uint32_t* var_bytes, uint32_t* d_dst_count,
+                     uint32_t dst_capacity, uint32_t* d_overflow, void* scratch, size_t scratch_bytes, void* stream) {
+    (void)ctx;
+    if (!d_dst_count || !d_src_off) return fail(FGX_ERR_ARG, "fgx_append_range: NULL device counters");
+    VarTable vt;
+    int rc = make_var_table(nvars, src, dst, var_bytes, &vt);
+    if (rc) return rc;
+    CompactArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.keep = 1;
+    a.d_n = d_n;
+    a.n_cap = n_cap;
+    a.births = 1;
+    a.id_var = -1;
+    a.d_dst_count = d_dst_count;
+    a.dst_capacity = dst_capacity;
+    a.d_overflow = d_overflow;
+    a.d_src_off = d_src_off;
     return launch_compact(a, vt, scratch, scratch_bytes, stream);
 }
 

@@ -248,6 +248,8 @@ inline void BatchSimulation::validate() {
                                                   "' changes the population or its state; batches need fixed populations");
             if (f->message_output_optional)
                 throw exception::InvalidOperation("BatchSimulation: optional message output ('" + f->name + "') cannot be batched");
+            if (f->condition)
+                throw exception::InvalidOperation("BatchSimulation: agent function conditions ('" + f->name + "') cannot be batched");
             if (auto mi = f->message_input.lock())
                 if (!written.count(mi->name))
                     throw exception::InvalidOperation("BatchSimulation: message list '" + mi->name +

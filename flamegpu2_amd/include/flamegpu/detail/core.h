@@ -224,6 +224,7 @@ struct FnArgs {
     const char* env;             // environment property buffer (device)
     const uint32_t* d_msg_in_count;
     const uint32_t* ids;         // the agent list's "_id" array
+    const uint32_t* d_first;     // function condition: agents [*d_first, n) run (NULL: all)
     int32_t msg_out_optional;
     uint32_t agent_mul, msg_in_mul, msg_out_mul, new_mul, env_mul;
     // neighbourhood staging (spatial message input)

@@ -53,8 +53,11 @@ typedef CONDITION_RESULT (*FLAMEGPU_EXIT_CONDITION_POINTER)(HostAPI*);
 
 namespace detail {
 typedef void (*LaunchFn)(const FnArgs& args, uint32_t grid, uint32_t block, uint32_t smem, void* stream);
+typedef void (*CondLaunchFn)(const FnArgs& args, uint32_t* flags, uint32_t grid, void* stream);
 typedef void (*LaunchSegFn)(const FnArgs& args, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t apt,
                             uint32_t grid, uint32_t block, void* stream);
+template <typename C>
+void launch_condition(const FnArgs& a, uint32_t* flags, uint32_t grid, void* stream);
 template <typename T>
 Variable make_variable(const std::string& name, size_type elements, const T* defaults) {
     static_assert(std::is_arithmetic<T>::value, "agent/message variables must be arithmetic types");
@@ -86,6 +89,7 @@ struct AgentFunctionData {
     AgentData* parent = nullptr;
     detail::LaunchFn launch = nullptr;
     detail::LaunchSegFn launch_seg = nullptr;   // batched (segmented) launch
+    detail::CondLaunchFn condition = nullptr;   // agent function condition (NULL: none)
     MsgKind in_kind = MsgKind::None, out_kind = MsgKind::None;
     std::weak_ptr<MessageData> message_input, message_output;
     bool message_output_optional = false;
@@ -282,6 +286,12 @@ class AgentFunctionDescription {
     void setAgentOutput(const std::string& agent_name, const std::string& state = ModelData::DEFAULT_STATE());
     void setAgentOutput(const AgentDescription& agent, const std::string& state = ModelData::DEFAULT_STATE());
     void setAllowAgentDeath(bool d) { d_->has_agent_death = d; }
+    // include/flamegpu/model/AgentFunctionDescription.h: setFunctionCondition(FLAMEGPU_AGENT_FUNCTION_CONDITION)
+    template <typename C>
+    void setFunctionCondition(C) {
+        d_->condition = &detail::launch_condition<C>;
+    }
+    bool hasFunctionCondition() const { return d_->condition != nullptr; }
     void dependsOn(const AgentFunctionDescription& other) {
         if (other.d_.get() == d_.get()) throw exception::InvalidDependencyGraph("A function cannot depend on itself");
         d_->dependencies.push_back(other.d_.get());

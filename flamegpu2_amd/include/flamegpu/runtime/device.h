@@ -50,6 +50,8 @@ struct ThreadCtx {
     const StageView* stage;     // null outside the staged kernel
     uint32_t msg_off;           // brute-force input range [msg_off, msg_off + msg_n)
     uint32_t msg_n;
+    uint32_t lidx;              // index among the agents this launch runs (output slots, flags)
+    uint32_t stream_id;         // RNG stream of the call
 };
 
 // Global-memory pointers (address space 1), so loads through pointers rebuilt from
@@ -159,7 +161,7 @@ class AgentRandom {
         const uint32_t blk = counter_ >> 2;
         if (blk != cached_) {
             const detail::FnArgs& a = c_.a;
-            uint32_t c[4] = {blk, a.stream_id, c_.d_step ? *c_.d_step : 0u, a.ids ? a.ids[c_.idx] : c_.idx};
+            uint32_t c[4] = {blk, c_.stream_id, c_.d_step ? *c_.d_step : 0u, a.ids ? a.ids[c_.idx] : c_.idx};
             detail::philox4x32_10(c, (uint32_t)c_.seed, (uint32_t)(c_.seed >> 32));
             r0_ = c[0];
             r1_ = c[1];
@@ -338,17 +340,17 @@ class MessageBruteForce::Out {
     __device__ __forceinline__ void setVariable(const char (&name)[N], T value) const {
         const bool on = c_.a.msg_out_flags != nullptr;
         c_.err.note(!on, detail::DERR_MSG_OUTPUT_DISABLED, 0);
-        detail::store_var<T>(c_.a.msg_out, c_.a.msg_out_mul, name, c_.idx, value, c_, detail::DERR_UNKNOWN_MSG_VAR,
+        detail::store_var<T>(c_.a.msg_out, c_.a.msg_out_mul, name, c_.lidx, value, c_, detail::DERR_UNKNOWN_MSG_VAR,
                              detail::DERR_MSG_VAR_TYPE);
-        if (on && c_.a.msg_out_optional) c_.a.msg_out_flags[c_.idx] = 1u;
+        if (on && c_.a.msg_out_optional) c_.a.msg_out_flags[c_.lidx] = 1u;
     }
     template <typename T, size_type L, unsigned N>
     __device__ __forceinline__ void setVariable(const char (&name)[N], unsigned int e, T value) const {
         const bool on = c_.a.msg_out_flags != nullptr;
         c_.err.note(!on, detail::DERR_MSG_OUTPUT_DISABLED, 0);
-        detail::store_elem<T>(c_.a.msg_out, c_.a.msg_out_mul, name, c_.idx, L, e, value, c_, detail::DERR_UNKNOWN_MSG_VAR,
+        detail::store_elem<T>(c_.a.msg_out, c_.a.msg_out_mul, name, c_.lidx, L, e, value, c_, detail::DERR_UNKNOWN_MSG_VAR,
                               detail::DERR_MSG_VAR_TYPE);
-        if (on && c_.a.msg_out_optional) c_.a.msg_out_flags[c_.idx] = 1u;
+        if (on && c_.a.msg_out_optional) c_.a.msg_out_flags[c_.lidx] = 1u;
     }
 
  protected:
@@ -387,9 +389,10 @@ constexpr uint32_t kHashId = name_hash("id", 3);
 //  Filter 3-D (MessageSpatial3DDevice.cuh:52-59,693-719): 9 x-strips, dy outer, dz inner;
 //  WrapFilter 3-D (:264-279,727-749): 27 cells, dx outer, dy, dz inner, modulo grid;
 //  Filter 2-D (MessageSpatial2DDevice.cuh:646-672): 3 strips; WrapFilter 2-D (:679-698): 9 cells.
-// Units (strips or cells) are numbered in visit order; a unit's neighbour offsets
-// follow from its number by constant divisions, and its bins from the search
-// origin's cell with selects only (no per-lane branches).
+// Units (strips or cells) are visited with up to three counters (c0 fastest) and the
+// matching coordinate parts of the unit's first bin (pre-multiplied by their stride
+// and wrapped in place for the wrapping iterators, raw row coordinates for the
+// clamped ones), advanced with selects only: no divisions, no per-lane branches.
 //
 // The iterator keeps one candidate of look-ahead: while the body runs on message k,
 // the loads of message k+1's x, y (, z) and id are in flight, and the bin-table
@@ -404,6 +407,12 @@ template <int DIMS, bool WRAP>
 class SpatialFilter {
  public:
     static constexpr int kUnits = WRAP ? (DIMS == 3 ? 27 : 9) : (DIMS == 3 ? 9 : 3);
+    // counter levels: wrap 3-D (dz, dy, dx), wrap 2-D (dy, dx), clamped 3-D (dz, dy), clamped 2-D (dy)
+    static constexpr int kLevels = WRAP ? (DIMS == 3 ? 3 : 2) : (DIMS == 3 ? 2 : 1);
+    struct Unit {
+        int c0, c1, c2;   // counters, 0..2
+        int p0, p1, p2;   // coordinate parts of the unit's first bin
+    };
 
     __device__ __forceinline__ SpatialFilter(const detail::ThreadCtx& c, float x, float y, float z, bool allow_stage)
         : c_(c) {
@@ -507,7 +516,8 @@ class SpatialFilter {
         // first message
         __device__ __forceinline__ iterator(const SpatialFilter& f, int) : f_(f) {
             pu_ = 0;
-            f_.load_unit(0, ps_, pe_, po_);
+            f_.first_unit(k_);
+            f_.load_unit(k_, ps_, pe_, po_);
             next_unit();
             fetch();
             shift();
@@ -531,7 +541,10 @@ class SpatialFilter {
                 e_ = pe_;
                 o_ = po_;
                 ++pu_;
-                if (pu_ < kUnits) f_.load_unit(pu_, ps_, pe_, po_);
+                if (pu_ < kUnits) {
+                    f_.next_unit(k_);
+                    f_.load_unit(k_, ps_, pe_, po_);
+                }
                 if (l_ < e_) {
                     more_ = true;
                     return;
@@ -564,8 +577,9 @@ class SpatialFilter {
         uint32_t l_ = 0, e_ = 0, o_ = 0;
         float nx_ = 0.f, ny_ = 0.f, nz_ = 0.f;
         uint32_t nid_ = 0;
-        // the unit after the look-ahead's: its number and loaded range
+        // the unit after the look-ahead's: its number, counters and loaded range
         int pu_ = 0;
+        Unit k_;
         uint32_t ps_ = 0, pe_ = 0, po_ = 0;
     };
     __device__ __forceinline__ iterator begin() const { return iterator(*this, 0); }
@@ -581,59 +595,101 @@ class SpatialFilter {
         gy_ = (int)m.grid[1];
         gz_ = (int)m.grid[2];
         cx_ = cx;
-        cy_ = cy;
-        cz_ = cz;
         ebase_ = ebase;
         xbase_ = xbase;
         ewidth_ = width;
-        if (!WRAP) {
+        const int sy = gx_, sz = gx_ * gy_;
+        if (WRAP) {
+            xs_ = 0;
+            span_ = 0;
+            if (DIMS == 3) {   // c0: z, c1: y, c2: x
+                st0_ = detail::wrap_cell(cz, -1, gz_) * sz;
+                st1_ = detail::wrap_cell(cy, -1, gy_) * sy;
+                st2_ = detail::wrap_cell(cx, -1, gx_);
+                m0_ = (gz_ - 1) * sz;
+                m1_ = (gy_ - 1) * sy;
+                m2_ = gx_ - 1;
+                s0_ = sz;
+                s1_ = sy;
+            } else {           // c0: y, c1: x
+                st0_ = detail::wrap_cell(cy, -1, gy_) * sy;
+                st1_ = detail::wrap_cell(cx, -1, gx_);
+                st2_ = 0;
+                m0_ = (gy_ - 1) * sy;
+                m1_ = gx_ - 1;
+                m2_ = 0;
+                s0_ = sy;
+                s1_ = 1;
+            }
+        } else {
             xs_ = cx - 1 < 0 ? 0 : cx - 1;
             const int xe = cx + 1 > gx_ - 1 ? gx_ - 1 : cx + 1;
             span_ = xe - xs_;
-        } else {
-            xs_ = 0;
-            span_ = 0;
+            // raw row coordinates; c0: z (3-D) or y (2-D), c1: y (3-D)
+            st0_ = DIMS == 3 ? cz - 1 : cy - 1;
+            st1_ = cy - 1;
+            st2_ = 0;
+            m0_ = m1_ = m2_ = 0;
+            s0_ = s1_ = 1;
         }
     }
-    // Range [s, e) of unit u in the source's index space; o = global - source index.
-    __device__ __forceinline__ void load_unit(int u, uint32_t& s, uint32_t& e, uint32_t& o) const {
-        int dx = 0, dy, dz = 0;
-        if (WRAP && DIMS == 3) {
-            dx = u / 9;
-            const int r = u - 9 * dx;
-            dy = r / 3;
-            dz = r - 3 * dy;
-        } else if (WRAP || DIMS == 3) {
-            const int q = u / 3;
-            if (WRAP) {
-                dx = q;
-                dy = u - 3 * q;
-            } else {
-                dy = q;
-                dz = u - 3 * q;
+    __device__ __forceinline__ void first_unit(Unit& k) const {
+        k.c0 = k.c1 = k.c2 = 0;
+        k.p0 = st0_;
+        k.p1 = st1_;
+        k.p2 = st2_;
+    }
+    __device__ __forceinline__ void next_unit(Unit& k) const {
+        const bool w0 = k.c0 == 2;
+        k.c0 = w0 ? 0 : k.c0 + 1;
+        if (WRAP) {
+            k.p0 = w0 ? st0_ : (k.p0 == m0_ ? 0 : k.p0 + s0_);
+            const bool w1 = w0 && k.c1 == 2;
+            k.c1 = w0 ? (w1 ? 0 : k.c1 + 1) : k.c1;
+            k.p1 = w0 ? (w1 ? st1_ : (k.p1 == m1_ ? 0 : k.p1 + s1_)) : k.p1;
+            if (kLevels == 3) {
+                k.c2 = w1 ? k.c2 + 1 : k.c2;
+                k.p2 = w1 ? (k.p2 == m2_ ? 0 : k.p2 + 1) : k.p2;
             }
         } else {
-            dy = u;
+            k.p0 = w0 ? st0_ : k.p0 + 1;
+            if (kLevels == 2) {
+                k.c1 = w0 ? k.c1 + 1 : k.c1;
+                k.p1 = w0 ? k.p1 + 1 : k.p1;
+            }
         }
-        // offsets are 0..2 here (neighbour offset + 1)
-        const int ay = cy_ + dy - 1;
-        const int az = cz_ + dz - 1;
+    }
+    // Range [s, e) of unit k in the source's index space; o = global - source index.
+    __device__ __forceinline__ void load_unit(const Unit& k, uint32_t& s, uint32_t& e, uint32_t& o) const {
         bool ok = true;
-        if (!WRAP) ok = ay >= 0 && ay < gy_ && (DIMS == 2 || (az >= 0 && az < gz_));
+        if (!WRAP) {
+            if (DIMS == 3) ok = k.p0 >= 0 && k.p0 < gz_ && k.p1 >= 0 && k.p1 < gy_;
+            else ok = k.p0 >= 0 && k.p0 < gy_;
+        }
         uint32_t first;
         if (sv_) {
-            const int row = DIMS == 3 ? 3 * dy + dz : dy;
-            const int vx = WRAP ? cx_ + dx - 1 : xs_;
+            // rows: dy * 3 + dz (3-D) or dy (2-D); x: virtual, from xbase
+            int row, vx;
+            if (WRAP) {
+                row = DIMS == 3 ? 3 * k.c1 + k.c0 : k.c0;
+                vx = cx_ - 1 + (DIMS == 3 ? k.c2 : k.c1);
+            } else {
+                row = DIMS == 3 ? 3 * k.c1 + k.c0 : k.c0;
+                vx = xs_;
+            }
             first = ok ? (uint32_t)(ebase_ + row * ewidth_ + (vx - xbase_)) : 0u;
             s = sv_->lstart[first];
             const uint32_t t = sv_->lstart[first + (uint32_t)span_ + 1u];
             o = sv_->gstart[first] - s;
             e = ok ? t : s;
         } else {
-            const int x = WRAP ? detail::wrap_cell(cx_, dx - 1, gx_) : xs_;
-            const int y = WRAP ? detail::wrap_cell(cy_, dy - 1, gy_) : ay;
-            const int z = DIMS == 3 ? (WRAP ? detail::wrap_cell(cz_, dz - 1, gz_) : az) : 0;
-            first = ok ? (uint32_t)((z * gy_ + y) * gx_ + x) : 0u;
+            if (WRAP) {
+                first = (uint32_t)(k.p0 + k.p1 + (DIMS == 3 ? k.p2 : 0));
+            } else if (DIMS == 3) {
+                first = ok ? (uint32_t)((k.p0 * gy_ + k.p1) * gx_ + xs_) : 0u;
+            } else {
+                first = ok ? (uint32_t)(k.p0 * gx_ + xs_) : 0u;
+            }
             uint32_t t;
             if (WRAP) {
                 uint32_t pair[2];
@@ -669,8 +725,10 @@ class SpatialFilter {
     float loc_[3];
     float width_[3];
     int gx_, gy_, gz_;              // grid
-    int cx_, cy_, cz_;              // search origin's cell
+    int cx_;                        // search origin's x cell (LDS source)
     int xs_, span_;                 // non-wrapping strips: first x, last x - first x
+    int st0_, st1_, st2_;           // coordinate parts of the first unit
+    int m0_, m1_, m2_, s0_, s1_;    // wrapping: last part value and stride per level
     int ebase_, xbase_, ewidth_;    // LDS source: covering group
     const uint32_t* pbm_;
     const float *px_, *py_, *pz_;
@@ -757,9 +815,9 @@ class DeviceAPI {
             }
             const bool on = c_.a.birth_flags != nullptr;
             c_.err.note(!on, detail::DERR_AGENT_OUTPUT_DISABLED, 0);
-            detail::store_var<T>(c_.a.new_agent, c_.a.new_mul, name, c_.idx, value, c_, detail::DERR_UNKNOWN_AGENT_VAR,
+            detail::store_var<T>(c_.a.new_agent, c_.a.new_mul, name, c_.lidx, value, c_, detail::DERR_UNKNOWN_AGENT_VAR,
                                  detail::DERR_AGENT_VAR_TYPE);
-            if (on) c_.a.birth_flags[c_.idx] = 1u;
+            if (on) c_.a.birth_flags[c_.lidx] = 1u;
         }
         template <typename T, size_type L, unsigned N>
         __device__ __forceinline__ void setVariable(const char (&name)[N], unsigned int e, T value) const {
@@ -769,9 +827,9 @@ class DeviceAPI {
             }
             const bool on = c_.a.birth_flags != nullptr;
             c_.err.note(!on, detail::DERR_AGENT_OUTPUT_DISABLED, 0);
-            detail::store_elem<T>(c_.a.new_agent, c_.a.new_mul, name, c_.idx, L, e, value, c_,
+            detail::store_elem<T>(c_.a.new_agent, c_.a.new_mul, name, c_.lidx, L, e, value, c_,
                                   detail::DERR_UNKNOWN_AGENT_VAR, detail::DERR_AGENT_VAR_TYPE);
-            if (on) c_.a.birth_flags[c_.idx] = 1u;
+            if (on) c_.a.birth_flags[c_.lidx] = 1u;
         }
         // New-agent IDs are nextID + the parent's rank among this launch's births, which is
         // only known after the launch; the ID can be read from the new agent next step.
@@ -829,6 +887,34 @@ class DeviceAPI {
     const detail::ThreadCtx& c_;
 };
 
+// ------------------------------------------------------------------ ReadOnlyDeviceAPI
+// include/flamegpu/runtime/DeviceAPI.cuh:40-160: the API of agent function conditions
+// (variables read-only, no messages, no agent output).  Conditions draw from their own
+// RNG stream (the function's stream with the top bit set).
+class ReadOnlyDeviceAPI {
+ public:
+    __device__ __forceinline__ explicit ReadOnlyDeviceAPI(const detail::ThreadCtx& c) : random(c), environment(c), c_(c) {}
+    template <typename T, unsigned N>
+    __device__ __forceinline__ T getVariable(const char (&name)[N]) const {
+        return detail::load_var<T>(c_.a.agent, c_.a.agent_mul, name, c_.idx, c_, detail::DERR_UNKNOWN_AGENT_VAR,
+                                   detail::DERR_AGENT_VAR_TYPE);
+    }
+    template <typename T, size_type L, unsigned N>
+    __device__ __forceinline__ T getVariable(const char (&name)[N], unsigned int e) const {
+        return detail::load_elem<T>(c_.a.agent, c_.a.agent_mul, name, c_.idx, L, e, c_, detail::DERR_UNKNOWN_AGENT_VAR,
+                                    detail::DERR_AGENT_VAR_TYPE);
+    }
+    __device__ __forceinline__ id_t getID() const { return c_.a.ids ? c_.a.ids[c_.idx] : ID_NOT_SET; }
+    __device__ __forceinline__ unsigned int getStepCounter() const { return c_.d_step ? *c_.d_step : 0u; }
+    __device__ __forceinline__ unsigned int getThreadIndex() const { return c_.idx; }
+
+    AgentRandom random;
+    DeviceEnvironment environment;
+
+ private:
+    const detail::ThreadCtx& c_;
+};
+
 // ------------------------------------------------------------------ the agent-function kernels
 // include/flamegpu/runtime/AgentFunction.cuh:60-120: one thread per agent of the
 // state list; the count is read from device memory so launches can be replayed
@@ -838,26 +924,29 @@ namespace detail {
 template <typename F>
 __device__ __forceinline__ void run_agent(const ThreadCtx& c) {
     const FnArgs& a = c.a;
-    const uint32_t i = c.idx;
-    if (a.msg_out_flags) a.msg_out_flags[i] = a.msg_out_optional ? 0u : 1u;
-    if (a.birth_flags) a.birth_flags[i] = 0u;
+    const uint32_t o = c.lidx;
+    if (a.msg_out_flags) a.msg_out_flags[o] = a.msg_out_optional ? 0u : 1u;
+    if (a.birth_flags) a.birth_flags[o] = 0u;
     DeviceAPI<typename F::MsgIn, typename F::MsgOut> api(c);
     const AGENT_STATUS s = F()(&api);
     if (a.death_flags)
-        a.death_flags[i] = (uint32_t)s;
+        a.death_flags[o] = (uint32_t)s;
     else
         c.err.note(s == DEAD, DERR_DEATH_DISABLED, 0);
 }
 
+// Agents [first, n) of the list run, first = the agents a function condition disabled.
 template <typename F>
 __global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
+    const uint32_t first = a.d_first ? load_count(a.d_first) : 0u;
     const uint32_t n = a.d_n ? load_count(a.d_n) : a.n_cap;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (first + o >= n) return;
+    const uint32_t i = first + o;
     uint32_t mn = 0u;
     if constexpr (F::MsgIn::kind == MsgKind::BruteForce) mn = a.d_msg_in_count ? load_count(a.d_msg_in_count) : 0u;
     const ThreadCtx c{a, ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step, a.seed, nullptr,
-                      0u, mn};
+                      0u, mn, o, a.stream_id};
     run_agent<F>(c);
 }
 
@@ -883,7 +972,7 @@ __global__ void __launch_bounds__(1024) agent_function_kernel_seg(const FnArgs a
             if (local < s.n) {
                 const uint32_t i = s.agent_off + local;
                 const ThreadCtx c{a,        ErrState{s.d_error, i}, a.zero_page, a.sink_page, i, s.meta, s.env, s.d_step,
-                                  s.seed,   nullptr,                s.msg_off,   s.msg_n};
+                                  s.seed,   nullptr,                s.msg_off,   s.msg_n,     i,     a.stream_id};
                 run_agent<F>(c);
             }
         }
@@ -1058,8 +1147,26 @@ __global__ void __launch_bounds__(256) agent_function_kernel_staged(const FnArgs
     const StageView sv{fgx_stage_smem, fgx_stage_smem + maxe + 1, fgx_stage_smem + 2 * (maxe + 1), a.stage_cap,
                        a.stage_nvars > (uint32_t)F::MsgIn::In::dims ? 1u : 0u, &sh};
     const ThreadCtx c{a,         ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step,
-                      a.seed,    staged ? &sv : nullptr, 0u, 0u};
+                      a.seed,    staged ? &sv : nullptr, 0u, 0u, i, a.stream_id};
     run_agent<F>(c);
+}
+
+// include/flamegpu/runtime/AgentFunctionCondition.cuh:44-91: flags[i] = !condition, so the
+// stable partition that follows puts the disabled agents first.
+template <typename C>
+__global__ void __launch_bounds__(256) agent_condition_kernel(const FnArgs a, uint32_t* __restrict__ flags) {
+    const uint32_t n = a.d_n ? load_count(a.d_n) : a.n_cap;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ThreadCtx c{a, ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step, a.seed, nullptr,
+                      0u, 0u, i, a.stream_id | 0x80000000u};
+    ReadOnlyDeviceAPI api(c);
+    flags[i] = C()(&api) ? 0u : 1u;
+}
+
+template <typename C>
+void launch_condition(const FnArgs& a, uint32_t* flags, uint32_t grid, void* stream) {
+    hipLaunchKernelGGL(agent_condition_kernel<C>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, flags);
 }
 
 template <typename F>
@@ -1097,6 +1204,14 @@ void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nse
     [[maybe_unused]] constexpr funcName##_impl funcName{};                                                           \
     __device__ __forceinline__ flamegpu::AGENT_STATUS funcName##_impl::operator()(                                  \
         flamegpu::DeviceAPI<message_in, message_out>* FLAMEGPU) const
+
+// include/flamegpu/runtime/AgentFunctionCondition_shim.cuh
+#define FLAMEGPU_AGENT_FUNCTION_CONDITION(funcName)                                                                 \
+    struct funcName##_cdn_impl {                                                                                    \
+        __device__ __forceinline__ bool operator()(flamegpu::ReadOnlyDeviceAPI* FLAMEGPU) const;                   \
+    };                                                                                                              \
+    [[maybe_unused]] constexpr funcName##_cdn_impl funcName{};                                                       \
+    __device__ __forceinline__ bool funcName##_cdn_impl::operator()(flamegpu::ReadOnlyDeviceAPI* FLAMEGPU) const
 
 #define FLAMEGPU_DEVICE_FUNCTION __device__ __forceinline__
 #define FLAMEGPU_HOST_DEVICE_FUNCTION __host__ __device__ __forceinline__

@@ -226,6 +226,12 @@ struct FnRT {
     size_t append_scratch_bytes = 0;
     void* trans_scratch = nullptr;
     size_t trans_scratch_bytes = 0;
+    // function condition: flags (!condition), disabled count, work word, partition scratch
+    uint32_t* cond_flags = nullptr;
+    uint32_t* d_first = nullptr;
+    uint32_t* d_work = nullptr;
+    void* cond_scratch = nullptr;
+    size_t cond_scratch_bytes = 0;
     void* stream = nullptr;
 };
 
@@ -615,6 +621,7 @@ class CUDASimulation {
     void enqueue_step();
     void enqueue_layer(const LayerData& layer);
     void launch_function(detail::FnRT& f, void* st);
+    void fill_condition_args(detail::FnRT& f, detail::StateList& l, detail::FnArgs& a);
     void stage_neighbourhood(detail::FnRT& f, detail::StateList& l, detail::FnArgs& a);
     static uint32_t stage_smem_bytes(const detail::FnArgs& a) {
         return a.stage_ok ? (2u * (a.stage_max_entries + 1u) + a.stage_nvars * a.stage_cap) * 4u : 0u;
@@ -779,6 +786,10 @@ inline void CUDASimulation::ensure_init() {
                 if (r.min->meta.dims == 2 && isf("x") && isf("y")) r.sort_dims = 2;
             }
             r.d_launch_count = counter();
+            if (f->condition) {
+                r.d_first = counter();
+                r.d_work = counter();
+            }
             fn_index_[f.get()] = fns_.size();
             fns_.push_back(std::move(r));
         }
@@ -832,6 +843,9 @@ inline void CUDASimulation::release() {
         free_dev(f.births_scratch);
         free_dev(f.append_scratch);
         free_dev(f.trans_scratch);
+        p = f.cond_flags;
+        free_dev(p);
+        free_dev(f.cond_scratch);
         if (f.stream) fgx_stream_destroy(f.stream);
     }
     for (auto e : join_events_) fgx_event_destroy(e);
@@ -928,6 +942,11 @@ inline void CUDASimulation::size_function(detail::FnRT& f, uint32_t cap) {
     if (f.in_state != f.end_state) {
         free_dev(f.trans_scratch);
         f.trans_scratch = new_scratch(FGX_OP_BIRTHS, cap, 0, &f.trans_scratch_bytes);
+    }
+    if (f.d->condition) {
+        realloc_u32(f.cond_flags);
+        free_dev(f.cond_scratch);
+        f.cond_scratch = new_scratch(FGX_OP_COMPACT, cap, 0, &f.cond_scratch_bytes);
     }
     f.cap = cap;
 }
@@ -1224,7 +1243,13 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
             ++i;
         }
     }
-    detail::fgx_check(fgx_memcpy_d2d(f.d_launch_count, l.d_count, 4, st), "launch count");
+    if (f.d->condition) {
+        a.d_first = f.d_first;
+        a.stage_ok = 0;
+        detail::fgx_check(fgx_sub_u32(f.d_launch_count, l.d_count, f.d_first, st), "launch count");
+    } else {
+        detail::fgx_check(fgx_memcpy_d2d(f.d_launch_count, l.d_count, 4, st), "launch count");
+    }
     const uint32_t block = 256;
     const uint32_t grid = std::max<uint32_t>(1, (l.cap + block - 1) / block);
     const std::string label = "fn:" + f.agent->d->name + "::" + f.d->name;
@@ -1234,6 +1259,36 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
     if (hipGetLastError() != hipSuccess) throw exception::CUDAError("agent function launch failed: " + f.d->name);
 }
 
+
+// Launch arguments of a function condition: the agent's variables (read-only), environment, RNG.
+inline void CUDASimulation::fill_condition_args(detail::FnRT& f, detail::StateList& l, detail::FnArgs& a) {
+    a.n_cap = l.cap;
+    a.zero_page = (const char*)d_pages_;
+    a.sink_page = (char*)d_pages_ + 128;
+    a.stream_id = f.d->stream_id;
+    a.d_n = l.d_count;
+    a.d_step = d_step_;
+    a.seed = config_.random_seed;
+    a.d_error = d_error_;
+    a.env = d_env_;
+    a.ids = l.id_slot >= 0 ? (const uint32_t*)l.vars[l.id_slot].cur : nullptr;
+    std::vector<uint32_t> hs;
+    for (auto& v : l.vars) hs.push_back(detail::name_hash(v.name));
+    a.agent_mul = detail::perfect_hash_mul(hs.data(), (unsigned)hs.size(), detail::VAR_SLOT_BITS);
+    for (size_t i = 0; i < l.vars.size(); ++i) {
+        const uint32_t s = detail::table_slot(hs[i], a.agent_mul, detail::VAR_SLOT_BITS);
+        a.agent[s] = {hs[i], (uint16_t)l.vars[i].bytes, 0, (char*)l.vars[i].cur};
+    }
+    std::vector<uint32_t> he;
+    for (auto& kv : m_->env.props) he.push_back(detail::name_hash(kv.first));
+    a.env_mul = detail::perfect_hash_mul(he.data(), (unsigned)he.size(), detail::ENV_SLOT_BITS);
+    size_t i = 0;
+    for (auto& kv : m_->env.props) {
+        const uint32_t s = detail::table_slot(he[i], a.env_mul, detail::ENV_SLOT_BITS);
+        a.env_b[s] = {he[i], (uint16_t)kv.second.offset, (uint16_t)(kv.second.type_size * kv.second.elements)};
+        ++i;
+    }
+}
 
 // LDS staging plan for a spatial-input function: the agent's own x/y(/z) is the
 // assumed search origin, and up to four 4-byte message variables are staged
@@ -1307,9 +1362,17 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
             by.push_back(v.bytes);
         }
         tick("death:" + f.d->name, st, true);
-        detail::fgx_check(fgx_compact(ctx_, f.death_flags, 1, l.d_count, l.cap, (uint32_t)l.vars.size(), src.data(),
-                                      dst.data(), by.data(), l.d_count, f.death_scratch, f.death_scratch_bytes, st),
-                          "death compaction");
+        if (f.d->condition) {
+            // disabled agents stay in front (CUDAFatAgentStateList::scatterDeath with disabledAgents)
+            detail::fgx_check(fgx_compact_prefix(ctx_, f.death_flags, 1, f.d_first, l.d_count, l.cap, (uint32_t)l.vars.size(),
+                                                 src.data(), dst.data(), by.data(), l.d_count, f.death_scratch,
+                                                 f.death_scratch_bytes, st),
+                              "death compaction");
+        } else {
+            detail::fgx_check(fgx_compact(ctx_, f.death_flags, 1, l.d_count, l.cap, (uint32_t)l.vars.size(), src.data(),
+                                          dst.data(), by.data(), l.d_count, f.death_scratch, f.death_scratch_bytes, st),
+                              "death compaction");
+        }
         tick("death:" + f.d->name, st, false);
         for (auto& v : l.vars) std::swap(v.cur, v.alt);
     }
@@ -1324,11 +1387,20 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
             dst.push_back(dl.vars[i].cur);
             by.push_back(l.vars[i].bytes);
         }
-        detail::fgx_check(fgx_append_births(ctx_, nullptr, l.d_count, l.cap, (uint32_t)l.vars.size(), src.data(), dst.data(),
-                                            by.data(), -1, dl.d_count, dl.cap, d_dummy_, nullptr, f.trans_scratch,
-                                            f.trans_scratch_bytes, st),
-                          "state transition");
-        detail::fgx_check(fgx_fill_u32(l.d_count, 0, 1, st), "state transition count");
+        if (f.d->condition) {
+            // only the agents that ran move; the disabled ones stay in the start state
+            detail::fgx_check(fgx_append_range(ctx_, f.d_first, l.d_count, l.cap, (uint32_t)l.vars.size(), src.data(),
+                                               dst.data(), by.data(), dl.d_count, dl.cap, nullptr, f.trans_scratch,
+                                               f.trans_scratch_bytes, st),
+                              "state transition");
+            detail::fgx_check(fgx_memcpy_d2d(l.d_count, f.d_first, 4, st), "state transition count");
+        } else {
+            detail::fgx_check(fgx_append_births(ctx_, nullptr, l.d_count, l.cap, (uint32_t)l.vars.size(), src.data(),
+                                                dst.data(), by.data(), -1, dl.d_count, dl.cap, d_dummy_, nullptr,
+                                                f.trans_scratch, f.trans_scratch_bytes, st),
+                              "state transition");
+            detail::fgx_check(fgx_fill_u32(l.d_count, 0, 1, st), "state transition count");
+        }
     }
     // 4. births (CUDAAgentStateList::scatterNew): appended after the survivors, ordered IDs
     if (f.out_agent) {
@@ -1407,6 +1479,33 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
             for (auto& v : m.vars) std::swap(v.cur, v.alt);
             m.pbm_dirty = false;
         }
+    }
+    // function conditions (CUDASimulation.cu:687-835): condition kernel, then a stable
+    // partition that puts the disabled agents in front; the function runs on the rest
+    for (auto& fd : layer.agent_functions) {
+        auto& f = fns_[fn_index_.at(fd.get())];
+        if (!f.d->condition) continue;
+        auto& l = f.agent->states.at(f.in_state);
+        detail::FnArgs a;
+        std::memset(&a, 0, sizeof(a));
+        fill_condition_args(f, l, a);
+        const uint32_t grid = std::max<uint32_t>(1, (l.cap + 255) / 256);
+        tick("condition:" + f.d->name, stream_, true);
+        f.d->condition(a, f.cond_flags, grid, stream_);
+        if (hipGetLastError() != hipSuccess) throw exception::CUDAError("agent function condition launch failed: " + f.d->name);
+        std::vector<const void*> src;
+        std::vector<void*> dst;
+        std::vector<uint32_t> by;
+        for (auto& v : l.vars) {
+            src.push_back(v.cur);
+            dst.push_back(v.alt);
+            by.push_back(v.bytes);
+        }
+        detail::fgx_check(fgx_partition(ctx_, f.cond_flags, l.d_count, l.cap, (uint32_t)l.vars.size(), src.data(), dst.data(),
+                                        by.data(), f.d_first, f.d_work, f.cond_scratch, f.cond_scratch_bytes, stream_),
+                          "function condition partition");
+        tick("condition:" + f.d->name, stream_, false);
+        for (auto& v : l.vars) std::swap(v.cur, v.alt);
     }
     const bool concurrent = cuda_config_.inLayerConcurrency && layer.agent_functions.size() > 1 && !timing_;
     if (!concurrent) {

@@ -169,6 +169,30 @@ int fgx_append_births(fgx_ctx* ctx, const uint32_t* flags, const uint32_t* d_n_p
                       int32_t id_var, uint32_t* d_dst_count, uint32_t dst_capacity, uint32_t* d_next_id,
                       uint32_t* d_overflow, void* scratch, size_t scratch_bytes, void* stream);
 
+/* ---------------------------------------------------------------- agent function conditions
+ * CUDAFatAgent::processFunctionCondition (src/flamegpu/simulation/detail/CUDAFatAgent.cu:186-236):
+ * stable partition of every variable, items with flags[i] != 0 (condition false) first, then
+ * the others, both in input order.  *d_first_count receives the size of the first part (the
+ * disabled agents); d_work is one device word of working space.  Scratch as fgx_compact. */
+int fgx_partition(fgx_ctx* ctx, const uint32_t* flags, const uint32_t* d_n, uint32_t n_cap, uint32_t nvars,
+                  const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes, uint32_t* d_first_count,
+                  uint32_t* d_work, void* scratch, size_t scratch_bytes, void* stream);
+/* Death compaction under a function condition (CUDAFatAgentStateList::scatterDeath with
+ * disabled agents, src/flamegpu/simulation/detail/CUDAFatAgentStateList.cu:155-178): the first
+ * *d_prefix items (disabled) are kept in front, item i >= *d_prefix is kept when
+ * (flags[i - *d_prefix] != 0) == (keep != 0). */
+int fgx_compact_prefix(fgx_ctx* ctx, const uint32_t* flags, uint32_t keep, const uint32_t* d_prefix, const uint32_t* d_n,
+                       uint32_t n_cap, uint32_t nvars, const void* const* vars_in, void* const* vars_out,
+                       const uint32_t* var_bytes, uint32_t* d_count_out, void* scratch, size_t scratch_bytes, void* stream);
+/* State transition under a function condition (CUDAFatAgent::transitionState,
+ * src/flamegpu/simulation/detail/CUDAFatAgent.cu:139-184, scatterAll from data_condition):
+ * appends items [*d_src_off, *d_n) of src after *d_dst_count items of dst. */
+int fgx_append_range(fgx_ctx* ctx, const uint32_t* d_src_off, const uint32_t* d_n, uint32_t n_cap, uint32_t nvars,
+                     const void* const* src, void* const* dst, const uint32_t* var_bytes, uint32_t* d_dst_count,
+                     uint32_t dst_capacity, uint32_t* d_overflow, void* scratch, size_t scratch_bytes, void* stream);
+/* *dst = max(*a - *b, 0) on the device (enabled-agent counts). */
+int fgx_sub_u32(uint32_t* dst, const uint32_t* a, const uint32_t* b, void* stream);
+
 /* ---------------------------------------------------------------- scan
  * Exclusive prefix sum of n u32 (cub::DeviceScan::ExclusiveSum call sites,
  * e.g. src/flamegpu/runtime/messaging/MessageSpatial3D.cu:134). */

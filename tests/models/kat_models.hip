@@ -241,6 +241,42 @@ FLAMEGPU_AGENT_FUNCTION(draw_random, MessageNone, MessageNone) {
 }
 FLAMEGPU_AGENT_FUNCTION(die_without_permission, MessageNone, MessageNone) { return DEAD; }
 
+// ---------------------------------------------------------------- function conditions
+// tests/test_cases/runtime/test_agent_function_conditions.cu:26-47
+FLAMEGPU_AGENT_FUNCTION(NullFn1, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<int>("x", FLAMEGPU->getVariable<int>("x") + 1);
+    FLAMEGPU->setVariable<int, 4>("y", 0, 3);
+    FLAMEGPU->setVariable<int, 4>("y", 1, 4);
+    FLAMEGPU->setVariable<int, 4>("y", 2, 5);
+    FLAMEGPU->setVariable<int, 4>("y", 3, 6);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(NullFn2, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<int>("x", FLAMEGPU->getVariable<int>("x") - 1);
+    FLAMEGPU->setVariable<int, 4>("y", 0, 23);
+    FLAMEGPU->setVariable<int, 4>("y", 1, 24);
+    FLAMEGPU->setVariable<int, 4>("y", 2, 25);
+    FLAMEGPU->setVariable<int, 4>("y", 3, 26);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION_CONDITION(Condition1) { return FLAMEGPU->getVariable<int>("x") == 1; }
+FLAMEGPU_AGENT_FUNCTION_CONDITION(Condition2) { return FLAMEGPU->getVariable<int>("x") != 1; }
+FLAMEGPU_AGENT_FUNCTION_CONDITION(AllFail) { return false; }
+// a condition with death, births and message output under it
+FLAMEGPU_AGENT_FUNCTION_CONDITION(NotMultipleOf3) { return FLAMEGPU->getVariable<int>("x") % 3 != 0; }
+FLAMEGPU_AGENT_FUNCTION(act_under_condition, MessageNone, MessageBruteForce) {
+    const int x = FLAMEGPU->getVariable<int>("x");
+    FLAMEGPU->message_out.setVariable<int>("x", x);
+    if (x % 5 == 0) FLAMEGPU->agent_out.setVariable<int>("x", x + 1000);
+    return x % 2 == 0 ? DEAD : ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(sum_messages, MessageBruteForce, MessageNone) {
+    int sum = 0;
+    for (const auto& m : FLAMEGPU->message_in) sum += m.getVariable<int>("x");
+    FLAMEGPU->setVariable<int>("sum", sum);
+    return ALIVE;
+}
+
 flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* p, uint32_t np) {
     const std::string v = variant_c ? variant_c : "";
     flamegpu::ModelDescription model("kat_" + v);
@@ -408,6 +444,49 @@ flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* 
         auto agent = model.newAgent("agent");
         agent.newVariable<float>("x");
         model.newLayer().addAgentFunction(agent.newFunction("die", die_without_permission));
+    } else if (v == "cond_split") {
+        // TestAgentFunctionConditions.SplitAgents (test_agent_function_conditions.cu:48-102)
+        auto a = model.newAgent("agent");
+        a.newVariable<int>("x");
+        a.newVariable<int, 4>("y");
+        a.newState("Start");
+        a.newState("End");
+        a.newState("End2");
+        auto af1 = a.newFunction("Function1", NullFn1);
+        af1.setInitialState("Start");
+        af1.setEndState("End");
+        af1.setFunctionCondition(Condition1);
+        auto af2 = a.newFunction("Function2", NullFn2);
+        af2.setInitialState("Start");
+        af2.setEndState("End2");
+        af2.setFunctionCondition(Condition2);
+        model.newLayer().addAgentFunction(af1);
+        model.newLayer().addAgentFunction(af2);
+    } else if (v == "cond_all_disabled") {
+        // TestAgentFunctionConditions.AllDisabled (test_agent_function_conditions.cu:107-129)
+        auto a = model.newAgent("agent");
+        a.newVariable<int>("x");
+        a.newVariable<int, 4>("y");
+        auto af1 = a.newFunction("Function1", NullFn1);
+        af1.setFunctionCondition(AllFail);
+        auto af2 = a.newFunction("Function2", NullFn2);
+        model.newLayer().addAgentFunction(af1);
+        model.newLayer().addAgentFunction(af2);
+    } else if (v == "cond_mixed") {
+        auto message = model.newMessage("msg");
+        message.newVariable<int>("x");
+        auto a = model.newAgent("agent");
+        a.newVariable<int>("x");
+        a.newVariable<int>("sum");
+        auto act = a.newFunction("act", act_under_condition);
+        act.setFunctionCondition(NotMultipleOf3);
+        act.setMessageOutput("msg");
+        act.setAgentOutput(a);
+        act.setAllowAgentDeath(true);
+        auto sum = a.newFunction("sum", sum_messages);
+        sum.setMessageInput("msg");
+        model.newLayer().addAgentFunction(act);
+        model.newLayer().addAgentFunction(sum);
     } else {
         throw flamegpu::exception::InvalidArgument("unknown test model variant '" + v + "'");
     }

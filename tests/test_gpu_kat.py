@@ -200,3 +200,57 @@ def test_device_error_wrap_out_of_bounds():
     s.set_agents("agent", [x, x, x, np.zeros(2, np.uint32)])
     with pytest.raises(FgxError, match="DeviceError"):
         s.step(1)
+
+
+# ---------------------------------------------------------------- function conditions
+# tests/test_cases/runtime/test_agent_function_conditions.cu
+
+
+def test_condition_split_agents():
+    n = 1000
+    s = _sim("cond_split")
+    x = (np.arange(2 * n) % 2).astype(np.int32)
+    y = np.tile(np.array([13, 14, 15, 16], np.int32), 2 * n)
+    s.set_agents("agent", [x, y], state="Start")
+    s.step(1)
+    assert s.count("agent", "Start") == 0
+    assert s.count("agent", "End") == n and s.count("agent", "End2") == n
+    (x2, y2), ids2 = s.get_agents("agent", "End")
+    (x3, y3), ids3 = s.get_agents("agent", "End2")
+    assert np.all(x2 == 2) and np.array_equal(y2.reshape(-1, 4), np.tile([3, 4, 5, 6], (n, 1)))
+    assert np.all(x3 == -1) and np.array_equal(y3.reshape(-1, 4), np.tile([23, 24, 25, 26], (n, 1)))
+    # agents keep their input order within each part
+    assert np.array_equal(ids2, np.arange(2, 2 * n + 1, 2)) and np.array_equal(ids3, np.arange(1, 2 * n, 2))
+
+
+def test_condition_all_disabled():
+    n = 1000
+    s = _sim("cond_all_disabled")
+    s.set_agents("agent", [np.zeros(n, np.int32), np.zeros(4 * n, np.int32)])
+    s.step(3)
+    (x, y), ids = s.get_agents("agent")
+    assert len(x) == n and np.all(x == -3)
+    assert np.array_equal(y.reshape(-1, 4), np.tile([23, 24, 25, 26], (n, 1)))
+    assert np.array_equal(ids, np.arange(1, n + 1))
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_condition_with_death_births_and_messages(graphs):
+    n = 3000
+    x = np.arange(n, dtype=np.int32)
+    s = _sim("cond_mixed")
+    s.set_option("graphs", graphs)
+    s.set_agents("agent", [x, np.zeros(n, np.int32)])
+    s.step(1)
+    (gx, gsum), gid = s.get_agents("agent")
+    ids = np.arange(1, n + 1, dtype=np.uint32)
+    enabled = x % 3 != 0
+    disabled_x, disabled_id = x[~enabled], ids[~enabled]
+    ex, eid = x[enabled], ids[enabled]
+    surv = ex % 2 == 1
+    born_x = ex[ex % 5 == 0] + 1000
+    exp_x = np.concatenate([disabled_x, ex[surv], born_x])
+    exp_id = np.concatenate([disabled_id, eid[surv], n + 1 + np.arange(len(born_x), dtype=np.uint32)])
+    assert np.array_equal(gx, exp_x)
+    assert np.array_equal(gid, exp_id)
+    assert np.all(gsum == int(ex.sum()))
