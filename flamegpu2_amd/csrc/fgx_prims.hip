@@ -378,6 +378,125 @@ __global__ void __launch_bounds__(kBlock) k_reduce_final(const Acc* __restrict__
     if (threadIdx.x == 0) *out = s[0];
 }
 
+// ================================================================ deterministic statistics
+// Sum, minimum and maximum in one pass, then the sum of squared deviations from the
+// mean (which needs the finished sum) in a second.  Each block owns a fixed strided
+// set of elements and folds it with a fixed tree, so the result does not depend on
+// scheduling.  Acc: double for f32/f64 (exact widening), int64 / uint64 for integers.
+template <typename Acc>
+struct StatLimits;
+template <>
+struct StatLimits<double> {
+    __device__ static double hi() { return __builtin_huge_val(); }
+    __device__ static double lo() { return -__builtin_huge_val(); }
+};
+template <>
+struct StatLimits<long long> {
+    __device__ static long long hi() { return 0x7FFFFFFFFFFFFFFFll; }
+    __device__ static long long lo() { return -0x7FFFFFFFFFFFFFFFll - 1; }
+};
+template <>
+struct StatLimits<unsigned long long> {
+    __device__ static unsigned long long hi() { return ~0ull; }
+    __device__ static unsigned long long lo() { return 0ull; }
+};
+
+template <typename T, typename Acc>
+__global__ void __launch_bounds__(kBlock) k_stats_partial(const T* __restrict__ in, const uint32_t* d_n, uint32_t n_cap,
+                                                          Acc* __restrict__ part) {
+    __shared__ Acc s_sum[kBlock], s_min[kBlock], s_max[kBlock];
+    const uint32_t n = load_count(d_n, n_cap);
+    const uint32_t stride = kReduceBlocks * kBlock;
+    Acc sum = 0, mn = StatLimits<Acc>::hi(), mx = StatLimits<Acc>::lo();
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const Acc v = (Acc)in[i];
+        sum += v;
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    s_sum[threadIdx.x] = sum;
+    s_min[threadIdx.x] = mn;
+    s_max[threadIdx.x] = mx;
+    __syncthreads();
+    for (uint32_t o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
+            const Acc a = s_min[threadIdx.x + o], b = s_max[threadIdx.x + o];
+            if (a < s_min[threadIdx.x]) s_min[threadIdx.x] = a;
+            if (b > s_max[threadIdx.x]) s_max[threadIdx.x] = b;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s_sum[0];
+        part[kReduceBlocks + blockIdx.x] = s_min[0];
+        part[2 * kReduceBlocks + blockIdx.x] = s_max[0];
+    }
+}
+
+// out[0] sum, out[1] min, out[2] max (as Acc)
+template <typename Acc>
+__global__ void __launch_bounds__(kBlock) k_stats_final(const Acc* __restrict__ part, Acc* __restrict__ out) {
+    static_assert(kReduceBlocks == kBlock, "one partial per thread");
+    __shared__ Acc s_sum[kBlock], s_min[kBlock], s_max[kBlock];
+    s_sum[threadIdx.x] = part[threadIdx.x];
+    s_min[threadIdx.x] = part[kReduceBlocks + threadIdx.x];
+    s_max[threadIdx.x] = part[2 * kReduceBlocks + threadIdx.x];
+    __syncthreads();
+    for (uint32_t o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
+            const Acc a = s_min[threadIdx.x + o], b = s_max[threadIdx.x + o];
+            if (a < s_min[threadIdx.x]) s_min[threadIdx.x] = a;
+            if (b > s_max[threadIdx.x]) s_max[threadIdx.x] = b;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s_sum[0];
+        out[1] = s_min[0];
+        out[2] = s_max[0];
+    }
+}
+
+// Σ ((double)x - mean)^2 with mean = sum / n (0 when n == 0), as the reference's
+// standard deviation (include/flamegpu/simulation/AgentLoggingConfig.h,
+// getAgentVariableStandardDevFunc; AgentLoggingConfig_Reductions.cuh).
+template <typename T, typename Acc>
+__global__ void __launch_bounds__(kBlock) k_sqdev_partial(const T* __restrict__ in, const uint32_t* d_n, uint32_t n_cap,
+                                                          const Acc* __restrict__ stats, double* __restrict__ part) {
+    __shared__ double s[kBlock];
+    const uint32_t n = load_count(d_n, n_cap);
+    const double mean = n ? (double)stats[0] / (double)n : 0.0;
+    const uint32_t stride = kReduceBlocks * kBlock;
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const double d = (double)in[i] - mean;
+        acc += d * d;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+template <typename T, typename Acc>
+static int run_stats(const void* in, const uint32_t* d_n, uint32_t n_cap, void* d_out, char* scratch, hipStream_t st) {
+    Acc* part = reinterpret_cast<Acc*>(scratch);
+    double* dpart = reinterpret_cast<double*>(scratch + 3 * kReduceBlocks * 8);
+    hipLaunchKernelGGL((k_stats_partial<T, Acc>), dim3(kReduceBlocks), dim3(kBlock), 0, st, static_cast<const T*>(in), d_n,
+                       n_cap, part);
+    hipLaunchKernelGGL((k_stats_final<Acc>), dim3(1), dim3(kBlock), 0, st, part, static_cast<Acc*>(d_out));
+    hipLaunchKernelGGL((k_sqdev_partial<T, Acc>), dim3(kReduceBlocks), dim3(kBlock), 0, st, static_cast<const T*>(in), d_n,
+                       n_cap, static_cast<const Acc*>(d_out), dpart);
+    hipLaunchKernelGGL((k_reduce_final<double>), dim3(1), dim3(kBlock), 0, st, dpart, static_cast<double*>(d_out) + 3);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
 // ================================================================ scratch layouts
 struct BuildLayout {
     size_t hist, key, sub, tmp, hdr, status, total;
@@ -544,6 +663,9 @@ int fgx_scratch_bytes(int op, uint32_t n, uint32_t bins, size_t* bytes) {
         case FGX_OP_BIRTHS:
         case FGX_OP_SCAN:
             *bytes = lookback_bytes(n) + align256(kReduceBlocks * 8);
+            return FGX_OK;
+        case FGX_OP_STATS:
+            *bytes = align256(4 * kReduceBlocks * 8);
             return FGX_OK;
         default:
             return fail(FGX_ERR_ARG, "fgx_scratch_bytes: unknown op");
@@ -759,6 +881,26 @@ int fgx_append_range(fgx_ctx* ctx, const uint32_t* d_src_off, const uint32_t* d_
     a.d_overflow = d_overflow;
     a.d_src_off = d_src_off;
     return launch_compact(a, vt, scratch, scratch_bytes, stream);
+}
+
+int fgx_reduce_stats(fgx_ctx* ctx, int type, const void* in, const uint32_t* d_n, uint32_t n_cap, void* d_out,
+                     void* scratch, size_t scratch_bytes, void* stream) {
+    (void)ctx;
+    if (!d_out) return fail(FGX_ERR_ARG, "fgx_reduce_stats: NULL output");
+    if (n_cap && !in) return fail(FGX_ERR_ARG, "fgx_reduce_stats: NULL input");
+    if (!scratch || scratch_bytes < align256(4 * kReduceBlocks * 8))
+        return fail(FGX_ERR_CAPACITY, "fgx_reduce_stats: scratch too small");
+    char* sc = static_cast<char*>(scratch);
+    hipStream_t st = (hipStream_t)stream;
+    switch (type) {
+        case 0: return run_stats<float, double>(in, d_n, n_cap, d_out, sc, st);
+        case 1: return run_stats<double, double>(in, d_n, n_cap, d_out, sc, st);
+        case 2: return run_stats<int32_t, long long>(in, d_n, n_cap, d_out, sc, st);
+        case 3: return run_stats<uint32_t, unsigned long long>(in, d_n, n_cap, d_out, sc, st);
+        case 4: return run_stats<long long, long long>(in, d_n, n_cap, d_out, sc, st);
+        case 5: return run_stats<unsigned long long, unsigned long long>(in, d_n, n_cap, d_out, sc, st);
+        default: return fail(FGX_ERR_ARG, "fgx_reduce_stats: unknown type");
+    }
 }
 
 int fgx_reduce_sum(fgx_ctx* ctx, int type, const void* in, const uint32_t* d_n, uint32_t n_cap,

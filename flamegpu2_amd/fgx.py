@@ -20,6 +20,7 @@ FGX_OP_COMPACT = 1
 FGX_OP_BIRTHS = 2
 FGX_OP_AGENT_SORT = 3
 FGX_OP_SCAN = 4
+FGX_OP_STATS = 5
 FGX_MAX_VARS = 32
 
 
@@ -71,6 +72,7 @@ _SIGS = {
     "fgx_exclusive_scan_u32": [vp, vp, vp, u32, vp, sz, vp],
     "fgx_assign_ids": [vp, u32, u32, vp],
     "fgx_reduce_sum": [vp, _c.c_int, vp, vp, u32, vp, vp, sz, vp],
+    "fgx_reduce_stats": [vp, _c.c_int, vp, vp, u32, vp, vp, sz, vp],
     "fgx_seg_key_init": [_c.c_int, _c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float,
                          _c.POINTER(SegKey)],
     "fgx_segments_pack": [_c.POINTER(SegKey), u32, _c.POINTER(u32), _c.POINTER(u32), _c.POINTER(u32)],
@@ -199,6 +201,12 @@ def scratch_clear(scratch, nbytes, stream=0):
 
 def reduce_sum(ctx, type_code, d_in, n_cap, d_out, scratch, scratch_len, stream=0, d_n=None):
     check(lib().fgx_reduce_sum(ctx.h, type_code, d_in, d_n, n_cap, d_out, scratch, scratch_len, stream))
+
+
+def reduce_stats(ctx, type_code, d_in, n_cap, d_out, scratch, scratch_len, stream=0, d_n=None):
+    """Four 8-byte words at d_out: sum, min, max (double or 64-bit integer) and the sum of
+    squared deviations from the mean (double)."""
+    check(lib().fgx_reduce_stats(ctx.h, type_code, d_in, d_n, n_cap, d_out, scratch, scratch_len, stream))
 
 
 def seg_key(kind, dims, mn, mx, radius, n=0):

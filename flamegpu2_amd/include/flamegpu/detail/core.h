@@ -74,6 +74,8 @@ FGX_DERIVED_EXCEPTION(EnsembleError)
 FGX_DERIVED_EXCEPTION(InvalidStateName)
 FGX_DERIVED_EXCEPTION(ReservedName)
 FGX_DERIVED_EXCEPTION(DuplicateEnvProperty)
+FGX_DERIVED_EXCEPTION(AgentIDCollision)
+FGX_DERIVED_EXCEPTION(InvalidInputFile)
 FGX_DERIVED_EXCEPTION(InvalidHostFunc)
 FGX_DERIVED_EXCEPTION(InvalidPopulationData)
 FGX_DERIVED_EXCEPTION(InvalidConfig)

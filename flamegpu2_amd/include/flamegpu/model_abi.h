@@ -19,6 +19,8 @@ struct fgxm_sim {
     std::unique_ptr<flamegpu::CUDASimulation> sim;
     void* ev0 = nullptr;
     void* ev1 = nullptr;
+    std::unique_ptr<flamegpu::StepLoggingConfig> step_log;
+    std::unique_ptr<flamegpu::LoggingConfig> exit_log;
 };
 
 namespace fgxm_detail {
@@ -222,6 +224,145 @@ FGXM_EXPORT int fgxm_timed_steps(fgxm_sim* s, uint32_t steps, double* ms) {
 
 FGXM_EXPORT int fgxm_step_counter(fgxm_sim* s, uint32_t* out) {
     return fgxm_detail::guard([&] { *out = s->sim->getStepCounter(); });
+}
+
+FGXM_EXPORT int fgxm_simulate(fgxm_sim* s, uint32_t steps) {
+    return fgxm_detail::guard([&] {
+        s->sim->SimulationConfig().steps = steps;
+        s->sim->simulate();
+    });
+}
+
+FGXM_EXPORT int fgxm_log_config(fgxm_sim* s, int exit_log, const char* what, const char* agent, const char* state,
+                                const char* name, uint32_t value) {
+    return fgxm_detail::guard([&] {
+        if (!s->step_log) s->step_log.reset(new flamegpu::StepLoggingConfig(*s->model));
+        if (!s->exit_log) s->exit_log.reset(new flamegpu::LoggingConfig(*s->model));
+        flamegpu::LoggingConfig& cfg = exit_log ? *s->exit_log : static_cast<flamegpu::LoggingConfig&>(*s->step_log);
+        const std::string w = what ? what : "";
+        const std::string st = state && *state ? state : flamegpu::ModelData::DEFAULT_STATE();
+        if (w == "frequency") {
+            if (exit_log) throw flamegpu::exception::InvalidArgument("an exit log has no frequency");
+            s->step_log->setFrequency(value);
+        } else if (w == "timing") {
+            cfg.logTiming(value != 0);
+        } else if (w == "env") {
+            cfg.logEnvironment(name ? name : "");
+        } else if (w == "count") {
+            cfg.agent(agent ? agent : "", st).logCount();
+        } else {
+            auto a = fgxm_detail::agent(s, agent);
+            auto vit = a->variables.find(name ? name : "");
+            if (vit == a->variables.end()) throw flamegpu::exception::InvalidAgentVar(std::string("unknown variable ") + (name ? name : ""));
+            auto alc = cfg.agent(agent, st);
+            auto dispatch = [&](auto tag) {
+                using T = decltype(tag);
+                if (w == "mean") alc.logMean<T>(name);
+                else if (w == "standard_deviation") alc.logStandardDev<T>(name);
+                else if (w == "min") alc.logMin<T>(name);
+                else if (w == "max") alc.logMax<T>(name);
+                else if (w == "sum") alc.logSum<T>(name);
+                else throw flamegpu::exception::InvalidArgument("unknown log item " + w);
+            };
+            const std::type_index t = vit->second.type;
+            if (t == typeid(float)) dispatch(float());
+            else if (t == typeid(double)) dispatch(double());
+            else if (t == typeid(int32_t)) dispatch(int32_t());
+            else if (t == typeid(uint32_t)) dispatch(uint32_t());
+            else if (t == typeid(int64_t)) dispatch(int64_t());
+            else if (t == typeid(uint64_t)) dispatch(uint64_t());
+            else throw flamegpu::exception::UnsupportedVarType("variable type has no logged reductions");
+        }
+        if (exit_log) s->sim->setExitLog(*s->exit_log);
+        else s->sim->setStepLog(*s->step_log);
+    });
+}
+
+FGXM_EXPORT int fgxm_export_state(fgxm_sim* s, const char* path, int pretty) {
+    return fgxm_detail::guard([&] { s->sim->exportData(path ? path : "", pretty != 0); });
+}
+
+FGXM_EXPORT int fgxm_load_state(fgxm_sim* s, const char* path) {
+    return fgxm_detail::guard([&] {
+        s->sim->SimulationConfig().input_file = path ? path : "";
+        s->sim->applyConfig();
+    });
+}
+
+// Parses a state file against the model without a simulation (host only) and
+// describes it as JSON: {"step": N|null, "env": {name: [values]}, "agents":
+// {agent: {state: {"n": count, "vars": {name: [sum of element k over agents]}}}}}.
+FGXM_EXPORT int fgxm_state_summary(const char* variant, const float* params, uint32_t nparams, const char* path, char* buf,
+                                   uint32_t buflen) {
+    return fgxm_detail::guard([&] {
+        flamegpu::ModelDescription model = fgxm_build_model(variant, params, nparams);
+        const auto md = model.data();
+        flamegpu::io::StateData d = flamegpu::io::read_state(path ? path : "");
+        std::string o = "{\"step\":" + (d.has_step ? std::to_string(d.step_count) : std::string("null")) + ",\"env\":{";
+        bool first = true;
+        for (auto& kv : d.environment) {
+            auto it = md->env.props.find(kv.first);
+            if (it == md->env.props.end()) throw flamegpu::exception::InvalidEnvProperty("unknown property " + kv.first);
+            std::vector<char> b(it->second.type_size * it->second.elements);
+            o += std::string(first ? "" : ",") + "\"" + kv.first + "\":[";
+            first = false;
+            for (size_t e = 0; e < kv.second.size(); ++e) {
+                flamegpu::io::detail::parse_token(it->second.type, kv.second[e], b.data(), e);
+                o += (e ? "," : "") + flamegpu::io::detail::format_value(it->second.type, b.data(), e);
+            }
+            o += "]";
+        }
+        o += "},\"agents\":{";
+        std::string cur;
+        first = true;
+        for (auto& kv : d.agents) {
+            auto a = md->agents.find(kv.first.first);
+            if (a == md->agents.end()) throw flamegpu::exception::InvalidAgentName("unknown agent " + kv.first.first);
+            if (first || kv.first.first != cur) {
+                if (!first) o += "},";
+                o += "\"" + kv.first.first + "\":{";
+                cur = kv.first.first;
+            } else {
+                o += ",";
+            }
+            first = false;
+            const std::string st = kv.first.second.empty() ? a->second->initial_state : kv.first.second;
+            o += "\"" + st + "\":{\"n\":" + std::to_string(kv.second.size()) + ",\"vars\":{";
+            std::map<std::string, std::vector<double>> sums;
+            for (auto& rec : kv.second)
+                for (auto& v : rec.vars) {
+                    auto vit = a->second->variables.find(v.first);
+                    if (vit == a->second->variables.end()) throw flamegpu::exception::InvalidAgentVar("unknown variable " + v.first);
+                    std::vector<char> b(vit->second.bytes());
+                    auto& acc = sums[v.first];
+                    acc.resize(vit->second.elements, 0.0);
+                    for (size_t e = 0; e < v.second.size(); ++e) {
+                        flamegpu::io::detail::parse_token(vit->second.type, v.second[e], b.data(), e);
+                        acc[e] += flamegpu::io::detail::as_double(vit->second.type, b.data(), e);
+                    }
+                }
+            bool fv = true;
+            for (auto& sv : sums) {
+                o += std::string(fv ? "" : ",") + "\"" + sv.first + "\":[";
+                fv = false;
+                for (size_t e = 0; e < sv.second.size(); ++e) {
+                    char t[64];
+                    snprintf(t, sizeof t, "%.17g", sv.second[e]);
+                    o += (e ? "," : "") + std::string(t);
+                }
+                o += "]";
+            }
+            o += "}}";
+        }
+        if (!first) o += "}";
+        o += "}}";
+        if (!buf || buflen < o.size() + 1) throw flamegpu::exception::OutOfBoundsException("summary buffer too small");
+        std::memcpy(buf, o.c_str(), o.size() + 1);
+    });
+}
+
+FGXM_EXPORT int fgxm_export_log(fgxm_sim* s, const char* path, int steps, int exit_log, int step_time, int exit_time) {
+    return fgxm_detail::guard([&] { s->sim->exportLog(path ? path : "", steps != 0, exit_log != 0, step_time != 0, exit_time != 0); });
 }
 
 }  // extern "C"

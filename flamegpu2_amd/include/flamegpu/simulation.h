@@ -25,6 +25,8 @@
 #include <vector>
 
 #include "flamegpu/runtime/device.h"
+#include "flamegpu/logging.h"
+#include "flamegpu/io.h"
 
 namespace flamegpu {
 
@@ -273,6 +275,11 @@ class HostEnvironment {
     T getProperty(const std::string& name) const;
     template <typename T>
     T setProperty(const std::string& name, T value);
+    // include/flamegpu/runtime/environment/HostEnvironment.cuh: array properties
+    template <typename T, unsigned int N>
+    std::array<T, N> getProperty(const std::string& name) const;
+    template <typename T, unsigned int N>
+    std::array<T, N> setProperty(const std::string& name, const std::array<T, N>& value);
 
  private:
     CUDASimulation& s_;
@@ -468,16 +475,26 @@ class CUDASimulation {
             if (a == "-s" || a == "--steps") config_.steps = (unsigned)std::stoul(next("--steps"));
             else if (a == "-r" || a == "--random") config_.random_seed = std::stoull(next("--random"));
             else if (a == "-d" || a == "--device") cuda_config_.device_id = std::stoi(next("--device"));
-            else if (a == "-i" || a == "--in") config_.input_file = next("--in");
+            else if (a == "-i" || a == "--in") {
+                // loaded here so that later arguments override the file's configuration
+                // (src/flamegpu/simulation/Simulation.cu: checkArgs)
+                config_.input_file = next("--in");
+                load_state(config_.input_file);
+            }
             else if (a == "-t" || a == "--timing") config_.timing = true;
             else if (a == "-v" || a == "--verbose") config_.verbose = true;
             else if (a == "-q" || a == "--quiet") config_.verbose = false;
             else if (a == "--no-graphs") cuda_config_.graphs = false;
         }
-        if (!config_.input_file.empty())
-            throw exception::InvalidArgument("state file input (-i) is not supported by this runtime");
     }
-    void applyConfig() { ensure_init(); }
+    // src/flamegpu/simulation/Simulation.cu:55-83: loads SimulationConfig().input_file once
+    void applyConfig() {
+        if (!config_.input_file.empty() && config_.input_file != loaded_input_file_) load_state(config_.input_file);
+        ensure_init();
+    }
+    // src/flamegpu/simulation/Simulation.cu exportData: configuration, step count,
+    // environment and every agent population to an .xml or .json state file.
+    void exportData(const std::string& path, bool prettyPrint = true);
 
     // src/flamegpu/simulation/CUDASimulation.cu:575-642
     bool step();
@@ -485,7 +502,10 @@ class CUDASimulation {
     void simulate() {
         const auto t0 = std::chrono::steady_clock::now();
         ensure_init();
+        reset_log();
         run_init_functions();
+        // the initial state is step log frame 0 (src/flamegpu/simulation/CUDASimulation.cu:1262-1263)
+        process_step_log(elapsed_init_);
         for (unsigned int s = 0; s < config_.steps || config_.steps == 0; ++s) {
             if (!step()) break;
             if (config_.steps == 0 && s > 1000000000u) break;
@@ -494,6 +514,7 @@ class CUDASimulation {
         synchronize();
         check_device_error();
         elapsed_sim_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        process_exit_log();
     }
     void simulate(const RunPlan& plan) {
         config_.random_seed = plan.getRandomSimulationSeed();
@@ -501,6 +522,36 @@ class CUDASimulation {
         for (auto& kv : plan.properties()) set_env_raw(kv.first, kv.second);
         simulate();
     }
+    // src/flamegpu/simulation/CUDASimulation.cu:1476-1491
+    void setStepLog(const StepLoggingConfig& cfg) {
+        check_log_model(cfg, "setStepLog");
+        flush_step_log();   // frames already taken keep the configuration they were taken with
+        step_log_cfg_ = std::make_shared<StepLoggingConfig>(cfg);
+        step_plan_.reset();
+        run_log_.step_log_frequency = cfg.frequency();
+    }
+    void setExitLog(const LoggingConfig& cfg) {
+        check_log_model(cfg, "setExitLog");
+        exit_log_cfg_ = std::make_shared<LoggingConfig>(cfg);
+        exit_plan_.reset();
+    }
+    // src/flamegpu/simulation/CUDASimulation.cu:2023-2025.  Reads back logged steps
+    // still held in device memory first.
+    const RunLog& getRunLog() const {
+        const_cast<CUDASimulation*>(this)->flush_step_log();
+        return run_log_;
+    }
+    // src/flamegpu/simulation/Simulation.cu exportLog + src/flamegpu/io/JSONLogger.cu (JSON only)
+    void exportLog(const std::string& path, bool steps, bool exit, bool stepTime, bool exitTime) {
+        const std::string j = detail::run_log_json(getRunLog(), steps, exit, stepTime, exitTime);
+        FILE* f = std::fopen(path.c_str(), "wb");
+        if (!f) throw exception::InvalidArgument("Unable to open file '" + path + "' for writing");
+        const size_t w = std::fwrite(j.data(), 1, j.size(), f);
+        std::fclose(f);
+        if (w != j.size()) throw exception::InvalidArgument("Unable to write file '" + path + "'");
+    }
+    double getElapsedTimeInitFunctions() const { return elapsed_init_; }
+    double getElapsedTimeExitFunctions() const { return elapsed_exit_; }
     // src/flamegpu/simulation/CUDASimulation.cu:1418-1452
     void setPopulationData(AgentVector& pop, const std::string& state = "");
     void getPopulationData(AgentVector& pop, const std::string& state = "");
@@ -525,7 +576,23 @@ class CUDASimulation {
         std::memcpy(&v, env_host_.data() + p.offset, sizeof(T));
         return v;
     }
-    void exportData(const std::string&) {}
+    template <typename T, unsigned int N>
+    std::array<T, N> getEnvironmentProperty(const std::string& name) const {
+        const EnvProp& p = env_prop<T>(name);
+        if (p.elements != N)
+            throw exception::OutOfBoundsException("Environment property '" + name + "' does not have " + std::to_string(N) + " elements");
+        std::array<T, N> v;
+        std::memcpy(v.data(), env_host_.data() + p.offset, sizeof(T) * N);
+        return v;
+    }
+    template <typename T, unsigned int N>
+    void setEnvironmentProperty(const std::string& name, const std::array<T, N>& value) {
+        const EnvProp& p = env_prop<T>(name);
+        if (p.elements != N)
+            throw exception::OutOfBoundsException("Environment property '" + name + "' does not have " + std::to_string(N) + " elements");
+        std::memcpy(env_host_.data() + p.offset, value.data(), sizeof(T) * N);
+        env_dirty_ = true;
+    }
 
     // ---- extensions used by the benchmark and the model C ABI
     void setKernelTiming(bool on) { timing_ = on; }
@@ -592,6 +659,8 @@ class CUDASimulation {
             throw exception::InvalidEnvPropertyType("Environment property '" + name + "' type mismatch");
         return it->second;
     }
+    void load_state(const std::string& path);
+    std::string loaded_input_file_;
     void set_env_raw(const std::string& name, const std::vector<char>& b) {
         auto it = m_->env.props.find(name);
         if (it == m_->env.props.end()) throw exception::InvalidEnvProperty("Environment property '" + name + "' was not found");
@@ -631,6 +700,42 @@ class CUDASimulation {
     void run_init_functions();
     void run_exit_functions();
     void upload_env();
+    // step / exit logs
+    struct LogVarPlan {
+        std::string name;
+        int type;               // fgx_reduce_stats type code
+        std::type_index vtype;  // variable type
+        size_t off;             // byte offset of its 4 words in a frame record
+    };
+    struct LogAgentPlan {
+        std::string agent, state;
+        bool count;
+        size_t count_off;
+        std::vector<LogVarPlan> vars;
+        std::set<LoggingConfig::NameReduction> reductions;
+    };
+    struct LogPlan {
+        std::vector<LogAgentPlan> agents;
+        size_t record_bytes = 0;
+    };
+    struct PendingLogFrame {
+        unsigned int step;
+        double step_time;
+        std::map<std::string, detail::LogValue> env;
+    };
+    void check_log_model(const LoggingConfig& cfg, const char* where) const {
+        if (cfg.model().get() != m_.get())
+            throw exception::InvalidArgument(std::string("Model descriptions attached to LoggingConfig and CUDASimulation do not match, in CUDASimulation::") +
+                                             where + "()");
+    }
+    std::unique_ptr<LogPlan> make_log_plan(const LoggingConfig& cfg);
+    void enqueue_log_frame(const LogPlan& plan, char* record);
+    void decode_log_frame(const LoggingConfig& cfg, const LogPlan& plan, const char* record, LogFrame& f) const;
+    std::map<std::string, detail::LogValue> log_env(const LoggingConfig& cfg) const;
+    void reset_log();
+    void process_step_log(double step_time);
+    void flush_step_log();
+    void process_exit_log();
     void tick(const std::string& label, void* st, bool begin);
     void collect_timings();
 
@@ -660,6 +765,19 @@ class CUDASimulation {
     unsigned int step_ = 0;
     std::vector<double> step_times_;
     double elapsed_sim_ = 0.0;
+    double elapsed_init_ = 0.0;
+    double elapsed_exit_ = 0.0;
+    // logs
+    std::shared_ptr<StepLoggingConfig> step_log_cfg_;
+    std::shared_ptr<LoggingConfig> exit_log_cfg_;
+    std::unique_ptr<LogPlan> step_plan_, exit_plan_;
+    RunLog run_log_;
+    char* d_log_ = nullptr;          // frame records of logged steps not yet read back
+    size_t log_cap_frames_ = 0;
+    std::vector<PendingLogFrame> log_pending_;
+    char* d_exit_log_ = nullptr;
+    size_t exit_log_bytes_ = 0;
+    void* log_scratch_ = nullptr;
     std::unique_ptr<HostAPI> host_api_;
     bool init_functions_ran_ = false;
     // graphs
@@ -858,6 +976,15 @@ inline void CUDASimulation::release() {
     free_dev(d_reduce_out_);
     free_dev(d_pages_);
     free_dev(reduce_scratch_);
+    p = d_log_;
+    free_dev(p);
+    d_log_ = nullptr;
+    log_cap_frames_ = 0;
+    p = d_exit_log_;
+    free_dev(p);
+    d_exit_log_ = nullptr;
+    exit_log_bytes_ = 0;
+    free_dev(log_scratch_);
     if (h_small_) fgx_host_free(h_small_);
     fgx_stream_destroy(stream_);
     fgx_ctx_destroy(ctx_);
@@ -1087,6 +1214,149 @@ inline void CUDASimulation::setPopulationData(AgentVector& pop, const std::strin
     detail::fgx_check(fgx_memcpy_h2d(a.d_next_id, &next, 4, stream_), "next id upload");
     synchronize();
     clear_graphs();
+}
+
+// src/flamegpu/simulation/Simulation.cu:55-83 + src/flamegpu/io/StateReader.cu
+// getFullModelState: the file's configuration, then environment values, then every
+// (agent, state) population of the model (empty when the file has none).
+inline void CUDASimulation::load_state(const std::string& path) {
+    io::StateData d = io::read_state(path);
+    auto as_bool = [](std::string v) {
+        for (auto& c : v) c = (char)std::tolower((unsigned char)c);
+        return v == "true" || (v != "false" && std::stoll(v) != 0);
+    };
+    for (auto& kv : d.simulation) {
+        if (kv.first == "random_seed") config_.random_seed = std::stoull(kv.second);
+        else if (kv.first == "steps") config_.steps = (unsigned int)std::stoul(kv.second);
+        else if (kv.first == "timing") config_.timing = as_bool(kv.second);
+        else if (kv.first == "truncate_log_files") config_.truncate_log_files = as_bool(kv.second);
+    }
+    for (auto& kv : d.cuda) {
+        if (kv.first == "device_id" && !initialised_) cuda_config_.device_id = std::stoi(kv.second);
+        else if (kv.first == "inLayerConcurrency") cuda_config_.inLayerConcurrency = as_bool(kv.second);
+    }
+    for (auto& kv : d.environment) {
+        auto it = m_->env.props.find(kv.first);
+        if (it == m_->env.props.end())
+            throw exception::InvalidEnvProperty("Input file contains unrecognised environment property '" + kv.first + "'");
+        if (kv.second.size() > it->second.elements)
+            throw exception::OutOfBoundsException("Input file contains environment property '" + kv.first + "' with too many elements");
+        for (size_t e = 0; e < kv.second.size(); ++e)
+            io::detail::parse_token(it->second.type, kv.second[e], env_host_.data() + it->second.offset, e);
+        env_dirty_ = true;
+    }
+    for (auto& kv : d.agents) {
+        auto a = m_->agents.find(kv.first.first);
+        const std::string st = a == m_->agents.end() ? kv.first.second
+                             : kv.first.second.empty() ? a->second->initial_state : kv.first.second;
+        if (a == m_->agents.end() || !a->second->states.count(st))
+            throw exception::InvalidAgentState("Agent '" + kv.first.first + "' with state '" + st + "', found in input file '" +
+                                               path + "', is not part of the model description hierarchy");
+    }
+    ensure_init();
+    for (auto& ap : m_->agents) {
+        std::set<id_t> seen;
+        for (auto& st : ap.second->states) {
+            AgentVector pop(AgentDescription(ap.second));
+            const std::vector<io::StateData::AgentRecord>* recs = nullptr;
+            auto it = d.agents.find({ap.first, st});
+            if (it != d.agents.end()) recs = &it->second;
+            if (st == ap.second->initial_state) {
+                auto it0 = d.agents.find({ap.first, std::string()});
+                if (it0 != d.agents.end()) {
+                    if (recs) throw exception::InvalidInputFile("Input file gives agent '" + ap.first + "' state '" + st + "' twice");
+                    recs = &it0->second;
+                }
+            }
+            const size_t n = recs ? recs->size() : 0;
+            pop.resize(n);
+            for (size_t i = 0; i < n; ++i) {
+                for (auto& v : (*recs)[i].vars) {
+                    auto vit = ap.second->variables.find(v.first);
+                    if (vit == ap.second->variables.end())
+                        throw exception::InvalidAgentVar("Input file agent '" + ap.first + "' has unrecognised variable '" + v.first + "'");
+                    if (v.second.size() > vit->second.elements)
+                        throw exception::OutOfBoundsException("Input file agent variable '" + v.first + "' has too many elements");
+                    char* dst = pop.raw(v.first).data() + i * vit->second.bytes();
+                    for (size_t e = 0; e < v.second.size(); ++e) io::detail::parse_token(vit->second.type, v.second[e], dst, e);
+                }
+                id_t id;
+                std::memcpy(&id, pop.raw("_id").data() + i * sizeof(id_t), sizeof(id_t));
+                // tests/test_cases/io/test_io.cu AgentID_FileInput_IDCollision
+                if (id != ID_NOT_SET && !seen.insert(id).second)
+                    throw exception::AgentIDCollision("Input file contains agent '" + ap.first + "' ID " + std::to_string(id) +
+                                                      " more than once");
+            }
+            setPopulationData(pop, st);
+        }
+    }
+    // Next agent IDs as exported (this build's stats/next_id), so births after a resume
+    // get the IDs they would have had; never below the loaded IDs.
+    for (auto& kv : d.next_id) {
+        auto it = agents_.find(kv.first);
+        if (it == agents_.end()) continue;
+        sync_counts();
+        uint32_t mx = 0;
+        for (auto& st : it->second.states) {
+            auto& l = st.second;
+            if (!l.h_count) continue;
+            std::vector<uint32_t> ids(l.h_count);
+            detail::fgx_check(fgx_memcpy_d2h(ids.data(), l.vars[l.id_slot].cur, (size_t)l.h_count * 4, stream_), "id readback");
+            synchronize();
+            for (auto v : ids) mx = std::max(mx, v);
+        }
+        it->second.h_next_id = std::max(kv.second, mx + 1);
+        detail::fgx_check(fgx_memcpy_h2d(it->second.d_next_id, &it->second.h_next_id, 4, stream_), "next id upload");
+        synchronize();
+    }
+    // The step counter resumes from the file's step count, so that an exported state
+    // continues bit-exactly (random streams are keyed by agent ID and step); the
+    // reference restarts a loaded state at step 0.
+    if (d.has_step) {
+        step_ = d.step_count;
+        detail::fgx_check(fgx_fill_u32(d_step_, step_, 1, stream_), "step counter");
+        synchronize();
+    }
+    loaded_input_file_ = path;
+}
+
+inline void CUDASimulation::exportData(const std::string& path, bool prettyPrint) {
+    ensure_init();
+    sync_counts();
+    io::StateOut s;
+    s.simulation = {{"input_file", config_.input_file, true},
+                    {"random_seed", std::to_string(config_.random_seed), false},
+                    {"steps", std::to_string(config_.steps), false},
+                    {"timing", config_.timing ? "true" : "false", false},
+                    {"truncate_log_files", config_.truncate_log_files ? "true" : "false", false}};
+    s.cuda = {{"device_id", std::to_string(cuda_config_.device_id), false},
+              {"inLayerConcurrency", cuda_config_.inLayerConcurrency ? "true" : "false", false}};
+    s.step_count = step_;
+    for (auto& kv : agents_) s.next_id[kv.first] = kv.second.h_next_id;   // read by sync_counts above
+    for (auto& kv : m_->env.props) {
+        io::StateOut::Env e;
+        e.name = kv.first;
+        e.type = kv.second.type;
+        e.elements = (unsigned int)kv.second.elements;
+        e.bytes.assign(env_host_.data() + kv.second.offset,
+                       env_host_.data() + kv.second.offset + kv.second.type_size * kv.second.elements);
+        s.environment.push_back(std::move(e));
+    }
+    std::vector<std::unique_ptr<AgentVector>> keep;
+    for (auto& ap : m_->agents)
+        for (auto& st : ap.second->states) {
+            keep.emplace_back(new AgentVector(AgentDescription(ap.second)));
+            getPopulationData(*keep.back(), st);
+            io::StateOut::Pop p;
+            p.agent = ap.first;
+            p.state = st;
+            p.n = (uint32_t)keep.back()->size();
+            for (auto& v : ap.second->variables)
+                p.vars.push_back(io::StateOut::Pop::Var{v.first, v.second.type, (unsigned int)v.second.elements,
+                                                        &keep.back()->raw(v.first)});
+            s.pops.push_back(std::move(p));
+        }
+    io::write_state(path, s, prettyPrint);
 }
 
 inline void CUDASimulation::getPopulationData(AgentVector& pop, const std::string& state_in) {
@@ -1637,25 +1907,214 @@ inline bool CUDASimulation::step() {
     }
     if (config_.timing) synchronize();
     step_times_.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    process_step_log(step_times_.back());
     return cont;
 }
 
 inline void CUDASimulation::run_init_functions() {
     if (init_functions_ran_) return;
     init_functions_ran_ = true;
+    const auto t0 = std::chrono::steady_clock::now();
     ensure_init();
     for (auto fn : m_->init_functions) {
         synchronize();
         fn(host_api_.get());
     }
     upload_env();
+    elapsed_init_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 inline void CUDASimulation::run_exit_functions() {
+    const auto t0 = std::chrono::steady_clock::now();
     for (auto fn : m_->exit_functions) {
         synchronize();
         fn(host_api_.get());
     }
+    elapsed_exit_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ------------------------------------------------------------------ step / exit logs
+// Record layout of one logged frame: per (agent, state) an 8-byte count slot when
+// the count is logged, then 32 bytes per logged variable (fgx_reduce_stats output).
+inline std::unique_ptr<CUDASimulation::LogPlan> CUDASimulation::make_log_plan(const LoggingConfig& cfg) {
+    std::unique_ptr<LogPlan> p(new LogPlan());
+    size_t off = 0;
+    for (auto& kv : cfg.agents()) {
+        LogAgentPlan a;
+        a.agent = kv.first.first;
+        a.state = kv.first.second;
+        a.count = kv.second->count;
+        a.reductions = kv.second->reductions;
+        a.count_off = off;   // captured even when not logged: mean and deviation divide by it
+        off += 8;
+        const auto& ad = m_->agents.at(a.agent);
+        std::set<std::string> names;
+        for (auto& r : a.reductions) names.insert(r.name);
+        for (auto& n : names) {
+            const auto& v = ad->variables.at(n);
+            a.vars.push_back(LogVarPlan{n, detail::stats_type_code(v.type), v.type, off});
+            off += 32;
+        }
+        p->agents.push_back(std::move(a));
+    }
+    p->record_bytes = std::max<size_t>(off, 8);
+    return p;
+}
+
+inline void CUDASimulation::enqueue_log_frame(const LogPlan& plan, char* record) {
+    size_t sb = 0;
+    detail::fgx_check(fgx_scratch_bytes(FGX_OP_STATS, 0, 0, &sb), "log scratch sizing");
+    if (!log_scratch_) alloc(&log_scratch_, sb);
+    for (auto& a : plan.agents) {
+        auto& ar = agent_rt(a.agent);
+        auto it = ar.states.find(a.state);
+        if (it == ar.states.end()) throw exception::InvalidAgentState("Agent '" + a.agent + "' has no state '" + a.state + "'");
+        detail::StateList& l = it->second;
+        detail::fgx_check(fgx_memcpy_d2d(record + a.count_off, l.d_count, 4, stream_), "log count");
+        for (auto& v : a.vars) {
+            const int vi = l.var_index(v.name);
+            detail::fgx_check(fgx_reduce_stats(ctx_, v.type, l.vars[vi].cur, l.d_count, l.cap, record + v.off, log_scratch_, sb,
+                                               stream_),
+                              "log reduction");
+        }
+    }
+}
+
+namespace detail {
+template <typename T, typename Acc>
+inline void decode_stats(const char* w, unsigned int n, LoggingConfig::Reduction r, LogValue& out) {
+    Acc sum, mn, mx;
+    double sq;
+    std::memcpy(&sum, w, 8);
+    std::memcpy(&mn, w + 8, 8);
+    std::memcpy(&mx, w + 16, 8);
+    std::memcpy(&sq, w + 24, 8);
+    switch (r) {
+        case LoggingConfig::Sum: out = LogValue::of<typename sum_input_t<T>::result_t>((typename sum_input_t<T>::result_t)sum); break;
+        case LoggingConfig::Min: out = LogValue::of<T>((T)mn); break;
+        case LoggingConfig::Max: out = LogValue::of<T>((T)mx); break;
+        // include/flamegpu/simulation/AgentLoggingConfig.h: mean 0 and standard deviation 0 for an empty list
+        case LoggingConfig::Mean: out = LogValue::of<double>(n ? (double)sum / (double)n : 0.0); break;
+        case LoggingConfig::StandardDev: out = LogValue::of<double>(n ? std::sqrt(sq / (double)n) : 0.0); break;
+    }
+}
+}  // namespace detail
+
+inline void CUDASimulation::decode_log_frame(const LoggingConfig& cfg, const LogPlan& plan, const char* record,
+                                             LogFrame& f) const {
+    (void)cfg;
+    for (auto& a : plan.agents) {
+        auto& entry = f.agents[{a.agent, a.state}];
+        uint32_t n = 0;
+        std::memcpy(&n, record + a.count_off, 4);
+        entry.second = a.count ? n : UINT_MAX;
+        for (auto& r : a.reductions) {
+            const LogVarPlan* vp = nullptr;
+            for (auto& v : a.vars)
+                if (v.name == r.name) vp = &v;
+            const char* w = record + vp->off;
+            const uint32_t cnt = n;
+            detail::LogValue v;
+            switch (vp->type) {
+                case 0: detail::decode_stats<float, double>(w, cnt, r.reduction, v); break;
+                case 1: detail::decode_stats<double, double>(w, cnt, r.reduction, v); break;
+                case 2: detail::decode_stats<int32_t, long long>(w, cnt, r.reduction, v); break;
+                case 3: detail::decode_stats<uint32_t, unsigned long long>(w, cnt, r.reduction, v); break;
+                case 4: detail::decode_stats<int64_t, long long>(w, cnt, r.reduction, v); break;
+                default: detail::decode_stats<uint64_t, unsigned long long>(w, cnt, r.reduction, v); break;
+            }
+            entry.first.emplace(r, std::move(v));
+        }
+    }
+}
+
+inline std::map<std::string, detail::LogValue> CUDASimulation::log_env(const LoggingConfig& cfg) const {
+    std::map<std::string, detail::LogValue> out;
+    for (auto& name : cfg.environment()) {
+        const auto& p = m_->env.props.at(name);
+        detail::LogValue v;
+        v.type = p.type;
+        v.elements = (unsigned int)p.elements;
+        v.bytes.assign(env_host_.data() + p.offset, env_host_.data() + p.offset + p.type_size * p.elements);
+        out.emplace(name, std::move(v));
+    }
+    return out;
+}
+
+// src/flamegpu/simulation/CUDASimulation.cu: resetLog()
+inline void CUDASimulation::reset_log() {
+    log_pending_.clear();
+    run_log_ = RunLog();
+    run_log_.random_seed = config_.random_seed;
+    run_log_.step_log_frequency = step_log_cfg_ ? step_log_cfg_->frequency() : 0;
+    run_log_.performance_specs.device_name = "AMD Instinct MI355X (gfx950)";
+    run_log_.performance_specs.device_cc_major = 9;
+    run_log_.performance_specs.device_cc_minor = 5;
+}
+
+// src/flamegpu/simulation/CUDASimulation.cu:1949-1983: a frame every `frequency` steps.
+inline void CUDASimulation::process_step_log(double step_time) {
+    if (!step_log_cfg_) return;
+    const unsigned int freq = step_log_cfg_->frequency();
+    if (freq == 0 || step_ % freq != 0) return;
+    if (!step_plan_) {
+        flush_step_log();
+        step_plan_ = make_log_plan(*step_log_cfg_);
+        void* p = d_log_;
+        free_dev(p);
+        d_log_ = nullptr;
+        log_cap_frames_ = 0;
+    }
+    if (!d_log_) {
+        log_cap_frames_ = 256;
+        void* p = nullptr;
+        alloc(&p, log_cap_frames_ * step_plan_->record_bytes);
+        d_log_ = static_cast<char*>(p);
+    }
+    if (log_pending_.size() == log_cap_frames_) flush_step_log();
+    enqueue_log_frame(*step_plan_, d_log_ + log_pending_.size() * step_plan_->record_bytes);
+    log_pending_.push_back(PendingLogFrame{step_, step_time, log_env(*step_log_cfg_)});
+}
+
+inline void CUDASimulation::flush_step_log() {
+    if (log_pending_.empty()) return;
+    std::vector<char> host(log_pending_.size() * step_plan_->record_bytes);
+    detail::fgx_check(fgx_memcpy_d2h(host.data(), d_log_, host.size(), stream_), "log readback");
+    synchronize();
+    for (size_t i = 0; i < log_pending_.size(); ++i) {
+        StepLogFrame f;
+        f.step_count = log_pending_[i].step;
+        f.step_time = log_pending_[i].step_time;
+        f.environment = std::move(log_pending_[i].env);
+        decode_log_frame(*step_log_cfg_, *step_plan_, host.data() + i * step_plan_->record_bytes, f);
+        run_log_.step.push_back(std::move(f));
+    }
+    log_pending_.clear();
+}
+
+// src/flamegpu/simulation/CUDASimulation.cu:1985-2021
+inline void CUDASimulation::process_exit_log() {
+    if (!exit_log_cfg_) return;
+    if (!exit_plan_) exit_plan_ = make_log_plan(*exit_log_cfg_);
+    if (exit_log_bytes_ < exit_plan_->record_bytes) {
+        void* p = d_exit_log_;
+        free_dev(p);
+        alloc(&p, exit_plan_->record_bytes);
+        d_exit_log_ = static_cast<char*>(p);
+        exit_log_bytes_ = exit_plan_->record_bytes;
+    }
+    enqueue_log_frame(*exit_plan_, d_exit_log_);
+    std::vector<char> host(exit_plan_->record_bytes);
+    detail::fgx_check(fgx_memcpy_d2h(host.data(), d_exit_log_, host.size(), stream_), "exit log readback");
+    synchronize();
+    ExitLogFrame f;
+    f.step_count = step_;
+    f.environment = log_env(*exit_log_cfg_);
+    decode_log_frame(*exit_log_cfg_, *exit_plan_, host.data(), f);
+    f.init_time = elapsed_init_;
+    f.exit_time = elapsed_exit_;
+    f.total_time = elapsed_sim_;
+    run_log_.exit = std::move(f);
 }
 
 // ------------------------------------------------------------------ HostAPI implementation
@@ -1750,9 +2209,20 @@ inline T HostEnvironment::setProperty(const std::string& name, T value) {
     s_.setEnvironmentProperty<T>(name, value);
     return old;
 }
+template <typename T, unsigned int N>
+inline std::array<T, N> HostEnvironment::getProperty(const std::string& name) const {
+    return s_.getEnvironmentProperty<T, N>(name);
+}
+template <typename T, unsigned int N>
+inline std::array<T, N> HostEnvironment::setProperty(const std::string& name, const std::array<T, N>& value) {
+    const std::array<T, N> old = s_.getEnvironmentProperty<T, N>(name);
+    s_.setEnvironmentProperty<T, N>(name, value);
+    return old;
+}
 
 // ------------------------------------------------------------------ ensembles
-struct RunLog {
+// Per-run bookkeeping of an ensemble (plan, seed, wall time, final counts, error).
+struct RunRecord {
     unsigned int plan_index = 0;
     uint64_t seed = 0;
     unsigned int steps = 0;
@@ -1789,9 +2259,22 @@ class CUDAEnsemble {
     const EnsembleConfig& getConfig() const { return config_; }
     // Population initialiser called for every run before it starts.
     void setPopulationInit(std::function<void(CUDASimulation&, const RunPlan&, unsigned int)> f) { init_ = std::move(f); }
+    // src/flamegpu/simulation/CUDAEnsemble.cu: setStepLog / setExitLog
+    void setStepLog(const StepLoggingConfig& cfg) {
+        if (cfg.model().get() != model_.data().get())
+            throw exception::InvalidArgument("Model descriptions attached to LoggingConfig and CUDAEnsemble do not match, in CUDAEnsemble::setStepLog()");
+        step_log_ = std::make_shared<StepLoggingConfig>(cfg);
+    }
+    void setExitLog(const LoggingConfig& cfg) {
+        if (cfg.model().get() != model_.data().get())
+            throw exception::InvalidArgument("Model descriptions attached to LoggingConfig and CUDAEnsemble do not match, in CUDAEnsemble::setExitLog()");
+        exit_log_ = std::make_shared<LoggingConfig>(cfg);
+    }
     unsigned int simulate(const RunPlanVector& plans) {
         logs_.clear();
         logs_.resize(plans.size());
+        run_logs_.clear();
+        std::mutex log_mutex;
         std::atomic<unsigned> next{0}, errors{0};
         std::atomic<bool> stop{false};
         const auto t0 = std::chrono::steady_clock::now();
@@ -1799,7 +2282,7 @@ class CUDAEnsemble {
             while (!stop) {
                 const unsigned i = next++;
                 if (i >= plans.size()) break;
-                RunLog& lg = logs_[i];
+                RunRecord& lg = logs_[i];
                 lg.plan_index = i;
                 lg.seed = plans[i].getRandomSimulationSeed();
                 lg.steps = plans[i].getSteps();
@@ -1811,8 +2294,13 @@ class CUDAEnsemble {
                     sim.SimulationConfig().steps = plans[i].getSteps();
                     for (auto& kv : plans[i].properties()) sim_set_env(sim, kv.first, kv.second);
                     if (init_) init_(sim, plans[i], i);
+                    if (step_log_) sim.setStepLog(*step_log_);
+                    if (exit_log_) sim.setExitLog(*exit_log_);
                     sim.simulate();
                     for (auto& a : model_.data()->agents) lg.final_counts[a.first] = sim_count(sim, a.first);
+                    const RunLog& rl = sim.getRunLog();
+                    std::lock_guard<std::mutex> lock(log_mutex);
+                    run_logs_[i] = rl;
                 } catch (const std::exception& e) {
                     lg.error = e.what();
                     ++errors;
@@ -1832,7 +2320,9 @@ class CUDAEnsemble {
         }
         return errors;
     }
-    const std::vector<RunLog>& getLogs() const { return logs_; }
+    // include/flamegpu/simulation/CUDAEnsemble.h: getLogs(), keyed by plan index
+    const std::map<unsigned int, RunLog>& getLogs() const { return run_logs_; }
+    const std::vector<RunRecord>& getRunRecords() const { return logs_; }
     double getEnsembleElapsedTime() const { return elapsed_; }
 
  private:
@@ -1854,7 +2344,10 @@ class CUDAEnsemble {
     }
     ModelDescription model_;
     EnsembleConfig config_;
-    std::vector<RunLog> logs_;
+    std::vector<RunRecord> logs_;
+    std::map<unsigned int, RunLog> run_logs_;
+    std::shared_ptr<StepLoggingConfig> step_log_;
+    std::shared_ptr<LoggingConfig> exit_log_;
     std::function<void(CUDASimulation&, const RunPlan&, unsigned int)> init_;
     double elapsed_ = 0.0;
 };

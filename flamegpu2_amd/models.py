@@ -41,6 +41,12 @@ def _lib(model):
         "fgxm_kernel_times": [vp, c.c_char_p, c.c_uint32, c.c_int],
         "fgxm_timed_steps": [vp, c.c_uint32, c.POINTER(c.c_double)],
         "fgxm_step_counter": [vp, u32p],
+        "fgxm_simulate": [vp, c.c_uint32],
+        "fgxm_log_config": [vp, c.c_int, c.c_char_p, c.c_char_p, c.c_char_p, c.c_char_p, c.c_uint32],
+        "fgxm_export_log": [vp, c.c_char_p, c.c_int, c.c_int, c.c_int, c.c_int],
+        "fgxm_export_state": [vp, c.c_char_p, c.c_int],
+        "fgxm_load_state": [vp, c.c_char_p],
+        "fgxm_state_summary": [c.c_char_p, c.POINTER(c.c_float), c.c_uint32, c.c_char_p, c.c_char_p, c.c_uint32],
         "fgxm_batch_create": [c.c_char_p, c.POINTER(c.c_float), c.c_uint32, c.c_int, c.POINTER(vp)],
         "fgxm_batch_destroy": [vp],
         "fgxm_batch_var_count": [vp, c.c_char_p, u32p],
@@ -180,6 +186,41 @@ class Simulation:
         n = ctypes.c_uint32()
         self._chk(self.L.fgxm_step_counter(self.h, ctypes.byref(n)))
         return n.value
+
+    def simulate(self, steps):
+        """CUDASimulation::simulate(): init functions, step log frame 0, `steps` steps,
+        exit functions, exit log."""
+        self._chk(self.L.fgxm_simulate(self.h, steps))
+
+    def log(self, what, agent=None, name=None, state=None, value=0, exit=False):
+        """Adds one item to the step log (exit=False) or the exit log: what is "count",
+        "mean", "standard_deviation", "min", "max", "sum" (agent, name = variable),
+        "env" (name = property), "timing" or "frequency" (value)."""
+        self._chk(self.L.fgxm_log_config(self.h, 1 if exit else 0, _b(what), _b(agent), _b(state), _b(name), int(value)))
+
+    def export_state(self, path, pretty=True):
+        """CUDASimulation::exportData to an .xml or .json state file."""
+        self._chk(self.L.fgxm_export_state(self.h, _b(str(path)), int(pretty)))
+
+    def load_state(self, path):
+        """Loads an .xml or .json state file (SimulationConfig().input_file + applyConfig)."""
+        self._chk(self.L.fgxm_load_state(self.h, _b(str(path))))
+
+    def export_log(self, path, steps=True, exit=True, step_time=False, exit_time=False):
+        self._chk(self.L.fgxm_export_log(self.h, _b(str(path)), int(steps), int(exit), int(step_time), int(exit_time)))
+
+
+def state_summary(model, path, params=(), variant=None):
+    """Parses a state file against a model on the host only (no GPU): step count,
+    environment values and per (agent, state) counts and per-variable sums."""
+    import json
+    L = _lib(model)
+    p = (ctypes.c_float * max(len(params), 1))(*params)
+    buf = ctypes.create_string_buffer(1 << 20)
+    rc = L.fgxm_state_summary(_b(variant), p, len(params), _b(str(path)), buf, len(buf))
+    if rc:
+        raise FgxError(rc, L.fgxm_last_error().decode())
+    return json.loads(buf.value.decode())
 
 
 class Batch:

@@ -95,7 +95,8 @@ enum {
     FGX_OP_COMPACT = 1,        /* n = max items */
     FGX_OP_BIRTHS = 2,         /* n = max parents */
     FGX_OP_AGENT_SORT = 3,     /* n = max agents, bins = 2^key_bits */
-    FGX_OP_SCAN = 4            /* n = max items */
+    FGX_OP_SCAN = 4,           /* n = max items */
+    FGX_OP_STATS = 5           /* fgx_reduce_stats; n and bins unused */
 };
 int fgx_scratch_bytes(int op, uint32_t n, uint32_t bins, size_t* bytes);
 int fgx_scratch_clear(void* scratch, size_t bytes, void* stream);
@@ -211,6 +212,18 @@ int fgx_assign_ids(uint32_t* ids, uint32_t n, uint32_t next_id, void* stream);
  * Writes a double (f32/f64) or 64-bit integer to *d_out. */
 int fgx_reduce_sum(fgx_ctx* ctx, int type, const void* in, const uint32_t* d_n, uint32_t n_cap,
                    void* d_out, void* scratch, size_t scratch_bytes, void* stream);
+
+/* Statistics of one agent variable for the step and exit logs
+ * (include/flamegpu/simulation/AgentLoggingConfig.h: getAgentVariableSum/Min/Max/Mean/
+ * StandardDevFunc; src/flamegpu/simulation/CUDASimulation.cu:1949-2021 processStepLog).
+ * Writes four 8-byte words to d_out (8-byte aligned, device memory):
+ *   [0] sum, [1] min, [2] max  as double (f32/f64), int64 (i32/i64) or uint64 (u32/u64);
+ *   [3] sum over items of ((double)x - sum / n)^2 as double (0 when n == 0).
+ * min/max of an empty list are the type's largest / lowest value (the reduction identities).
+ * Deterministic: a fixed-shape tree, so the bits do not depend on scheduling.  Four
+ * launches, no host synchronisation.  Scratch: fgx_scratch_bytes(FGX_OP_STATS, 0, 0). */
+int fgx_reduce_stats(fgx_ctx* ctx, int type, const void* in, const uint32_t* d_n, uint32_t n_cap,
+                     void* d_out, void* scratch, size_t scratch_bytes, void* stream);
 
 /* ---------------------------------------------------------------- batched (segmented) primitives
  * A batch runs many independent simulations in the same launches.  Their lists

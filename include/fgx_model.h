@@ -57,6 +57,27 @@ int fgxm_kernel_times(fgxm_sim* sim, char* buf, uint32_t buflen, int reset);
  * returns the elapsed milliseconds (inputs already resident on the device). */
 int fgxm_timed_steps(fgxm_sim* sim, uint32_t steps, double* ms);
 int fgxm_step_counter(fgxm_sim* sim, uint32_t* out);
+/* CUDASimulation::simulate() for `steps` steps: init functions, step log frame 0,
+ * the steps, exit functions, exit log (src/flamegpu/simulation/CUDASimulation.cu:1220-1344). */
+int fgxm_simulate(fgxm_sim* sim, uint32_t steps);
+/* Step / exit logging (include/flamegpu/simulation/LoggingConfig.h, AgentLoggingConfig.h):
+ * exit_log 0 configures the step log, 1 the exit log.  what:
+ *   "count" (agent, state), "mean" | "standard_deviation" | "min" | "max" | "sum" (agent,
+ *   state, name = variable), "env" (name = property), "timing" (value 0/1),
+ *   "frequency" (step log only, value = steps).  state NULL or "" = "default". */
+int fgxm_log_config(fgxm_sim* sim, int exit_log, const char* what, const char* agent, const char* state,
+                    const char* name, uint32_t value);
+/* State files (.xml / .json by extension; src/flamegpu/io/{XML,JSON}State{Reader,Writer}.cu):
+ * export = CUDASimulation::exportData; load = SimulationConfig().input_file + applyConfig
+ * (configuration, environment, every agent population; the step counter resumes from
+ * the file's step count).  fgxm_state_summary parses a file against the model on the
+ * host only and writes a JSON description (counts, per-variable sums) to buf. */
+int fgxm_export_state(fgxm_sim* sim, const char* path, int pretty);
+int fgxm_load_state(fgxm_sim* sim, const char* path);
+int fgxm_state_summary(const char* variant, const float* params, uint32_t nparams, const char* path, char* buf,
+                       uint32_t buflen);
+/* Writes the run log as JSON (src/flamegpu/io/JSONLogger.cu layout). */
+int fgxm_export_log(fgxm_sim* sim, const char* path, int steps, int exit_log, int step_time, int exit_time);
 
 /* ---------------------------------------------------------------- batches
  * Many simulations of this model co-scheduled in shared segmented launches

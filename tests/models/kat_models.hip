@@ -277,6 +277,29 @@ FLAMEGPU_AGENT_FUNCTION(sum_messages, MessageBruteForce, MessageNone) {
     return ALIVE;
 }
 
+FLAMEGPU_AGENT_FUNCTION(AddFloatOne, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<float>("float_var", FLAMEGPU->getVariable<float>("float_var") + 1.0f);
+    return ALIVE;
+}
+// tests/test_cases/io/test_logging.cu:18-36
+FLAMEGPU_AGENT_FUNCTION(agent_fn1, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<float>("float_var", FLAMEGPU->getVariable<float>("float_var") + 1.0f);
+    FLAMEGPU->setVariable<int>("int_var", FLAMEGPU->getVariable<int>("int_var") + 1);
+    FLAMEGPU->setVariable<unsigned int>("uint_var", FLAMEGPU->getVariable<unsigned int>("uint_var") + 1);
+    return ALIVE;
+}
+FLAMEGPU_STEP_FUNCTION(step_fn1) {
+    FLAMEGPU->environment.setProperty<float>("float_prop", FLAMEGPU->environment.getProperty<float>("float_prop") + 1.0f);
+    FLAMEGPU->environment.setProperty<int>("int_prop", FLAMEGPU->environment.getProperty<int>("int_prop") + 1);
+    FLAMEGPU->environment.setProperty<unsigned int>("uint_prop", FLAMEGPU->environment.getProperty<unsigned int>("uint_prop") + 1);
+    auto a = FLAMEGPU->environment.getProperty<float, 2>("float_prop_array");
+    auto b = FLAMEGPU->environment.getProperty<int, 3>("int_prop_array");
+    auto c = FLAMEGPU->environment.getProperty<unsigned int, 4>("uint_prop_array");
+    FLAMEGPU->environment.setProperty<float, 2>("float_prop_array", {a[0] + 1.0f, a[1] + 1.0f});
+    FLAMEGPU->environment.setProperty<int, 3>("int_prop_array", {b[0] + 1, b[1] + 1, b[2] + 1});
+    FLAMEGPU->environment.setProperty<unsigned int, 4>("uint_prop_array", {c[0] + 1, c[1] + 1, c[2] + 1, c[3] + 1});
+}
+
 flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* p, uint32_t np) {
     const std::string v = variant_c ? variant_c : "";
     flamegpu::ModelDescription model("kat_" + v);
@@ -487,6 +510,26 @@ flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* 
         sum.setMessageInput("msg");
         model.newLayer().addAgentFunction(act);
         model.newLayer().addAgentFunction(sum);
+    } else if (v == "logging" || v == "logging_empty") {
+        // tests/test_cases/io/test_logging.cu:49-78 (CUDASimulationStep), :336-356 (EmptyMean)
+        auto a = model.newAgent("Agent1");
+        a.newVariable<float>("float_var");
+        if (v == "logging") {
+            a.newVariable<int>("int_var");
+            a.newVariable<unsigned int>("uint_var");
+            auto f1 = a.newFunction("Func1", agent_fn1);
+            model.newLayer().addAgentFunction(f1);
+            model.addStepFunction(step_fn1);
+            model.Environment().newProperty<float>("float_prop", 1.0f);
+            model.Environment().newProperty<int>("int_prop", 1);
+            model.Environment().newProperty<unsigned int>("uint_prop", 1);
+            model.Environment().newProperty<float, 2>("float_prop_array", {1.0f, 2.0f});
+            model.Environment().newProperty<int, 3>("int_prop_array", {2, 3, 4});
+            model.Environment().newProperty<unsigned int, 4>("uint_prop_array", {3, 4, 5, 6});
+        } else {
+            auto f1 = a.newFunction("Func1", AddFloatOne);
+            model.newLayer().addAgentFunction(f1);
+        }
     } else {
         throw flamegpu::exception::InvalidArgument("unknown test model variant '" + v + "'");
     }

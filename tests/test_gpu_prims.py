@@ -251,3 +251,47 @@ def test_reduce_sum_deterministic(ctx):
     fgx.reduce_sum(ctx, 3, ptr(d_u), n, ptr(o), ptr(s), sb)
     sync()
     assert int(host(o, np.uint64)[0]) == int(u.astype(np.uint64).sum())
+
+
+@pytest.mark.parametrize("code,dtype,acc", [(0, np.float32, np.float64), (1, np.float64, np.float64),
+                                            (2, np.int32, np.int64), (3, np.uint32, np.uint64),
+                                            (4, np.int64, np.int64), (5, np.uint64, np.uint64)])
+@pytest.mark.parametrize("n", [0, 1, 101, 1_000_003])
+def test_reduce_stats(ctx, code, dtype, acc, n):
+    """Sum / min / max exact (integers) or to double rounding (floats); the squared
+    deviation sum against numpy's two-pass formula; min/max of an empty list are the
+    reduction identities."""
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(5 + code)
+    if np.issubdtype(dtype, np.floating):
+        a = rng.normal(3.0, 40.0, size=n).astype(dtype)
+    elif np.issubdtype(dtype, np.signedinteger):
+        a = rng.integers(-100000, 100000, size=n).astype(dtype)
+    else:
+        a = rng.integers(0, 200000, size=n).astype(dtype)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_STATS, 0)
+    s = scratch(sb)
+    d_a = dev(a if n else np.zeros(1, dtype))
+    o = empty(4, np.uint64)
+    fgx.reduce_stats(ctx, code, ptr(d_a), n, ptr(o), ptr(s), sb)
+    sync()
+    raw = host(o, np.uint64, 4)
+    sm, mn, mx = raw[:3].view(acc)
+    sq = raw[3:].view(np.float64)[0]
+    if n == 0:
+        assert sm == 0 and sq == 0.0
+        if acc is np.float64:
+            assert mn == np.inf and mx == -np.inf
+        else:
+            assert mn == np.iinfo(acc).max and mx == np.iinfo(acc).min
+        return
+    w = a.astype(acc)
+    assert mn == w.min() and mx == w.max()
+    if acc is np.float64:
+        assert abs(sm - w.sum()) <= 1e-12 * np.abs(w).sum()
+    else:
+        assert int(sm) == int(w.sum())
+    mean = float(sm) / n
+    d = a.astype(np.float64) - mean
+    ref = float((d * d).sum())
+    assert abs(sq - ref) <= 1e-10 * max(ref, 1.0)
