@@ -29,15 +29,23 @@ __device__ __forceinline__ uint32_t load_count(const uint32_t* d_n, uint32_t n_c
 struct KeyArgs {
     int32_t mode;        // 0: message bin (clamped, MessageSpatial3DDevice.cuh:646-672)
                          // 1: agent sort key (unclamped, CUDASimulation.cu:372-408)
+                         // 2: bucket key, x holds int32 keys (MessageBucket.cu:58-72)
     int32_t dims;
     float min[3];
     float radius;
     float width[3];
     uint32_t grid[3];
     uint32_t key_mask;   // mode 1: (1 << key_bits) - 1
+    int32_t key_min;     // mode 2: lowest key
+    uint32_t keys;       // mode 2: number of keys
 };
 
 __device__ __forceinline__ uint32_t bin_key(const KeyArgs& k, float x, float y, float z) {
+    if (k.mode == 2) {
+        // keys outside the range were reported by setKey; clamp so they stay in the table
+        const int v = __float_as_int(x) - k.key_min;
+        return v < 0 ? 0u : ((uint32_t)v >= k.keys ? k.keys - 1u : (uint32_t)v);
+    }
     if (k.mode == 0) {
         int c[3];
         const float p[3] = {x, y, z};
@@ -378,6 +386,53 @@ __global__ void __launch_bounds__(kBlock) k_reduce_final(const Acc* __restrict__
     if (threadIdx.x == 0) *out = s[0];
 }
 
+// ================================================================ array messages
+// src/flamegpu/simulation/detail/CUDAScatter.cu:540-640 (reorder_array_messages): message i
+// goes to array element index[i]; elements nobody wrote are zero; two writes to one
+// element, or more messages than elements, are an error (ArrayMessageWriteConflict).
+constexpr uint32_t kErrArrayConflict = 18;   // DERR_ARRAY_CONFLICT of the device API
+
+__device__ __forceinline__ void array_report(uint32_t* d_error, uint32_t detail) {
+    if (d_error && atomicCAS(&d_error[0], 0u, kErrArrayConflict) == 0u) {
+        d_error[1] = detail;
+        d_error[2] = 0xFFFFFFFFu;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_array_zero(VarTable vt, uint32_t length, uint32_t* __restrict__ counts) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t i = t0; i < length; i += stride) counts[i] = 0u;
+    for (uint32_t v = 0; v < vt.n; ++v) {
+        const size_t bytes = (size_t)length * vt.bytes[v];
+        if ((vt.bytes[v] & 3u) == 0u) {
+            uint32_t* o = reinterpret_cast<uint32_t*>(vt.out[v]);
+            for (size_t w = t0; w < bytes / 4; w += stride) o[w] = 0u;
+        } else {
+            for (size_t b = t0; b < bytes; b += stride) vt.out[v][b] = 0;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_array_scatter(const uint32_t* d_n, uint32_t n_cap, const uint32_t* __restrict__ index,
+                                                          uint32_t length, VarTable vt, uint32_t* __restrict__ counts,
+                                                          uint32_t* d_error) {
+    const uint32_t n = load_count(d_n, n_cap);
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 == 0 && n > length) array_report(d_error, n);
+    for (uint32_t i = t0; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t e = index[i];
+        if (e >= length) continue;   // reported by setIndex
+        copy_element(vt, i, e);
+        atomicAdd(&counts[e], 1u);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_array_check(const uint32_t* __restrict__ counts, uint32_t length, uint32_t* d_error) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < length; i += gridDim.x * blockDim.x)
+        if (counts[i] > 1u) array_report(d_error, i);
+}
+
 // ================================================================ deterministic statistics
 // Sum, minimum and maximum in one pass, then the sum of squared deviations from the
 // mean (which needs the finished sum) in a second.  Each block owns a fixed strided
@@ -700,6 +755,46 @@ int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t
         ka.grid[a] = meta->grid[a] ? meta->grid[a] : 1u;
     }
     return run_bin_sort(ctx, ka, meta->bins, d_n, n_cap, x, y, z, vt, pbm, scratch, scratch_bytes, stream);
+}
+
+int fgx_bucket_build(fgx_ctx* ctx, int32_t key_min, uint32_t keys, const uint32_t* d_n, uint32_t n_cap, const int32_t* key,
+                     uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                     uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream) {
+    if (!pbm || !keys) return fail(FGX_ERR_ARG, "fgx_bucket_build: NULL bucket table or no keys");
+    if (n_cap && !key) return fail(FGX_ERR_ARG, "fgx_bucket_build: missing key array");
+    VarTable vt;
+    int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
+    if (rc) return rc;
+    KeyArgs ka;
+    std::memset(&ka, 0, sizeof(ka));
+    ka.mode = 2;
+    ka.dims = 1;
+    ka.key_min = key_min;
+    ka.keys = keys;
+    const float* kf = reinterpret_cast<const float*>(key);
+    return run_bin_sort(ctx, ka, keys, d_n, n_cap, kf, kf, nullptr, vt, pbm, scratch, scratch_bytes, stream);
+}
+
+int fgx_array_build(fgx_ctx* ctx, uint32_t length, const uint32_t* d_n, uint32_t n_cap, const uint32_t* index,
+                    uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                    uint32_t* counts, uint32_t* d_error, void* stream) {
+    if (!length || !counts) return fail(FGX_ERR_ARG, "fgx_array_build: zero length or NULL counts");
+    if (n_cap && !index) return fail(FGX_ERR_ARG, "fgx_array_build: missing index array");
+    VarTable vt;
+    int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t gz = stream_grid(ctx ? &ctx->c : nullptr, length);
+    hipLaunchKernelGGL(k_array_zero, dim3(gz), dim3(kBlock), 0, st, vt, length, counts);
+    FGX_HIP(hipGetLastError());
+    if (n_cap) {
+        const uint32_t g = stream_grid(ctx ? &ctx->c : nullptr, n_cap);
+        hipLaunchKernelGGL(k_array_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, index, length, vt, counts, d_error);
+        FGX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_array_check, dim3(gz), dim3(kBlock), 0, st, counts, length, d_error);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
 }
 
 int fgx_agent_sort(fgx_ctx* ctx, int dims, const float* mn, const float* mx, float radius,

@@ -262,6 +262,10 @@ inline void BatchSimulation::validate() {
         }
     }
     for (auto& mm : m_->messages)
+        if (mm.second->kind == MsgKind::Bucket || is_array_kind(mm.second->kind))
+            throw exception::InvalidOperation("BatchSimulation: bucket and array message lists ('" + mm.first +
+                                              "') cannot be batched");
+    for (auto& mm : m_->messages)
         if (mm.second->persistent)
             throw exception::InvalidOperation("BatchSimulation: persistent message lists cannot be batched");
     for (auto& a : m_->agents)

@@ -75,6 +75,8 @@ FGX_DERIVED_EXCEPTION(InvalidStateName)
 FGX_DERIVED_EXCEPTION(ReservedName)
 FGX_DERIVED_EXCEPTION(DuplicateEnvProperty)
 FGX_DERIVED_EXCEPTION(AgentIDCollision)
+FGX_DERIVED_EXCEPTION(ArrayMessageWriteConflict)
+FGX_DERIVED_EXCEPTION(InvalidMessage)
 FGX_DERIVED_EXCEPTION(InvalidInputFile)
 FGX_DERIVED_EXCEPTION(InvalidHostFunc)
 FGX_DERIVED_EXCEPTION(InvalidPopulationData)
@@ -160,6 +162,9 @@ struct alignas(8) EnvBinding {
 };
 
 // Device mirror of the spatial message metadata (MessageSpatial3D.h:38-68).
+// Message-input metadata.  Spatial: min/max/radius/width/grid/wrap_ok and the bin
+// table pbm.  Bucket: key_min, keys and pbm (keys + 1 bucket starts).  Array: grid =
+// the array dimensions (unused ones 1).
 struct SpatialDev {
     float min[3];
     float max[3];
@@ -168,6 +173,8 @@ struct SpatialDev {
     uint32_t grid[3];
     int32_t wrap_ok;
     const uint32_t* pbm;
+    int32_t key_min;
+    uint32_t keys;
 };
 
 // Device error codes (the device half of exception::DeviceError, replacing DTHROW).
@@ -188,6 +195,10 @@ enum DeviceErrorCode : uint32_t {
     DERR_ARRAY_INDEX = 13,
     DERR_RANDOM_RANGE = 14,
     DERR_MSG_OUTPUT_DISABLED = 15,
+    DERR_BUCKET_KEY = 16,
+    DERR_ARRAY_MSG_INDEX = 17,
+    DERR_ARRAY_CONFLICT = 18,
+    DERR_ARRAY_RADIUS = 19,
 };
 
 inline const char* device_error_text(uint32_t code) {
@@ -196,6 +207,10 @@ inline const char* device_error_text(uint32_t code) {
         case DERR_AGENT_VAR_TYPE: return "agent variable accessed with the wrong type or length";
         case DERR_UNKNOWN_MSG_VAR: return "unknown message variable";
         case DERR_MSG_VAR_TYPE: return "message variable accessed with the wrong type or length";
+        case DERR_BUCKET_KEY: return "bucket message key out of range (MessageBucket)";
+        case DERR_ARRAY_MSG_INDEX: return "array message index out of bounds (MessageArray*::setIndex / iteration origin)";
+        case DERR_ARRAY_CONFLICT: return "multiple messages output to the same array message index, or more messages than array elements";
+        case DERR_ARRAY_RADIUS: return "invalid radius for array message iteration";
         case DERR_WRAP_OUT_OF_BOUNDS: return "location exceeds environment bounds, unsupported by the wrapped iterator (MessageSpatial::In::wrap())";
         case DERR_WRAP_NOT_FACTOR: return "spatial messaging radius is not a factor of the environment dimensions, unsupported by the wrapped iterator";
         case DERR_DEATH_DISABLED: return "agent death must be enabled per agent function when defining the model";
@@ -237,6 +252,10 @@ struct FnArgs {
     const float* stage_pos[3];        // agent x, y, z (the assumed search origin)
     const uint32_t* stage_src[STAGE_MAX_VARS];
     SpatialDev in_meta;
+    // message output checks: array dimensions / bucket key range of the output list
+    uint32_t out_dims[3];
+    int32_t out_key_min;
+    uint32_t out_keys;
     VarBinding agent[VAR_SLOTS];
     VarBinding msg_in[VAR_SLOTS];
     VarBinding msg_out[VAR_SLOTS];

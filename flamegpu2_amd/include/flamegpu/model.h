@@ -24,7 +24,8 @@
 
 namespace flamegpu {
 
-enum class MsgKind { None = 0, BruteForce = 1, Spatial2D = 2, Spatial3D = 3 };
+enum class MsgKind { None = 0, BruteForce = 1, Spatial2D = 2, Spatial3D = 3, Bucket = 4, Array = 5, Array2D = 6, Array3D = 7 };
+inline bool is_array_kind(MsgKind k) { return k == MsgKind::Array || k == MsgKind::Array2D || k == MsgKind::Array3D; }
 
 struct Variable {
     std::string name;
@@ -81,6 +82,12 @@ struct MessageData {
     float max[3] = {0, 0, 0};
     bool radius_set = false, min_set = false, max_set = false;
     bool persistent = false;
+    // bucket: keys [lower, upper] (both inclusive; include/flamegpu/runtime/messaging/MessageBucket/MessageBucketHost.h)
+    int32_t lower = 0;
+    int32_t upper = 0x7FFFFFFF;
+    bool upper_set = false;
+    // array: dimensions (1-D uses dims[0]; unused ones 1)
+    uint32_t dims[3] = {0, 1, 1};
     ModelData* model = nullptr;
 };
 
@@ -185,7 +192,7 @@ struct MessageBruteForce {
                 throw exception::InvalidMessageVar("Message '" + d_->name + "' already contains variable '" + v.name +
                                                    "', in MessageDescription::newVariable()");
             if (!v.name.empty() && v.name[0] == '_')
-                throw exception::InvalidMessageVar("Message variable names starting with '_' are reserved");
+                throw exception::ReservedName("Message variable names starting with '_' are reserved");
             d_->variables.emplace(v.name, v);
         }
         std::shared_ptr<MessageData> d_;
@@ -259,6 +266,99 @@ struct MessageSpatial3D {
         float getMaxX() const { return d_->max[0]; }
         float getMaxY() const { return d_->max[1]; }
         float getMaxZ() const { return d_->max[2]; }
+    };
+    typedef Description CDescription;
+};
+
+// include/flamegpu/runtime/messaging/MessageBucket/MessageBucketHost.h;
+// src/flamegpu/runtime/messaging/MessageBucket.cu:190-215 (bound validation)
+struct MessageBucket {
+    static constexpr MsgKind kind = MsgKind::Bucket;
+    class In;
+    class Out;
+    class Description : public MessageBruteForce::Description {
+     public:
+        explicit Description(std::shared_ptr<MessageData> d) : MessageBruteForce::Description(std::move(d)) {}
+        void setLowerBound(int32_t key) {
+            if (d_->upper_set && key >= d_->upper)
+                throw exception::InvalidArgument("Bucket messaging minimum bound must be lower than upper bound");
+            d_->lower = key;
+        }
+        void setUpperBound(int32_t key) {
+            if (key <= d_->lower)
+                throw exception::InvalidArgument("Bucket messaging upperBound bound must be greater than lower bound");
+            d_->upper = key;
+            d_->upper_set = true;
+        }
+        void setBounds(int32_t min, int32_t max) {
+            if (min >= max) throw exception::InvalidArgument("Bucket messaging minimum bound must be lower than upper bound");
+            d_->lower = min;
+            d_->upper = max;
+            d_->upper_set = true;
+        }
+        int32_t getLowerBound() const { return d_->lower; }
+        int32_t getUpperBound() const { return d_->upper; }
+    };
+    typedef Description CDescription;
+};
+
+// include/flamegpu/runtime/messaging/MessageArray/MessageArrayHost.h; MessageArray.cu setLength
+struct MessageArray {
+    static constexpr MsgKind kind = MsgKind::Array;
+    class In;
+    class Out;
+    class Description : public MessageBruteForce::Description {
+     public:
+        explicit Description(std::shared_ptr<MessageData> d) : MessageBruteForce::Description(std::move(d)) {}
+        void setLength(size_type len) {
+            if (len == 0) throw exception::InvalidArgument("Array messaging length must not be zero");
+            d_->dims[0] = len;
+        }
+        size_type getLength() const { return d_->dims[0]; }
+    };
+    typedef Description CDescription;
+};
+
+// include/flamegpu/runtime/messaging/MessageArray2D/MessageArray2DHost.h
+struct MessageArray2D {
+    static constexpr MsgKind kind = MsgKind::Array2D;
+    class In;
+    class Out;
+    class Description : public MessageBruteForce::Description {
+     public:
+        explicit Description(std::shared_ptr<MessageData> d) : MessageBruteForce::Description(std::move(d)) {}
+        void setDimensions(size_type x, size_type y) {
+            if (x == 0 || y == 0) throw exception::InvalidArgument("All dimensions must be above zero in array2D message");
+            d_->dims[0] = x;
+            d_->dims[1] = y;
+        }
+        void setDimensions(const std::array<size_type, 2>& d) { setDimensions(d[0], d[1]); }
+        std::array<size_type, 2> getDimensions() const { return {d_->dims[0], d_->dims[1]}; }
+        size_type getDimX() const { return d_->dims[0]; }
+        size_type getDimY() const { return d_->dims[1]; }
+    };
+    typedef Description CDescription;
+};
+
+// include/flamegpu/runtime/messaging/MessageArray3D/MessageArray3DHost.h
+struct MessageArray3D {
+    static constexpr MsgKind kind = MsgKind::Array3D;
+    class In;
+    class Out;
+    class Description : public MessageBruteForce::Description {
+     public:
+        explicit Description(std::shared_ptr<MessageData> d) : MessageBruteForce::Description(std::move(d)) {}
+        void setDimensions(size_type x, size_type y, size_type z) {
+            if (x == 0 || y == 0 || z == 0) throw exception::InvalidArgument("All dimensions must be above zero in array3D message");
+            d_->dims[0] = x;
+            d_->dims[1] = y;
+            d_->dims[2] = z;
+        }
+        void setDimensions(const std::array<size_type, 3>& d) { setDimensions(d[0], d[1], d[2]); }
+        std::array<size_type, 3> getDimensions() const { return {d_->dims[0], d_->dims[1], d_->dims[2]}; }
+        size_type getDimX() const { return d_->dims[0]; }
+        size_type getDimY() const { return d_->dims[1]; }
+        size_type getDimZ() const { return d_->dims[2]; }
     };
     typedef Description CDescription;
 };
@@ -649,6 +749,10 @@ class ModelDescription {
             d->variables.emplace("y", detail::make_variable<float>("y", 1, nullptr));
             if (M::kind == MsgKind::Spatial3D) d->variables.emplace("z", detail::make_variable<float>("z", 1, nullptr));
         }
+        // hidden variables: the bucket key (MessageBucket.cu Data()), the array element index
+        // (MessageArray.cu Data(): "___INDEX")
+        if (M::kind == MsgKind::Bucket) d->variables.emplace("_key", detail::make_variable<int32_t>("_key", 1, nullptr));
+        if (is_array_kind(M::kind)) d->variables.emplace("___INDEX", detail::make_variable<uint32_t>("___INDEX", 1, nullptr));
         m_->messages.emplace(name, d);
         return typename M::Description(d);
     }

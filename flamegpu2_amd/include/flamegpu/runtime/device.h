@@ -288,39 +288,42 @@ class MessageNone::Out {
 class MessageBruteForce::In {
  public:
     __device__ __forceinline__ explicit In(const detail::ThreadCtx& c) : c_(c), off_(c.msg_off), len_(c.msg_n) {}
+    class iterator;
     class Message {
      public:
-        __device__ __forceinline__ Message(const detail::ThreadCtx& c, uint32_t i) : c_(c), i_(i) {}
+        __device__ __forceinline__ Message(const detail::ThreadCtx& c, uint32_t i) : c_(&c), i_(i) {}
         template <typename T, unsigned N>
         __device__ __forceinline__ T getVariable(const char (&name)[N]) const {
-            return detail::load_var<T>(c_.a.msg_in, c_.a.msg_in_mul, name, i_, c_, detail::DERR_UNKNOWN_MSG_VAR,
+            return detail::load_var<T>(c_->a.msg_in, c_->a.msg_in_mul, name, i_, *c_, detail::DERR_UNKNOWN_MSG_VAR,
                                        detail::DERR_MSG_VAR_TYPE);
         }
         template <typename T, size_type L, unsigned N>
         __device__ __forceinline__ T getVariable(const char (&name)[N], unsigned int e) const {
-            return detail::load_elem<T>(c_.a.msg_in, c_.a.msg_in_mul, name, i_, L, e, c_, detail::DERR_UNKNOWN_MSG_VAR,
+            return detail::load_elem<T>(c_->a.msg_in, c_->a.msg_in_mul, name, i_, L, e, *c_, detail::DERR_UNKNOWN_MSG_VAR,
                                         detail::DERR_MSG_VAR_TYPE);
         }
-        __device__ __forceinline__ size_type getIndex() const { return i_ - c_.msg_off; }
+        __device__ __forceinline__ size_type getIndex() const { return i_ - c_->msg_off; }
 
      private:
-        const detail::ThreadCtx& c_;
+        friend class iterator;
+        const detail::ThreadCtx* c_;
         uint32_t i_;
     };
+    // operator* gives a Message& (MessageBruteForceDevice.cuh iterator), so loops may bind `auto&`
     class iterator {
      public:
-        __device__ __forceinline__ iterator(const detail::ThreadCtx& c, uint32_t i) : c_(c), i_(i) {}
+        __device__ __forceinline__ iterator(const detail::ThreadCtx& c, uint32_t i) : m_(c, i) {}
         __device__ __forceinline__ iterator& operator++() {
-            ++i_;
+            ++m_.i_;
             return *this;
         }
-        __device__ __forceinline__ bool operator!=(const iterator& o) const { return i_ != o.i_; }
-        __device__ __forceinline__ bool operator==(const iterator& o) const { return i_ == o.i_; }
-        __device__ __forceinline__ Message operator*() const { return Message(c_, i_); }
+        __device__ __forceinline__ bool operator!=(const iterator& o) const { return m_.i_ != o.m_.i_; }
+        __device__ __forceinline__ bool operator==(const iterator& o) const { return m_.i_ == o.m_.i_; }
+        __device__ __forceinline__ Message& operator*() { return m_; }
+        __device__ __forceinline__ Message* operator->() { return &m_; }
 
      private:
-        const detail::ThreadCtx& c_;
-        uint32_t i_;
+        Message m_;
     };
     __device__ __forceinline__ iterator begin() const { return iterator(c_, off_); }
     __device__ __forceinline__ iterator end() const { return iterator(c_, off_ + len_); }
@@ -463,6 +466,7 @@ class SpatialFilter {
         plan(m, cx, cy, cz, 0, 0, 0);
     }
 
+    class iterator;
     class Message {
      public:
         __device__ __forceinline__ Message(const SpatialFilter& f, uint32_t g, float x, float y, float z, uint32_t id)
@@ -503,18 +507,20 @@ class SpatialFilter {
         __device__ __forceinline__ size_type getIndex() const { return g_; }
 
      private:
+        friend class iterator;
         const SpatialFilter& f_;
         uint32_t g_;
         float x_, y_, z_;
         uint32_t id_;
     };
 
+    // operator* gives a Message& (MessageSpatial3DDevice.cuh iterator), so loops may bind `auto&`
     class iterator {
      public:
         // end sentinel
-        __device__ __forceinline__ explicit iterator(const SpatialFilter& f) : f_(f), valid_(false) {}
+        __device__ __forceinline__ explicit iterator(const SpatialFilter& f) : f_(f), m_(f, 0, 0.f, 0.f, 0.f, 0), valid_(false) {}
         // first message
-        __device__ __forceinline__ iterator(const SpatialFilter& f, int) : f_(f) {
+        __device__ __forceinline__ iterator(const SpatialFilter& f, int) : f_(f), m_(f, 0, 0.f, 0.f, 0.f, 0) {
             pu_ = 0;
             f_.first_unit(k_);
             f_.load_unit(k_, ps_, pe_, po_);
@@ -529,7 +535,8 @@ class SpatialFilter {
             return *this;
         }
         __device__ __forceinline__ bool operator!=(const iterator&) const { return valid_; }
-        __device__ __forceinline__ Message operator*() const { return Message(f_, g_, x_, y_, z_, id_); }
+        __device__ __forceinline__ Message& operator*() { return m_; }
+        __device__ __forceinline__ Message* operator->() { return &m_; }
 
      private:
         // Move the look-ahead to the first message of the next non-empty unit.
@@ -560,19 +567,16 @@ class SpatialFilter {
         __device__ __forceinline__ void fetch() { f_.fetch(more_ ? l_ : 0u, nx_, ny_, nz_, nid_); }
         __device__ __forceinline__ void shift() {
             valid_ = more_;
-            g_ = l_ + o_;
-            x_ = nx_;
-            y_ = ny_;
-            z_ = nz_;
-            id_ = nid_;
+            m_.g_ = l_ + o_;
+            m_.x_ = nx_;
+            m_.y_ = ny_;
+            m_.z_ = nz_;
+            m_.id_ = nid_;
         }
 
         const SpatialFilter& f_;
+        Message m_;   // current message
         bool valid_ = false, more_ = false;
-        // current message
-        uint32_t g_ = 0;
-        float x_ = 0.f, y_ = 0.f, z_ = 0.f;
-        uint32_t id_ = 0;
         // look-ahead: source index l_ in unit [.., e_), global index l_ + o_, its values
         uint32_t l_ = 0, e_ = 0, o_ = 0;
         float nx_ = 0.f, ny_ = 0.f, nz_ = 0.f;
@@ -795,6 +799,403 @@ class MessageSpatial3D::Out : public MessageBruteForce::Out {
         setVariable<float>("x", x);
         setVariable<float>("y", y);
         setVariable<float>("z", z);
+    }
+};
+
+// ------------------------------------------------------------------ bucket messages
+// include/flamegpu/runtime/messaging/MessageBucket/MessageBucketDevice.cuh:190-330.
+// Keys [lower, upper] both inclusive; the bucket table holds keys + 1 starts.  The
+// reference's Filter drops the last bucket (its bound test is endKey < max); here every
+// bucket in the declared range is readable.
+class MessageBucket::In {
+ public:
+    __device__ __forceinline__ explicit In(const detail::ThreadCtx& c) : c_(c) {}
+    class Filter {
+     public:
+        class iterator;
+        class Message {
+         public:
+            __device__ __forceinline__ Message(const detail::ThreadCtx& c, uint32_t i) : c_(&c), i_(i) {}
+            template <typename T, unsigned N>
+            __device__ __forceinline__ T getVariable(const char (&name)[N]) const {
+                return detail::load_var<T>(c_->a.msg_in, c_->a.msg_in_mul, name, i_, *c_, detail::DERR_UNKNOWN_MSG_VAR,
+                                           detail::DERR_MSG_VAR_TYPE);
+            }
+            template <typename T, size_type L, unsigned N>
+            __device__ __forceinline__ T getVariable(const char (&name)[N], unsigned int e) const {
+                return detail::load_elem<T>(c_->a.msg_in, c_->a.msg_in_mul, name, i_, L, e, *c_, detail::DERR_UNKNOWN_MSG_VAR,
+                                            detail::DERR_MSG_VAR_TYPE);
+            }
+
+         private:
+            friend class iterator;
+            const detail::ThreadCtx* c_;
+            uint32_t i_;
+        };
+        class iterator {
+         public:
+            __device__ __forceinline__ iterator(const detail::ThreadCtx& c, uint32_t i) : m_(c, i) {}
+            __device__ __forceinline__ iterator& operator++() {
+                ++m_.i_;
+                return *this;
+            }
+            __device__ __forceinline__ bool operator!=(const iterator& o) const { return m_.i_ != o.m_.i_; }
+            __device__ __forceinline__ bool operator==(const iterator& o) const { return m_.i_ == o.m_.i_; }
+            __device__ __forceinline__ Message& operator*() { return m_; }
+            __device__ __forceinline__ Message* operator->() { return &m_; }
+
+         private:
+            Message m_;
+        };
+        __device__ __forceinline__ Filter(const detail::ThreadCtx& c, uint32_t b, uint32_t e) : c_(c), b_(b), e_(e) {}
+        __device__ __forceinline__ iterator begin() const { return iterator(c_, b_); }
+        __device__ __forceinline__ iterator end() const { return iterator(c_, e_); }
+        __device__ __forceinline__ size_type size() const { return e_ - b_; }
+
+     private:
+        const detail::ThreadCtx& c_;
+        uint32_t b_, e_;
+    };
+    // message_in(key): the bucket of `key`
+    __device__ __forceinline__ Filter operator()(int32_t key) const {
+        const int64_t k = (int64_t)key - c_.meta.key_min;
+        const bool bad = k < 0 || k >= (int64_t)c_.meta.keys;
+        c_.err.note(bad, detail::DERR_BUCKET_KEY, (uint32_t)key);
+        if (bad) return Filter(c_, 0, 0);
+        return Filter(c_, c_.meta.pbm[k], c_.meta.pbm[k + 1]);
+    }
+    // message_in(begin, end): buckets [begin, end)
+    __device__ __forceinline__ Filter operator()(int32_t begin_key, int32_t end_key) const {
+        const int64_t b = (int64_t)begin_key - c_.meta.key_min, e = (int64_t)end_key - c_.meta.key_min;
+        const bool bad = b < 0 || e > (int64_t)c_.meta.keys || e <= b;
+        c_.err.note(bad, detail::DERR_BUCKET_KEY, (uint32_t)(bad && b >= 0 ? end_key : begin_key));
+        if (bad) return Filter(c_, 0, 0);
+        return Filter(c_, c_.meta.pbm[b], c_.meta.pbm[e]);
+    }
+
+ private:
+    const detail::ThreadCtx& c_;
+};
+
+class MessageBucket::Out : public MessageBruteForce::Out {
+ public:
+    __device__ __forceinline__ explicit Out(const detail::ThreadCtx& c) : MessageBruteForce::Out(c) {}
+    // MessageBucketDevice.cuh:303-318
+    __device__ __forceinline__ void setKey(int32_t key) const {
+        const int64_t k = (int64_t)key - c_.a.out_key_min;
+        const bool bad = k < 0 || k >= (int64_t)c_.a.out_keys;
+        c_.err.note(bad, detail::DERR_BUCKET_KEY, (uint32_t)key);
+        if (bad) return;
+        setVariable<int32_t>("_key", key);
+    }
+};
+
+// ------------------------------------------------------------------ array messages
+// include/flamegpu/runtime/messaging/MessageArray{,2D,3D}/MessageArray*Device.cuh.
+namespace detail {
+// A message of an array list at element i (i >= length: reads give zero).
+class ArrayMessage {
+ public:
+    __device__ __forceinline__ ArrayMessage(const ThreadCtx& c, uint32_t i) : c_(c), i_(i) {}
+    template <typename T, unsigned N>
+    __device__ __forceinline__ T getVariable(const char (&name)[N]) const {
+        const T v = load_var<T>(c_.a.msg_in, c_.a.msg_in_mul, name, i_ < len() ? i_ : 0u, c_, DERR_UNKNOWN_MSG_VAR,
+                                DERR_MSG_VAR_TYPE);
+        return i_ < len() ? v : T(0);
+    }
+    template <typename T, size_type L, unsigned N>
+    __device__ __forceinline__ T getVariable(const char (&name)[N], unsigned int e) const {
+        const T v = load_elem<T>(c_.a.msg_in, c_.a.msg_in_mul, name, i_ < len() ? i_ : 0u, L, e, c_, DERR_UNKNOWN_MSG_VAR,
+                                 DERR_MSG_VAR_TYPE);
+        return i_ < len() ? v : T(0);
+    }
+    __device__ __forceinline__ size_type getIndex() const { return i_; }
+    __device__ __forceinline__ size_type getX() const { return i_ % c_.meta.grid[0]; }
+    __device__ __forceinline__ size_type getY() const { return (i_ / c_.meta.grid[0]) % c_.meta.grid[1]; }
+    __device__ __forceinline__ size_type getZ() const { return i_ / (c_.meta.grid[0] * c_.meta.grid[1]); }
+
+ private:
+    __device__ __forceinline__ uint32_t len() const { return c_.meta.grid[0] * c_.meta.grid[1] * c_.meta.grid[2]; }
+    const ThreadCtx& c_;
+    uint32_t i_;
+};
+
+// The Moore (or von Neumann) neighbourhood of an array element, in the reference's
+// order: x offset outermost, then y, then z innermost, the origin skipped
+// (MessageArray2DDevice.cuh:1380-1448, MessageArray3DDevice.cuh:1536-1571).  Clamped
+// filters cover the offsets that stay inside the array; wrapping ones cover [-r, r]
+// on every axis with the element taken modulo the dimension.
+template <int D, bool WRAP, bool VN>
+class ArrayFilter {
+ public:
+    class Message {
+     public:
+        __device__ __forceinline__ Message(const ArrayFilter& f, int x, int y, int z) : f_(f) {
+            rel_[0] = x;
+            rel_[1] = y;
+            rel_[2] = z;
+        }
+        __device__ __forceinline__ bool operator==(const Message& o) const {
+            return rel_[0] == o.rel_[0] && rel_[1] == o.rel_[1] && rel_[2] == o.rel_[2];
+        }
+        __device__ __forceinline__ bool operator!=(const Message& o) const { return !(*this == o); }
+        __device__ __forceinline__ size_type getX() const { return coord(0); }
+        __device__ __forceinline__ size_type getY() const { return coord(1); }
+        __device__ __forceinline__ size_type getZ() const { return coord(2); }
+        __device__ __forceinline__ int getOffsetX() const { return rel_[0]; }
+        __device__ __forceinline__ int getOffsetY() const { return rel_[1]; }
+        __device__ __forceinline__ int getOffsetZ() const { return rel_[2]; }
+        template <typename T, unsigned N>
+        __device__ __forceinline__ T getVariable(const char (&name)[N]) const {
+            return load_var<T>(f_.c_.a.msg_in, f_.c_.a.msg_in_mul, name, index(), f_.c_, DERR_UNKNOWN_MSG_VAR,
+                               DERR_MSG_VAR_TYPE);
+        }
+        template <typename T, size_type L, unsigned N>
+        __device__ __forceinline__ T getVariable(const char (&name)[N], unsigned int e) const {
+            return load_elem<T>(f_.c_.a.msg_in, f_.c_.a.msg_in_mul, name, index(), L, e, f_.c_, DERR_UNKNOWN_MSG_VAR,
+                                DERR_MSG_VAR_TYPE);
+        }
+        __device__ __forceinline__ Message& operator++() {
+            do {
+                if (D == 1) {
+                    ++rel_[0];
+                } else if (D == 2) {
+                    if (rel_[1] >= f_.hi_[1]) {
+                        rel_[1] = f_.lo_[1];
+                        ++rel_[0];
+                    } else {
+                        ++rel_[1];
+                    }
+                } else {
+                    if (rel_[2] >= f_.hi_[2]) {
+                        rel_[2] = f_.lo_[2];
+                        if (rel_[1] >= f_.hi_[1]) {
+                            rel_[1] = f_.lo_[1];
+                            ++rel_[0];
+                        } else {
+                            ++rel_[1];
+                        }
+                    } else {
+                        ++rel_[2];
+                    }
+                }
+            } while (rel_[0] <= f_.hi_[0] && skip());
+            return *this;
+        }
+        __device__ __forceinline__ bool skip() const {
+            const bool origin = rel_[0] == 0 && rel_[1] == 0 && rel_[2] == 0;
+            const int l1 = abs(rel_[0]) + abs(rel_[1]) + abs(rel_[2]);
+            return origin || (VN && l1 > f_.r_);
+        }
+
+     private:
+        __device__ __forceinline__ uint32_t coord(int a) const {
+            int v = (int)f_.loc_[a] + rel_[a];
+            if (WRAP) {
+                const int g = (int)f_.c_.meta.grid[a];
+                v = v < 0 ? v + g : (v >= g ? v - g : v);
+            }
+            return (uint32_t)v;
+        }
+        __device__ __forceinline__ uint32_t index() const {
+            uint32_t i = coord(0);
+            if (D >= 2) i += coord(1) * f_.c_.meta.grid[0];
+            if (D == 3) i += coord(2) * f_.c_.meta.grid[0] * f_.c_.meta.grid[1];
+            return i;
+        }
+        const ArrayFilter& f_;
+        int rel_[3];
+    };
+    class iterator {
+     public:
+        __device__ __forceinline__ explicit iterator(const Message& m) : m_(m) {}
+        __device__ __forceinline__ iterator& operator++() {
+            ++m_;
+            return *this;
+        }
+        __device__ __forceinline__ bool operator==(const iterator& o) const { return m_ == o.m_; }
+        __device__ __forceinline__ bool operator!=(const iterator& o) const { return m_ != o.m_; }
+        __device__ __forceinline__ Message& operator*() { return m_; }
+        __device__ __forceinline__ Message* operator->() { return &m_; }
+
+     private:
+        Message m_;
+    };
+    // valid == false: an empty filter (the call's arguments were reported as an error)
+    __device__ __forceinline__ ArrayFilter(const ThreadCtx& c, const uint32_t* loc, uint32_t r, bool valid) : c_(c), r_((int)r) {
+        for (int a = 0; a < 3; ++a) {
+            loc_[a] = a < D ? loc[a] : 0u;
+            if (a >= D) {
+                lo_[a] = hi_[a] = 0;
+            } else if (WRAP) {
+                lo_[a] = -(int)r;
+                hi_[a] = (int)r;
+            } else {
+                const int x = (int)loc_[a], g = (int)c.meta.grid[a];
+                lo_[a] = x - (int)r < 0 ? -x : -(int)r;
+                hi_[a] = x + (int)r >= g ? g - 1 - x : (int)r;
+            }
+        }
+        if (!valid) {
+            lo_[0] = 1;
+            hi_[0] = 0;
+        }
+    }
+    __device__ __forceinline__ iterator begin() const {
+        Message m(*this, lo_[0], lo_[1], lo_[2]);
+        if (lo_[0] <= hi_[0] && m.skip()) ++m;
+        return iterator(m);
+    }
+    __device__ __forceinline__ iterator end() const { return iterator(Message(*this, hi_[0] + 1, lo_[1], lo_[2])); }
+
+ private:
+    const ThreadCtx& c_;
+    uint32_t loc_[3];
+    int lo_[3], hi_[3];
+    int r_;
+};
+
+// Argument checks of the iteration entry points (MessageArray2DDevice.cuh:977-1092):
+// radius 0, an origin outside the array, or (wrapping) a diameter above a dimension.
+template <int D>
+__device__ __forceinline__ bool array_args_ok(const ThreadCtx& c, const uint32_t* loc, uint32_t r, bool wrap) {
+    bool in = true, fits = true;
+    for (int a = 0; a < D; ++a) {
+        in = in && loc[a] < c.meta.grid[a];
+        fits = fits && 2u * r + 1u <= c.meta.grid[a];
+    }
+    c.err.note(!in, DERR_ARRAY_MSG_INDEX, loc[0]);
+    c.err.note(r == 0u || (wrap && !fits), DERR_ARRAY_RADIUS, r);
+    return in && r != 0u && (!wrap || fits);
+}
+}  // namespace detail
+
+class MessageArray::In {
+ public:
+    __device__ __forceinline__ explicit In(const detail::ThreadCtx& c) : c_(c) {}
+    typedef detail::ArrayMessage Message;
+    typedef detail::ArrayFilter<1, false, false> Filter;
+    typedef detail::ArrayFilter<1, true, false> WrapFilter;
+    __device__ __forceinline__ size_type size() const { return c_.meta.grid[0]; }
+    __device__ __forceinline__ Message at(size_type i) const {
+        c_.err.note(i >= c_.meta.grid[0], detail::DERR_ARRAY_MSG_INDEX, i);
+        return Message(c_, i >= c_.meta.grid[0] ? 0xFFFFFFFFu : i);
+    }
+    __device__ __forceinline__ Filter operator()(size_type x, size_type radius = 1) const {
+        const uint32_t loc[1] = {x};
+        const bool ok = detail::array_args_ok<1>(c_, loc, radius, false);
+        return Filter(c_, loc, radius, ok);
+    }
+    __device__ __forceinline__ WrapFilter wrap(size_type x, size_type radius = 1) const {
+        const uint32_t loc[1] = {x};
+        const bool ok = detail::array_args_ok<1>(c_, loc, radius, true);
+        return WrapFilter(c_, loc, radius, ok);
+    }
+
+ private:
+    const detail::ThreadCtx& c_;
+};
+
+class MessageArray2D::In {
+ public:
+    __device__ __forceinline__ explicit In(const detail::ThreadCtx& c) : c_(c) {}
+    typedef detail::ArrayMessage Message;
+    typedef detail::ArrayFilter<2, false, false> Filter;
+    typedef detail::ArrayFilter<2, true, false> WrapFilter;
+    typedef detail::ArrayFilter<2, false, true> VonNeumannFilter;
+    typedef detail::ArrayFilter<2, true, true> VonNeumannWrapFilter;
+    __device__ __forceinline__ size_type getDimX() const { return c_.meta.grid[0]; }
+    __device__ __forceinline__ size_type getDimY() const { return c_.meta.grid[1]; }
+    __device__ __forceinline__ size_type size() const { return c_.meta.grid[0] * c_.meta.grid[1]; }
+    __device__ __forceinline__ Message at(size_type x, size_type y) const {
+        const bool bad = x >= c_.meta.grid[0] || y >= c_.meta.grid[1];
+        c_.err.note(bad, detail::DERR_ARRAY_MSG_INDEX, x);
+        return Message(c_, bad ? 0xFFFFFFFFu : y * c_.meta.grid[0] + x);
+    }
+    __device__ __forceinline__ Filter operator()(size_type x, size_type y, size_type radius = 1) const { return make<Filter>(x, y, radius, false); }
+    __device__ __forceinline__ WrapFilter wrap(size_type x, size_type y, size_type radius = 1) const { return make<WrapFilter>(x, y, radius, true); }
+    __device__ __forceinline__ VonNeumannFilter vn(size_type x, size_type y, size_type radius = 1) const { return make<VonNeumannFilter>(x, y, radius, false); }
+    __device__ __forceinline__ VonNeumannWrapFilter vn_wrap(size_type x, size_type y, size_type radius = 1) const {
+        return make<VonNeumannWrapFilter>(x, y, radius, true);
+    }
+
+ private:
+    template <typename F>
+    __device__ __forceinline__ F make(size_type x, size_type y, size_type r, bool wrap) const {
+        const uint32_t loc[2] = {x, y};
+        const bool ok = detail::array_args_ok<2>(c_, loc, r, wrap);
+        return F(c_, loc, r, ok);
+    }
+    const detail::ThreadCtx& c_;
+};
+
+class MessageArray3D::In {
+ public:
+    __device__ __forceinline__ explicit In(const detail::ThreadCtx& c) : c_(c) {}
+    typedef detail::ArrayMessage Message;
+    typedef detail::ArrayFilter<3, false, false> Filter;
+    typedef detail::ArrayFilter<3, true, false> WrapFilter;
+    typedef detail::ArrayFilter<3, false, true> VonNeumannFilter;
+    typedef detail::ArrayFilter<3, true, true> VonNeumannWrapFilter;
+    __device__ __forceinline__ size_type getDimX() const { return c_.meta.grid[0]; }
+    __device__ __forceinline__ size_type getDimY() const { return c_.meta.grid[1]; }
+    __device__ __forceinline__ size_type getDimZ() const { return c_.meta.grid[2]; }
+    __device__ __forceinline__ size_type size() const { return c_.meta.grid[0] * c_.meta.grid[1] * c_.meta.grid[2]; }
+    __device__ __forceinline__ Message at(size_type x, size_type y, size_type z) const {
+        const bool bad = x >= c_.meta.grid[0] || y >= c_.meta.grid[1] || z >= c_.meta.grid[2];
+        c_.err.note(bad, detail::DERR_ARRAY_MSG_INDEX, x);
+        return Message(c_, bad ? 0xFFFFFFFFu : (z * c_.meta.grid[1] + y) * c_.meta.grid[0] + x);
+    }
+    __device__ __forceinline__ Filter operator()(size_type x, size_type y, size_type z, size_type radius = 1) const {
+        return make<Filter>(x, y, z, radius, false);
+    }
+    __device__ __forceinline__ WrapFilter wrap(size_type x, size_type y, size_type z, size_type radius = 1) const {
+        return make<WrapFilter>(x, y, z, radius, true);
+    }
+    __device__ __forceinline__ VonNeumannFilter vn(size_type x, size_type y, size_type z, size_type radius = 1) const {
+        return make<VonNeumannFilter>(x, y, z, radius, false);
+    }
+    __device__ __forceinline__ VonNeumannWrapFilter vn_wrap(size_type x, size_type y, size_type z, size_type radius = 1) const {
+        return make<VonNeumannWrapFilter>(x, y, z, radius, true);
+    }
+
+ private:
+    template <typename F>
+    __device__ __forceinline__ F make(size_type x, size_type y, size_type z, size_type r, bool wrap) const {
+        const uint32_t loc[3] = {x, y, z};
+        const bool ok = detail::array_args_ok<3>(c_, loc, r, wrap);
+        return F(c_, loc, r, ok);
+    }
+    const detail::ThreadCtx& c_;
+};
+
+// MessageArray*Device.cuh setIndex: the element this agent's message goes to; an index
+// outside the array is reported and the message is dropped.
+class MessageArray::Out : public MessageBruteForce::Out {
+ public:
+    __device__ __forceinline__ explicit Out(const detail::ThreadCtx& c) : MessageBruteForce::Out(c) {}
+    __device__ __forceinline__ void setIndex(size_type i) const {
+        const bool bad = i >= c_.a.out_dims[0];
+        c_.err.note(bad, detail::DERR_ARRAY_MSG_INDEX, i);
+        setVariable<uint32_t>("___INDEX", bad ? 0xFFFFFFFFu : i);
+    }
+};
+class MessageArray2D::Out : public MessageBruteForce::Out {
+ public:
+    __device__ __forceinline__ explicit Out(const detail::ThreadCtx& c) : MessageBruteForce::Out(c) {}
+    __device__ __forceinline__ void setIndex(size_type x, size_type y) const {
+        const bool bad = x >= c_.a.out_dims[0] || y >= c_.a.out_dims[1];
+        c_.err.note(bad, detail::DERR_ARRAY_MSG_INDEX, x);
+        setVariable<uint32_t>("___INDEX", bad ? 0xFFFFFFFFu : y * c_.a.out_dims[0] + x);
+    }
+};
+class MessageArray3D::Out : public MessageBruteForce::Out {
+ public:
+    __device__ __forceinline__ explicit Out(const detail::ThreadCtx& c) : MessageBruteForce::Out(c) {}
+    __device__ __forceinline__ void setIndex(size_type x, size_type y, size_type z) const {
+        const bool bad = x >= c_.a.out_dims[0] || y >= c_.a.out_dims[1] || z >= c_.a.out_dims[2];
+        c_.err.note(bad, detail::DERR_ARRAY_MSG_INDEX, x);
+        setVariable<uint32_t>("___INDEX", bad ? 0xFFFFFFFFu : (z * c_.a.out_dims[1] + y) * c_.a.out_dims[0] + x);
     }
 };
 

@@ -193,7 +193,13 @@ struct MessageRT {
     uint32_t* d_count = nullptr;
     fgx_spatial_meta meta{};
     bool spatial = false;
-    uint32_t* pbm = nullptr;
+    MsgKind kind = MsgKind::BruteForce;
+    int32_t key_min = 0;            // bucket: keys [key_min, key_min + keys)
+    uint32_t keys = 0;
+    uint32_t dims[3] = {1, 1, 1};   // array: dimensions; length = their product
+    uint32_t length = 0;
+    uint32_t* counts = nullptr;     // array: per-element write counts (build scratch)
+    uint32_t* pbm = nullptr;        // spatial bin table / bucket start table
     void* build_scratch = nullptr;
     size_t build_scratch_bytes = 0;
     bool truncate = true;
@@ -873,6 +879,30 @@ inline void CUDASimulation::ensure_init() {
             alloc((void**)&mr.pbm, (size_t)(mr.meta.bins + 1) * 4);
             detail::fgx_check(fgx_fill_u32(mr.pbm, 0, mr.meta.bins + 1, stream_), "PBM init");
         }
+        mr.kind = kv.second->kind;
+        if (mr.kind == MsgKind::Bucket) {
+            // src/flamegpu/runtime/messaging/MessageBucket.cu:38-46
+            if (!kv.second->upper_set)
+                throw exception::InvalidMessage("Bucket message '" + kv.first + "' needs an upper bound (setUpperBound / setBounds)");
+            mr.key_min = kv.second->lower;
+            const int64_t keys = (int64_t)kv.second->upper - kv.second->lower + 1;
+            if (keys <= 0 || keys > (1ll << 30)) throw exception::InvalidMessage("Bucket message '" + kv.first + "' key range is too large");
+            mr.keys = (uint32_t)keys;
+            alloc((void**)&mr.pbm, (size_t)(mr.keys + 1) * 4);
+            detail::fgx_check(fgx_fill_u32(mr.pbm, 0, mr.keys + 1, stream_), "bucket table init");
+        } else if (is_array_kind(mr.kind)) {
+            // src/flamegpu/runtime/messaging/MessageArray*.cu: Data() (zero length is an error)
+            uint64_t len = 1;
+            const int nd = mr.kind == MsgKind::Array ? 1 : (mr.kind == MsgKind::Array2D ? 2 : 3);
+            for (int a = 0; a < 3; ++a) {
+                mr.dims[a] = a < nd ? kv.second->dims[a] : 1u;
+                len *= mr.dims[a];
+            }
+            if (len == 0) throw exception::InvalidMessage("Length must not be zero in array message '" + kv.first + "'");
+            if (len > 0x7FFFFFFFull) throw exception::InvalidMessage("Array message '" + kv.first + "' is too large");
+            mr.length = (uint32_t)len;
+            alloc((void**)&mr.counts, (size_t)mr.length * 4);
+        }
         msgs_.emplace(kv.first, std::move(mr));
     }
     // functions, in layer order
@@ -946,6 +976,8 @@ inline void CUDASimulation::release() {
             free_dev(v.alt);
         }
         void* p = kv.second.pbm;
+        free_dev(p);
+        p = kv.second.counts;
         free_dev(p);
         free_dev(kv.second.build_scratch);
     }
@@ -1027,6 +1059,8 @@ inline void CUDASimulation::grow_message(detail::MessageRT& m, uint32_t cap) {
     free_dev(m.build_scratch);
     if (m.spatial) {
         m.build_scratch = new_scratch(FGX_OP_SPATIAL_BUILD, cap, m.meta.bins, &m.build_scratch_bytes);
+    } else if (m.kind == MsgKind::Bucket) {
+        m.build_scratch = new_scratch(FGX_OP_SPATIAL_BUILD, cap, m.keys, &m.build_scratch_bytes);
     }
     m.cap = cap;
 }
@@ -1148,6 +1182,7 @@ inline bool CUDASimulation::ensure_capacity() {
     for (auto& kv : msgs_) {
         auto& m = kv.second;
         uint64_t want = mub[&m] + (m.d->persistent ? m.cap : 0);
+        want = std::max<uint64_t>(want, m.length);   // an array list holds every element
         if (uint32_t c = need(std::max<uint64_t>(want, 1), m.cap)) {
             grow_message(m, c);
             changed = true;
@@ -1384,6 +1419,10 @@ inline void CUDASimulation::check_device_error() {
     if (e[0] != detail::DERR_NONE) {
         detail::fgx_check(fgx_fill_u32(d_error_, 0, 3, stream_), "error reset");
         synchronize();
+        // src/flamegpu/simulation/detail/CUDAScatter.cu arrayMessageReorder raises this type
+        if (e[0] == detail::DERR_ARRAY_CONFLICT)
+            throw exception::ArrayMessageWriteConflict(detail::format("Device error %u (%s), array index %u", e[0],
+                                                                      detail::device_error_text(e[0]), e[1]));
         throw exception::DeviceError(detail::format("Device error %u (%s), name hash 0x%08x, agent index %u", e[0],
                                                     detail::device_error_text(e[0]), e[1], e[2]));
     }
@@ -1478,9 +1517,18 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
             a.in_meta.wrap_ok = m.wrap_ok;
             a.in_meta.pbm = f.min->pbm;
             stage_neighbourhood(f, l, a);
+        } else if (f.min->kind == MsgKind::Bucket) {
+            a.in_meta.key_min = f.min->key_min;
+            a.in_meta.keys = f.min->keys;
+            a.in_meta.pbm = f.min->pbm;
+        } else if (is_array_kind(f.min->kind)) {
+            for (int k = 0; k < 3; ++k) a.in_meta.grid[k] = f.min->dims[k];
         }
     }
     if (f.mout) {
+        for (int k = 0; k < 3; ++k) a.out_dims[k] = f.mout->dims[k];
+        a.out_key_min = f.mout->key_min;
+        a.out_keys = f.mout->keys;
         fill_table(a.msg_out, &a.msg_out_mul, f.mout->vars, true);
         a.msg_out_flags = f.msg_flags;
         a.msg_out_optional = f.d->message_output_optional ? 1 : 0;
@@ -1745,6 +1793,35 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
                                                 (uint32_t)m.vars.size(), src.data(), dst.data(), by.data(), m.pbm,
                                                 m.build_scratch, m.build_scratch_bytes, stream_),
                               "message index build");
+            tick("build:" + m.d->name, stream_, false);
+            for (auto& v : m.vars) std::swap(v.cur, v.alt);
+            m.pbm_dirty = false;
+        } else if (f.min && (f.min->kind == MsgKind::Bucket || is_array_kind(f.min->kind)) && f.min->pbm_dirty) {
+            auto& m = *f.min;
+            std::vector<const void*> src;
+            std::vector<void*> dst;
+            std::vector<uint32_t> by;
+            for (auto& v : m.vars) {
+                src.push_back(v.cur);
+                dst.push_back(v.alt);
+                by.push_back(v.bytes);
+            }
+            tick("build:" + m.d->name, stream_, true);
+            if (m.kind == MsgKind::Bucket) {
+                const int ik = m.var_index("_key");
+                detail::fgx_check(fgx_bucket_build(ctx_, m.key_min, m.keys, m.d_count, m.cap, (const int32_t*)m.vars[ik].cur,
+                                                   (uint32_t)m.vars.size(), src.data(), dst.data(), by.data(), m.pbm,
+                                                   m.build_scratch, m.build_scratch_bytes, stream_),
+                                  "bucket message index build");
+            } else {
+                const int ii = m.var_index("___INDEX");
+                detail::fgx_check(fgx_array_build(ctx_, m.length, m.d_count, m.cap, (const uint32_t*)m.vars[ii].cur,
+                                                  (uint32_t)m.vars.size(), src.data(), dst.data(), by.data(), m.counts,
+                                                  d_error_, stream_),
+                                  "array message build");
+                // the list now holds every element (MessageArray.cu buildIndex: count = length)
+                detail::fgx_check(fgx_fill_u32(m.d_count, m.length, 1, stream_), "array message count");
+            }
             tick("build:" + m.d->name, stream_, false);
             for (auto& v : m.vars) std::swap(v.cur, v.alt);
             m.pbm_dirty = false;

@@ -131,6 +131,28 @@ int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t
                       uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
                       uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream);
 
+/* Bucket message index (src/flamegpu/runtime/messaging/MessageBucket.cu:58-150,
+ * MessageBucket::CUDAModelHandler::buildIndex): a stable sort of the messages by integer
+ * key into `keys` buckets [key_min, key_min + keys), with pbm[0..keys] the bucket start
+ * table (pbm[keys] = n).  Keys outside the range are clamped (the device API reports
+ * them when they are set).  Within a bucket messages keep their output order, where
+ * the reference's order is that of atomics.  Scratch: fgx_scratch_bytes(FGX_OP_SPATIAL_BUILD,
+ * n_cap, keys). */
+int fgx_bucket_build(fgx_ctx* ctx, int32_t key_min, uint32_t keys, const uint32_t* d_n, uint32_t n_cap, const int32_t* key,
+                     uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                     uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream);
+
+/* Array message list (src/flamegpu/runtime/messaging/MessageArray.cu buildIndex,
+ * src/flamegpu/simulation/detail/CUDAScatter.cu:540-640 arrayMessageReorder): message i
+ * is copied to element index[i] of the output arrays (length elements, every element
+ * nobody wrote is zero).  Two messages to one element, or more messages than elements,
+ * set the error word d_error[0..2] to {18, element or count, 0xFFFFFFFF}
+ * (exception::ArrayMessageWriteConflict at the next host check).  counts: `length`
+ * words of caller scratch. */
+int fgx_array_build(fgx_ctx* ctx, uint32_t length, const uint32_t* d_n, uint32_t n_cap, const uint32_t* index,
+                    uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                    uint32_t* counts, uint32_t* d_error, void* stream);
+
 /* ---------------------------------------------------------------- auto spatial agent sort
  * Replaces CUDASimulation::spatialSortAgent_async (src/flamegpu/simulation/CUDASimulation.cu:463-573):
  * key = floor(((p-min)/width)*grid) unclamped, masked to floor(log2(bins))+1 bits
