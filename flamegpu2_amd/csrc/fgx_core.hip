@@ -66,6 +66,20 @@ __global__ void k_add_u32(uint32_t* p, uint32_t delta) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p += delta;
 }
 
+struct SetCounts {
+    uint32_t n;
+    uint32_t* dst[FGX_SET_COUNTS_MAX];
+    uint32_t value[FGX_SET_COUNTS_MAX];
+    uint32_t* inc;
+};
+
+__global__ void k_set_counts(SetCounts a) {
+    const uint32_t t = threadIdx.x;
+    if (blockIdx.x != 0) return;
+    if (t < a.n) *a.dst[t] = a.value[t];
+    if (t == 63 && a.inc) *a.inc += 1u;
+}
+
 __global__ void k_sub_u32(uint32_t* dst, const uint32_t* a, const uint32_t* b) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const uint32_t x = *a, y = *b;
@@ -196,6 +210,24 @@ int fgx_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* stream) {
     if (!n) return FGX_OK;
     const uint32_t blocks = std::min<uint32_t>(ceil_div(n, 256), 4096);
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, (hipStream_t)stream, dst, value, n);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+int fgx_set_counts(uint32_t n, uint32_t* const* dst, const uint32_t* values, uint32_t* inc, void* stream) {
+    if (n > FGX_SET_COUNTS_MAX) return fail(FGX_ERR_ARG, "fgx_set_counts: too many counters");
+    if (n && (!dst || !values)) return fail(FGX_ERR_ARG, "fgx_set_counts: NULL arrays");
+    SetCounts a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = n;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (!dst[k]) return fail(FGX_ERR_ARG, "fgx_set_counts: NULL counter");
+        a.dst[k] = dst[k];
+        a.value[k] = values[k];
+    }
+    a.inc = inc;
+    if (!n && !inc) return FGX_OK;
+    hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, (hipStream_t)stream, a);
     FGX_HIP(hipGetLastError());
     return FGX_OK;
 }

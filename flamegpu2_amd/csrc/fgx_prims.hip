@@ -258,6 +258,79 @@ __global__ void __launch_bounds__(kBlock) k_scan_excl(const uint32_t* __restrict
     retire_tile(hdr, num_tiles);
 }
 
+// ================================================================ two-pass exclusive scan
+// For up to kTwoPassTiles tiles: per-tile sums, then every tile adds the sums of the
+// tiles before it.  The kernel boundary orders the two passes, where the single-pass
+// scan's look-back waits on status words written by other CUs.
+constexpr uint32_t kTwoPassTiles = 1024;
+
+__global__ void __launch_bounds__(kBlock) k_tile_sums(const uint32_t* __restrict__ in, uint32_t n,
+                                                      uint32_t* __restrict__ tile_sum) {
+    __shared__ uint32_t s_wave[kBlock / 64];
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kItems;
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) sum += (base + j < n) ? in[base + j] : 0u;
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t tot = wave_sum_u32(sum);
+    if (lane_id() == 0) s_wave[w] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t k = 0; k < kBlock / 64; ++k) t += s_wave[k];
+        tile_sum[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                       uint32_t n, const uint32_t* __restrict__ tile_sum) {
+    __shared__ uint32_t s_wave[kBlock / 64];
+    __shared__ uint32_t s_prefix;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    if (w == 0) {
+        uint32_t p = 0;
+        for (uint32_t k = lane; k < tile; k += 64) p += tile_sum[k];
+        p = wave_sum_u32(p);
+        if (lane == 0) s_prefix = p;
+    }
+    const uint32_t base = tile * kTile + threadIdx.x * kItems;
+    uint32_t v[kItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        v[j] = (base + j < n) ? in[base + j] : 0u;
+        sum += v[j];
+    }
+    const uint32_t incl = wave_inclusive_u32(sum);
+    if (lane == 63) s_wave[w] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0;
+    for (uint32_t k = 0; k < w; ++k) wave_off += s_wave[k];
+    uint32_t run = s_prefix + wave_off + incl - sum;
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+}
+
+// Exclusive scan of `n` counts into `out` (two-pass when the tiles fit, else single-pass).
+static int scan_counts(const uint32_t* in, uint32_t* out, uint32_t n, LookbackHeader* hdr, uint64_t* status, hipStream_t st) {
+    const uint32_t tiles = ceil_div(n, kTile);
+    if (tiles <= kTwoPassTiles) {
+        uint32_t* ts = reinterpret_cast<uint32_t*>(status);   // the status words are free in this mode
+        hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kBlock), 0, st, in, n, ts);
+        FGX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kBlock), 0, st, in, out, n, ts);
+        FGX_HIP(hipGetLastError());
+        return FGX_OK;
+    }
+    hipLaunchKernelGGL(k_scan_excl, dim3(tiles), dim3(kBlock), 0, st, in, out, n, hdr, status, tiles);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
 // ================================================================ compaction / births
 // Striped arrangement: item (j, wave, lane) = tile*2048 + j*256 + wave*64 + lane, so loads
 // and the compacted stores of a wave are contiguous.  Index order == (j, wave, lane) order.
@@ -618,9 +691,8 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
     {
         auto* hdr = reinterpret_cast<LookbackHeader*>(s + L.hdr);
         auto* status = reinterpret_cast<uint64_t*>(s + L.status);
-        const uint32_t tiles = ceil_div(bins + 1, kTile);
-        hipLaunchKernelGGL(k_scan_excl, dim3(tiles), dim3(kBlock), 0, st, hist, start_out, bins + 1, hdr, status, tiles);
-        FGX_HIP(hipGetLastError());
+        rc = scan_counts(hist, start_out, bins + 1, hdr, status, st);
+        if (rc) return rc;
     }
     if (n_cap) {
         hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, sub, start_out, tmp, unsorted);
@@ -654,10 +726,8 @@ static int run_bin_sort_seg(fgx_ctx* ctx, int32_t mode, const fgx_seg_key* d_seg
     {
         auto* hdr = reinterpret_cast<LookbackHeader*>(s + L.hdr);
         auto* status = reinterpret_cast<uint64_t*>(s + L.status);
-        const uint32_t st_tiles = ceil_div(keys_total + 1, kTile);
-        hipLaunchKernelGGL(k_scan_excl, dim3(st_tiles), dim3(kBlock), 0, st, hist, start_out, keys_total + 1, hdr, status,
-                           st_tiles);
-        FGX_HIP(hipGetLastError());
+        rc = scan_counts(hist, start_out, keys_total + 1, hdr, status, st);
+        if (rc) return rc;
     }
     if (n_total) {
         const uint32_t g = stream_grid(ctx ? &ctx->c : nullptr, n_total);

@@ -243,6 +243,7 @@ struct FnArgs {
     const uint32_t* ids;         // the agent list's "_id" array
     const uint32_t* d_first;     // function condition: agents [*d_first, n) run (NULL: all)
     int32_t msg_out_optional;
+    uint32_t* d_msg_count_out;   // mandatory output: the launch writes the message count (agent count) here
     uint32_t agent_mul, msg_in_mul, msg_out_mul, new_mul, env_mul;
     // neighbourhood staging (spatial message input)
     int32_t stage_ok;                 // agent x/y(/z) present and staged vars chosen

@@ -1342,6 +1342,7 @@ __global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
     const uint32_t first = a.d_first ? load_count(a.d_first) : 0u;
     const uint32_t n = a.d_n ? load_count(a.d_n) : a.n_cap;
     const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.d_msg_count_out && o == 0) *a.d_msg_count_out = n;   // one message per agent (no condition here)
     if (first + o >= n) return;
     const uint32_t i = first + o;
     uint32_t mn = 0u;
@@ -1539,6 +1540,7 @@ __global__ void __launch_bounds__(256) agent_function_kernel_staged(const FnArgs
     __shared__ StageHeader sh;
     const uint32_t n = a.d_n ? load_count(a.d_n) : a.n_cap;
     const uint32_t b0 = blockIdx.x * blockDim.x;
+    if (a.d_msg_count_out && b0 == 0 && threadIdx.x == 0) *a.d_msg_count_out = n;
     if (b0 >= n) return;
     const uint32_t i = b0 + threadIdx.x;
     const bool active = i < n;

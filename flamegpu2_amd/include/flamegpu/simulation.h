@@ -225,6 +225,8 @@ struct FnRT {
     uint32_t* birth_flags = nullptr;
     uint32_t* msg_flags = nullptr;
     uint32_t* d_launch_count = nullptr;
+    const uint32_t* launch_count = nullptr;   // count the last launch ran over: d_launch_count or the list's own count
+    bool msg_count_in_kernel = false;        // the last launch wrote the message count itself
     std::vector<VarBuf> newbuf;     // output agent variables (cur only)
     void* death_scratch = nullptr;
     size_t death_scratch_bytes = 0;
@@ -1561,13 +1563,24 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
             ++i;
         }
     }
+    // The count the launch runs over, for the post-processing in unmap_function: a
+    // function that neither removes, adds nor moves agents leaves its list's count as
+    // it is, so that count is read directly instead of being copied.
+    const bool keeps_count = !f.d->has_agent_death && !f.out_agent && f.end_state == f.in_state;
     if (f.d->condition) {
         a.d_first = f.d_first;
         a.stage_ok = 0;
         detail::fgx_check(fgx_sub_u32(f.d_launch_count, l.d_count, f.d_first, st), "launch count");
+        f.launch_count = f.d_launch_count;
+    } else if (keeps_count) {
+        f.launch_count = l.d_count;
     } else {
         detail::fgx_check(fgx_memcpy_d2d(f.d_launch_count, l.d_count, 4, st), "launch count");
+        f.launch_count = f.d_launch_count;
     }
+    // mandatory output into an emptied list: the kernel sets the message count
+    f.msg_count_in_kernel = f.mout && f.mout->truncate && !f.d->message_output_optional && !f.d->condition;
+    a.d_msg_count_out = f.msg_count_in_kernel ? f.mout->d_count : nullptr;
     const uint32_t block = 256;
     const uint32_t grid = std::max<uint32_t>(1, (l.cap + block - 1) / block);
     const std::string label = "fn:" + f.agent->d->name + "::" + f.d->name;
@@ -1649,7 +1662,8 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
         tick("msgout:" + f.d->name, st, true);
         if (m.truncate && !f.d->message_output_optional) {
             for (auto& v : m.vars) std::swap(v.cur, v.alt);
-            detail::fgx_check(fgx_memcpy_d2d(m.d_count, f.d_launch_count, 4, st), "message count");
+            if (!f.msg_count_in_kernel)
+                detail::fgx_check(fgx_memcpy_d2d(m.d_count, f.launch_count, 4, st), "message count");
         } else {
             if (m.truncate) detail::fgx_check(fgx_fill_u32(m.d_count, 0, 1, st), "message truncate");
             std::vector<const void*> src;
@@ -1660,7 +1674,7 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
                 dst.push_back(v.cur);
                 by.push_back(v.bytes);
             }
-            detail::fgx_check(fgx_append_births(ctx_, f.msg_flags, f.d_launch_count, l.cap, (uint32_t)m.vars.size(),
+            detail::fgx_check(fgx_append_births(ctx_, f.msg_flags, f.launch_count, l.cap, (uint32_t)m.vars.size(),
                                                 src.data(), dst.data(), by.data(), -1, m.d_count, m.cap, d_dummy_,
                                                 nullptr, f.append_scratch, f.append_scratch_bytes, st),
                               "message append");
@@ -1732,7 +1746,7 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
             by.push_back(dl.vars[i].bytes);
         }
         tick("births:" + f.d->name, st, true);
-        detail::fgx_check(fgx_append_births(ctx_, f.birth_flags, f.d_launch_count, f.cap, (uint32_t)dl.vars.size(), src.data(),
+        detail::fgx_check(fgx_append_births(ctx_, f.birth_flags, f.launch_count, f.cap, (uint32_t)dl.vars.size(), src.data(),
                                             dst.data(), by.data(), dl.id_slot, dl.d_count, dl.cap, f.out_agent->d_next_id,
                                             d_error_ + 2, f.births_scratch, f.births_scratch_bytes, st),
                           "agent births");
@@ -1885,15 +1899,25 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
 
 inline void CUDASimulation::enqueue_end_of_step() {
     // CUDASimulation.cu:619-624: non-persistent message lists are emptied
+    std::vector<uint32_t*> zero;
     for (auto& kv : msgs_) {
         auto& m = kv.second;
         if (!m.d->persistent) {
-            detail::fgx_check(fgx_fill_u32(m.d_count, 0, 1, stream_), "message reset");
+            zero.push_back(m.d_count);
             m.truncate = true;
             m.pbm_dirty = true;
         }
     }
-    detail::fgx_check(fgx_add_u32(d_step_, 1, stream_), "step counter");
+    // one launch: message counts to zero and the step counter incremented
+    size_t k = 0;
+    do {
+        const uint32_t nz = (uint32_t)std::min<size_t>(zero.size() - k, FGX_SET_COUNTS_MAX);
+        const std::vector<uint32_t> vals(nz, 0u);
+        const bool last = k + nz >= zero.size();
+        detail::fgx_check(fgx_set_counts(nz, nz ? zero.data() + k : nullptr, vals.data(), last ? d_step_ : nullptr, stream_),
+                          "end of step counters");
+        k += nz;
+    } while (k < zero.size());
 }
 
 inline void CUDASimulation::enqueue_step() {

@@ -65,6 +65,12 @@ int fgx_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 int fgx_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* stream);
 /* *p += delta on the device (step counters inside captured graphs). */
 int fgx_add_u32(uint32_t* p, uint32_t delta, void* stream);
+/* Step bookkeeping in one launch (the end of CUDASimulation::step,
+ * src/flamegpu/simulation/CUDASimulation.cu:619-630: message counts reset, step counter
+ * incremented): *dst[k] = values[k] for k < n, then *inc += 1 when inc != NULL.
+ * n <= FGX_SET_COUNTS_MAX; the pointers are device memory. */
+#define FGX_SET_COUNTS_MAX 32
+int fgx_set_counts(uint32_t n, uint32_t* const* dst, const uint32_t* values, uint32_t* inc, void* stream);
 /* Fill `nvars` variable arrays of n elements with per-variable default values
  * (element bytes var_bytes[v], default value bytes at defaults + def_off[v]);
  * replaces CUDAScatter::broadcastInit (src/flamegpu/simulation/detail/CUDAScatter.cu:404-422). */
