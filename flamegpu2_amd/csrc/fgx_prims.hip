@@ -83,6 +83,9 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist(KeyArgs ka, const uint32_t*
                                                      const float* __restrict__ z, uint32_t* __restrict__ key,
                                                      uint32_t* __restrict__ sub, uint32_t* __restrict__ hist,
                                                      uint32_t* __restrict__ unsorted) {
+    __shared__ uint32_t s_down;
+    if (threadIdx.x == 0) s_down = 0u;
+    __syncthreads();
     const uint32_t n = load_count(d_n, n_cap);
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -97,7 +100,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist(KeyArgs ka, const uint32_t*
             uint32_t before = prev;
             if (lane == 0 && base > 0) before = bin_key(ka, x[base - 1], y[base - 1], ka.dims == 3 ? z[base - 1] : 0.0f);
             const bool down = valid && (lane > 0 || base > 0) && k < before;
-            if (__ballot(down) != 0ull && lane == 0) atomicOr(unsorted, 1u);
+            if (__ballot(down) != 0ull && lane == 0) s_down = 1u;
         }
         const bool head = valid && (lane == 0 || k != prev);
         const uint64_t hm = __ballot(head);
@@ -115,6 +118,10 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist(KeyArgs ka, const uint32_t*
             sub[i] = b + (lane - hl);
         }
     }
+    // one flag update per block, skipped once another block has set it
+    __syncthreads();
+    if (threadIdx.x == 0 && s_down && !__hip_atomic_load(unsorted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicOr(unsorted, 1u);
 }
 
 // Segmented variant: block t is tile t of the packed batch; its simulation s (last
@@ -125,6 +132,9 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist_seg(int32_t mode, const fgx
                                                          const float* __restrict__ y, const float* __restrict__ z,
                                                          uint32_t* __restrict__ key, uint32_t* __restrict__ sub,
                                                          uint32_t* __restrict__ hist, uint32_t* __restrict__ unsorted) {
+    __shared__ uint32_t s_down;
+    if (threadIdx.x == 0) s_down = 0u;
+    __syncthreads();
     const uint32_t t = blockIdx.x;
     uint32_t lo = 0, hi = nsegs;
     while (hi - lo > 1u) {
@@ -162,7 +172,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist_seg(int32_t mode, const fgx
             uint32_t before = prev;
             if (lane == 0 && has_pred) before = koff + bin_key(ka, x[c0 - 1], y[c0 - 1], ka.dims == 3 ? z[c0 - 1] : 0.0f);
             const bool down = valid && (lane > 0 || has_pred) && k < before;
-            if (__ballot(down) != 0ull && lane == 0) atomicOr(unsorted, 1u);
+            if (__ballot(down) != 0ull && lane == 0) s_down = 1u;
         }
         const bool head = valid && (lane == 0 || k != prev);
         const uint64_t hm = __ballot(head);
@@ -180,6 +190,9 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist_seg(int32_t mode, const fgx
             sub[i] = b + (lane - hl);
         }
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_down && !__hip_atomic_load(unsorted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicOr(unsorted, 1u);
 }
 
 __global__ void __launch_bounds__(kBlock) k_index_scatter(const uint32_t* d_n, uint32_t n_cap,

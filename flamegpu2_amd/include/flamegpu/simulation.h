@@ -764,6 +764,7 @@ class CUDASimulation {
     uint32_t* d_error_ = nullptr;
     uint32_t* d_step_ = nullptr;
     uint32_t* d_dummy_ = nullptr;
+    uint32_t* d_overflow_ = nullptr;   // set when an append would pass a list's capacity (an internal bound error)
     uint32_t small_used_ = 8;
     uint32_t* h_small_ = nullptr;    // pinned mirror
     void* d_reduce_out_ = nullptr;
@@ -829,6 +830,7 @@ inline void CUDASimulation::ensure_init() {
     d_error_ = d_small_;
     d_step_ = d_small_ + 3;
     d_dummy_ = d_small_ + 4;
+    d_overflow_ = d_small_ + 5;
     alloc(&d_reduce_out_, 64);
     alloc(&d_pages_, 256);
     detail::fgx_check(fgx_fill_u32((uint32_t*)d_pages_, 0, 64, stream_), "zero page");
@@ -1135,6 +1137,8 @@ inline void CUDASimulation::sync_counts() {
     }
     (void)k;
     (void)order;
+    if (h_small_[5] != 0)
+        throw exception::InvalidOperation("an agent list append passed its capacity: the step's population bound was too low");
     if (h_small_[0] != 0) check_device_error();
 }
 
@@ -1164,7 +1168,8 @@ inline bool CUDASimulation::ensure_capacity() {
                 auto* dst = &f.agent->states.at(f.end_state);
                 ub[dst] += n;
                 peak[dst] = std::max(peak[dst], ub[dst]);
-                ub[in] = 0;
+                // with a function condition the disabled agents stay behind
+                if (!f.d->condition) ub[in] = 0;
             }
         }
     }
@@ -1676,7 +1681,7 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
             }
             detail::fgx_check(fgx_append_births(ctx_, f.msg_flags, f.launch_count, l.cap, (uint32_t)m.vars.size(),
                                                 src.data(), dst.data(), by.data(), -1, m.d_count, m.cap, d_dummy_,
-                                                nullptr, f.append_scratch, f.append_scratch_bytes, st),
+                                                d_overflow_, f.append_scratch, f.append_scratch_bytes, st),
                               "message append");
         }
         tick("msgout:" + f.d->name, st, false);
@@ -1722,13 +1727,13 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
         if (f.d->condition) {
             // only the agents that ran move; the disabled ones stay in the start state
             detail::fgx_check(fgx_append_range(ctx_, f.d_first, l.d_count, l.cap, (uint32_t)l.vars.size(), src.data(),
-                                               dst.data(), by.data(), dl.d_count, dl.cap, nullptr, f.trans_scratch,
+                                               dst.data(), by.data(), dl.d_count, dl.cap, d_overflow_, f.trans_scratch,
                                                f.trans_scratch_bytes, st),
                               "state transition");
             detail::fgx_check(fgx_memcpy_d2d(l.d_count, f.d_first, 4, st), "state transition count");
         } else {
             detail::fgx_check(fgx_append_births(ctx_, nullptr, l.d_count, l.cap, (uint32_t)l.vars.size(), src.data(),
-                                                dst.data(), by.data(), -1, dl.d_count, dl.cap, d_dummy_, nullptr,
+                                                dst.data(), by.data(), -1, dl.d_count, dl.cap, d_dummy_, d_overflow_,
                                                 f.trans_scratch, f.trans_scratch_bytes, st),
                               "state transition");
             detail::fgx_check(fgx_fill_u32(l.d_count, 0, 1, st), "state transition count");
@@ -1748,7 +1753,7 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
         tick("births:" + f.d->name, st, true);
         detail::fgx_check(fgx_append_births(ctx_, f.birth_flags, f.launch_count, f.cap, (uint32_t)dl.vars.size(), src.data(),
                                             dst.data(), by.data(), dl.id_slot, dl.d_count, dl.cap, f.out_agent->d_next_id,
-                                            d_error_ + 2, f.births_scratch, f.births_scratch_bytes, st),
+                                            d_overflow_, f.births_scratch, f.births_scratch_bytes, st),
                           "agent births");
         tick("births:" + f.d->name, st, false);
     }
