@@ -334,6 +334,26 @@ struct RunLog {
 };
 
 namespace detail {
+template <typename T, typename Acc>
+inline void decode_stats(const char* w, unsigned int n, LoggingConfig::Reduction r, LogValue& out) {
+    Acc sum, mn, mx;
+    double sq;
+    std::memcpy(&sum, w, 8);
+    std::memcpy(&mn, w + 8, 8);
+    std::memcpy(&mx, w + 16, 8);
+    std::memcpy(&sq, w + 24, 8);
+    switch (r) {
+        case LoggingConfig::Sum: out = LogValue::of<typename sum_input_t<T>::result_t>((typename sum_input_t<T>::result_t)sum); break;
+        case LoggingConfig::Min: out = LogValue::of<T>((T)mn); break;
+        case LoggingConfig::Max: out = LogValue::of<T>((T)mx); break;
+        // include/flamegpu/simulation/AgentLoggingConfig.h: mean 0 and standard deviation 0 for an empty list
+        case LoggingConfig::Mean: out = LogValue::of<double>(n ? (double)sum / (double)n : 0.0); break;
+        case LoggingConfig::StandardDev: out = LogValue::of<double>(n ? std::sqrt(sq / (double)n) : 0.0); break;
+    }
+}
+}  // namespace detail
+
+namespace detail {
 // JSON text of a run log, in the layout of src/flamegpu/io/JSONLogger.cu (config,
 // performance_specs, steps[], exit; per frame step_index, env, agents[name][state][]
 // with count and variables[name][reduction]).  Differs in one place: "count" is written

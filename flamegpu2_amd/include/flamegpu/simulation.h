@@ -265,12 +265,18 @@ class HostAgentAPI {
     T min(const std::string& variable);
     template <typename T>
     T max(const std::string& variable);
+    // include/flamegpu/runtime/agent/HostAgentAPI.cuh: meanStandardDeviation (population standard deviation)
+    template <typename T>
+    std::pair<double, double> meanStandardDeviation(const std::string& variable);
     template <typename T>
     std::vector<T> getVariableValues(const std::string& variable);
 
  private:
     template <typename T>
     const detail::VarBuf& var(const std::string& name);
+    // sum, min, max and squared deviation of a variable on the device (fgx_reduce_stats)
+    template <typename T>
+    void stats(const std::string& name, char out[32]);
     CUDASimulation& s_;
     detail::AgentRT& a_;
     detail::StateList& l_;
@@ -796,6 +802,7 @@ class CUDASimulation {
     };
     std::vector<GraphEntry> graphs_;
     bool graph_eligible_ = false;
+    bool populations_set_ = false;   // setPopulationData since the last step: re-check capacities
     void clear_graphs() {
         for (auto& g : graphs_) fgx_graph_destroy(g.g);
         graphs_.clear();
@@ -1256,6 +1263,7 @@ inline void CUDASimulation::setPopulationData(AgentVector& pop, const std::strin
     detail::fgx_check(fgx_memcpy_h2d(a.d_next_id, &next, 4, stream_), "next id upload");
     synchronize();
     clear_graphs();
+    populations_set_ = true;
 }
 
 // src/flamegpu/simulation/Simulation.cu:55-83 + src/flamegpu/io/StateReader.cu
@@ -1966,7 +1974,8 @@ inline bool CUDASimulation::step() {
     ensure_init();
     if (!init_functions_ran_) run_init_functions();
     bool changed = false;
-    if (model_can_grow() || step_ == 0) {
+    if (model_can_grow() || step_ == 0 || populations_set_) {
+        populations_set_ = false;
         sync_counts();
         changed = ensure_capacity();
     }
@@ -2086,25 +2095,6 @@ inline void CUDASimulation::enqueue_log_frame(const LogPlan& plan, char* record)
     }
 }
 
-namespace detail {
-template <typename T, typename Acc>
-inline void decode_stats(const char* w, unsigned int n, LoggingConfig::Reduction r, LogValue& out) {
-    Acc sum, mn, mx;
-    double sq;
-    std::memcpy(&sum, w, 8);
-    std::memcpy(&mn, w + 8, 8);
-    std::memcpy(&mx, w + 16, 8);
-    std::memcpy(&sq, w + 24, 8);
-    switch (r) {
-        case LoggingConfig::Sum: out = LogValue::of<typename sum_input_t<T>::result_t>((typename sum_input_t<T>::result_t)sum); break;
-        case LoggingConfig::Min: out = LogValue::of<T>((T)mn); break;
-        case LoggingConfig::Max: out = LogValue::of<T>((T)mx); break;
-        // include/flamegpu/simulation/AgentLoggingConfig.h: mean 0 and standard deviation 0 for an empty list
-        case LoggingConfig::Mean: out = LogValue::of<double>(n ? (double)sum / (double)n : 0.0); break;
-        case LoggingConfig::StandardDev: out = LogValue::of<double>(n ? std::sqrt(sq / (double)n) : 0.0); break;
-    }
-}
-}  // namespace detail
 
 inline void CUDASimulation::decode_log_frame(const LoggingConfig& cfg, const LogPlan& plan, const char* record,
                                              LogFrame& f) const {
@@ -2295,14 +2285,64 @@ inline std::vector<T> HostAgentAPI::getVariableValues(const std::string& name) {
 }
 
 template <typename T>
+inline void HostAgentAPI::stats(const std::string& name, char out[32]) {
+    const auto& v = var<T>(name);
+    const int type = detail::stats_type_code(std::type_index(typeid(T)));
+    size_t sb = 0;
+    detail::fgx_check(fgx_scratch_bytes(FGX_OP_STATS, 0, 0, &sb), "statistics scratch");
+    void* scratch = s_.reduce_scratch(sb);
+    detail::fgx_check(fgx_reduce_stats(s_.ctx(), type, v.cur, l_.d_count, l_.cap, s_.reduce_out(), scratch, sb, s_.stream()),
+                      "statistics");
+    detail::fgx_check(fgx_memcpy_d2h(out, s_.reduce_out(), 32, s_.stream()), "statistics readback");
+    s_.synchronize();
+}
+template <typename T>
 inline T HostAgentAPI::min(const std::string& name) {
-    auto v = getVariableValues<T>(name);
-    return v.empty() ? std::numeric_limits<T>::max() : *std::min_element(v.begin(), v.end());
+    if (detail::stats_type_code(std::type_index(typeid(T))) < 0) {
+        auto v = getVariableValues<T>(name);
+        return v.empty() ? std::numeric_limits<T>::max() : *std::min_element(v.begin(), v.end());
+    }
+    char w[32];
+    stats<T>(name, w);
+    detail::LogValue r;
+    if (std::is_floating_point<T>::value) detail::decode_stats<T, double>(w, 0, LoggingConfig::Min, r);
+    else if (std::is_signed<T>::value) detail::decode_stats<T, long long>(w, 0, LoggingConfig::Min, r);
+    else detail::decode_stats<T, unsigned long long>(w, 0, LoggingConfig::Min, r);
+    return r.get<T>();
 }
 template <typename T>
 inline T HostAgentAPI::max(const std::string& name) {
-    auto v = getVariableValues<T>(name);
-    return v.empty() ? std::numeric_limits<T>::lowest() : *std::max_element(v.begin(), v.end());
+    if (detail::stats_type_code(std::type_index(typeid(T))) < 0) {
+        auto v = getVariableValues<T>(name);
+        return v.empty() ? std::numeric_limits<T>::lowest() : *std::max_element(v.begin(), v.end());
+    }
+    char w[32];
+    stats<T>(name, w);
+    detail::LogValue r;
+    if (std::is_floating_point<T>::value) detail::decode_stats<T, double>(w, 0, LoggingConfig::Max, r);
+    else if (std::is_signed<T>::value) detail::decode_stats<T, long long>(w, 0, LoggingConfig::Max, r);
+    else detail::decode_stats<T, unsigned long long>(w, 0, LoggingConfig::Max, r);
+    return r.get<T>();
+}
+template <typename T>
+inline std::pair<double, double> HostAgentAPI::meanStandardDeviation(const std::string& name) {
+    if (detail::stats_type_code(std::type_index(typeid(T))) < 0)
+        throw exception::UnsupportedVarType("meanStandardDeviation supports float, double and 32/64-bit integers");
+    const unsigned int n = count();
+    char w[32];
+    stats<T>(name, w);
+    detail::LogValue mean, sd;
+    if (std::is_floating_point<T>::value) {
+        detail::decode_stats<T, double>(w, n, LoggingConfig::Mean, mean);
+        detail::decode_stats<T, double>(w, n, LoggingConfig::StandardDev, sd);
+    } else if (std::is_signed<T>::value) {
+        detail::decode_stats<T, long long>(w, n, LoggingConfig::Mean, mean);
+        detail::decode_stats<T, long long>(w, n, LoggingConfig::StandardDev, sd);
+    } else {
+        detail::decode_stats<T, unsigned long long>(w, n, LoggingConfig::Mean, mean);
+        detail::decode_stats<T, unsigned long long>(w, n, LoggingConfig::StandardDev, sd);
+    }
+    return {mean.get<double>(), sd.get<double>()};
 }
 
 template <typename T>
