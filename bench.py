@@ -154,6 +154,92 @@ def bench_serving(args):
     return out
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "boids_inputdata_traffic.md")
+COPY_BYTES = 1 << 30   # calibration copy: 1 GiB read + 1 GiB written
+
+
+def measured_traffic():
+    """HBM bytes per neighbour-iteration launch from the committed PMC summary, or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def traffic_probe(args):
+    """The program that the FETCH_SIZE / WRITE_SIZE passes profile: three eager boids
+    steps at the bench size, then three launches of a plain elementwise kernel over
+    COPY_BYTES (negation: a copy kernel with known traffic), which calibrates the
+    counters."""
+    import torch
+    from flamegpu2_amd import workloads
+    from flamegpu2_amd.models import Simulation
+    params, arrs = workloads.boids(args.agents, seed=7)
+    sim = Simulation("boids_spatial3D", params, seed=7)
+    sim.set_option("graphs", 0)
+    sim.set_agents("Boid", arrs)
+    sim.step(3)
+    sim.sync()
+    sim.close()
+    print("traffic probe: boids steps done", file=sys.stderr, flush=True)
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.neg(src, out=dst)   # a plain elementwise kernel: COPY_BYTES read, COPY_BYTES written
+    torch.cuda.synchronize()
+    print("traffic probe: copies done", file=sys.stderr, flush=True)
+
+
+def traffic_report(fetch_dir, write_dir, command):
+    """Turn the two counter passes into the committed traffic summary.  Counters are KiB
+    per dispatch; each is scaled by what it reads for the calibration copy, whose true
+    traffic is COPY_BYTES each way."""
+    import csv
+    import glob
+
+    def per_dispatch(d, counter):
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            raise SystemExit(f"no counter_collection.csv under {d}")
+        vals, names = {}, set()
+        for f in files:
+            for i, r in enumerate(csv.DictReader(open(f))):
+                if r["Counter_Name"] != counter:
+                    continue
+                name = r["Kernel_Name"]
+                names.add(name[:120])
+                key = "inputdata" if "inputdata" in name else ("copy" if "neg_kernel" in name else None)
+                if key:
+                    d_id = (f, r.get("Dispatch_Id", i))
+                    vals.setdefault(key, {}).setdefault(d_id, 0.0)
+                    vals[key][d_id] += float(r["Counter_Value"])
+        if set(vals) != {"inputdata", "copy"}:
+            raise SystemExit(f"{counter}: kernels found: {sorted(names)}")
+        return {k: (sum(v.values()) / len(v), len(v)) for k, v in vals.items()}
+
+    fetch = per_dispatch(fetch_dir, "FETCH_SIZE")
+    write = per_dispatch(write_dir, "WRITE_SIZE")
+    f_scale = fetch["copy"][0] * 1024 / COPY_BYTES
+    w_scale = write["copy"][0] * 1024 / COPY_BYTES
+    f_bytes = fetch["inputdata"][0] * 1024 / f_scale
+    w_bytes = write["inputdata"][0] * 1024 / w_scale
+    with open(TRAFFIC_FILE, "w") as o:
+        o.write("# HBM traffic of the neighbour-iteration kernel (boids, 1,048,576 agents)\n\n")
+        o.write("Commands (one counter pass each):\n\n")
+        o.write("```\n" + command + "\n```\n\n")
+        o.write("| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB |\n|---|---|---|---|\n")
+        for k in ("inputdata", "copy"):
+            o.write(f"| {k} | {fetch[k][1]} | {fetch[k][0]:.1f} | {write[k][0]:.1f} |\n")
+        o.write(f"\nCalibration: the copy moves {COPY_BYTES} B each way; FETCH_SIZE reads "
+                f"{f_scale:.4f} x and WRITE_SIZE {w_scale:.4f} x the true bytes.\n\n")
+        o.write(f"fetch_bytes_per_launch: {f_bytes:.0f}\n\nwrite_bytes_per_launch: {w_bytes:.0f}\n\n")
+        o.write(f"bytes_per_launch: {f_bytes + w_bytes:.0f}\n")
+    print(open(TRAFFIC_FILE).read())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,10 +256,20 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--no-serving", dest="serving", action="store_false")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run the short program profiled by the FETCH_SIZE/WRITE_SIZE passes")
+    ap.add_argument("--traffic-report", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "COMMAND"),
+                    help="write the traffic summary from two rocprofv3 --pmc output folders")
     args = ap.parse_args()
+    if args.traffic_report:
+        traffic_report(*args.traffic_report)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (single-GPU build).", file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
 
     from flamegpu2_amd import workloads
     from flamegpu2_amd.models import Simulation
@@ -208,13 +304,7 @@ def main():
     cand = 0.5 * (ca + cb)
     bytes_per_agent = 28 + 8 * 27 + cand * 28 + 24
     achieved = n * bytes_per_agent / avg_s / 1e9
-    traffic = None
-    pmc = os.path.join(ROOT, "profiles", "boids_inputdata_traffic.json")
-    if os.path.exists(pmc):
-        try:
-            traffic = json.load(open(pmc)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic = measured_traffic()
 
     out = {
         "metric": METRIC,

@@ -28,8 +28,6 @@ int make_var_table(uint32_t nvars, const void* const* in, void* const* out, cons
         t->in[v] = static_cast<const char*>(in ? in[v] : nullptr);
         t->out[v] = static_cast<char*>(out ? out[v] : nullptr);
     }
-    t->all4 = nvars > 0 && nvars <= 16 && in && out;
-    for (uint32_t v = 0; v < nvars; ++v) t->all4 &= bytes[v] == 4 && t->in[v] && t->out[v];
     return FGX_OK;
 }
 

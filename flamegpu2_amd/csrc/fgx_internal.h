@@ -32,7 +32,6 @@ struct Ctx {
 // and read through the scalar unit.
 struct VarTable {
     uint32_t n;
-    uint32_t all4;   // every variable is 4 bytes and n <= 16 (copy_element fast path)
     uint32_t bytes[FGX_MAX_VARS];
     const char* in[FGX_MAX_VARS];
     char* out[FGX_MAX_VARS];
@@ -41,23 +40,7 @@ struct VarTable {
 int make_var_table(uint32_t nvars, const void* const* in, void* const* out, const uint32_t* bytes, VarTable* t);
 
 // Copy element `src_i` of every variable (except `skip`) to element `dst_i` of its output array.
-// Copy element `src_i` of every variable (except `skip`) to element `dst_i` of its output array.
-// Tables of up to 16 four-byte variables (the common case, flagged by make_var_table)
-// issue every load before the first store, so the loads' latencies overlap.
 __device__ __forceinline__ void copy_element(const VarTable& t, uint32_t src_i, uint32_t dst_i, int32_t skip = -1) {
-    if (t.all4) {
-        constexpr uint32_t kMax = 16;
-        uint32_t val[kMax];
-#pragma unroll
-        for (uint32_t k = 0; k < kMax; ++k) {
-            const char* base = k < t.n ? t.in[k] : t.in[0];   // loads stay unconditional
-            val[k] = reinterpret_cast<const uint32_t*>(base)[src_i];
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < kMax; ++k)
-            if (k < t.n && (int32_t)k != skip) reinterpret_cast<uint32_t*>(t.out[k])[dst_i] = val[k];
-        return;
-    }
     for (uint32_t v = 0; v < t.n; ++v) {
         if ((int32_t)v == skip) continue;
         const uint32_t b = t.bytes[v];

@@ -74,11 +74,12 @@ __device__ __forceinline__ Bound resolve(const VarBinding (&tab)[VAR_SLOTS], uin
     // one 16-byte load of the whole binding, then selects on the values (a select
     // of addresses would chain two dependent scalar loads)
     const uint4 raw = *reinterpret_cast<const uint4*>(&tab[table_slot(h, mul, VAR_SLOT_BITS)]);
-    const bool known = raw.x == h;
-    const bool typed = (raw.y & 0xFFFFu) == (uint32_t)sizeof(T) * len;
-    c.err.note(!known, err_unknown, h);
-    c.err.note(known && !typed, err_type, h);
-    const bool ok = known && typed;
+    // name hash and element bytes (whose reserved upper half-word is zero) checked by
+    // one 64-bit compare against a compile-time constant; one branch to the report
+    const uint64_t got = ((uint64_t)raw.y << 32) | raw.x;
+    const uint64_t want = ((uint64_t)((uint32_t)sizeof(T) * len) << 32) | h;
+    const bool ok = got == want;
+    if (__builtin_expect(!ok, 0)) report_error(c.err.d_error, raw.x != h ? err_unknown : err_type, h, c.err.index);
     const uint64_t bound = ((uint64_t)raw.w << 32) | raw.z;
     const uint64_t fallback = write ? (uint64_t)c.sink_page : (uint64_t)c.zero_page;
     return Bound{ok ? bound : fallback, ok ? 0xFFFFFFFFu : 0u};
@@ -1336,12 +1337,24 @@ __device__ __forceinline__ void run_agent(const ThreadCtx& c) {
         c.err.note(s == DEAD, DERR_DEATH_DISABLED, 0);
 }
 
+// Workgroups go to the 8 XCDs round-robin (workgroup w runs on XCD w % 8), and each
+// XCD has its own L2.  This renumbering gives XCD x one contiguous range of logical
+// blocks, started in order, so the bin-sorted agents that share neighbourhoods (and
+// so messages) also share an L2: XCD x gets q + (x < r) blocks, q = G / 8, r = G % 8,
+// starting at x*q + min(x, r).
+__device__ __forceinline__ uint32_t xcd_block() {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    const uint32_t x = b % kXcds, k = b / kXcds, q = g / kXcds, r = g % kXcds;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Agents [first, n) of the list run, first = the agents a function condition disabled.
 template <typename F>
 __global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
     const uint32_t first = a.d_first ? load_count(a.d_first) : 0u;
     const uint32_t n = a.d_n ? load_count(a.d_n) : a.n_cap;
-    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t o = xcd_block() * blockDim.x + threadIdx.x;
     if (a.d_msg_count_out && o == 0) *a.d_msg_count_out = n;   // one message per agent (no condition here)
     if (first + o >= n) return;
     const uint32_t i = first + o;
@@ -1539,7 +1552,7 @@ __global__ void __launch_bounds__(256) agent_function_kernel_staged(const FnArgs
     extern __shared__ uint32_t fgx_stage_smem[];
     __shared__ StageHeader sh;
     const uint32_t n = a.d_n ? load_count(a.d_n) : a.n_cap;
-    const uint32_t b0 = blockIdx.x * blockDim.x;
+    const uint32_t b0 = xcd_block() * blockDim.x;
     if (a.d_msg_count_out && b0 == 0 && threadIdx.x == 0) *a.d_msg_count_out = n;
     if (b0 >= n) return;
     const uint32_t i = b0 + threadIdx.x;
