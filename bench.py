@@ -86,10 +86,22 @@ def bench_batched(args):
     made = [workloads.circles(n, seed=s, radius=r) for (n, r, s) in plans]
     agents = sum(n for (n, _, _) in plans)
     total = float(agents) * steps
-    # batched: all simulations in one batch; the tuner's trial steps are part of the run
-    b = Batch("circles_spatial3D", made[0][0])
-    hs = [b.add(p, s, steps, {"Circle": a}) for (p, a), (_, _, s) in zip(made, plans)]
+    # batched: all simulations in one batch; the tuner's trial steps are part of the run.
+    # The bin-width multiplier is chosen first, on throwaway copies (it changes the
+    # summation order, so it cannot change inside the run); its trials count too.
     t0 = time.perf_counter()
+    bpr, bw_timings = 1, {}
+    if args.autotune:
+        def make(k):
+            c = Batch("circles_spatial3D", made[0][0])
+            c.set_option("bins_per_radius", k)
+            for (p, a), (_, _, s) in zip(made, plans):
+                c.add(p, s, 8, {"Circle": a})
+            return c
+        bpr, bw_timings = autotune.choose_bin_width(make)
+    b = Batch("circles_spatial3D", made[0][0])
+    b.set_option("bins_per_radius", bpr)
+    hs = [b.add(p, s, steps, {"Circle": a}) for (p, a), (_, _, s) in zip(made, plans)]
     b.step(1)
     b.sync()
     tuned, timings = None, {}
@@ -104,6 +116,7 @@ def bench_batched(args):
     print(f"bench: batched {wall_b:.2f} s", file=sys.stderr, flush=True)
     # steady-state step time of the tuned batch (separate short run, same simulations)
     b2 = Batch("circles_spatial3D", made[0][0])
+    b2.set_option("bins_per_radius", bpr)
     if tuned:
         b2.set_tuning(*tuned)
     for (p, a), (_, _, s) in zip(made, plans):
@@ -124,7 +137,8 @@ def bench_batched(args):
     print(f"bench: plain {wall_p:.2f} s", file=sys.stderr, flush=True)
     return {
         "workload": f"{len(plans)} circles simulations, N log-uniform [1024, 262144] (seeded), r ~ U[1, 3], "
-                    f"density 1, {steps} steps each",
+                    f"density 1, {steps} steps each; batched run autotuned over block {{64..1024}} x agents per "
+                    f"thread {{1,2,4}} x bin width {{1.0, 0.5}} x radius; plain runs use bins as wide as the radius",
         "agents": agents,
         "batched_agent_steps_per_s": total / wall_b,
         "plain_agent_steps_per_s": total / wall_p,
@@ -134,7 +148,9 @@ def bench_batched(args):
         "batched_wall_s": wall_b,
         "plain_wall_s": wall_p,
         "batched_ms_per_step_steady": ms_step_b,
-        "tuning": {"block": tuned[0], "agents_per_thread": tuned[1], "blocks_per_cu": tuned[2]} if tuned else None,
+        "tuning": {"block": tuned[0], "agents_per_thread": tuned[1], "blocks_per_cu": tuned[2],
+                   "bin_width_multiplier": 1.0 / bpr} if tuned else None,
+        "bin_width_ms_per_step": {f"{1.0 / k:g}": round(v, 4) for k, v in bw_timings.items()},
         "tuning_ms_per_step": {f"{k[0]}x{k[1]}/{k[2]}": round(v, 4) for k, v in timings.items()},
     }
 

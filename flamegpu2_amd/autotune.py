@@ -9,11 +9,18 @@ Search space: block size {64, 128, 256, 512, 1024} x agents per thread {1, 2, 4}
 plus persistent grids (blocks per CU) for the best one-block-per-tile shape.  Each
 candidate runs two untimed steps (repack, graph capture of both buffer
 arrangements) and `steps` timed steps.
+
+The bin-width multiplier (1 or 0.5: spatial bins as wide as the radius, or half
+as wide and searched two bins either side) is the third dimension.  It changes the
+neighbour visit order, so the FP32 summation order, so it cannot be switched inside
+a run: choose_bin_width() times each multiplier on throwaway copies of the batch
+before the run starts, and the run is then built with the winner.
 """
 
 BLOCKS = (64, 128, 256, 512, 1024)
 AGENTS_PER_THREAD = (1, 2, 4)
 PERSISTENT = (2, 4, 8)
+BINS_PER_RADIUS = (1, 2)   # bin-width multipliers 1.0 and 0.5
 
 
 def candidates(blocks=BLOCKS, apts=AGENTS_PER_THREAD, persistent=PERSISTENT):
@@ -42,3 +49,18 @@ def autotune(batch, steps=2, blocks=BLOCKS, apts=AGENTS_PER_THREAD, persistent=P
     best = min(timings, key=timings.get)
     batch.set_tuning(*best)
     return best, timings
+
+
+def choose_bin_width(make_batch, steps=2, bins_per_radius=BINS_PER_RADIUS):
+    """make_batch(bins_per_radius) builds a models.Batch holding copies of the run's
+    simulations.  Each copy runs two untimed steps and `steps` timed steps with the
+    default launch shape, and is then closed.  Returns (best bins_per_radius,
+    {bins_per_radius: ms per step})."""
+    timings = {}
+    for k in bins_per_radius:
+        b = make_batch(k)
+        try:
+            timings[k] = b.timed_steps(steps) / steps
+        finally:
+            b.close()
+    return min(timings, key=timings.get), timings

@@ -450,12 +450,16 @@ static bool approx_exactly_divisible(float x, float y) {
     return v <= eps || v > y - eps;
 }
 
-int fgx_spatial_meta_init(int dims, const float* mn, const float* mx, float radius, fgx_spatial_meta* m) {
+int fgx_spatial_meta_init_mult(int dims, const float* mn, const float* mx, float radius, float mult,
+                               fgx_spatial_meta* m) {
     if (dims != 2 && dims != 3) return fail(FGX_ERR_ARG, "spatial messages are 2-D or 3-D");
     if (!(radius > 0.0f)) return fail(FGX_ERR_ARG, "spatial message radius must be > 0");
+    if (!(mult == 1.0f || mult == 0.5f)) return fail(FGX_ERR_ARG, "bin-width multiplier must be 1 or 0.5");
     std::memset(m, 0, sizeof(*m));
     m->dims = dims;
     m->radius = radius;
+    m->cell = radius * mult;
+    m->span = mult == 1.0f ? 1 : 2;
     m->bins = 1;
     m->wrap_ok = 1;
     for (int a = 0; a < 3; ++a) {
@@ -464,17 +468,21 @@ int fgx_spatial_meta_init(int dims, const float* mn, const float* mx, float radi
             m->min[a] = mn[a];
             m->max[a] = mx[a];
             m->width[a] = mx[a] - mn[a];
-            const float q = m->width[a] / radius;
+            const float q = m->width[a] / m->cell;
             m->grid[a] = (uint32_t)std::ceil(q);
             const uint64_t bins = (uint64_t)m->bins * m->grid[a];
             if (bins >= (1ull << 31)) return fail(FGX_ERR_ARG, "spatial grid too large");
             m->bins = (uint32_t)bins;
-            m->wrap_ok = m->wrap_ok && approx_exactly_divisible(m->width[a], radius);
+            m->wrap_ok = m->wrap_ok && approx_exactly_divisible(m->width[a], m->cell);
         } else {
             m->grid[a] = 1;
         }
     }
     return FGX_OK;
+}
+
+int fgx_spatial_meta_init(int dims, const float* mn, const float* mx, float radius, fgx_spatial_meta* m) {
+    return fgx_spatial_meta_init_mult(dims, mn, mx, radius, 1.0f, m);
 }
 
 int fgx_agent_sort_key_bits(int dims, const float* mn, const float* mx, float radius, uint32_t* key_bits) {

@@ -59,6 +59,18 @@ __device__ __forceinline__ void copy_element(const VarTable& t, uint32_t src_i, 
     }
 }
 
+// ---------------------------------------------------------------- XCD-contiguous block order
+// Workgroups go to the 8 XCDs round-robin (workgroup w runs on XCD w % 8), each with
+// its own L2.  Renumbered, XCD x runs one contiguous range of logical blocks, in order
+// (q + (x < r) of them, q = G / 8, r = G % 8, from x*q + min(x, r)), so neighbouring
+// elements, bins and scatter targets share an L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    const uint32_t x = b % kXcds, k = b / kXcds, q = g / kXcds, r = g % kXcds;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // ---------------------------------------------------------------- wave64 helpers
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 

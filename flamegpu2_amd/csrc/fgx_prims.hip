@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist(KeyArgs ka, const uint32_t*
     __syncthreads();
     const uint32_t n = load_count(d_n, n_cap);
     const uint32_t lane = lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t wave = (xcd_block() * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
         const uint32_t i = base + lane;
@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_hist_seg(int32_t mode, const fgx
     __shared__ uint32_t s_down;
     if (threadIdx.x == 0) s_down = 0u;
     __syncthreads();
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = xcd_block();   // tiles in XCD-contiguous order
     uint32_t lo = 0, hi = nsegs;
     while (hi - lo > 1u) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(kBlock) k_index_scatter(const uint32_t* d_n, u
     if (!*unsorted) return;   // in bin order already: the permutation is the identity
     const uint32_t n = load_count(d_n, n_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    for (uint32_t i = xcd_block() * blockDim.x + threadIdx.x; i < n; i += stride)
         tmp[start[key[i]] + sub[i]] = i;
 }
 
@@ -217,10 +217,10 @@ __global__ void __launch_bounds__(kBlock) k_stable_scatter(const uint32_t* d_n, 
     const uint32_t n = load_count(d_n, n_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
     if (!*unsorted) {
-        for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) copy_element(vt, s, s);
+        for (uint32_t s = xcd_block() * blockDim.x + threadIdx.x; s < n; s += stride) copy_element(vt, s, s);
         return;
     }
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
+    for (uint32_t s = xcd_block() * blockDim.x + threadIdx.x; s < n; s += stride) {
         const uint32_t e = tmp[s];
         const uint32_t b = key[e];
         const uint32_t st = start[b];
@@ -328,18 +328,74 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles(const uint32_t* __restric
     }
 }
 
-// Exclusive scan of `n` counts into `out` (two-pass when the tiles fit, else single-pass).
-static int scan_counts(const uint32_t* in, uint32_t* out, uint32_t n, LookbackHeader* hdr, uint64_t* status, hipStream_t st) {
-    const uint32_t tiles = ceil_div(n, kTile);
-    if (tiles <= kTwoPassTiles) {
-        uint32_t* ts = reinterpret_cast<uint32_t*>(status);   // the status words are free in this mode
-        hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kBlock), 0, st, in, n, ts);
-        FGX_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kBlock), 0, st, in, out, n, ts);
-        FGX_HIP(hipGetLastError());
-        return FGX_OK;
+// In-place exclusive scan of the `t` tile sums by one block: thread k sums a contiguous
+// chunk, the block scans the chunk totals, and each thread rewrites its chunk.
+constexpr uint32_t kSumsBlock = 1024;
+__global__ void __launch_bounds__(kSumsBlock) k_scan_sums(uint32_t* __restrict__ ts, uint32_t t) {
+    __shared__ uint32_t s_wave[kSumsBlock / 64];
+    const uint32_t per = (t + kSumsBlock - 1) / kSumsBlock;
+    const uint32_t lo = threadIdx.x * per;
+    const uint32_t hi = lo + per < t ? lo + per : t;
+    uint32_t sum = 0;
+    for (uint32_t k = lo; k < hi; ++k) sum += ts[k];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t incl = wave_inclusive_u32(sum);
+    if (lane == 63) s_wave[w] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    for (uint32_t k = 0; k < w; ++k) run += s_wave[k];
+    for (uint32_t k = lo; k < hi; ++k) {
+        const uint32_t v = ts[k];
+        ts[k] = run;
+        run += v;
     }
-    hipLaunchKernelGGL(k_scan_excl, dim3(tiles), dim3(kBlock), 0, st, in, out, n, hdr, status, tiles);
+}
+
+// The second pass when the tile sums are already scanned: tile t's prefix is ts[t].
+__global__ void __launch_bounds__(kBlock) k_scan_tiles_pre(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                           uint32_t n, const uint32_t* __restrict__ ts) {
+    __shared__ uint32_t s_wave[kBlock / 64];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t base = tile * kTile + threadIdx.x * kItems;
+    uint32_t v[kItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        v[j] = (base + j < n) ? in[base + j] : 0u;
+        sum += v[j];
+    }
+    const uint32_t incl = wave_inclusive_u32(sum);
+    if (lane == 63) s_wave[w] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0;
+    for (uint32_t k = 0; k < w; ++k) wave_off += s_wave[k];
+    uint32_t run = ts[tile] + wave_off + incl - sum;
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+}
+
+// Exclusive scan of `n` counts into `out`, ordered by kernel boundaries only (no
+// inter-block waits): up to kTwoPassTiles tiles, tile sums then a pass in which each
+// tile adds up the sums before it; beyond that, tile sums, one block scanning them,
+// then a pass that reads each tile's prefix.  `status` holds the tile sums.
+static int scan_counts(const uint32_t* in, uint32_t* out, uint32_t n, LookbackHeader* hdr, uint64_t* status, hipStream_t st) {
+    (void)hdr;
+    const uint32_t tiles = ceil_div(n, kTile);
+    if (!tiles) return FGX_OK;
+    uint32_t* ts = reinterpret_cast<uint32_t*>(status);   // 8 bytes per tile are reserved; sums use 4
+    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kBlock), 0, st, in, n, ts);
+    FGX_HIP(hipGetLastError());
+    if (tiles <= kTwoPassTiles) {
+        hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kBlock), 0, st, in, out, n, ts);
+    } else {
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kSumsBlock), 0, st, ts, tiles);
+        FGX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_scan_tiles_pre, dim3(tiles), dim3(kBlock), 0, st, in, out, n, ts);
+    }
     FGX_HIP(hipGetLastError());
     return FGX_OK;
 }
@@ -674,10 +730,7 @@ static int run_scan(const uint32_t* in, uint32_t* out, uint32_t n, char* scratch
     if (!n) return FGX_OK;
     auto* hdr = reinterpret_cast<LookbackHeader*>(scratch);
     auto* status = reinterpret_cast<uint64_t*>(scratch + sizeof(LookbackHeader));
-    const uint32_t tiles = ceil_div(n, kTile);
-    hipLaunchKernelGGL(k_scan_excl, dim3(tiles), dim3(kBlock), 0, (hipStream_t)stream, in, out, n, hdr, status, tiles);
-    FGX_HIP(hipGetLastError());
-    return FGX_OK;
+    return scan_counts(in, out, n, hdr, status, (hipStream_t)stream);
 }
 
 // Shared by the message build and the agent sort.
@@ -831,7 +884,7 @@ int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t
     std::memset(&ka, 0, sizeof(ka));
     ka.mode = 0;
     ka.dims = meta->dims;
-    ka.radius = meta->radius;
+    ka.radius = meta->cell > 0.0f ? meta->cell : meta->radius;   // bin width
     for (int a = 0; a < 3; ++a) {
         ka.min[a] = meta->min[a];
         ka.width[a] = meta->width[a];

@@ -32,7 +32,8 @@ class FgxError(RuntimeError):
 
 class SpatialMeta(_c.Structure):
     _fields_ = [("dims", i32), ("min", _c.c_float * 3), ("max", _c.c_float * 3), ("radius", _c.c_float),
-                ("width", _c.c_float * 3), ("grid", u32 * 3), ("bins", u32), ("wrap_ok", i32)]
+                ("width", _c.c_float * 3), ("grid", u32 * 3), ("bins", u32), ("wrap_ok", i32), ("cell", _c.c_float),
+                ("span", i32)]
 
 
 class SegKey(_c.Structure):
@@ -60,6 +61,8 @@ _SIGS = {
     "fgx_scratch_clear": [vp, sz, vp],
     "fgx_spatial_meta_init": [_c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float,
                               _c.POINTER(SpatialMeta)],
+    "fgx_spatial_meta_init_mult": [_c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float, _c.c_float,
+                                   _c.POINTER(SpatialMeta)],
     "fgx_spatial_build": [vp, _c.POINTER(SpatialMeta), vp, u32, vp, vp, vp, u32, _c.POINTER(vp), _c.POINTER(vp),
                           _c.POINTER(u32), vp, vp, sz, vp],
     "fgx_agent_sort_key_bits": [_c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float,
@@ -137,11 +140,12 @@ def _arr(t, vals):
     return (t * max(len(vals), 1))(*vals)
 
 
-def spatial_meta(dims, mn, mx, radius):
+def spatial_meta(dims, mn, mx, radius, mult=1.0):
     m = SpatialMeta()
     mn = list(mn) + [0.0] * (3 - len(mn))
     mx = list(mx) + [0.0] * (3 - len(mx))
-    check(lib().fgx_spatial_meta_init(dims, _arr(_c.c_float, mn), _arr(_c.c_float, mx), radius, _c.byref(m)))
+    check(lib().fgx_spatial_meta_init_mult(dims, _arr(_c.c_float, mn), _arr(_c.c_float, mx), radius, mult,
+                                           _c.byref(m)))
     return m
 
 

@@ -91,6 +91,13 @@ class BatchSimulation {
         graphs_on_ = on;
         clear_graphs();
     }
+    // Bin-width multiplier (1 or 0.5; CUDASimulation::setBinWidthMultiplier) of the
+    // simulations added from now on.
+    void setBinWidthMultiplier(float mult) {
+        if (!(mult == 1.0f || mult == 0.5f)) throw exception::InvalidArgument("bin-width multiplier must be 1 or 0.5");
+        bin_mult_ = mult;
+    }
+    float binWidthMultiplier() const { return bin_mult_; }
     void* stream() {
         init();
         return stream_;
@@ -227,6 +234,7 @@ class BatchSimulation {
         fgx_graph* g = nullptr;
     };
     std::vector<GraphEntry> graphs_;
+    float bin_mult_ = 1.0f;
     bool graphs_on_ = true;
     uint32_t stable_steps_ = 0;   // steps since the last repack
 };
@@ -395,7 +403,8 @@ inline int BatchSimulation::add(const Plan& p) {
             b.radius = md->radius;
         }
         fgx_spatial_meta m;
-        detail::fgx_check(fgx_spatial_meta_init(kv.second.dims, b.min, b.max, b.radius, &m), "spatial message metadata");
+        detail::fgx_check(fgx_spatial_meta_init_mult(kv.second.dims, b.min, b.max, b.radius, bin_mult_, &m),
+                          "spatial message metadata");
         s.meta[kv.first] = m;
     }
     for (auto& kv : agents_) {
@@ -685,7 +694,7 @@ inline void BatchSimulation::rebuild() {
             mb.keys.assign(S, fgx_seg_key{});
             for (uint32_t s = 0; s < S; ++s) {
                 const fgx_spatial_meta& m = sims_.at(neworder[s]).meta.at(kv.first);
-                detail::fgx_check(fgx_seg_key_init(FGX_SEG_MESSAGE_BINS, mb.dims, m.min, m.max, m.radius, &mb.keys[s]),
+                detail::fgx_check(fgx_seg_key_init(FGX_SEG_MESSAGE_BINS, mb.dims, m.min, m.max, m.cell, &mb.keys[s]),
                                   "message segment");
                 mb.keys[s].n = cnt[s];
             }
@@ -813,6 +822,8 @@ inline void BatchSimulation::rebuild() {
                         d.meta.grid[k] = m.grid[k] ? m.grid[k] : 1u;
                     }
                     d.meta.radius = m.radius;
+                    d.meta.cell = m.cell;
+                    d.meta.span = m.span;
                     d.meta.wrap_ok = m.wrap_ok;
                     d.meta.pbm = f.in->pbm + f.in->keys[s].key_off;
                 }

@@ -175,6 +175,8 @@ struct SpatialDev {
     const uint32_t* pbm;
     int32_t key_min;
     uint32_t keys;
+    float cell;      // bin width (the radius, or half of it for half-width bins)
+    int32_t span;    // bins searched either side of the query's bin (1, or 2 for half-width bins)
 };
 
 // Device error codes (the device half of exception::DeviceError, replacing DTHROW).

@@ -165,6 +165,12 @@ FGXM_EXPORT int fgxm_step(fgxm_sim* s, uint32_t nsteps) {
     });
 }
 
+// option "bins_per_radius": 1 (bins as wide as the radius) or 2 (half-width bins)
+inline float bins_per_radius_mult(int value) {
+    if (value != 1 && value != 2) throw flamegpu::exception::InvalidArgument("bins_per_radius must be 1 or 2");
+    return value == 1 ? 1.0f : 0.5f;
+}
+
 FGXM_EXPORT int fgxm_set_option(fgxm_sim* s, const char* key, int value) {
     return fgxm_detail::guard([&] {
         const std::string k = key ? key : "";
@@ -172,6 +178,7 @@ FGXM_EXPORT int fgxm_set_option(fgxm_sim* s, const char* key, int value) {
         else if (k == "concurrency") s->sim->CUDAConfig().inLayerConcurrency = value != 0;
         else if (k == "kernel_timing") s->sim->setKernelTiming(value != 0);
         else if (k == "staging") s->sim->setNeighbourhoodStaging(value != 0);
+        else if (k == "bins_per_radius") s->sim->setBinWidthMultiplier(bins_per_radius_mult(value));
         else throw flamegpu::exception::InvalidArgument("unknown option " + k);
     });
 }
@@ -565,6 +572,7 @@ FGXM_EXPORT int fgxm_batch_set_option(fgxm_batch* b, const char* key, int value)
     return fgxm_detail::guard([&] {
         const std::string k = key ? key : "";
         if (k == "graphs") b->batch->setGraphs(value != 0);
+        else if (k == "bins_per_radius") b->batch->setBinWidthMultiplier(bins_per_radius_mult(value));
         else throw flamegpu::exception::InvalidArgument("unknown batch option " + k);
     });
 }

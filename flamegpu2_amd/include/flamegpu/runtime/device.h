@@ -364,7 +364,7 @@ class MessageBruteForce::Out {
 namespace detail {
 // getGridPosition / getHash (MessageSpatial3DDevice.cuh:646-672).
 __device__ __forceinline__ int grid_pos(const SpatialDev& m, int a, float p) {
-    const int g = (int)floorf((p - m.min[a]) / m.radius);
+    const int g = (int)floorf((p - m.min[a]) / m.cell);   // the bin width: the radius, or half of it
     const int hi = (int)m.grid[a] - 1;
     return g < 0 ? 0 : (g > hi ? hi : g);
 }
@@ -380,6 +380,11 @@ __device__ __forceinline__ int wrap_cell(int c, int d, int g) {
     int a = c + d;
     a = a < 0 ? a + g : a;
     return a >= g ? a - g : a;
+}
+// (c + d) mod g for any sign and size of d (a search reach may exceed a tiny grid)
+__device__ __forceinline__ int wrap_mod(int c, int d, int g) {
+    const int a = (c + d) % g;
+    return a < 0 ? a + g : a;
 }
 constexpr uint32_t kHashX = name_hash("x", 2);
 constexpr uint32_t kHashY = name_hash("y", 2);
@@ -410,8 +415,9 @@ constexpr uint32_t kHashId = name_hash("id", 3);
 template <int DIMS, bool WRAP>
 class SpatialFilter {
  public:
-    static constexpr int kUnits = WRAP ? (DIMS == 3 ? 27 : 9) : (DIMS == 3 ? 9 : 3);
-    // counter levels: wrap 3-D (dz, dy, dx), wrap 2-D (dy, dx), clamped 3-D (dz, dy), clamped 2-D (dy)
+    // counter levels: wrap 3-D (dz, dy, dx), wrap 2-D (dy, dx), clamped 3-D (dz, dy), clamped 2-D (dy);
+    // each counter runs over 2R+1 values for a reach of R bins either side (R = 1 in the
+    // reference; 2 with half-width bins), so there are (2R+1)^levels units
     static constexpr int kLevels = WRAP ? (DIMS == 3 ? 3 : 2) : (DIMS == 3 ? 2 : 1);
     struct Unit {
         int c0, c1, c2;   // counters, 0..2
@@ -544,12 +550,12 @@ class SpatialFilter {
         __device__ __forceinline__ void next_unit() {
             more_ = false;
 #pragma clang loop unroll(disable)
-            while (pu_ < kUnits) {
+            while (pu_ < f_.units_) {
                 l_ = ps_;
                 e_ = pe_;
                 o_ = po_;
                 ++pu_;
-                if (pu_ < kUnits) {
+                if (pu_ < f_.units_) {
                     f_.next_unit(k_);
                     f_.load_unit(k_, ps_, pe_, po_);
                 }
@@ -599,6 +605,10 @@ class SpatialFilter {
         gx_ = (int)m.grid[0];
         gy_ = (int)m.grid[1];
         gz_ = (int)m.grid[2];
+        const int R = m.span;
+        last_ = 2 * R;
+        units_ = kLevels == 3 ? (last_ + 1) * (last_ + 1) * (last_ + 1)
+                              : (kLevels == 2 ? (last_ + 1) * (last_ + 1) : last_ + 1);
         cx_ = cx;
         ebase_ = ebase;
         xbase_ = xbase;
@@ -608,17 +618,17 @@ class SpatialFilter {
             xs_ = 0;
             span_ = 0;
             if (DIMS == 3) {   // c0: z, c1: y, c2: x
-                st0_ = detail::wrap_cell(cz, -1, gz_) * sz;
-                st1_ = detail::wrap_cell(cy, -1, gy_) * sy;
-                st2_ = detail::wrap_cell(cx, -1, gx_);
+                st0_ = detail::wrap_mod(cz, -R, gz_) * sz;
+                st1_ = detail::wrap_mod(cy, -R, gy_) * sy;
+                st2_ = detail::wrap_mod(cx, -R, gx_);
                 m0_ = (gz_ - 1) * sz;
                 m1_ = (gy_ - 1) * sy;
                 m2_ = gx_ - 1;
                 s0_ = sz;
                 s1_ = sy;
             } else {           // c0: y, c1: x
-                st0_ = detail::wrap_cell(cy, -1, gy_) * sy;
-                st1_ = detail::wrap_cell(cx, -1, gx_);
+                st0_ = detail::wrap_mod(cy, -R, gy_) * sy;
+                st1_ = detail::wrap_mod(cx, -R, gx_);
                 st2_ = 0;
                 m0_ = (gy_ - 1) * sy;
                 m1_ = gx_ - 1;
@@ -627,12 +637,12 @@ class SpatialFilter {
                 s1_ = 1;
             }
         } else {
-            xs_ = cx - 1 < 0 ? 0 : cx - 1;
-            const int xe = cx + 1 > gx_ - 1 ? gx_ - 1 : cx + 1;
+            xs_ = cx - R < 0 ? 0 : cx - R;
+            const int xe = cx + R > gx_ - 1 ? gx_ - 1 : cx + R;
             span_ = xe - xs_;
             // raw row coordinates; c0: z (3-D) or y (2-D), c1: y (3-D)
-            st0_ = DIMS == 3 ? cz - 1 : cy - 1;
-            st1_ = cy - 1;
+            st0_ = DIMS == 3 ? cz - R : cy - R;
+            st1_ = cy - R;
             st2_ = 0;
             m0_ = m1_ = m2_ = 0;
             s0_ = s1_ = 1;
@@ -645,11 +655,11 @@ class SpatialFilter {
         k.p2 = st2_;
     }
     __device__ __forceinline__ void next_unit(Unit& k) const {
-        const bool w0 = k.c0 == 2;
+        const bool w0 = k.c0 == last_;
         k.c0 = w0 ? 0 : k.c0 + 1;
         if (WRAP) {
             k.p0 = w0 ? st0_ : (k.p0 == m0_ ? 0 : k.p0 + s0_);
-            const bool w1 = w0 && k.c1 == 2;
+            const bool w1 = w0 && k.c1 == last_;
             k.c1 = w0 ? (w1 ? 0 : k.c1 + 1) : k.c1;
             k.p1 = w0 ? (w1 ? st1_ : (k.p1 == m1_ ? 0 : k.p1 + s1_)) : k.p1;
             if (kLevels == 3) {
@@ -732,6 +742,7 @@ class SpatialFilter {
     int gx_, gy_, gz_;              // grid
     int cx_;                        // search origin's x cell (LDS source)
     int xs_, span_;                 // non-wrapping strips: first x, last x - first x
+    int last_, units_;              // last counter value (2R) and unit count ((2R+1)^levels)
     int st0_, st1_, st2_;           // coordinate parts of the first unit
     int m0_, m1_, m2_, s0_, s1_;    // wrapping: last part value and stride per level
     int ebase_, xbase_, ewidth_;    // LDS source: covering group
@@ -1369,11 +1380,15 @@ __global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
 // apt agents) belongs to the simulation s with segs[s].tile_off <= t <
 // segs[s+1].tile_off and runs with that simulation's metadata, environment, step,
 // seed and error words; each thread runs apt agents, blockDim apart.  A grid
-// smaller than the tile count loops over tiles (persistent blocks).
+// smaller than the tile count makes the blocks persistent: logical block b (in
+// XCD-contiguous order) runs tiles [b*T/G, (b+1)*T/G).
 template <typename F>
 __global__ void __launch_bounds__(1024) agent_function_kernel_seg(const FnArgs a, const SegDev* __restrict__ segs,
                                                                   uint32_t nsegs, uint32_t ntiles, uint32_t apt) {
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint32_t lb = xcd_block();
+    const uint32_t t0 = (uint32_t)((uint64_t)lb * ntiles / gridDim.x);
+    const uint32_t t1 = (uint32_t)((uint64_t)(lb + 1) * ntiles / gridDim.x);
+    for (uint32_t t = t0; t < t1; ++t) {
         uint32_t lo = 0, hi = nsegs;
         while (hi - lo > 1u) {
             const uint32_t mid = (lo + hi) >> 1;

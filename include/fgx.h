@@ -118,9 +118,18 @@ typedef struct {
     float width[3];
     uint32_t grid[3];      /* grid[2] == 1 in 2-D */
     uint32_t bins;
-    int32_t wrap_ok;       /* approxExactlyDivisible(width, radius) on every axis */
+    int32_t wrap_ok;       /* approxExactlyDivisible(width, cell) on every axis */
+    float cell;            /* bin width: radius x bin-width multiplier */
+    int32_t span;          /* bins searched either side of a query's bin: 1, or 2 for half-width bins */
 } fgx_spatial_meta;
 int fgx_spatial_meta_init(int dims, const float* min, const float* max, float radius, fgx_spatial_meta* out);
+/* Half-width bins (mult = 0.5; mult = 1 is fgx_spatial_meta_init).  No counterpart in the
+ * reference, whose bin width is always the radius: bins of width radius/2 (exact in FP32),
+ * grid ceil(width / cell), and the iterators search 2 bins either side in the reference's
+ * order, so a query visits 5^3 (5^2) bins of 1/8 (1/4) the volume instead of 3^3 (3^2).
+ * The in-radius neighbours are the same; their visit order, so FP summation order, is not. */
+int fgx_spatial_meta_init_mult(int dims, const float* min, const float* max, float radius, float mult,
+                               fgx_spatial_meta* out);
 
 /* ---------------------------------------------------------------- message-list build (PBM)
  * Replaces MessageSpatial3D/2D::CUDAModelHandler::buildIndex

@@ -46,7 +46,9 @@ int fgxm_step(fgxm_sim* sim, uint32_t nsteps);
 /* options: "graphs" (capture steps as hipGraphs, default 1), "concurrency"
  * (in-layer streams, default 1), "kernel_timing" (HIP events around every
  * launch, eager steps only, default 0), "staging" (LDS staging of spatial
- * message neighbourhoods, default 0). */
+ * message neighbourhoods, default 0), "bins_per_radius" (1: spatial bins as wide
+ * as the radius, the reference's grid; 2: half-width bins, searched 2 bins either
+ * side; no counterpart in the reference; before the first step). */
 int fgxm_set_option(fgxm_sim* sim, const char* key, int value);
 int fgxm_sync(fgxm_sim* sim);
 /* Bin start table of a spatial message list as last built. */
@@ -110,7 +112,8 @@ int fgxm_batch_remove(fgxm_batch* b, int32_t handle);
 /* Launch shape of the batched agent functions: block size (multiple of 64, <= 1024),
  * agents per thread, persistent blocks per CU (0: one block per tile). */
 int fgxm_batch_set_tuning(fgxm_batch* b, uint32_t block, uint32_t agents_per_thread, uint32_t blocks_per_cu);
-/* options: "graphs" (capture batched steps as hipGraphs, default 1). */
+/* options: "graphs" (capture batched steps as hipGraphs, default 1), "bins_per_radius"
+ * (1 or 2, as fgxm_set_option, for the simulations added afterwards). */
 int fgxm_batch_set_option(fgxm_batch* b, const char* key, int value);
 int fgxm_batch_info(fgxm_batch* b, uint32_t* resident, uint32_t* staged, uint64_t* resident_agents,
                     uint64_t* resident_bytes);

@@ -601,3 +601,12 @@ extern "C" double orc_sim_last_candidates(const orc_sim* s, int msg) {
     if (msg < 0 || msg >= (int)s->s->msgs.size()) return 0.0;
     return s->s->msgs[msg].last_candidates;
 }
+
+extern "C" int orc_sim_set_bin_mult(orc_sim* s, float mult) {
+    for (auto& m : s->s->msgs) {
+        if (!m.spatial) continue;
+        const orc_spatial_meta old = m.meta;
+        if (orc_spatial_meta_init_mult(old.dims, old.min, old.max, old.radius, mult, &m.meta)) return -1;
+    }
+    return 0;
+}

@@ -53,9 +53,17 @@ typedef struct {
     uint32_t grid[3];      /* grid[2] == 1 for 2-D */
     uint32_t bins;
     int32_t wrap_ok;
+    float cell;            /* bin width: radius x bin-width multiplier (radius when the multiplier is 1) */
+    int32_t span;          /* bins searched either side of the query's bin: ceil(1 / multiplier) */
 } orc_spatial_meta;
 
 int orc_spatial_meta_init(int dims, const float* mn, const float* mx, float radius, orc_spatial_meta* out);
+/* Bins of width radius x mult (mult = 1 or 1/2): the grid is ceil(width / (radius x mult)) and
+ * the iterators search span = ceil(1/mult) bins either side, in the reference's order
+ * (strips of 2*span+1 bins along x, dy outer, dz inner; wrapped cells dx, dy, dz).
+ * There is no multiplier in the reference; mult = 1 is its grid exactly. */
+int orc_spatial_meta_init_mult(int dims, const float* mn, const float* mx, float radius, float mult,
+                               orc_spatial_meta* out);
 uint32_t orc_spatial_hash(const orc_spatial_meta* m, float x, float y, float z);
 
 /* Stable counting sort of a message list by bin.  pbm has bins+1 entries.
@@ -111,6 +119,8 @@ int orc_sim_get_agents(const orc_sim* s, int t, void* const* vars, uint32_t* ids
 int orc_sim_step(orc_sim* s, uint32_t nsteps);
 /* PBM of message list `msg` as last built (bins+1 entries), and its bin count. */
 uint32_t orc_sim_bins(const orc_sim* s, int msg);
+/* Rebuild every spatial message grid with bin width radius x mult (before the first step). */
+int orc_sim_set_bin_mult(orc_sim* s, float mult);
 int orc_sim_get_pbm(const orc_sim* s, int msg, uint32_t* pbm);
 /* Mean candidates per agent of the last neighbour pass of message list msg. */
 double orc_sim_last_candidates(const orc_sim* s, int msg);

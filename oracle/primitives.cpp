@@ -61,11 +61,18 @@ static bool approx_exactly_divisible(float x, float y) {
     return v <= eps || v > y - eps;
 }
 
-extern "C" int orc_spatial_meta_init(int dims, const float* mn, const float* mx, float radius, orc_spatial_meta* m) {
+// With mult = 1 (orc_spatial_meta_init) the grid is the reference's: MessageSpatial3D.cu:31-52.
+// mult = 1/2 halves the bin width (an exact FP32 scaling), so the grid is ceil(width / (r/2))
+// and the searched neighbourhood widens to 2 bins either side.
+extern "C" int orc_spatial_meta_init_mult(int dims, const float* mn, const float* mx, float radius, float mult,
+                                          orc_spatial_meta* m) {
     if (dims != 2 && dims != 3) return -1;
+    if (!(mult == 1.0f || mult == 0.5f)) return -1;
     std::memset(m, 0, sizeof(*m));
     m->dims = dims;
     m->radius = radius;
+    m->cell = radius * mult;
+    m->span = mult == 1.0f ? 1 : 2;
     m->bins = 1;
     m->wrap_ok = 1;
     for (int a = 0; a < 3; ++a) {
@@ -73,10 +80,10 @@ extern "C" int orc_spatial_meta_init(int dims, const float* mn, const float* mx,
             m->min[a] = mn[a];
             m->max[a] = mx[a];
             m->width[a] = mx[a] - mn[a];
-            const float q = m->width[a] / radius;
+            const float q = m->width[a] / m->cell;
             m->grid[a] = (uint32_t)std::ceil(q);
             m->bins *= m->grid[a];
-            m->wrap_ok = m->wrap_ok && approx_exactly_divisible(m->width[a], radius);
+            m->wrap_ok = m->wrap_ok && approx_exactly_divisible(m->width[a], m->cell);
         } else {
             m->grid[a] = 1;
         }
@@ -84,9 +91,14 @@ extern "C" int orc_spatial_meta_init(int dims, const float* mn, const float* mx,
     return 0;
 }
 
-// getGridPosition3D/2D (MessageSpatial3DDevice.cuh:646-659): floor((p-min)/r), clamped to [0, grid-1].
+extern "C" int orc_spatial_meta_init(int dims, const float* mn, const float* mx, float radius, orc_spatial_meta* m) {
+    return orc_spatial_meta_init_mult(dims, mn, mx, radius, 1.0f, m);
+}
+
+// getGridPosition3D/2D (MessageSpatial3DDevice.cuh:646-659): floor((p-min)/r), clamped to [0, grid-1]
+// (r is the bin width, the radius when the multiplier is 1).
 static inline int grid_pos(const orc_spatial_meta* m, int a, float p) {
-    const int g = (int)std::floor((p - m->min[a]) / m->radius);
+    const int g = (int)std::floor((p - m->min[a]) / m->cell);
     const int hi = (int)m->grid[a] - 1;
     return g < 0 ? 0 : (g > hi ? hi : g);
 }
@@ -134,12 +146,12 @@ extern "C" int orc_spatial_build(const orc_spatial_meta* m, uint32_t n,
     return 0;
 }
 
-// Neighbour visit order.
+// Neighbour visit order, S = m->span (1 in the reference).
 //  Filter 3-D (MessageSpatial3DDevice.cuh:52-59,693-719): strips along x, dy outer, dz inner,
-//    strip = [PBM[hash(cx-1,..)], PBM[hash(cx+1,..)+1]), out-of-range y/z strips skipped.
-//  WrapFilter 3-D (:264-279,727-749): 27 single cells, dx outer, dy, dz inner, modulo grid.
-//  Filter 2-D (MessageSpatial2DDevice.cuh:51-53,646-672): 3 strips, dy ascending.
-//  WrapFilter 2-D (:253-260,679-698): 9 cells, dx outer, dy inner.
+//    strip = [PBM[hash(cx-S,..)], PBM[hash(cx+S,..)+1]), out-of-range y/z strips skipped.
+//  WrapFilter 3-D (:264-279,727-749): (2S+1)^3 single cells, dx outer, dy, dz inner, modulo grid.
+//  Filter 2-D (MessageSpatial2DDevice.cuh:51-53,646-672): 2S+1 strips, dy ascending.
+//  WrapFilter 2-D (:253-260,679-698): (2S+1)^2 cells, dx outer, dy inner.
 template <typename Visit>
 static void visit_candidates(const orc_spatial_meta* m, const uint32_t* pbm, int mode,
                              float x, float y, float z, Visit&& visit) {
@@ -147,43 +159,41 @@ static void visit_candidates(const orc_spatial_meta* m, const uint32_t* pbm, int
     const int cy = grid_pos(m, 1, y);
     const int cz = m->dims == 3 ? grid_pos(m, 2, z) : 0;
     const int gx = (int)m->grid[0], gy = (int)m->grid[1], gz = (int)m->grid[2];
+    const int S = m->span;
     if (mode == 0) {
         if (m->dims == 3) {
-            for (int dy = -1; dy <= 1; ++dy) {
-                for (int dz = -1; dz <= 1; ++dz) {
+            for (int dy = -S; dy <= S; ++dy) {
+                for (int dz = -S; dz <= S; ++dz) {
                     const int ay = cy + dy, az = cz + dz;
                     if (ay < 0 || az < 0 || ay >= gy || az >= gz) continue;
-                    const uint32_t s = pbm[hash_cell(m, cx - 1, ay, az)];
-                    const uint32_t e = pbm[hash_cell(m, cx + 1, ay, az) + 1];
+                    const uint32_t s = pbm[hash_cell(m, cx - S, ay, az)];
+                    const uint32_t e = pbm[hash_cell(m, cx + S, ay, az) + 1];
                     for (uint32_t i = s; i < e; ++i) visit(i);
                 }
             }
         } else {
-            for (int dy = -1; dy <= 1; ++dy) {
+            for (int dy = -S; dy <= S; ++dy) {
                 const int ay = cy + dy;
                 if (ay < 0 || ay >= gy) continue;
-                const uint32_t s = pbm[hash_cell(m, cx - 1, ay, 0)];
-                const uint32_t e = pbm[hash_cell(m, cx + 1, ay, 0) + 1];
+                const uint32_t s = pbm[hash_cell(m, cx - S, ay, 0)];
+                const uint32_t e = pbm[hash_cell(m, cx + S, ay, 0) + 1];
                 for (uint32_t i = s; i < e; ++i) visit(i);
             }
         }
     } else {
+        // modulo of a possibly negative cell (S may exceed the grid on tiny grids)
+        auto wrapc = [](int c, int g) { return ((c % g) + g) % g; };
         if (m->dims == 3) {
-            for (int dx = -1; dx <= 1; ++dx)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dz = -1; dz <= 1; ++dz) {
-                        const int ax = (cx + dx + gx) % gx;
-                        const int ay = (cy + dy + gy) % gy;
-                        const int az = (cz + dz + gz) % gz;
-                        const uint32_t h = hash_cell(m, ax, ay, az);
+            for (int dx = -S; dx <= S; ++dx)
+                for (int dy = -S; dy <= S; ++dy)
+                    for (int dz = -S; dz <= S; ++dz) {
+                        const uint32_t h = hash_cell(m, wrapc(cx + dx, gx), wrapc(cy + dy, gy), wrapc(cz + dz, gz));
                         for (uint32_t i = pbm[h]; i < pbm[h + 1]; ++i) visit(i);
                     }
         } else {
-            for (int dx = -1; dx <= 1; ++dx)
-                for (int dy = -1; dy <= 1; ++dy) {
-                    const int ax = (cx + dx + gx) % gx;
-                    const int ay = (cy + dy + gy) % gy;
-                    const uint32_t h = hash_cell(m, ax, ay, 0);
+            for (int dx = -S; dx <= S; ++dx)
+                for (int dy = -S; dy <= S; ++dy) {
+                    const uint32_t h = hash_cell(m, wrapc(cx + dx, gx), wrapc(cy + dy, gy), 0);
                     for (uint32_t i = pbm[h]; i < pbm[h + 1]; ++i) visit(i);
                 }
         }
@@ -212,9 +222,9 @@ extern "C" double orc_spatial_mean_candidates(const orc_spatial_meta* m, const u
         const uint32_t by = (b / m->grid[0]) % m->grid[1];
         const uint32_t bz = b / (m->grid[0] * m->grid[1]);
         // A query point at the centre of bin b lands in bin b.
-        const float qx = m->min[0] + ((float)bx + 0.5f) * m->radius;
-        const float qy = m->min[1] + ((float)by + 0.5f) * m->radius;
-        const float qz = m->dims == 3 ? m->min[2] + ((float)bz + 0.5f) * m->radius : 0.0f;
+        const float qx = m->min[0] + ((float)bx + 0.5f) * m->cell;
+        const float qy = m->min[1] + ((float)by + 0.5f) * m->cell;
+        const float qz = m->dims == 3 ? m->min[2] + ((float)bz + 0.5f) * m->cell : 0.0f;
         uint64_t c = 0;
         visit_candidates(m, pbm, mode, qx, qy, qz, [&](uint32_t) { ++c; });
         total += (double)nb * (double)c;

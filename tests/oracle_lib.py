@@ -20,7 +20,8 @@ f32p = ctypes.POINTER(ctypes.c_float)
 class SpatialMeta(ctypes.Structure):
     _fields_ = [("dims", ctypes.c_int32), ("min", ctypes.c_float * 3), ("max", ctypes.c_float * 3),
                 ("radius", ctypes.c_float), ("width", ctypes.c_float * 3), ("grid", ctypes.c_uint32 * 3),
-                ("bins", ctypes.c_uint32), ("wrap_ok", ctypes.c_int32)]
+                ("bins", ctypes.c_uint32), ("wrap_ok", ctypes.c_int32), ("cell", ctypes.c_float),
+                ("span", ctypes.c_int32)]
 
 
 def lib():
@@ -58,6 +59,7 @@ def lib():
         L.orc_sim_bins.restype = ctypes.c_uint32
         L.orc_sim_bins.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_sim_get_pbm.argtypes = [ctypes.c_void_p, ctypes.c_int, u32p]
+        L.orc_sim_set_bin_mult.argtypes = [ctypes.c_void_p, ctypes.c_float]
         L.orc_sim_last_candidates.restype = ctypes.c_double
         L.orc_sim_last_candidates.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _lib = L
@@ -84,11 +86,11 @@ def philox(ctr, key):
     return list(O)
 
 
-def spatial_meta(dims, mn, mx, radius):
+def spatial_meta(dims, mn, mx, radius, mult=1.0):
     m = SpatialMeta()
     mn = (ctypes.c_float * 3)(*(list(mn) + [0.0] * (3 - len(mn))))
     mx = (ctypes.c_float * 3)(*(list(mx) + [0.0] * (3 - len(mx))))
-    rc = lib().orc_spatial_meta_init(dims, mn, mx, ctypes.c_float(radius), ctypes.byref(m))
+    rc = lib().orc_spatial_meta_init_mult(dims, mn, mx, ctypes.c_float(radius), ctypes.c_float(mult), ctypes.byref(m))
     assert rc == 0
     return m
 
@@ -175,10 +177,12 @@ def stable_sort_perm(keys):
 class Sim:
     """Whole-model oracle simulation (model ids: 0 circles, 1 boids, 2 predator-prey, 3 circles brute force)."""
 
-    def __init__(self, model, params, seed):
+    def __init__(self, model, params, seed, bin_mult=1.0):
         p = _f32(params)
         self.h = lib().orc_sim_create(model, ptr(p, ctypes.c_float), len(p), seed)
         assert self.h, "bad oracle model"
+        if bin_mult != 1.0:
+            assert lib().orc_sim_set_bin_mult(self.h, bin_mult) == 0
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -166,6 +166,42 @@ def test_batched_circles_same_as_alone(tuning):
             assert np.array_equal(g, e), f"tuning {tuning}: simulation {h} differs from its unbatched run"
 
 
+def test_batched_half_width_bins_same_as_alone():
+    sims = _circles_sims()
+    b = Batch("circles_spatial3D", sims[0][0])
+    b.set_option("bins_per_radius", 2)
+    b.set_tuning(128, 2, 0)
+    hs = [b.add(p, seed, 5, {"Circle": a}) for (p, a, seed) in sims]
+    b.step(5)
+    for h, (p, a, seed) in zip(hs, sims):
+        got, gid = b.get_agents(h, "Circle")
+        s = Simulation("circles_spatial3D", p, seed=seed)
+        s.set_option("bins_per_radius", 2)
+        s.set_agents("Circle", a)
+        s.step(5)
+        exp, eid = s.get_agents("Circle")
+        assert np.array_equal(gid, eid)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e), f"simulation {h} differs from its unbatched half-width-bin run"
+
+
+def test_batched_half_width_boids_against_oracle():
+    sims = _boids_sims()
+    b = Batch("boids_spatial3D", sims[0][0])
+    b.set_option("bins_per_radius", 2)
+    hs = [b.add(p, seed, 3, {"Boid": a}) for (p, a, seed) in sims]
+    b.step(3)
+    for h, (p, a, seed) in zip(hs, sims):
+        ref = O.Sim(1, p, seed, bin_mult=0.5)
+        ref.set_agents(0, a)
+        ref.step(3)
+        got, gid = b.get_agents(h, "Boid")
+        exp, eid = ref.get_agents(0, [np.float32] * 6)
+        assert np.array_equal(gid, eid)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e)
+
+
 def test_batched_join_and_leave_same_as_alone():
     sims = _circles_sims()
     b = Batch("circles_spatial3D", sims[0][0])

@@ -129,3 +129,64 @@ def test_graph_and_eager_agree():
     for x, y in zip(ga, gb):
         assert np.array_equal(x, y)
     assert a.step_counter() == b.step_counter() == 7
+
+
+# ---------------------------------------------------------------- half-width bins
+# Bin-width multiplier 1/2 (option "bins_per_radius" = 2; no counterpart in the
+# reference): the oracle runs the same finer grid and visit order (orc_sim_set_bin_mult).
+def _run_half(model, oracle_model, params, arrs, name, steps, seed):
+    sim = Simulation(model, params, seed=seed)
+    sim.set_option("bins_per_radius", 2)
+    ref = O.Sim(oracle_model, params, seed, bin_mult=0.5)
+    sim.set_agents(name, arrs)
+    ref.set_agents(0, arrs)
+    sim.step(steps)
+    ref.step(steps)
+    return sim, ref
+
+
+def test_half_width_bins_circles():
+    params, arrs = workloads.circles(16384, seed=12)
+    sim, ref = _run_half("circles_spatial3D", 0, params, arrs, "Circle", 1, 12)
+    pbm = sim.pbm("location")
+    assert len(pbm) == 26 ** 3 + 1   # ceil(25.398 / 1.0) bins per axis
+    assert np.array_equal(pbm, ref.pbm(0))
+    _compare(sim, ref, 0, "Circle", exact=False)
+
+
+@pytest.mark.parametrize("n,steps", [(32768, 4), (1 << 20, 1)])
+def test_half_width_bins_boids_bit_exact(n, steps):
+    params, arrs = workloads.boids(n, seed=7)
+    sim, ref = _run_half("boids_spatial3D", 1, params, arrs, "Boid", steps, 7)
+    assert len(sim.pbm("location")) == 128 ** 3 + 1
+    assert np.array_equal(sim.pbm("location"), ref.pbm(0))
+    _compare(sim, ref, 0, "Boid", exact=True)
+
+
+def test_half_width_bins_agree_with_full_bins():
+    # same in-radius neighbours, different visit order: FP32 agreement after one step
+    params, arrs = workloads.boids(32768, seed=8)
+    a = Simulation("boids_spatial3D", params, seed=8)
+    b = Simulation("boids_spatial3D", params, seed=8)
+    b.set_option("bins_per_radius", 2)
+    a.set_agents("Boid", arrs)
+    b.set_agents("Boid", arrs)
+    a.step(1)
+    b.step(1)
+    (ga, ia), (gb, ib) = a.get_agents("Boid"), b.get_agents("Boid")
+    oa, ob = np.argsort(ia), np.argsort(ib)
+    assert np.array_equal(ia[oa], ib[ob])
+    for x, y in zip(ga, gb):
+        np.testing.assert_allclose(x[oa], y[ob], rtol=1e-5, atol=1e-6)
+
+
+def test_half_width_bins_only_before_first_step():
+    from flamegpu2_amd.fgx import FgxError
+    params, arrs = workloads.boids(1000, seed=8)
+    s = Simulation("boids_spatial3D", params, seed=8)
+    s.set_agents("Boid", arrs)
+    s.step(1)
+    with pytest.raises(FgxError, match="before the first step"):
+        s.set_option("bins_per_radius", 2)
+    with pytest.raises(FgxError, match="1 or 2"):
+        s.set_option("bins_per_radius", 3)

@@ -101,15 +101,16 @@ def test_spatial_build_matches_counting_sort(dims, n, width, radius, seed):
 
 
 def _bin_coords(meta, p):
-    r = np.float32(meta.radius)
+    r = np.float32(meta.cell)
     return [np.clip(np.floor((np.float32(p[a]) - np.float32(meta.min[a])) / r).astype(np.int64), 0, meta.grid[a] - 1)
             for a in range(meta.dims)]
 
 
-def _check_filter_against_bruteforce(dims, n, width, radius, seed, nq=300):
+def _check_filter_against_bruteforce(dims, n, width, radius, seed, nq=300, mult=1.0):
     rng = np.random.default_rng(seed)
     P = rng.uniform(0, width, size=(dims, n)).astype(np.float32)
-    m = O.spatial_meta(dims, [0.0] * dims, [float(width)] * dims, radius)
+    m = O.spatial_meta(dims, [0.0] * dims, [float(width)] * dims, radius, mult)
+    S = m.span
     pbm, perm, _ = O.spatial_build(m, P[0], P[1], P[2] if dims == 3 else None)
     sortedP = P[:, perm]
     cells = np.stack(_bin_coords(m, sortedP))
@@ -117,12 +118,12 @@ def _check_filter_against_bruteforce(dims, n, width, radius, seed, nq=300):
         qp = P[:, q]
         qc = np.array([c[0] for c in _bin_coords(m, qp[:, None])])
         got = O.candidates(m, pbm, 0, *[float(v) for v in qp])
-        # brute force: every message whose (clamped) bin is within +-1 on every axis
-        mask = np.all(np.abs(cells - qc[:, None]) <= 1, axis=0)
+        # brute force: every message whose (clamped) bin is within +-S on every axis
+        mask = np.all(np.abs(cells - qc[:, None]) <= S, axis=0)
         assert sorted(got.tolist()) == np.nonzero(mask)[0].tolist()
         # visit order: strips (dy outer, dz inner), ascending x then stable within bin
         if dims == 3:
-            strip_key = (cells[1, got] - qc[1]) * 3 + (cells[2, got] - qc[2])
+            strip_key = (cells[1, got] - qc[1]) * (2 * S + 1) + (cells[2, got] - qc[2])
         else:
             strip_key = cells[1, got] - qc[1]
         assert np.all(np.diff(strip_key) >= 0)
@@ -188,6 +189,36 @@ def test_wrapped_3d_known_answer(offset):
         ok = (dist <= np.float32(3.5)) & np.all((np.abs(d21) == 2.0) | (v == qp[:, None]), axis=0)
         assert ok.sum() == 27
         assert np.all(d21[:, ok].sum(axis=1) == 0)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_filter_half_width_bins(dims):
+    """Bin-width multiplier 1/2 (no reference counterpart): bins of width r/2, searched 2
+    bins either side, same strip order; still a superset of the in-radius set."""
+    m = O.spatial_meta(dims, [0.0] * dims, [5.0] * dims, 1.0, 0.5)
+    assert m.span == 2 and m.cell == 0.5 and list(m.grid)[:dims] == [10] * dims
+    _check_filter_against_bruteforce(dims, 3000, 5.0, 1.0, seed=4, nq=200, mult=0.5)
+    _check_filter_against_bruteforce(dims, 3000, float(np.float32(np.cbrt(16384.0))), 2.0, seed=5, nq=100, mult=0.5)
+
+
+def test_half_width_grid_dims():
+    # ceil(width / (r/2)) in FP32: 25.398/1 -> 26 (= 2 x 13); 12.3/0.5 -> 25 (not 2 x ceil(12.3) = 26)
+    m = O.spatial_meta(3, [0, 0, 0], [float(np.float32(np.cbrt(16384.0)))] * 3, 2.0, 0.5)
+    assert list(m.grid) == [26, 26, 26]
+    m = O.spatial_meta(3, [0, 0, 0], [12.3, 12.3, 12.3], 1.0, 0.5)
+    assert list(m.grid) == [25, 25, 25] and not m.wrap_ok
+
+
+def test_wrap_half_width_order():
+    m = O.spatial_meta(3, [0, 0, 0], [4, 4, 4], 1.0, 0.5)
+    assert m.wrap_ok and list(m.grid) == [8, 8, 8]
+    c = np.arange(8, dtype=np.float32) * np.float32(0.5) + np.float32(0.25)
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    pbm, perm, _ = O.spatial_build(m, X.ravel(), Y.ravel(), Z.ravel())
+    got = O.candidates(m, pbm, 1, 0.25, 0.25, 0.25)
+    r = range(-2, 3)
+    expect = [((dz % 8) * 64 + (dy % 8) * 8 + (dx % 8)) for dx in r for dy in r for dz in r]
+    assert got.tolist() == expect
 
 
 def test_wrap_order_is_dx_dy_dz():
@@ -275,6 +306,22 @@ def test_circles_spatial_equals_bruteforce():
     assert np.array_equal(ida[oa], idb[ob])
     for k in range(4):
         np.testing.assert_allclose(a[k][oa], b[k][ob], rtol=1e-5, atol=1e-6)
+
+
+def test_circles_half_width_bins_match_full_bins():
+    # The same in-radius neighbours are found with half-width bins; only the visit order
+    # (so the FP32 summation order) differs.
+    from flamegpu2_amd import workloads
+    params, arrs = workloads.circles(3000, 12)
+    a = O.Sim(0, params, 12)
+    b = O.Sim(0, params, 12, bin_mult=0.5)
+    a.set_agents(0, arrs); b.set_agents(0, arrs)
+    a.step(2); b.step(2)
+    (x, ia), (y, ib) = a.get_agents(0, [np.float32] * 4), b.get_agents(0, [np.float32] * 4)
+    oa, ob = np.argsort(ia), np.argsort(ib)
+    assert np.array_equal(ia[oa], ib[ob])
+    for k in range(4):
+        np.testing.assert_allclose(x[k][oa], y[k][ob], rtol=1e-5, atol=1e-6)
 
 
 def test_circles_isolated_agent_stays():
