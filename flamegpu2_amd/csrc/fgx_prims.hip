@@ -231,49 +231,9 @@ __global__ void __launch_bounds__(kBlock) k_stable_scatter(const uint32_t* d_n, 
     }
 }
 
-// ================================================================ single-pass exclusive scan
-// Blocked arrangement: thread t owns items [tile*2048 + t*8, +8).
-__global__ void __launch_bounds__(kBlock) k_scan_excl(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                      uint32_t n, LookbackHeader* hdr, uint64_t* status,
-                                                      uint32_t num_tiles) {
-    __shared__ uint32_t s_tile, s_epoch, s_prefix;
-    __shared__ uint32_t s_wave[kBlock / 64];
-    acquire_tile(hdr, num_tiles, &s_tile, &s_epoch);
-    const uint32_t tile = s_tile;
-    const uint32_t base = tile * kTile + threadIdx.x * kItems;
-    uint32_t v[kItems];
-    uint32_t sum = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kItems; ++j) {
-        v[j] = (base + j < n) ? in[base + j] : 0u;
-        sum += v[j];
-    }
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    const uint32_t incl = wave_inclusive_u32(sum);
-    if (lane == 63) s_wave[w] = incl;
-    __syncthreads();
-    uint32_t wave_off = 0, total = 0;
-    for (uint32_t k = 0; k < kBlock / 64; ++k) {
-        if (k < w) wave_off += s_wave[k];
-        total += s_wave[k];
-    }
-    if (w == 0) {
-        const uint32_t p = lookback_wave0(status, tile, s_epoch, total, hdr);
-        if (lane == 0) s_prefix = p;
-    }
-    __syncthreads();
-    uint32_t run = s_prefix + wave_off + incl - sum;
-#pragma unroll
-    for (uint32_t j = 0; j < kItems; ++j) {
-        if (base + j < n) out[base + j] = run;
-        run += v[j];
-    }
-    retire_tile(hdr, num_tiles);
-}
-
-// ================================================================ two-pass exclusive scan
+// ================================================================ kernel-boundary exclusive scan
 // For up to kTwoPassTiles tiles: per-tile sums, then every tile adds the sums of the
-// tiles before it.  The kernel boundary orders the two passes, where the single-pass
+// tiles before it.  The kernel boundary orders the two passes, where a single-pass
 // scan's look-back waits on status words written by other CUs.
 constexpr uint32_t kTwoPassTiles = 1024;
 
