@@ -356,6 +356,18 @@ def main():
         },
         "kernels_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in sorted(kt.items())},
     }
+    # PCIe-inclusive rate (not `value`): the same steps plus uploading the population
+    # from host arrays and reading the final state back (fgxm_set_agents / fgxm_get_agents)
+    sim.sync()
+    t_dn = time.perf_counter()
+    sim.get_agents("Boid")
+    t_dn = time.perf_counter() - t_dn
+    t_up = time.perf_counter()
+    sim.set_agents("Boid", arrs)
+    sim.sync()
+    t_up = time.perf_counter() - t_up
+    out["pcie_inclusive"] = {"upload_s": round(t_up, 5), "download_s": round(t_dn, 5),
+                             "agent_steps_per_s": n * args.steps / (ms / 1e3 + t_up + t_dn)}
     sim.close()
     if args.batched:
         print("bench: batched ensemble", file=sys.stderr, flush=True)

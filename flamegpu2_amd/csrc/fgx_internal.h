@@ -89,107 +89,15 @@ __device__ __forceinline__ uint32_t wave_inclusive_u32(uint32_t v) {
     return v;
 }
 
-// ---------------------------------------------------------------- decoupled look-back
-// Tile t is block t.  Workgroups are dispatched in index order on each XCD, so
-// the lowest unfinished tile is always resident and the chain of waits makes
-// progress.  Tile status words pack {flag:2, epoch:30, value:32}; the epoch
-// advances when the last block of a launch retires, so words from an older
-// launch read as "not yet published" and the status array never needs
-// clearing between launches (graph replays included).  The header must be
-// zeroed once before first use.  Spins are bounded: a tile that waits too long
-// records an error in the header instead of hanging the device.
-struct LookbackHeader {
-    uint32_t done;
-    uint32_t epoch;
-    uint32_t error;
-    uint32_t pad[61];
+// ---------------------------------------------------------------- scan / compaction scratch
+// Scans and compactions run as passes ordered by kernel boundaries, with no waits
+// between blocks.  Their scratch starts with this 256-byte header (unused words kept
+// so that the per-tile words after it stay 256-byte aligned), then 8 bytes per tile:
+// tile sums, or a compaction's stashed inputs and per-tile counts.
+struct ScratchHeader {
+    uint32_t pad[64];
 };
-static_assert(sizeof(LookbackHeader) == 256, "header keeps the status array 256-B aligned");
-
-constexpr uint64_t kFlagAggregate = 1ull;
-constexpr uint64_t kFlagInclusive = 2ull;
-
-__device__ __forceinline__ uint64_t pack_status(uint64_t flag, uint32_t epoch, uint32_t value) {
-    return (flag << 62) | ((uint64_t)(epoch & 0x3FFFFFFFu) << 32) | value;
-}
-
-__device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Called by every thread of the block at kernel start: the tile is blockIdx.x,
-// the epoch is read once and broadcast through shared memory.
-__device__ __forceinline__ void acquire_tile(LookbackHeader* hdr, uint32_t num_tiles, uint32_t* s_tile, uint32_t* s_epoch) {
-    (void)num_tiles;
-    if (threadIdx.x == 0) {
-        const uint32_t ep = __hip_atomic_load(&hdr->epoch, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        *s_tile = blockIdx.x;
-        *s_epoch = ep & 0x3FFFFFFFu;
-    }
-    __syncthreads();
-}
-
-// Called by every thread of the block after its last status access.  The block
-// that retires last (all others have read the epoch and finished) advances it.
-__device__ __forceinline__ void retire_tile(LookbackHeader* hdr, uint32_t num_tiles) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t d = __hip_atomic_fetch_add(&hdr->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (d == num_tiles - 1) {
-            __hip_atomic_store(&hdr->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&hdr->epoch, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-}
-
-constexpr uint32_t kMaxSpins = 1u << 22;
-
-// Exclusive prefix of `aggregate` over tiles 0..tile-1.  Executed by wave 0 of
-// the block (all 64 lanes, uniform control flow); returns the prefix in every lane.
-__device__ __forceinline__ uint32_t lookback_wave0(uint64_t* status, uint32_t tile, uint32_t epoch, uint32_t aggregate,
-                                                   LookbackHeader* hdr) {
-    const uint32_t lane = lane_id();
-    if (tile == 0) {
-        if (lane == 0) status_store(&status[0], pack_status(kFlagInclusive, epoch, aggregate));
-        return 0;
-    }
-    if (lane == 0) status_store(&status[tile], pack_status(kFlagAggregate, epoch, aggregate));
-    uint32_t excl = 0;
-    int64_t top = (int64_t)tile - 1;
-    uint32_t spins = 0;
-    while (true) {
-        const int64_t idx = top - (int64_t)lane;
-        uint64_t s;
-        uint32_t flag;
-        if (idx >= 0) {
-            s = status_load(&status[idx]);
-            flag = (((s >> 32) & 0x3FFFFFFFu) == epoch) ? (uint32_t)(s >> 62) : 0u;
-        } else {
-            s = 0;
-            flag = (uint32_t)kFlagInclusive;   // before tile 0: inclusive prefix 0
-        }
-        const uint64_t incl = __ballot(flag == kFlagInclusive);
-        const uint64_t valid = __ballot(flag != 0);
-        const int j = incl ? (int)__builtin_ctzll(incl) : 63;
-        const uint64_t need = (j == 63) ? ~0ull : ((2ull << j) - 1ull);
-        if ((valid & need) != need) {
-            if (++spins > kMaxSpins) {
-                if (lane == 0) __hip_atomic_store(&hdr->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        excl += wave_sum_u32((int)lane <= j ? (uint32_t)s : 0u);
-        if (incl) break;
-        top -= 64;
-    }
-    if (lane == 0) status_store(&status[tile], pack_status(kFlagInclusive, epoch, excl + aggregate));
-    return excl;
-}
+static_assert(sizeof(ScratchHeader) == 256, "header keeps the per-tile words 256-B aligned");
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 

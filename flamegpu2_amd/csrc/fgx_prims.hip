@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles_pre(const uint32_t* __res
 // inter-block waits): up to kTwoPassTiles tiles, tile sums then a pass in which each
 // tile adds up the sums before it; beyond that, tile sums, one block scanning them,
 // then a pass that reads each tile's prefix.  `status` holds the tile sums.
-static int scan_counts(const uint32_t* in, uint32_t* out, uint32_t n, LookbackHeader* hdr, uint64_t* status, hipStream_t st) {
+static int scan_counts(const uint32_t* in, uint32_t* out, uint32_t n, ScratchHeader* hdr, uint64_t* status, hipStream_t st) {
     (void)hdr;
     const uint32_t tiles = ceil_div(n, kTile);
     if (!tiles) return FGX_OK;
@@ -369,8 +369,6 @@ struct CompactArgs {
     const uint32_t* d_n;
     uint32_t n_cap;
     uint32_t num_tiles;
-    LookbackHeader* hdr;
-    uint64_t* status;
     // compaction output
     uint32_t* d_count_out;
     // births (mode 1)
@@ -384,42 +382,81 @@ struct CompactArgs {
     // flag k - *d_keep_prefix, and the first *d_keep_prefix items are kept unconditionally
     const uint32_t* d_src_off;
     const uint32_t* d_keep_prefix;
+    // scratch: the inputs as read by the first pass, then the per-tile kept counts
+    uint32_t* stash;           // [0] items n, [1] source offset, [2] kept prefix, [3] dst count, [4] next ID
+    uint32_t* counts;          // num_tiles words
 };
 
-__global__ void __launch_bounds__(kBlock) k_compact(CompactArgs a, VarTable vt) {
-    __shared__ uint32_t s_tile, s_epoch, s_prefix, s_dst0, s_id0, s_off, s_pre;
-    __shared__ uint32_t s_cnt[kItems * (kBlock / 64)];
-    if (threadIdx.x == 0) {
-        if (a.births) {
-            s_dst0 = __hip_atomic_load(a.d_dst_count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            s_id0 = a.d_next_id ? __hip_atomic_load(a.d_next_id, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-        }
-        s_off = a.d_src_off ? __hip_atomic_load(a.d_src_off, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-        s_pre = a.d_keep_prefix ? __hip_atomic_load(a.d_keep_prefix, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-    }
-    acquire_tile(a.hdr, a.num_tiles, &s_tile, &s_epoch);   // (its barrier publishes the values above)
-    const uint32_t tile = s_tile;
-    const uint32_t n_all = load_count(a.d_n, a.n_cap);
-    const uint32_t off = s_off < n_all ? s_off : n_all;
-    const uint32_t n = n_all - off;   // items k in [0, n), source element off + k
-    const uint32_t pre = s_pre;
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    const uint64_t lanes_below = (1ull << lane) - 1ull;
-    uint64_t masks[kItems];
+// Kept-item masks of tile `tile`, in the striped arrangement, for items [0, n).
+__device__ __forceinline__ void compact_masks(const CompactArgs& a, uint32_t tile, uint32_t n, uint32_t pre,
+                                              uint64_t (&masks)[kItems]) {
 #pragma unroll
     for (uint32_t j = 0; j < kItems; ++j) {
         const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
         const bool keep = i < n && (i < pre || !a.flags || ((a.flags[i - pre] != 0u) == (a.keep != 0u)));
         masks[j] = __ballot(keep);
-        if (lane == 0) s_cnt[j * (kBlock / 64) + w] = (uint32_t)__builtin_popcountll(masks[j]);
     }
+}
+
+// Pass 1: kept items per tile.  Block 0 also records the device-resident inputs, so that
+// pass 2 does not read counters that its own last tile rewrites (the output count may
+// be the input count).
+__global__ void __launch_bounds__(kBlock) k_compact_counts(CompactArgs a) {
+    __shared__ uint32_t s_cnt[kBlock / 64];
+    const uint32_t n_all = load_count(a.d_n, a.n_cap);
+    const uint32_t off0 = a.d_src_off ? __hip_atomic_load(a.d_src_off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    const uint32_t off = off0 < n_all ? off0 : n_all;
+    const uint32_t n = n_all - off;
+    const uint32_t pre = a.d_keep_prefix ? __hip_atomic_load(a.d_keep_prefix, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.stash[0] = n;
+        a.stash[1] = off;
+        a.stash[2] = pre;
+        a.stash[3] = a.births ? __hip_atomic_load(a.d_dst_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        a.stash[4] = a.births && a.d_next_id ? __hip_atomic_load(a.d_next_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    }
+    uint64_t masks[kItems];
+    compact_masks(a, blockIdx.x, n, pre, masks);
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) c += (uint32_t)__builtin_popcountll(masks[j]);
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 0) s_cnt[w] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t k = 0; k < kBlock / 64; ++k) t += s_cnt[k];
+        a.counts[blockIdx.x] = t;
+    }
+}
+
+// Pass 2 (after k_scan_sums when `scanned`): each tile's prefix, its items' ranks, and
+// the copy of every variable; the last tile writes the output counts.
+__global__ void __launch_bounds__(kBlock) k_compact_scatter(CompactArgs a, VarTable vt, uint32_t scanned) {
+    __shared__ uint32_t s_prefix;
+    __shared__ uint32_t s_cnt[kItems * (kBlock / 64)];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t n = a.stash[0], off = a.stash[1], pre = a.stash[2], dst0 = a.stash[3], id0 = a.stash[4];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
+    uint64_t masks[kItems];
+    compact_masks(a, tile, n, pre, masks);
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j)
+        if (lane == 0) s_cnt[j * (kBlock / 64) + w] = (uint32_t)__builtin_popcountll(masks[j]);
     __syncthreads();
     if (w == 0) {
-        // exclusive scan of the 32 (j, wave) counts in index order, then look-back
+        // exclusive scan of the 32 (j, wave) counts in index order; the tile's prefix
         const uint32_t c = lane < kItems * (kBlock / 64) ? s_cnt[lane] : 0u;
         const uint32_t inc = wave_inclusive_u32(c);
         const uint32_t total = __shfl(inc, 63, 64);
-        const uint32_t p = lookback_wave0(a.status, tile, s_epoch, total, a.hdr);
+        uint32_t p = 0;
+        if (scanned) {
+            p = a.counts[tile];
+        } else {
+            for (uint32_t k = lane; k < tile; k += 64) p += a.counts[k];
+            p = wave_sum_u32(p);
+        }
         if (lane < kItems * (kBlock / 64)) s_cnt[lane] = inc - c;
         if (lane == 0) s_prefix = p;
         if (lane == 0 && tile == a.num_tiles - 1) {
@@ -427,13 +464,13 @@ __global__ void __launch_bounds__(kBlock) k_compact(CompactArgs a, VarTable vt) 
             if (!a.births) {
                 *a.d_count_out = kept;
             } else {
-                uint32_t nd = s_dst0 + kept;
+                uint32_t nd = dst0 + kept;
                 if (nd > a.dst_capacity) {
                     nd = a.dst_capacity;
                     if (a.d_overflow) *a.d_overflow = 1u;
                 }
                 *a.d_dst_count = nd;
-                if (a.d_next_id) *a.d_next_id = s_id0 + kept;
+                if (a.d_next_id) *a.d_next_id = id0 + kept;
             }
         }
     }
@@ -447,13 +484,12 @@ __global__ void __launch_bounds__(kBlock) k_compact(CompactArgs a, VarTable vt) 
         if (!a.births) {
             copy_element(vt, off + i, r);
         } else {
-            const uint32_t d = s_dst0 + r;
+            const uint32_t d = dst0 + r;
             if (d >= a.dst_capacity) continue;
             copy_element(vt, off + i, d, a.id_var);
-            if (a.id_var >= 0) reinterpret_cast<uint32_t*>(vt.out[a.id_var])[d] = s_id0 + r;
+            if (a.id_var >= 0) reinterpret_cast<uint32_t*>(vt.out[a.id_var])[d] = id0 + r;
         }
     }
-    retire_tile(a.hdr, a.num_tiles);
 }
 
 // ================================================================ deterministic reduction
@@ -666,14 +702,14 @@ static BuildLayout build_layout(uint32_t n, uint32_t bins) {
     L.key = off; off += align256((size_t)std::max(n, 1u) * 4);
     L.sub = off; off += align256((size_t)std::max(n, 1u) * 4);
     L.tmp = off; off += align256((size_t)std::max(n, 1u) * 4);
-    L.hdr = off; off += sizeof(LookbackHeader);
+    L.hdr = off; off += sizeof(ScratchHeader);
     L.status = off; off += align256((size_t)ceil_div(bins + 1, kTile) * 8);
     L.total = off;
     return L;
 }
 
-static size_t lookback_bytes(uint32_t n) {
-    return sizeof(LookbackHeader) + align256((size_t)std::max(ceil_div(n, kTile), 1u) * 8);
+static size_t tile_scratch_bytes(uint32_t n) {
+    return sizeof(ScratchHeader) + align256((size_t)std::max(ceil_div(n, kTile), 1u) * 8);
 }
 
 static uint32_t stream_grid(const Ctx* c, uint32_t n) {
@@ -688,8 +724,8 @@ using namespace fgx;
 
 static int run_scan(const uint32_t* in, uint32_t* out, uint32_t n, char* scratch, void* stream) {
     if (!n) return FGX_OK;
-    auto* hdr = reinterpret_cast<LookbackHeader*>(scratch);
-    auto* status = reinterpret_cast<uint64_t*>(scratch + sizeof(LookbackHeader));
+    auto* hdr = reinterpret_cast<ScratchHeader*>(scratch);
+    auto* status = reinterpret_cast<uint64_t*>(scratch + sizeof(ScratchHeader));
     return scan_counts(in, out, n, hdr, status, (hipStream_t)stream);
 }
 
@@ -715,7 +751,7 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
     }
     // scan hist -> start (bins+1 entries, start[bins] = n)
     {
-        auto* hdr = reinterpret_cast<LookbackHeader*>(s + L.hdr);
+        auto* hdr = reinterpret_cast<ScratchHeader*>(s + L.hdr);
         auto* status = reinterpret_cast<uint64_t*>(s + L.status);
         rc = scan_counts(hist, start_out, bins + 1, hdr, status, st);
         if (rc) return rc;
@@ -750,7 +786,7 @@ static int run_bin_sort_seg(fgx_ctx* ctx, int32_t mode, const fgx_seg_key* d_seg
         FGX_HIP(hipGetLastError());
     }
     {
-        auto* hdr = reinterpret_cast<LookbackHeader*>(s + L.hdr);
+        auto* hdr = reinterpret_cast<ScratchHeader*>(s + L.hdr);
         auto* status = reinterpret_cast<uint64_t*>(s + L.status);
         rc = scan_counts(hist, start_out, keys_total + 1, hdr, status, st);
         if (rc) return rc;
@@ -813,7 +849,7 @@ int fgx_scratch_bytes(int op, uint32_t n, uint32_t bins, size_t* bytes) {
         case FGX_OP_COMPACT:
         case FGX_OP_BIRTHS:
         case FGX_OP_SCAN:
-            *bytes = lookback_bytes(n) + align256(kReduceBlocks * 8);
+            *bytes = tile_scratch_bytes(n) + align256(kReduceBlocks * 8);
             return FGX_OK;
         case FGX_OP_STATS:
             *bytes = align256(4 * kReduceBlocks * 8);
@@ -932,17 +968,28 @@ int fgx_agent_sort(fgx_ctx* ctx, int dims, const float* mn, const float* mx, flo
 int fgx_exclusive_scan_u32(fgx_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
                            void* scratch, size_t scratch_bytes, void* stream) {
     (void)ctx;
-    if (!scratch || scratch_bytes < lookback_bytes(n)) return fail(FGX_ERR_CAPACITY, "fgx_exclusive_scan_u32: scratch too small");
+    if (!scratch || scratch_bytes < tile_scratch_bytes(n)) return fail(FGX_ERR_CAPACITY, "fgx_exclusive_scan_u32: scratch too small");
     return run_scan(in, out, n, static_cast<char*>(scratch), stream);
 }
 
+// Stable compaction in passes ordered by kernel boundaries: kept counts per tile, a
+// block-level scan of them when there are more than kTwoPassTiles tiles (otherwise each
+// tile adds up the counts before it), then ranks and copies.
 static int launch_compact(CompactArgs a, const VarTable& vt, void* scratch, size_t scratch_bytes, void* stream) {
-    if (!scratch || scratch_bytes < lookback_bytes(a.n_cap)) return fail(FGX_ERR_CAPACITY, "compaction scratch too small");
-    char* s = static_cast<char*>(scratch);
-    a.hdr = reinterpret_cast<LookbackHeader*>(s);
-    a.status = reinterpret_cast<uint64_t*>(s + sizeof(LookbackHeader));
+    if (!scratch || scratch_bytes < tile_scratch_bytes(a.n_cap)) return fail(FGX_ERR_CAPACITY, "compaction scratch too small");
+    uint32_t* words = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) + sizeof(ScratchHeader));
+    a.stash = words;          // 8 words
+    a.counts = words + 8;     // num_tiles words: the region holds 8 bytes per tile, at least 256
     a.num_tiles = std::max(ceil_div(a.n_cap, kTile), 1u);
-    hipLaunchKernelGGL(k_compact, dim3(a.num_tiles), dim3(kBlock), 0, (hipStream_t)stream, a, vt);
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_compact_counts, dim3(a.num_tiles), dim3(kBlock), 0, st, a);
+    FGX_HIP(hipGetLastError());
+    const uint32_t scanned = a.num_tiles > kTwoPassTiles ? 1u : 0u;
+    if (scanned) {
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kSumsBlock), 0, st, a.counts, a.num_tiles);
+        FGX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_compact_scatter, dim3(a.num_tiles), dim3(kBlock), 0, st, a, vt, scanned);
     FGX_HIP(hipGetLastError());
     return FGX_OK;
 }
@@ -1097,9 +1144,9 @@ int fgx_reduce_stats(fgx_ctx* ctx, int type, const void* in, const uint32_t* d_n
 int fgx_reduce_sum(fgx_ctx* ctx, int type, const void* in, const uint32_t* d_n, uint32_t n_cap,
                    void* d_out, void* scratch, size_t scratch_bytes, void* stream) {
     (void)ctx;
-    const size_t need = lookback_bytes(n_cap) + align256(kReduceBlocks * 8);
+    const size_t need = tile_scratch_bytes(n_cap) + align256(kReduceBlocks * 8);
     if (!scratch || scratch_bytes < need) return fail(FGX_ERR_CAPACITY, "fgx_reduce_sum: scratch too small");
-    char* part = static_cast<char*>(scratch) + lookback_bytes(n_cap);
+    char* part = static_cast<char*>(scratch) + tile_scratch_bytes(n_cap);
     hipStream_t st = (hipStream_t)stream;
     switch (type) {
         case 0:

@@ -184,9 +184,10 @@ int fgx_agent_sort(fgx_ctx* ctx, int dims, const float* min, const float* max, f
  * Replaces CUDAScatter::scatter + the death scan (scatter_generic,
  * src/flamegpu/simulation/detail/CUDAScatter.cu:68-88,140-179;
  * CUDAFatAgent::processDeath, src/flamegpu/simulation/detail/CUDAFatAgent.cu:105-137):
- * one pass (decoupled look-back scan fused with the scatter) that keeps the
- * items whose flag != 0 equals keep != 0, stably, over every variable.  The
- * kept count is written to *d_count_out (device memory). */
+ * keeps the items whose flag != 0 equals keep != 0, stably, over every variable,
+ * in passes ordered by kernel boundaries (per-tile counts, their scan, then ranks
+ * and copies; no waits between blocks).  The kept count is written to
+ * *d_count_out (device memory), which may be d_n itself. */
 int fgx_compact(fgx_ctx* ctx, const uint32_t* flags, uint32_t keep, const uint32_t* d_n, uint32_t n_cap,
                 uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
                 uint32_t* d_count_out, void* scratch, size_t scratch_bytes, void* stream);

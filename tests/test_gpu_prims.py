@@ -39,7 +39,7 @@ def _build_case(ctx, dims, n, width, radius, seed, use_dn=False, spread=0.0):
         cnt = dev(np.array([n], np.uint32))
         d_n = ptr(cnt)
         cap = n  # capacity == n; the count is read on the device
-    for _ in range(2):   # second call reuses the scratch (look-back epoch)
+    for _ in range(2):   # the second call reuses the scratch as the first left it
         fgx.spatial_build(ctx, m, cap, ptr(d_in[0]), ptr(d_in[1]), ptr(d_in[2]) if dims == 3 else None,
                           [ptr(t) for t in d_in], [ptr(t) for t in d_out], nbytes, ptr(pbm), ptr(s), sb, d_n=d_n)
     sync()
