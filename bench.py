@@ -185,8 +185,9 @@ def measured_traffic():
 
 
 def traffic_probe(args):
-    """The program that the FETCH_SIZE / WRITE_SIZE passes profile: three eager boids
-    steps at the bench size, then three launches of a plain elementwise kernel over
+    """The program that the FETCH_SIZE / WRITE_SIZE passes profile: six eager boids
+    steps at the bench size (with LDS neighbourhood staging when --staging), then
+    three launches of a plain elementwise kernel over
     COPY_BYTES (negation: a copy kernel with known traffic), which calibrates the
     counters."""
     import torch
@@ -195,8 +196,9 @@ def traffic_probe(args):
     params, arrs = workloads.boids(args.agents, seed=7)
     sim = Simulation("boids_spatial3D", params, seed=7)
     sim.set_option("graphs", 0)
+    sim.set_option("staging", 1 if args.staging else 0)
     sim.set_agents("Boid", arrs)
-    sim.step(3)
+    sim.step(6)   # the first steps sort a random population; the last ones are the steady state
     sim.sync()
     sim.close()
     print("traffic probe: boids steps done", file=sys.stderr, flush=True)
@@ -274,6 +276,7 @@ def main():
     ap.add_argument("--no-serving", dest="serving", action="store_false")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run the short program profiled by the FETCH_SIZE/WRITE_SIZE passes")
+    ap.add_argument("--staging", action="store_true", help="traffic probe: stage neighbourhoods in LDS")
     ap.add_argument("--traffic-report", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "COMMAND"),
                     help="write the traffic summary from two rocprofv3 --pmc output folders")
     args = ap.parse_args()

@@ -127,7 +127,8 @@ def test_agent_sort_bit_exact(ctx, dims, n, mn, mx, r, seed):
 
 
 @pytest.mark.parametrize("n,p,keep", [(1, 1.0, 1), (2047, 0.5, 1), (2048, 0.9, 1), (2049, 0.1, 0),
-                                      (1_250_000, 0.7, 1), (0, 0.5, 1)])
+                                      (1_250_000, 0.7, 1), (0, 0.5, 1),
+                                      (3_000_000, 0.5, 1)])   # > 1,024 tiles: the tile counts are block-scanned
 def test_compact_stable(ctx, n, p, keep):
     from flamegpu2_amd import fgx
     rng = np.random.default_rng(n)
@@ -171,7 +172,32 @@ def test_compact_device_count(ctx):
     assert cnt == cnt_o and np.array_equal(host(d_o, np.uint32, cnt), outs_o[0])
 
 
-@pytest.mark.parametrize("n_par,existing,p", [(5000, 1234, 0.3), (1_000_000, 500_000, 0.02), (3, 0, 1.0)])
+def test_compact_count_in_place(ctx):
+    # the output count is the input count (a list compacted in place): the second pass
+    # must not see the count its own last tile writes
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(5)
+    for cap, n in [(5_000_000, 4_500_001), (300_000, 299_999)]:
+        flags = (rng.uniform(size=cap) < 0.3).astype(np.uint32)
+        b = np.arange(cap, dtype=np.uint32)
+        cnt_o, outs_o = O.compact(flags[:n], 1, [b[:n]])
+        d_f, d_b, d_o = dev(flags), dev(b), empty(cap, np.uint32)
+        d_n = dev(np.array([n], np.uint32))
+        sb = fgx.scratch_bytes(fgx.FGX_OP_COMPACT, cap)
+        s = scratch(sb)
+        fgx.compact(ctx, ptr(d_f), 1, cap, [ptr(d_b)], [ptr(d_o)], [4], ptr(d_n), ptr(s), sb, d_n=ptr(d_n))
+        sync()
+        cnt = int(host(d_n, np.uint32)[0])
+        assert cnt == cnt_o and np.array_equal(host(d_o, np.uint32, cnt), outs_o[0])
+
+
+def test_spatial_build_large_bin_table(ctx):
+    # 4.1 M bins (> 1,024 scan tiles): the bin counts go through the block-scanned path
+    _build_case(ctx, 3, 1 << 20, 16.0, 0.1, 21)
+
+
+@pytest.mark.parametrize("n_par,existing,p", [(5000, 1234, 0.3), (1_000_000, 500_000, 0.02), (3, 0, 1.0),
+                                             (3_000_000, 100, 0.01)])
 def test_births_bit_exact(ctx, n_par, existing, p):
     from flamegpu2_amd import fgx
     rng = np.random.default_rng(n_par)
