@@ -50,19 +50,37 @@ def wrap_candidates(pbm, grid):
     return float((counts * nb).sum()) / float(n)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(params, arrs, steps):
+    """The serial oracle (g++ -O3, one thread pinned to one core) on `steps` steps of the
+    same workload, timed on this host."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     ref = O.Sim(1, params, 7)
     ref.set_agents(0, arrs)
     ref.step(1)   # warm: first sort of an unsorted population
-    t0 = time.perf_counter()
-    ref.step(steps)
-    dt = time.perf_counter() - t0
+    allowed = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, {min(allowed)})
+    try:
+        t0 = time.perf_counter()
+        ref.step(steps)
+        dt = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, allowed)
     n = len(arrs[0])
     return {"value": n * steps / dt, "unit": "agent-steps/s", "cores": 1, "kind": "port",
-            "sample": f"serial C++ oracle, boids {n} agents, {steps} steps after 1 warm step, one thread, "
-                      f"{dt:.1f} s"}
+            "sample": f"serial C++ oracle (g++ -O3), boids {n} agents, {steps} steps after 1 warm step, one thread "
+                      f"pinned to one core, {dt:.1f} s",
+            "host_cores": os.cpu_count(), "host_cpu": _cpu_model()}
 
 
 def ensemble_plans(count=256, seed=2024, n_lo=1024, n_hi=262144, r_lo=1.0, r_hi=3.0):
