@@ -15,9 +15,9 @@
  * facade rethrows them as flamegpu::exception::* types.
  *
  * Calls on one context may run on several streams at once as long as each
- * concurrent call has its own scratch buffer.  Scratch buffers must be
- * zero-filled once (fgx_scratch_clear) before their first use; they keep
- * themselves consistent afterwards.
+ * concurrent call has its own scratch buffer.  Scratch contents need no
+ * preparation: every call initialises what it reads (fgx_scratch_clear is
+ * kept for callers that want defined contents).
  */
 #ifndef FGX_H_
 #define FGX_H_
