@@ -11,6 +11,15 @@ namespace fgx {
 
 static thread_local std::string g_last_error;
 
+std::mutex& clean_mutex() {
+    static std::mutex m;
+    return m;
+}
+std::unordered_map<const void*, uint32_t>& clean_record() {
+    static std::unordered_map<const void*, uint32_t> r;
+    return r;
+}
+
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 int fail(int code, const std::string& msg) {
@@ -151,6 +160,13 @@ int fgx_ctx_create(int device, fgx_ctx** out) {
 
 int fgx_ctx_destroy(fgx_ctx* ctx) {
     delete ctx;
+    return FGX_OK;
+}
+
+int fgx_ctx_set_option(fgx_ctx* ctx, int option, int value) {
+    if (!ctx) return fail(FGX_ERR_ARG, "fgx_ctx_set_option: NULL context");
+    if (option != FGX_OPT_KEEP_SCRATCH_ZERO) return fail(FGX_ERR_ARG, "fgx_ctx_set_option: unknown option");
+    ctx->c.keep_zero = value != 0;
     return FGX_OK;
 }
 

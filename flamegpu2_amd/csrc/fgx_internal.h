@@ -5,6 +5,8 @@
 #include <cstdint>
 #include <cstddef>
 #include <string>
+#include <mutex>
+#include <unordered_map>
 
 #include "../../include/fgx.h"
 
@@ -25,7 +27,15 @@ struct Ctx {
     int cu_count = 0;
     int xcd_count = 8;
     size_t hbm_bytes = 0;
+    // FGX_OPT_KEEP_SCRATCH_ZERO: bin sorts leave their histogram zeroed
+    int keep_zero = 0;
 };
+
+// Host-side record, per scratch buffer, of the bin count whose histogram region a sort
+// left zero (kAllZero after fgx_scratch_clear).  Consulted only with FGX_OPT_KEEP_SCRATCH_ZERO.
+constexpr uint32_t kAllZero = 0xFFFFFFFFu;
+std::mutex& clean_mutex();
+std::unordered_map<const void*, uint32_t>& clean_record();
 
 // ---------------------------------------------------------------- variable tables
 // Passed by value in the kernel argument segment, so every field is wave-uniform

@@ -237,9 +237,16 @@ __global__ void __launch_bounds__(kBlock) k_stable_scatter(const uint32_t* d_n, 
 // scan's look-back waits on status words written by other CUs.
 constexpr uint32_t kTwoPassTiles = 1024;
 
+// flag_src/flag_dst (bin sorts; NULL otherwise): block 0 moves the histogram's "unsorted"
+// word to the word the scatters read, leaving the histogram region all zero behind it.
 __global__ void __launch_bounds__(kBlock) k_tile_sums(const uint32_t* __restrict__ in, uint32_t n,
-                                                      uint32_t* __restrict__ tile_sum) {
+                                                      uint32_t* __restrict__ tile_sum, uint32_t* flag_src,
+                                                      uint32_t* flag_dst) {
     __shared__ uint32_t s_wave[kBlock / 64];
+    if (flag_src && blockIdx.x == 0 && threadIdx.x == 0) {
+        *flag_dst = *flag_src;
+        *flag_src = 0u;
+    }
     const uint32_t base = blockIdx.x * kTile + threadIdx.x * kItems;
     uint32_t sum = 0;
 #pragma unroll
@@ -255,8 +262,10 @@ __global__ void __launch_bounds__(kBlock) k_tile_sums(const uint32_t* __restrict
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                       uint32_t n, const uint32_t* __restrict__ tile_sum) {
+// zero_in: each element is cleared once read (the bin histograms of self-cleaning sorts)
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                       uint32_t n, const uint32_t* __restrict__ tile_sum,
+                                                       uint32_t zero_in) {
     __shared__ uint32_t s_wave[kBlock / 64];
     __shared__ uint32_t s_prefix;
     const uint32_t tile = blockIdx.x;
@@ -274,6 +283,11 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles(const uint32_t* __restric
     for (uint32_t j = 0; j < kItems; ++j) {
         v[j] = (base + j < n) ? in[base + j] : 0u;
         sum += v[j];
+    }
+    if (zero_in) {
+#pragma unroll
+        for (uint32_t j = 0; j < kItems; ++j)
+            if (base + j < n && v[j]) in[base + j] = 0u;
     }
     const uint32_t incl = wave_inclusive_u32(sum);
     if (lane == 63) s_wave[w] = incl;
@@ -312,8 +326,9 @@ __global__ void __launch_bounds__(kSumsBlock) k_scan_sums(uint32_t* __restrict__
 }
 
 // The second pass when the tile sums are already scanned: tile t's prefix is ts[t].
-__global__ void __launch_bounds__(kBlock) k_scan_tiles_pre(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                           uint32_t n, const uint32_t* __restrict__ ts) {
+__global__ void __launch_bounds__(kBlock) k_scan_tiles_pre(uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                           uint32_t n, const uint32_t* __restrict__ ts,
+                                                           uint32_t zero_in) {
     __shared__ uint32_t s_wave[kBlock / 64];
     const uint32_t tile = blockIdx.x;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
@@ -324,6 +339,11 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles_pre(const uint32_t* __res
     for (uint32_t j = 0; j < kItems; ++j) {
         v[j] = (base + j < n) ? in[base + j] : 0u;
         sum += v[j];
+    }
+    if (zero_in) {
+#pragma unroll
+        for (uint32_t j = 0; j < kItems; ++j)
+            if (base + j < n && v[j]) in[base + j] = 0u;
     }
     const uint32_t incl = wave_inclusive_u32(sum);
     if (lane == 63) s_wave[w] = incl;
@@ -342,19 +362,22 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles_pre(const uint32_t* __res
 // inter-block waits): up to kTwoPassTiles tiles, tile sums then a pass in which each
 // tile adds up the sums before it; beyond that, tile sums, one block scanning them,
 // then a pass that reads each tile's prefix.  `status` holds the tile sums.
-static int scan_counts(const uint32_t* in, uint32_t* out, uint32_t n, ScratchHeader* hdr, uint64_t* status, hipStream_t st) {
+// flag_src/flag_dst and zero_in: see k_tile_sums and k_scan_tiles (bin sorts only).
+static int scan_counts(const uint32_t* in, uint32_t* out, uint32_t n, ScratchHeader* hdr, uint64_t* status, hipStream_t st,
+                       uint32_t zero_in = 0u, uint32_t* flag_src = nullptr, uint32_t* flag_dst = nullptr) {
     (void)hdr;
     const uint32_t tiles = ceil_div(n, kTile);
     if (!tiles) return FGX_OK;
     uint32_t* ts = reinterpret_cast<uint32_t*>(status);   // 8 bytes per tile are reserved; sums use 4
-    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kBlock), 0, st, in, n, ts);
+    uint32_t* inw = const_cast<uint32_t*>(in);           // written only when zero_in
+    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kBlock), 0, st, in, n, ts, flag_src, flag_dst);
     FGX_HIP(hipGetLastError());
     if (tiles <= kTwoPassTiles) {
-        hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kBlock), 0, st, in, out, n, ts);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kBlock), 0, st, inw, out, n, ts, zero_in);
     } else {
         hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kSumsBlock), 0, st, ts, tiles);
         FGX_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_scan_tiles_pre, dim3(tiles), dim3(kBlock), 0, st, in, out, n, ts);
+        hipLaunchKernelGGL(k_scan_tiles_pre, dim3(tiles), dim3(kBlock), 0, st, inw, out, n, ts, zero_in);
     }
     FGX_HIP(hipGetLastError());
     return FGX_OK;
@@ -698,7 +721,8 @@ struct BuildLayout {
 static BuildLayout build_layout(uint32_t n, uint32_t bins) {
     BuildLayout L;
     size_t off = 0;
-    L.hist = off; off += align256((size_t)(bins + 2) * 4);   // counts [0, bins], "unsorted" flag at [bins + 1]
+    // counts [0, bins], "unsorted" flag at [bins + 1], the flag as the scatters read it at [bins + 2]
+    L.hist = off; off += align256((size_t)(bins + 3) * 4);
     L.key = off; off += align256((size_t)std::max(n, 1u) * 4);
     L.sub = off; off += align256((size_t)std::max(n, 1u) * 4);
     L.tmp = off; off += align256((size_t)std::max(n, 1u) * 4);
@@ -742,8 +766,22 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
     auto* tmp = reinterpret_cast<uint32_t*>(s + L.tmp);
     hipStream_t st = (hipStream_t)stream;
     uint32_t* unsorted = hist + bins + 1;
-    int rc = fgx_fill_u32(hist, 0u, (size_t)bins + 2, stream);
-    if (rc) return rc;
+    uint32_t* unsorted_read = hist + bins + 2;
+    // With FGX_OPT_KEEP_SCRATCH_ZERO every sort leaves [0, bins + 1] zero (the scan clears
+    // the counts, k_tile_sums moves the flag), so a buffer last sorted with this bin count
+    // needs no clearing launch.
+    bool clean = false;
+    if (ctx && ctx->c.keep_zero) {
+        std::lock_guard<std::mutex> lk(clean_mutex());
+        auto it = clean_record().find(scratch);
+        clean = it != clean_record().end() && (it->second == bins || it->second == kAllZero);
+        if (it != clean_record().end()) clean_record().erase(it);   // re-recorded once every launch is enqueued
+    }
+    int rc = FGX_OK;
+    if (!clean) {
+        rc = fgx_fill_u32(hist, 0u, (size_t)bins + 2, stream);
+        if (rc) return rc;
+    }
     const uint32_t g = stream_grid(ctx ? &ctx->c : nullptr, n_cap);
     if (n_cap) {
         hipLaunchKernelGGL(k_bin_hist, dim3(g), dim3(kBlock), 0, st, ka, d_n, n_cap, x, y, z, key, sub, hist, unsorted);
@@ -753,14 +791,21 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
     {
         auto* hdr = reinterpret_cast<ScratchHeader*>(s + L.hdr);
         auto* status = reinterpret_cast<uint64_t*>(s + L.status);
-        rc = scan_counts(hist, start_out, bins + 1, hdr, status, st);
+        const uint32_t zero = ctx && ctx->c.keep_zero ? 1u : 0u;
+        rc = scan_counts(hist, start_out, bins + 1, hdr, status, st, zero, unsorted, unsorted_read);
         if (rc) return rc;
     }
     if (n_cap) {
-        hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, sub, start_out, tmp, unsorted);
+        hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, sub, start_out, tmp,
+                           unsorted_read);
         FGX_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, start_out, tmp, vt, unsorted);
+        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, start_out, tmp, vt,
+                           unsorted_read);
         FGX_HIP(hipGetLastError());
+    }
+    if (ctx && ctx->c.keep_zero) {
+        std::lock_guard<std::mutex> lk(clean_mutex());
+        clean_record()[scratch] = bins;
     }
     return FGX_OK;
 }
@@ -862,7 +907,12 @@ int fgx_scratch_bytes(int op, uint32_t n, uint32_t bins, size_t* bytes) {
 int fgx_scratch_clear(void* scratch, size_t bytes, void* stream) {
     if (!bytes) return FGX_OK;
     if (bytes & 3u) return fail(FGX_ERR_ARG, "fgx_scratch_clear: size must be a multiple of 4");
-    return fgx_fill_u32(static_cast<uint32_t*>(scratch), 0u, bytes / 4, stream);
+    const int rc = fgx_fill_u32(static_cast<uint32_t*>(scratch), 0u, bytes / 4, stream);
+    if (rc == FGX_OK) {
+        std::lock_guard<std::mutex> lk(clean_mutex());
+        clean_record()[scratch] = kAllZero;
+    }
+    return rc;
 }
 
 int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t* d_n, uint32_t n_cap,

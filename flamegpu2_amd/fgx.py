@@ -22,6 +22,7 @@ FGX_OP_AGENT_SORT = 3
 FGX_OP_SCAN = 4
 FGX_OP_STATS = 5
 FGX_MAX_VARS = 32
+FGX_OPT_KEEP_SCRATCH_ZERO = 1
 
 
 class FgxError(RuntimeError):
@@ -61,6 +62,7 @@ _SIGS = {
     "fgx_scratch_clear": [vp, sz, vp],
     "fgx_spatial_meta_init": [_c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float,
                               _c.POINTER(SpatialMeta)],
+    "fgx_ctx_set_option": [vp, _c.c_int, _c.c_int],
     "fgx_spatial_meta_init_mult": [_c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float, _c.c_float,
                                    _c.POINTER(SpatialMeta)],
     "fgx_spatial_build": [vp, _c.POINTER(SpatialMeta), vp, u32, vp, vp, vp, u32, _c.POINTER(vp), _c.POINTER(vp),
@@ -123,6 +125,9 @@ class Context:
         h = vp()
         check(lib().fgx_ctx_create(device, _c.byref(h)))
         self.h = h
+
+    def set_option(self, option, value):
+        check(lib().fgx_ctx_set_option(self.h, option, int(value)))
 
     def close(self):
         if self.h:

@@ -172,6 +172,7 @@ struct StateList {
     uint32_t h_count = 0;
     void* sort_scratch = nullptr;
     size_t sort_scratch_bytes = 0;
+    uint32_t sort_scratch_keys = 0;   // the largest key range the scratch was sized for
     int var_index(const std::string& n) const {
         for (size_t i = 0; i < vars.size(); ++i)
             if (vars[i].name == n) return (int)i;
@@ -863,6 +864,8 @@ inline void CUDASimulation::setBinWidthMultiplier(float mult) {
 inline void CUDASimulation::ensure_init() {
     if (initialised_) return;
     detail::fgx_check(fgx_ctx_create(cuda_config_.device_id, &ctx_), "fgx_ctx_create");
+    // every build and sort scratch buffer comes from new_scratch (zeroed) and serves only builds/sorts
+    detail::fgx_check(fgx_ctx_set_option(ctx_, FGX_OPT_KEEP_SCRATCH_ZERO, 1), "context option");
     detail::fgx_check(fgx_stream_create(&stream_), "stream create");
     alloc((void**)&d_small_, 4096 * 4);
     detail::fgx_check(fgx_fill_u32(d_small_, 0, 4096, stream_), "zero counters");
@@ -1246,15 +1249,18 @@ inline bool CUDASimulation::ensure_capacity() {
             changed = true;
         }
     }
-    // sort scratch depends on list capacity
+    // sort scratch depends on list capacity and on the largest key range of the functions
+    // that sort the list (they may read different spatial messages)
     for (auto& f : fns_) {
         if (!f.sort_dims) continue;
         auto& l = f.agent->states.at(f.in_state);
-        if (!l.sort_scratch) {
-            uint32_t bits = 0;
-            detail::fgx_check(fgx_agent_sort_key_bits(f.sort_dims, f.min->meta.min, f.min->meta.max, f.min->meta.radius, &bits),
-                              "sort key bits");
-            l.sort_scratch = new_scratch(FGX_OP_AGENT_SORT, l.cap, 1u << bits, &l.sort_scratch_bytes);
+        uint32_t bits = 0;
+        detail::fgx_check(fgx_agent_sort_key_bits(f.sort_dims, f.min->meta.min, f.min->meta.max, f.min->meta.radius, &bits),
+                          "sort key bits");
+        if (!l.sort_scratch || l.sort_scratch_keys < (1u << bits)) {
+            free_dev(l.sort_scratch);
+            l.sort_scratch_keys = std::max(l.sort_scratch_keys, 1u << bits);
+            l.sort_scratch = new_scratch(FGX_OP_AGENT_SORT, l.cap, l.sort_scratch_keys, &l.sort_scratch_bytes);
             changed = true;
         }
     }

@@ -51,6 +51,15 @@ int fgx_ctx_destroy(fgx_ctx* ctx);
 const char* fgx_last_error(void);
 /* Device properties used for launch shaping. */
 int fgx_device_info(fgx_ctx* ctx, int* cu_count, int* xcd_count, size_t* hbm_bytes);
+/* Context options (no counterpart in the reference).
+ * FGX_OPT_KEEP_SCRATCH_ZERO (default 0): the caller promises that every scratch buffer
+ * it passes to fgx_spatial_build, fgx_bucket_build and fgx_agent_sort on this context
+ * was zeroed when allocated (fgx_scratch_clear) and is written by nothing but those
+ * calls.  The bin sorts then leave their histogram zeroed behind them, and a sort on a
+ * buffer that fgx_scratch_clear zeroed, or that the last sort used with the same bin
+ * count, skips the clearing launch. */
+enum { FGX_OPT_KEEP_SCRATCH_ZERO = 1 };
+int fgx_ctx_set_option(fgx_ctx* ctx, int option, int value);
 
 /* ---------------------------------------------------------------- memory, streams, events, graphs
  * Thin wrappers so the C++ runtime never includes HIP headers. */

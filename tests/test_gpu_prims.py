@@ -191,6 +191,34 @@ def test_compact_count_in_place(ctx):
         assert cnt == cnt_o and np.array_equal(host(d_o, np.uint32, cnt), outs_o[0])
 
 
+def test_spatial_build_keep_scratch_zero():
+    """FGX_OPT_KEEP_SCRATCH_ZERO: sorts leave their histogram zeroed and skip the clearing
+    launch when a buffer is reused with the same bin count; a different bin count on the
+    same buffer clears again.  Every build must still equal the oracle's."""
+    from flamegpu2_amd import fgx
+    c = fgx.Context(0)
+    c.set_option(fgx.FGX_OPT_KEEP_SCRATCH_ZERO, 1)
+    rng = np.random.default_rng(17)
+    n = 50000
+    sb = max(fgx.scratch_bytes(fgx.FGX_OP_SPATIAL_BUILD, n, 40 ** 3), fgx.scratch_bytes(fgx.FGX_OP_SPATIAL_BUILD, n, 10 ** 3))
+    s = scratch(sb)   # zeroed on allocation
+    for width, radius in [(4.0, 0.1), (4.0, 0.1), (4.0, 0.4), (4.0, 0.1), (4.0, 0.1)]:
+        P = rng.uniform(0, width, size=(3, n)).astype(np.float32)
+        mo = O.spatial_meta(3, [0.0] * 3, [width] * 3, radius)
+        pbm_o, perm_o, outs_o = O.spatial_build(mo, P[0], P[1], P[2])
+        m = fgx.spatial_meta(3, [0.0] * 3, [width] * 3, radius)
+        d_in = [dev(P[a]) for a in range(3)]
+        d_out = [empty(n, np.float32) for _ in range(3)]
+        pbm = empty(m.bins + 1, np.uint32)
+        fgx.spatial_build(c, m, n, ptr(d_in[0]), ptr(d_in[1]), ptr(d_in[2]), [ptr(t) for t in d_in],
+                          [ptr(t) for t in d_out], [4, 4, 4], ptr(pbm), ptr(s), sb)
+        sync()
+        assert np.array_equal(host(pbm, np.uint32, m.bins + 1), pbm_o), (width, radius)
+        for a in range(3):
+            assert np.array_equal(host(d_out[a], np.float32, n), outs_o[a])
+    c.close()
+
+
 def test_spatial_build_large_bin_table(ctx):
     # 4.1 M bins (> 1,024 scan tiles): the bin counts go through the block-scanned path
     _build_case(ctx, 3, 1 << 20, 16.0, 0.1, 21)
