@@ -176,7 +176,8 @@ def bench_batched(args):
 def bench_serving(args):
     from flamegpu2_amd import serving
     reqs = serving.poisson_requests(args.serve_rate, args.serve_seconds, seed=99)
-    srv = serving.Server(max_chunk=16)
+    cap = args.serve_max_agents or None
+    srv = serving.Server(max_chunk=16, policy=args.serve_policy, max_active_agents=cap)
     t0 = time.perf_counter()
     res = srv.run(reqs, progress=lambda t: print("bench: " + t, file=sys.stderr, flush=True))
     wall = time.perf_counter() - t0
@@ -184,6 +185,8 @@ def bench_serving(args):
     out["host_phase_s"] = {k: round(v, 3) for k, v in srv.phase.items()}
     out["workload"] = (f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s, circles/boids 50/50, "
                        f"N log-uniform [2^10, 2^20], steps U[10, 1000], random seeds; continuous batching")
+    out["policy"] = args.serve_policy
+    out["max_active_agents"] = cap
     out["offered_agent_steps_per_s"] = sum(r.n * r.steps for r in reqs) / args.serve_seconds
     return out
 
@@ -291,6 +294,10 @@ def main():
     ap.add_argument("--no-batched", dest="batched", action="store_false")
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-policy", choices=("fifo", "shortest"), default="shortest",
+                    help="admission order: arrival, or least work (agents x steps) first")
+    ap.add_argument("--serve-max-agents", type=int, default=2_000_000,
+                    help="cap on the agents of the runs resident at once; 0: HBM budget only")
     ap.add_argument("--no-serving", dest="serving", action="store_false")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run the short program profiled by the FETCH_SIZE/WRITE_SIZE passes")

@@ -1,8 +1,12 @@
 """Serving loop: simulation runs arrive over time and are served by continuous batching.
 
 Each request is one simulation run (model, population size, step count, seed).
-Requests are admitted in arrival order while the HBM they need (agent, message,
-bin-table and scratch bytes, from the batch's own estimate) fits the budget;
+Requests are admitted while the HBM they need (agent, message, bin-table and
+scratch bytes, from the batch's own estimate) fits the budget, and, when a cap
+on resident agents is set, while the agents fit the cap.  Policy "fifo" admits
+in arrival order; policy "shortest" admits the queued run with the least work
+(agents x steps) first, which shortens typical latency under overload at the
+cost of the longest runs' latency;
 an admitted run joins its model's batch at the next step and shares every launch
 with the runs already resident, so runs that arrive while others are on the GPU
 overlap with them.  The batches of different models run on their own streams.
@@ -57,8 +61,17 @@ def poisson_requests(rate, seconds, seed, n_range=(1 << 10, 1 << 20), steps_rang
 
 
 class Server:
-    def __init__(self, hbm_budget=160 << 30, max_chunk=16, tuning=None, device=0):
+    POLICIES = ("fifo", "shortest")
+
+    def __init__(self, hbm_budget=160 << 30, max_chunk=16, tuning=None, device=0, policy="fifo",
+                 max_active_agents=None):
+        if policy not in self.POLICIES:
+            raise ValueError(f"unknown admission policy '{policy}' (one of {', '.join(self.POLICIES)})")
+        if max_active_agents is not None and max_active_agents <= 0:
+            raise ValueError("max_active_agents must be positive")
         self.budget = hbm_budget
+        self.policy = policy
+        self.max_active_agents = max_active_agents
         self.max_chunk = max_chunk
         self.tuning = tuning or {}
         self.device = device
@@ -84,6 +97,7 @@ class Server:
         queue = collections.deque()
         active = {}        # (model, handle) -> [request, bytes, admitted, steps issued]
         used = 0
+        agents = 0         # agents of the active runs
         results = []
         made = {}          # request id -> (params, populations) while it waits for HBM
         nxt = 0
@@ -93,9 +107,12 @@ class Server:
             while nxt < len(reqs) and reqs[nxt].arrival <= now:
                 queue.append(reqs[nxt])
                 nxt += 1
-            # admission in arrival order while the HBM budget allows
+            # admission, by policy, while the HBM budget and the agent cap allow; a run
+            # that does not fit waits (it is always admitted when nothing is active)
             while queue:
-                r = queue[0]
+                r = queue[0] if self.policy == "fifo" else min(queue, key=lambda q: (q.n * q.steps, q.arrival))
+                if active and self.max_active_agents is not None and agents + r.n > self.max_active_agents:
+                    break
                 lib, agent, make = MODELS[r.model]
                 b = self._batch(r.model)
                 tp = clock()
@@ -110,8 +127,9 @@ class Server:
                     break
                 del made[r.id]
                 used += need
+                agents += r.n
                 active[(r.model, h)] = [r, need, clock() - t0, 0]
-                queue.popleft()
+                queue.remove(r)
             if not active:
                 if nxt < len(reqs):
                     time.sleep(max(0.0, reqs[nxt].arrival - (clock() - t0)))
@@ -149,6 +167,7 @@ class Server:
                     rec["latency"] = rec["completed"] - r.arrival
                     b.remove(h)
                     used -= need
+                    agents -= r.n
                     results.append(rec)
                     self.phase["complete_s"] += clock() - tp
             if progress and clock() - last_report > 5.0:

@@ -281,3 +281,22 @@ def test_serving_hbm_budget_admits_in_order():
     assert [r["id"] for r in order] == list(range(5))
     for a, b in zip(order, order[1:]):
         assert b["admitted"] >= a["completed"] - 1e-3
+
+
+def test_serving_shortest_first_under_agent_cap():
+    from flamegpu2_amd import serving
+    # all arrive together; the cap holds one run at a time, so completions follow work (n x steps)
+    reqs = [serving.Request(0, 0.0, "circles", 6000, 9, 21), serving.Request(1, 0.0, "boids", 3000, 4, 22),
+            serving.Request(2, 0.0, "circles", 2000, 3, 23), serving.Request(3, 0.0, "boids", 5000, 8, 24)]
+    srv = serving.Server(max_chunk=4, policy="shortest", max_active_agents=6000)
+    res = srv.run(reqs, collect=True)
+    order = [r["id"] for r in sorted(res, key=lambda r: r["completed"])]
+    assert order == [2, 1, 3, 0]
+    for rec in res:
+        r = reqs[rec["id"]]
+        lib, agent, make = serving.MODELS[r.model]
+        params, pops = make(r.n, r.seed)
+        exp, eid = _alone(lib, agent, params, pops[agent], r.seed, r.steps)
+        assert np.array_equal(rec["ids"], eid), f"request {r.id}"
+        for g, e in zip(rec["agents"], exp):
+            assert np.array_equal(g, e), f"request {r.id} differs from its run alone"
