@@ -189,7 +189,13 @@ int fgx_malloc(fgx_ctx* ctx, size_t bytes, void** out) {
 
 int fgx_free(fgx_ctx* ctx, void* p) {
     (void)ctx;
-    if (p) FGX_HIP(hipFree(p));
+    if (!p) return FGX_OK;
+    {
+        // a later allocation at this address starts with unknown contents
+        std::lock_guard<std::mutex> lk(clean_mutex());
+        clean_record().erase(p);
+    }
+    FGX_HIP(hipFree(p));
     return FGX_OK;
 }
 

@@ -1,10 +1,13 @@
 // libfgx: hot-path primitives for gfx950 (wave64).
 //
-//   message-list build : bin key + run-aggregated wave64 histogram -> single-pass
-//                        decoupled look-back scan -> PBM -> index scatter ->
-//                        stable in-bin ranking fused with the SoA payload scatter
+//   message-list build : bin key + run-aggregated wave64 histogram -> exclusive scan in
+//                        passes ordered by kernel boundaries (tile sums, then tile
+//                        prefixes) -> bin start table -> index scatter (bins grouped) ->
+//                        in-bin order restored by index (large bins sorted by one block
+//                        each) -> SoA payload scatter
 //   agent spatial sort : same pipeline with the unclamped sort key
-//   compaction/births  : one pass, decoupled look-back scan fused with the scatter
+//   compaction/births  : kept count per tile, then per-tile prefixes, ballot ranks and
+//                        the copy of every variable (two passes, no inter-block waits)
 //   reduction          : fixed-shape (deterministic) tree
 //
 // Nothing here allocates or synchronises with the host.
@@ -17,7 +20,7 @@ namespace fgx {
 
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kItems = 8;
-constexpr uint32_t kTile = kBlock * kItems;   // 2048 items per look-back tile
+constexpr uint32_t kTile = kBlock * kItems;   // 2048 items per scan / compaction tile
 
 __device__ __forceinline__ uint32_t load_count(const uint32_t* d_n, uint32_t n_cap) {
     if (!d_n) return n_cap;
@@ -206,13 +209,206 @@ __global__ void __launch_bounds__(kBlock) k_index_scatter(const uint32_t* d_n, u
         tmp[start[key[i]] + sub[i]] = i;
 }
 
-// For output slot s: the message e = tmp[s] sits in bin b = key[e]; its stable
-// rank in the bin is the number of bin members with a smaller input index.
-// Writes every variable of e to start[b] + rank.
+// ---------------------------------------------------------------- in-bin order
+// After k_index_scatter every bin's members sit in its slots in atomic order.  The
+// stable order is ascending input index.  Bins of at most kSmallBin members are put in
+// order by k_stable_scatter itself (each slot counts the smaller members of its bin);
+// larger bins are ordered first by k_sort_big_bins, one block per bin, into `sorted`:
+//   * up to kLdsBin members: the bin is staged in LDS and each member counts the smaller
+//     ones (every thread reads the same LDS word at once, a broadcast);
+//   * beyond: a bitmap over a window of kMapWords * 32 input indices is set in LDS, and a
+//     member's rank is the number of set bits below its own (per-8-word prefix counts
+//     plus at most 8 word popcounts), window by window over [min, max] of the bin.
+// Work per bin is O(n_b) per window, so a build costs O(n) however the bins are filled.
+constexpr uint32_t kSmallBin = 32;
+constexpr uint32_t kLdsBin = 2 * kBlock;
+constexpr uint32_t kMapWords = 16384;
+constexpr uint32_t kMapGroups = kMapWords / 8;
+
+__device__ __forceinline__ void block_min_max(uint32_t& lo, uint32_t& hi, uint32_t* s_red) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s_red[w] = lo;
+        s_red[kBlock / 64 + w] = hi;
+    }
+    __syncthreads();
+    lo = s_red[0];
+    hi = s_red[kBlock / 64];
+    for (uint32_t k = 1; k < kBlock / 64; ++k) {
+        lo = s_red[k] < lo ? s_red[k] : lo;
+        hi = s_red[kBlock / 64 + k] > hi ? s_red[kBlock / 64 + k] : hi;
+    }
+    __syncthreads();
+}
+
+// Exclusive block scan of one value per thread; *total receives the block's sum.
+__device__ __forceinline__ uint32_t block_exclusive_u32(uint32_t v, uint32_t* s_red, uint32_t* total) {
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t incl = wave_inclusive_u32(v);
+    if (lane_id() == 63) s_red[w] = incl;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (uint32_t k = 0; k < kBlock / 64; ++k) {
+        off += k < w ? s_red[k] : 0u;
+        tot += s_red[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - v;
+}
+
+// Block-wide: members tmp[st, en) of one bin (en - st > kSmallBin) to sorted[st, en) in
+// ascending order.  Ends with a barrier, so LDS is free again for the next bin.
+__device__ void sort_one_bin(uint32_t st, uint32_t en, const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
+                             uint32_t* s_map, uint32_t* s_gpre, uint32_t* s_red) {
+    const uint32_t nb = en - st, tid = threadIdx.x;
+    if (nb <= kLdsBin) {
+        for (uint32_t i = tid; i < nb; i += kBlock) s_map[i] = tmp[st + i];
+        __syncthreads();
+        const uint32_t e0 = tid < nb ? s_map[tid] : 0xFFFFFFFFu;
+        const uint32_t e1 = tid + kBlock < nb ? s_map[tid + kBlock] : 0xFFFFFFFFu;
+        uint32_t r0 = 0, r1 = 0;
+        for (uint32_t t = 0; t < nb; ++t) {
+            const uint32_t v = s_map[t];
+            r0 += v < e0 ? 1u : 0u;
+            r1 += v < e1 ? 1u : 0u;
+        }
+        if (tid < nb) sorted[st + r0] = e0;
+        if (tid + kBlock < nb) sorted[st + r1] = e1;
+        __syncthreads();
+        return;
+    }
+    // Member passes load kUnroll members per thread before using them, so that one block
+    // keeps enough loads in flight to stream a large bin.
+    constexpr uint32_t kUnroll = 8;
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    for (uint32_t i0 = 0; i0 < nb; i0 += kBlock * kUnroll) {
+        uint32_t v[kUnroll];
+#pragma unroll
+        for (uint32_t j = 0; j < kUnroll; ++j) {
+            const uint32_t i = i0 + j * kBlock + tid;
+            v[j] = i < nb ? tmp[st + i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kUnroll; ++j) {
+            if (i0 + j * kBlock + tid < nb) {
+                lo = v[j] < lo ? v[j] : lo;
+                hi = v[j] > hi ? v[j] : hi;
+            }
+        }
+    }
+    block_min_max(lo, hi, s_red);
+    constexpr uint64_t kSpan = (uint64_t)kMapWords * 32u;
+    uint32_t running = 0;
+    for (uint64_t w0 = lo; w0 <= hi; w0 += kSpan) {
+        const uint32_t span = (uint32_t)((uint64_t)hi - w0 + 1u < kSpan ? (uint64_t)hi - w0 + 1u : kSpan);
+        const uint32_t words = (span + 31u) >> 5;
+        const uint32_t base = (uint32_t)w0;
+        for (uint32_t i = tid; i < words; i += kBlock) s_map[i] = 0u;
+        __syncthreads();
+        for (uint32_t i0 = 0; i0 < nb; i0 += kBlock * kUnroll) {
+            uint32_t v[kUnroll];
+#pragma unroll
+            for (uint32_t j = 0; j < kUnroll; ++j) {
+                const uint32_t i = i0 + j * kBlock + tid;
+                v[j] = i < nb ? tmp[st + i] : 0u;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kUnroll; ++j) {
+                const uint32_t off = v[j] - base;   // wraps above span for members below the window
+                if (i0 + j * kBlock + tid < nb && off < span) atomicOr(&s_map[off >> 5], 1u << (off & 31u));
+            }
+        }
+        __syncthreads();
+        // thread t owns groups [8t, 8t + 8) of 8 words each
+        uint32_t gs[8];
+        uint32_t tsum = 0;
+#pragma unroll
+        for (uint32_t g = 0; g < 8; ++g) {
+            uint32_t c = 0;
+            const uint32_t w8 = (tid * 8u + g) * 8u;
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) c += w8 + k < words ? (uint32_t)__builtin_popcount(s_map[w8 + k]) : 0u;
+            gs[g] = c;
+            tsum += c;
+        }
+        uint32_t total = 0;
+        uint32_t run = block_exclusive_u32(tsum, s_red, &total);
+#pragma unroll
+        for (uint32_t g = 0; g < 8; ++g) {
+            s_gpre[tid * 8u + g] = run;
+            run += gs[g];
+        }
+        __syncthreads();
+        for (uint32_t i0 = 0; i0 < nb; i0 += kBlock * kUnroll) {
+            uint32_t v[kUnroll];
+#pragma unroll
+            for (uint32_t j = 0; j < kUnroll; ++j) {
+                const uint32_t i = i0 + j * kBlock + tid;
+                v[j] = i < nb ? tmp[st + i] : 0u;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kUnroll; ++j) {
+                const uint32_t off = v[j] - base;
+                if (i0 + j * kBlock + tid >= nb || off >= span) continue;
+                const uint32_t wi = off >> 5, gi = wi >> 3;
+                uint32_t r = running + s_gpre[gi];
+                for (uint32_t x = gi * 8u; x < wi; ++x) r += (uint32_t)__builtin_popcount(s_map[x]);
+                r += (uint32_t)__builtin_popcount(s_map[wi] & ((1u << (off & 31u)) - 1u));
+                sorted[st + r] = v[j];
+            }
+        }
+        running += total;
+        __syncthreads();
+    }
+}
+
+// Block k handles the bins b = k, k + G, k + 2G, ... (G = gridDim.x), so that the few
+// large bins of a small table land on different blocks.  Each block queues its bins with
+// more than kSmallBin members in LDS and orders them one after another.
+__global__ void __launch_bounds__(kBlock) k_sort_big_bins(const uint32_t* __restrict__ start, uint32_t bins,
+                                                          const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
+                                                          const uint32_t* __restrict__ unsorted) {
+    __shared__ uint32_t s_map[kMapWords];
+    __shared__ uint32_t s_gpre[kMapGroups];
+    __shared__ uint32_t s_q[kBlock];
+    __shared__ uint32_t s_red[2 * (kBlock / 64)];
+    __shared__ uint32_t s_qn;
+    if (!*unsorted) return;
+    const uint32_t G = gridDim.x, k = blockIdx.x;
+    const uint32_t mine = k < bins ? (bins - k + G - 1u) / G : 0u;
+    for (uint32_t j0 = 0; j0 < mine; j0 += kBlock) {
+        if (threadIdx.x == 0) s_qn = 0u;
+        __syncthreads();
+        const uint32_t j = j0 + threadIdx.x;
+        if (j < mine) {
+            const uint32_t b = j * G + k;
+            if (start[b + 1] - start[b] > kSmallBin) s_q[atomicAdd(&s_qn, 1u)] = b;
+        }
+        __syncthreads();
+        const uint32_t qn = s_qn;
+        for (uint32_t q = 0; q < qn; ++q) {
+            const uint32_t b = s_q[q];
+            sort_one_bin(start[b], start[b + 1], tmp, sorted, s_map, s_gpre, s_red);
+        }
+        __syncthreads();
+    }
+}
+
+// For output slot s: the message e = tmp[s] sits in bin b = key[e].  In a bin of at most
+// kSmallBin members its stable rank is the number of bin members with a smaller input
+// index, and every variable of e goes to start[b] + rank; a larger bin was ordered by
+// k_sort_big_bins, and slot s takes sorted[s].
 __global__ void __launch_bounds__(kBlock) k_stable_scatter(const uint32_t* d_n, uint32_t n_cap,
                                                            const uint32_t* __restrict__ key,
                                                            const uint32_t* __restrict__ start,
-                                                           const uint32_t* __restrict__ tmp, VarTable vt,
+                                                           const uint32_t* __restrict__ tmp,
+                                                           const uint32_t* __restrict__ sorted, VarTable vt,
                                                            const uint32_t* __restrict__ unsorted) {
     const uint32_t n = load_count(d_n, n_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -225,6 +421,10 @@ __global__ void __launch_bounds__(kBlock) k_stable_scatter(const uint32_t* d_n, 
         const uint32_t b = key[e];
         const uint32_t st = start[b];
         const uint32_t en = start[b + 1];
+        if (en - st > kSmallBin) {
+            copy_element(vt, sorted[s], s);
+            continue;
+        }
         uint32_t rank = 0;
         for (uint32_t t = st; t < en; ++t) rank += tmp[t] < e ? 1u : 0u;
         copy_element(vt, e, st + rank);
@@ -736,6 +936,12 @@ static size_t tile_scratch_bytes(uint32_t n) {
     return sizeof(ScratchHeader) + align256((size_t)std::max(ceil_div(n, kTile), 1u) * 8);
 }
 
+// k_sort_big_bins: at most two blocks per CU (73 KB of LDS each), at most one per bin.
+static uint32_t big_bin_grid(const Ctx* c, uint32_t bins) {
+    const uint32_t cap = (uint32_t)std::max(c ? c->cu_count : 256, 1) * 2u;
+    return std::max(1u, std::min(bins, cap));
+}
+
 static uint32_t stream_grid(const Ctx* c, uint32_t n) {
     // Enough waves to cover HBM latency on every CU; grid-stride beyond that.
     const uint32_t cap = (uint32_t)std::max(c ? c->cu_count : 256, 1) * 8u;
@@ -769,12 +975,17 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
     uint32_t* unsorted_read = hist + bins + 2;
     // With FGX_OPT_KEEP_SCRATCH_ZERO every sort leaves [0, bins + 1] zero (the scan clears
     // the counts, k_tile_sums moves the flag), so a buffer last sorted with this bin count
-    // needs no clearing launch.
+    // needs no clearing launch.  The record describes the buffer after the work already
+    // enqueued eagerly; a captured graph replays later, in an order the record cannot
+    // follow, so a sort under capture always clears and leaves no record behind.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    FGX_HIP(hipStreamIsCapturing((hipStream_t)stream, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
     bool clean = false;
     if (ctx && ctx->c.keep_zero) {
         std::lock_guard<std::mutex> lk(clean_mutex());
         auto it = clean_record().find(scratch);
-        clean = it != clean_record().end() && (it->second == bins || it->second == kAllZero);
+        clean = !capturing && it != clean_record().end() && (it->second == bins || it->second == kAllZero);
         if (it != clean_record().end()) clean_record().erase(it);   // re-recorded once every launch is enqueued
     }
     int rc = FGX_OK;
@@ -799,11 +1010,15 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
         hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, sub, start_out, tmp,
                            unsorted_read);
         FGX_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, start_out, tmp, vt,
+        // sub is free once the index scatter has run: it holds the ordered large bins
+        hipLaunchKernelGGL(k_sort_big_bins, dim3(big_bin_grid(ctx ? &ctx->c : nullptr, bins)), dim3(kBlock), 0, st,
+                           start_out, bins, tmp, sub, unsorted_read);
+        FGX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, start_out, tmp, sub, vt,
                            unsorted_read);
         FGX_HIP(hipGetLastError());
     }
-    if (ctx && ctx->c.keep_zero) {
+    if (ctx && ctx->c.keep_zero && !capturing) {
         std::lock_guard<std::mutex> lk(clean_mutex());
         clean_record()[scratch] = bins;
     }
@@ -841,7 +1056,10 @@ static int run_bin_sort_seg(fgx_ctx* ctx, int32_t mode, const fgx_seg_key* d_seg
         hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, nullptr, n_total, key, sub, start_out, tmp,
                            unsorted);
         FGX_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, nullptr, n_total, key, start_out, tmp, vt,
+        hipLaunchKernelGGL(k_sort_big_bins, dim3(big_bin_grid(ctx ? &ctx->c : nullptr, keys_total)), dim3(kBlock), 0, st,
+                           start_out, keys_total, tmp, sub, unsorted);
+        FGX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, nullptr, n_total, key, start_out, tmp, sub, vt,
                            unsorted);
         FGX_HIP(hipGetLastError());
     }

@@ -67,6 +67,8 @@ _SIGS = {
                                    _c.POINTER(SpatialMeta)],
     "fgx_spatial_build": [vp, _c.POINTER(SpatialMeta), vp, u32, vp, vp, vp, u32, _c.POINTER(vp), _c.POINTER(vp),
                           _c.POINTER(u32), vp, vp, sz, vp],
+    "fgx_bucket_build": [vp, i32, u32, vp, u32, vp, u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), vp, vp, sz,
+                         vp],
     "fgx_agent_sort_key_bits": [_c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float,
                                 _c.POINTER(u32)],
     "fgx_agent_sort": [vp, _c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float, vp, u32, vp, vp,
@@ -165,6 +167,13 @@ def spatial_build(ctx, meta, n_cap, x, y, z, vars_in, vars_out, var_bytes, pbm, 
     nv = len(vars_in)
     check(lib().fgx_spatial_build(ctx.h, _c.byref(meta), d_n, n_cap, x, y, z, nv, _arr(vp, vars_in),
                                   _arr(vp, vars_out), _arr(u32, var_bytes), pbm, scratch, scratch_len, stream))
+
+
+def bucket_build(ctx, key_min, keys, n_cap, key, vars_in, vars_out, var_bytes, pbm, scratch, scratch_len, stream=0,
+                 d_n=None):
+    nv = len(vars_in)
+    check(lib().fgx_bucket_build(ctx.h, key_min, keys, d_n, n_cap, key, nv, _arr(vp, vars_in), _arr(vp, vars_out),
+                                 _arr(u32, var_bytes), pbm, scratch, scratch_len, stream))
 
 
 def agent_sort(ctx, dims, mn, mx, radius, n_cap, x, y, z, vars_in, vars_out, var_bytes, scratch, scratch_len,

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <memory>
@@ -807,8 +808,15 @@ class CUDASimulation {
     struct GraphEntry {
         std::vector<void*> start, end;
         fgx_graph* g = nullptr;
+        uint64_t last_use = 0;
     };
+    // Step graphs kept per simulation: one per double-buffer arrangement at step start.
+    // Models alternate between a few arrangements; beyond this many the least recently
+    // used graph is replaced (and a warning is printed once).
+    static constexpr size_t kMaxStepGraphs = 8;
     std::vector<GraphEntry> graphs_;
+    uint64_t graph_uses_ = 0;
+    bool graph_evict_warned_ = false;
     bool graph_eligible_ = false;
     bool populations_set_ = false;   // setPopulationData since the last step: re-check capacities
     void clear_graphs() {
@@ -2031,7 +2039,21 @@ inline bool CUDASimulation::step() {
         GraphEntry* hit = nullptr;
         for (auto& g : graphs_)
             if (g.start == start) hit = &g;
-        if (!hit && graphs_.size() < 4) {
+        if (!hit && graphs_.size() >= kMaxStepGraphs) {
+            size_t lru = 0;
+            for (size_t k = 1; k < graphs_.size(); ++k)
+                if (graphs_[k].last_use < graphs_[lru].last_use) lru = k;
+            if (!graph_evict_warned_) {
+                std::fprintf(stderr,
+                             "flamegpu: more than %zu buffer arrangements at step start; replacing the least "
+                             "recently used step graph (each new arrangement is captured once)\n",
+                             kMaxStepGraphs);
+                graph_evict_warned_ = true;
+            }
+            fgx_graph_destroy(graphs_[lru].g);
+            graphs_.erase(graphs_.begin() + (std::ptrdiff_t)lru);
+        }
+        if (!hit) {
             GraphEntry e;
             e.start = start;
             detail::fgx_check(fgx_graph_begin(stream_), "graph begin");
@@ -2042,12 +2064,9 @@ inline bool CUDASimulation::step() {
             graphs_.push_back(std::move(e));
             hit = &graphs_.back();
         }
-        if (hit) {
-            detail::fgx_check(fgx_graph_launch(hit->g, stream_), "graph launch");
-            set_buffer_state(hit->end);
-        } else {
-            enqueue_step();
-        }
+        hit->last_use = ++graph_uses_;
+        detail::fgx_check(fgx_graph_launch(hit->g, stream_), "graph launch");
+        set_buffer_state(hit->end);
     } else {
         enqueue_step();
     }

@@ -98,6 +98,118 @@ def test_spatial_build_clumped_bins(ctx):
     assert np.array_equal(host(d_out[3], np.uint32, n), outs_o[3])
 
 
+def _timed(fn, reps=3):
+    """Best wall time of `reps` synchronised calls, after one untimed call."""
+    import time
+    fn()
+    sync()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        sync()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def _spatial_build_check(ctx, P, mn, mx, radius, max_seconds):
+    """Builds the message list of positions P (3 x n) twice (scratch reused), checks the bin
+    table and the stable message order bit for bit against the oracle, and bounds the time."""
+    from flamegpu2_amd import fgx
+    n = P.shape[1]
+    ids = np.arange(n, dtype=np.uint32)
+    mo = O.spatial_meta(3, mn, mx, radius)
+    pbm_o, perm_o, outs_o = O.spatial_build(mo, P[0], P[1], P[2], extra=[ids])
+    m = fgx.spatial_meta(3, mn, mx, radius)
+    d_in = [dev(P[a]) for a in range(3)] + [dev(ids)]
+    d_out = [empty(n, np.float32) for _ in range(3)] + [empty(n, np.uint32)]
+    pbm = empty(m.bins + 1, np.uint32)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_SPATIAL_BUILD, n, m.bins)
+    s = scratch(sb)
+
+    def run():
+        fgx.spatial_build(ctx, m, n, ptr(d_in[0]), ptr(d_in[1]), ptr(d_in[2]), [ptr(t) for t in d_in],
+                          [ptr(t) for t in d_out], [4, 4, 4, 4], ptr(pbm), ptr(s), sb)
+    t = _timed(run)
+    assert np.array_equal(host(pbm, np.uint32, m.bins + 1), pbm_o)
+    assert np.array_equal(host(d_out[3], np.uint32, n), outs_o[3])
+    for a in range(3):
+        assert np.array_equal(host(d_out[a], np.float32, n), outs_o[a])
+    print(f"spatial build n={n} bins={m.bins} largest bin={int(np.diff(pbm_o).max())}: {t * 1e3:.3f} ms")
+    assert t < max_seconds, (t, max_seconds)
+    return pbm_o
+
+
+def test_spatial_build_1M_in_64_bins(ctx):
+    # 2^20 messages in random order over a 4^3 grid: every bin holds ~16 K members, so the
+    # in-bin order comes from the per-bin bitmap ranks (a quadratic rank would take seconds)
+    P = np.random.default_rng(64).uniform(0, 4.0, size=(3, 1 << 20)).astype(np.float32)
+    pbm = _spatial_build_check(ctx, P, [0.0] * 3, [4.0] * 3, 1.0, 0.05)
+    assert len(pbm) == 65
+
+
+def test_spatial_build_bins_of_every_size(ctx):
+    # a 64^3 grid with 1.5 M messages: one bin holds 700 K of them spread over the whole
+    # input (three bitmap windows), 100 bins hold 200 each (LDS ranks), 40 bins hold 1,000
+    # each (one bitmap window) and the rest are uniform (at most a few per bin)
+    rng = np.random.default_rng(11)
+    n = 1_500_000
+    P = rng.uniform(0, 3.2, size=(3, n)).astype(np.float32)
+    idx = rng.permutation(n)
+    giant, k = idx[:700_000], 700_000
+    P[:, giant] = np.float32(1.6125)
+    cell = np.float32(0.05)
+    for j in range(100):
+        sel = idx[k:k + 200]
+        k += 200
+        c = rng.integers(0, 64, size=3).astype(np.float32)
+        P[:, sel] = ((c[:, None] + 0.25 + 0.5 * rng.uniform(size=(1, 200))) * cell).astype(np.float32)
+    for j in range(40):
+        sel = idx[k:k + 1000]
+        k += 1000
+        c = rng.integers(0, 64, size=3).astype(np.float32)
+        P[:, sel] = ((c[:, None] + 0.25 + 0.5 * rng.uniform(size=(3, 1000))) * cell).astype(np.float32)
+    pbm = _spatial_build_check(ctx, P, [0.0] * 3, [3.2] * 3, 0.05, 0.05)
+    sizes = np.diff(pbm)
+    assert sizes.max() >= 700_000 and ((sizes > 32) & (sizes <= 512)).sum() >= 50 and ((sizes > 512) & (sizes < 5000)).sum() >= 30
+
+
+def test_spatial_build_one_bin(ctx):
+    # every message in the same bin
+    P = np.full((3, 300_000), 0.5, np.float32)
+    _spatial_build_check(ctx, P, [0.0] * 3, [3.2] * 3, 0.05, 0.05)
+
+
+@pytest.mark.parametrize("n,keys", [(1 << 20, 8), (100_000, 1), (70_000, 3000)])
+def test_bucket_build_few_keys(ctx, n, keys):
+    # bucket index (a stable sort by integer key) with few keys, random order
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(n + keys)
+    key_min = -3
+    key = rng.integers(key_min, key_min + keys, size=n).astype(np.int32)
+    val = rng.uniform(size=n).astype(np.float32)
+    ids = np.arange(n, dtype=np.uint32)
+    perm = O.stable_sort_perm((key - key_min).astype(np.uint32))
+    table = np.zeros(keys + 1, np.uint32)
+    table[1:] = np.cumsum(np.bincount(key - key_min, minlength=keys)).astype(np.uint32)
+    d_key, d_val, d_ids = dev(key), dev(val), dev(ids)
+    o_key, o_val, o_ids = empty(n, np.int32), empty(n, np.float32), empty(n, np.uint32)
+    pbm = empty(keys + 1, np.uint32)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_SPATIAL_BUILD, n, keys)
+    s = scratch(sb)
+
+    def run():
+        fgx.bucket_build(ctx, key_min, keys, n, ptr(d_key), [ptr(d_key), ptr(d_val), ptr(d_ids)],
+                         [ptr(o_key), ptr(o_val), ptr(o_ids)], [4, 4, 4], ptr(pbm), ptr(s), sb)
+    t = _timed(run)
+    assert np.array_equal(host(pbm, np.uint32, keys + 1), table)
+    assert np.array_equal(host(o_ids, np.uint32, n), perm)
+    assert np.array_equal(host(o_key, np.int32, n), key[perm])
+    assert np.array_equal(host(o_val, np.float32, n), val[perm])
+    print(f"bucket build n={n} keys={keys}: {t * 1e3:.3f} ms")
+    assert t < 0.05, t
+
+
 @pytest.mark.parametrize("dims,n,mn,mx,r,seed", [
     (3, 1 << 20, [0] * 3, [3.2] * 3, 0.05, 7),
     (3, 16384, [0] * 3, [25.398416] * 3, 2.0, 12),
