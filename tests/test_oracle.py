@@ -264,6 +264,25 @@ def test_births_append_with_scan_ids():
     assert len(set(dstid[:c].tolist())) == c
 
 
+def test_births_ids_by_parent_index():
+    # once a parent read its child's ID in-kernel, the launch reserved one ID per parent:
+    # child ID = next_id + parent index, next_id advances by the parent count; children are
+    # still appended in parent order
+    rng = np.random.default_rng(7)
+    n_par, existing = 3000, 500
+    flags = (rng.uniform(size=n_par) < 0.4).astype(np.uint32)
+    newx = rng.uniform(size=n_par).astype(np.float32)
+    cap = existing + n_par
+    dstx = np.zeros(cap, np.float32)
+    dstid = np.zeros(cap, np.uint32); dstid[:existing] = np.arange(existing) + 1
+    c, nid = O.births(flags, [newx, None], [dstx, dstid], existing, 1, existing + 1, by_parent_index=True)
+    par = np.nonzero(flags)[0]
+    assert c == existing + len(par) and nid == existing + 1 + n_par
+    assert np.array_equal(dstx[existing:c], newx[par])
+    assert np.array_equal(dstid[existing:c], existing + 1 + par)
+    assert len(set(dstid[:c].tolist())) == c
+
+
 def test_agent_sort_reverse_known_answer():
     # test_spatial_agent_sort.cu:70-115: 4 agents at (-i,-i,-i), env [-5,5]^3, r=0.2 -> order 3,2,1,0
     x = -np.arange(4, dtype=np.float32)
@@ -372,3 +391,323 @@ def test_predprey_population_accounting_small():
         seen_pred |= set(ids_pred.tolist())
         assert np.all(pa[4] < 50)
         assert np.all((pa[0] >= 0) & (pa[0] <= 100))
+
+
+# ---------------------------------------------------------------- model math, closed forms
+# Each case is small enough that the expected state follows from the model's rules by hand
+# (float64 here, FP32 in the oracle), so a dropped term, a wrong sign or a swapped operand in
+# the oracle's agent functions fails one of them.  The rules: circles
+# examples/cpp/circles_spatial3D/src/main.cu:13-48; boids
+# examples/python_native/boids_spatial3D_wrapped/boids_spatial3D.py:123-256; predator-prey as
+# specified in oracle/models.cpp.
+def _agents_by_id(sim, t, dtypes):
+    a, ids = sim.get_agents(t, dtypes)
+    o = np.argsort(ids)
+    return [x[o].astype(np.float64) if x.dtype.kind == "f" else x[o] for x in a], ids[o]
+
+
+def _circle_k(d, r=2.0, repulse=0.05):
+    return np.sin(d / r * 3.141 * -2.0) * repulse
+
+
+@pytest.mark.parametrize("d", [0.3, 0.5, 0.9, 1.2, 1.5, 1.99])
+def test_circles_two_agents_closed_form(d):
+    # A at (4,5,5), B at (4+d,5,5): each moves along x by the force k(d) = sin(-2*3.141*d/r)*0.05,
+    # A by +k and B by -k (repulsion for d < r/2, attraction beyond); drift = |k|.
+    params = np.array([10.0, 2.0, 0.05], np.float32)
+    s = O.Sim(0, params, 1)
+    x = np.array([4.0, 4.0 + d], np.float32)
+    c = np.array([5.0, 5.0], np.float32)
+    s.set_agents(0, [x, c.copy(), c.copy(), np.zeros(2, np.float32)])
+    s.step(1)
+    (X, Y, Z, D), ids = _agents_by_id(s, 0, [np.float32] * 4)
+    dd = float(np.float32(4.0 + d)) - 4.0
+    k = _circle_k(dd)
+    assert (k < 0) == (dd < 1.0)
+    np.testing.assert_allclose(X, [4.0 + k, 4.0 + dd - k], rtol=0, atol=2e-6)
+    np.testing.assert_allclose(D, [abs(k)] * 2, rtol=1e-5)
+    assert np.all(Y == 5.0) and np.all(Z == 5.0)
+
+
+def test_circles_three_agents_and_outside_radius():
+    # A (5,5,5), B (5.6,5,5), C (5,5.8,5): B-C is 1.0 apart; D (8.5,5,5) is 2.9 from B, inside the
+    # searched bins but outside r, so it feels nothing and moves nothing.  Forces are averaged
+    # over the in-radius neighbours.
+    params = np.array([10.0, 2.0, 0.05], np.float32)
+    s = O.Sim(0, params, 1)
+    P = np.array([[5.0, 5.0, 5.0], [5.6, 5.0, 5.0], [5.0, 5.8, 5.0], [8.5, 5.0, 5.0]], np.float32)
+    s.set_agents(0, [P[:, 0].copy(), P[:, 1].copy(), P[:, 2].copy(), np.zeros(4, np.float32)])
+    s.step(1)
+    (X, Y, Z, D), ids = _agents_by_id(s, 0, [np.float32] * 4)
+    P64 = P.astype(np.float64)
+    exp = []
+    for i in range(4):
+        f, cnt = np.zeros(3), 0
+        for j in range(4):
+            v = P64[j] - P64[i]
+            dist = np.linalg.norm(v)
+            if i != j and 0 < dist < 2.0:
+                f += _circle_k(dist) * v / dist
+                cnt += 1
+        exp.append(f / max(cnt, 1))
+    exp = np.array(exp)
+    np.testing.assert_allclose(np.stack([X, Y, Z], 1), P64 + exp, rtol=0, atol=3e-6)
+    np.testing.assert_allclose(D, np.linalg.norm(exp, axis=1), rtol=1e-5, atol=1e-9)
+    assert D[3] == 0.0 and X[3] == np.float32(8.5)
+    # the averaged force on A: B pushes (d=0.6 < 1), C pushes (d=0.8 < 1), half each
+    assert exp[0][0] < 0 and exp[0][1] < 0
+
+
+def test_circles_clamped_to_box():
+    # pushed out through the low face: the position is clamped to 0 (the box is [0, max]^3)
+    params = np.array([10.0, 2.0, 0.05], np.float32)
+    s = O.Sim(0, params, 1)
+    x = np.array([0.02, 0.32], np.float32)
+    c = np.array([5.0, 5.0], np.float32)
+    s.set_agents(0, [x, c.copy(), c.copy(), np.zeros(2, np.float32)])
+    s.step(1)
+    (X, _, _, D), _ = _agents_by_id(s, 0, [np.float32] * 4)
+    k = _circle_k(0.3)
+    assert 0.02 + k < 0 and X[0] == 0.0
+    np.testing.assert_allclose(X[1], 0.32 - k, atol=2e-6)
+
+
+BOIDS_P = None
+
+
+def _boids_params():
+    from flamegpu2_amd import workloads
+    p, _ = workloads.boids(4)
+    return p   # [min 0, max 3.2, r 0.05, sep 0.015, dt 1, global 0.15, steer 0.005, coll 0.01, match 0.125, vmin 0.025, vmax 0.05]
+
+
+def _boids_run(P, V, steps=1):
+    params = _boids_params()
+    s = O.Sim(1, params, 7)
+    P = np.asarray(P, np.float32)
+    V = np.asarray(V, np.float32)
+    s.set_agents(0, [P[:, 0].copy(), P[:, 1].copy(), P[:, 2].copy(), V[:, 0].copy(), V[:, 1].copy(), V[:, 2].copy()])
+    s.step(steps)
+    a, ids = _agents_by_id(s, 0, [np.float32] * 6)
+    return np.stack(a[:3], 1), np.stack(a[3:], 1)
+
+
+def _boids_expect(P, V, W=3.2):
+    """One boids step by hand, float64, minimum-image displacement across the wrap."""
+    G, STEER, COLL, MATCH, SEP, R = 0.15, 0.005, 0.01, 0.125, 0.015, 0.05
+    VMIN, VMAX = 0.025, 0.05
+    P = np.asarray(P, np.float32).astype(np.float64)
+    V = np.asarray(V, np.float32).astype(np.float64)
+    outP, outV = [], []
+    for i in range(len(P)):
+        near = []
+        for j in range(len(P)):
+            if j == i:
+                continue
+            d = P[j] - P[i]
+            d -= W * np.round(d / W)          # position of j seen from i, across the wrap
+            if np.linalg.norm(d) < R:
+                near.append((d, V[j]))
+        steer = np.zeros(3)
+        if near:
+            centre = np.mean([d for d, _ in near], axis=0)                 # cohesion: towards the centre
+            steer += centre * STEER
+            steer += np.mean([v for _, v in near], axis=0) * MATCH - V[i]  # alignment
+            for d, _ in near:                                              # separation: away, (1 - d/SEP)^2
+                dist = np.linalg.norm(d)
+                if dist < SEP:
+                    steer -= COLL * d * (1.0 - dist / SEP) ** 2
+        v = V[i] + G * steer
+        sp = np.linalg.norm(v)
+        v = v / sp * VMAX if sp > VMAX else (v / sp * VMIN if sp < VMIN else v)
+        outV.append(v)
+        outP.append(np.mod(P[i] + v, W))
+    return np.array(outP), np.array(outV)
+
+
+@pytest.mark.parametrize("s,sign", [(0.003, -1.0), (0.006, +1.0)])
+def test_boids_separation_against_cohesion(s, sign):
+    # Two boids at rest, s apart on x: cohesion pulls with STEER*s, separation pushes with
+    # COLL*s*(1 - s/SEP)^2; they balance where (1 - s/SEP)^2 = STEER/COLL, s* = SEP*(1 - 1/sqrt 2)
+    # = 0.00439.  The resulting speed is tiny, so the minimum-speed clamp sets |v| = VMIN:
+    # below s* A moves away from B (-x), above it towards B (+x).
+    P = [[1.6, 1.6, 1.6], [1.6 + s, 1.6, 1.6]]
+    V = [[0.0, 0.0, 0.0], [0.0, 0.0, 0.0]]
+    gotP, gotV = _boids_run(P, V)
+    np.testing.assert_allclose(gotV[0], [sign * 0.025, 0, 0], rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(gotV[1], [-sign * 0.025, 0, 0], rtol=1e-6, atol=1e-9)
+    eP, eV = _boids_expect(P, V)
+    np.testing.assert_allclose(gotV, eV, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(gotP, eP, rtol=0, atol=1e-6)
+
+
+def test_boids_alignment_and_cohesion_no_separation():
+    # 0.03 apart (outside SEP): A steers towards B's place and towards MATCH x B's velocity,
+    # and away from its own velocity.  Speeds stay inside [VMIN, VMAX].
+    P = [[1.0, 1.0, 1.0], [1.03, 1.0, 1.0]]
+    V = [[0.04, 0.0, 0.0], [0.0, 0.04, 0.01]]
+    gotP, gotV = _boids_run(P, V)
+    eP, eV = _boids_expect(P, V)
+    dv = gotV - np.asarray(V, np.float32).astype(np.float64)
+    edv = eV - np.asarray(V, np.float32).astype(np.float64)
+    np.testing.assert_allclose(dv, edv, rtol=1e-5, atol=1e-9)
+    # by hand for A: 0.15 * ((0.03*0.005, 0, 0) + 0.125*(0, 0.04, 0.01) - (0.04, 0, 0))
+    np.testing.assert_allclose(edv[0], 0.15 * np.array([0.03 * 0.005 - 0.04, 0.005, 0.00125]), rtol=1e-5)
+    np.testing.assert_allclose(gotP, eP, rtol=0, atol=1e-6)
+
+
+def test_boids_three_body_means():
+    # A sees B and C (both within r, C within SEP): cohesion and alignment use the mean of the two
+    P = [[2.0, 2.0, 2.0], [2.02, 2.01, 2.0], [2.0, 1.99, 2.008]]
+    V = [[0.03, 0.0, 0.0], [0.0, 0.03, 0.0], [0.0, 0.0, -0.03]]
+    gotP, gotV = _boids_run(P, V)
+    eP, eV = _boids_expect(P, V)
+    dv = gotV - np.asarray(V, np.float32).astype(np.float64)
+    edv = eV - np.asarray(V, np.float32).astype(np.float64)
+    np.testing.assert_allclose(dv, edv, rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(gotP, eP, rtol=0, atol=1e-6)
+
+
+def test_boids_speed_clamps():
+    # alone: no steering; 0.1 is clamped down to VMAX, 0.02 and 0.01 raised to VMIN, 0.04 kept,
+    # direction kept
+    P = [[0.5, 0.5, 0.5], [2.5, 2.5, 2.5], [1.5, 0.5, 2.5], [0.5, 2.5, 1.5]]
+    V = [[0.06, 0.08, 0.0], [0.0, 0.012, -0.016], [0.006, 0.0, 0.008], [0.024, 0.0, -0.032]]
+    gotP, gotV = _boids_run(P, V)
+    np.testing.assert_allclose(gotV[0], [0.03, 0.04, 0.0], rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(gotV[1], [0.0, 0.015, -0.02], rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(gotV[2], [0.015, 0.0, 0.02], rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(gotV[3], np.asarray(V[3], np.float32), rtol=0, atol=0)
+    np.testing.assert_allclose(gotP, np.asarray(P) + gotV, rtol=0, atol=1e-6)
+
+
+def test_boids_pair_across_the_wrap():
+    # A pair straddling x = 0 / 3.2 interacts through the wrap exactly as the same pair away
+    # from the boundary: velocities agree, and A's position wraps to the far side.
+    Pw = [[0.002, 1.0, 1.0], [3.196, 1.0, 1.0]]     # 0.006 apart across the wrap
+    Pi = [[1.602, 1.0, 1.0], [1.596, 1.0, 1.0]]     # the same pair, shifted by 1.6
+    V = [[-0.03, 0.0, 0.0], [0.0, 0.03, 0.0]]
+    gw, vw = _boids_run(Pw, V)
+    gi, vi = _boids_run(Pi, V)
+    np.testing.assert_allclose(vw, vi, rtol=1e-4, atol=1e-8)
+    eP, eV = _boids_expect(Pw, V)
+    np.testing.assert_allclose(vw, eV, rtol=1e-4, atol=1e-8)
+    assert gw[0][0] > 3.0   # moved from 0.002 by about -0.026: wrapped to about 3.176
+    np.testing.assert_allclose(gw, eP, rtol=0, atol=1e-5)
+
+
+def _pp_params(width=1000.0):
+    from flamegpu2_amd import workloads
+    p, _, _ = workloads.predprey(1, 1, seed=3, width=width)
+    return p   # [width, radius 5, eat_r 1, p_prey_death .005, p_prey_repro .02, p_pred_repro .01, starve 50, jitter 1, flee .5, hunt .5]
+
+
+def _uniform(seed, aid, step, stream, k):
+    """The keyed agent stream from the published Philox4x32-10 (pinned above) and the
+    reference's float mapping (AgentRandom.cuh:80-83)."""
+    blk = O.philox([k >> 2, stream, step, aid], [seed & 0xFFFFFFFF, seed >> 32])
+    return float(np.float32((blk[k & 3] & 0xFFFFFF) / 16777216.0))
+
+
+def _pp_sim(seed, prey, pred, width=1000.0):
+    s = O.Sim(2, _pp_params(width), seed)
+    s.set_agents(0, [np.asarray(a, np.float32) for a in prey])
+    s.set_agents(1, [np.asarray(a, np.float32) for a in pred[:4]] + [np.asarray(pred[4], np.uint32)])
+    return s
+
+
+@pytest.mark.parametrize("birth", [True, False])
+def test_predprey_predator_eats_and_reproduces(birth):
+    # One prey 0.5 from one predator: the prey is eaten whatever its draws; the predator's
+    # hunger resets, it turns towards the prey (hunt 0.5, speed capped at 1), moves, and
+    # reproduces iff its draw (stream 7, draw 0) is below 0.01.  The child sits at the
+    # parent's new place plus (u - 0.5) * jitter, with velocity 2u - 1, hunger 0, ID 2.
+    seed = next(sd for sd in range(1, 20000) if (_uniform(sd, 1, 0, 7, 0) < 0.01) == birth)
+    prey = [[500.0], [500.0], [0.3], [0.0]]
+    pred = [[500.5], [500.0], [0.2], [0.4], [7]]
+    s = _pp_sim(seed, prey, pred)
+    s.step(1)
+    assert s.count(0) == 0
+    (PX, PY, PVX, PVY, LIFE), ids = _agents_by_id(s, 1, [np.float32] * 4 + [np.uint32])
+    vx, vy = 0.2 + (-0.5 / 0.5) * 0.5, 0.4        # nearest prey at dx = -0.5
+    sp = np.hypot(vx, vy)
+    if sp > 1.0:
+        vx, vy = vx / sp, vy / sp
+    px, py = 500.5 + vx, 500.0 + vy
+    assert ids[0] == 1
+    np.testing.assert_allclose([PX[0], PY[0], PVX[0], PVY[0]], [px, py, vx, vy], atol=1e-4)
+    assert LIFE[0] == 0
+    if birth:
+        assert list(ids) == [1, 2]
+        u = [_uniform(seed, 1, 0, 7, k) for k in range(5)]
+        np.testing.assert_allclose([PX[1], PY[1], PVX[1], PVY[1]],
+                                   [px + (u[1] - 0.5), py + (u[2] - 0.5), 2 * u[3] - 1, 2 * u[4] - 1], atol=1e-4)
+        assert LIFE[1] == 0
+    else:
+        assert list(ids) == [1]
+
+
+@pytest.mark.parametrize("life0", [0, 13, 49])
+def test_predprey_predator_starves_after_50_hungry_steps(life0):
+    # no prey: hunger rises by one per step and the predator dies at the end of the step in
+    # which it reaches 50; it never reproduces (it has not eaten)
+    s = _pp_sim(5, [[], [], [], []], [[100.0], [100.0], [0.5], [0.5], [life0]])
+    left = 50 - life0
+    s.step(left - 1)
+    (_, _, _, _, LIFE), ids = _agents_by_id(s, 1, [np.float32] * 4 + [np.uint32])
+    assert list(ids) == [1] and LIFE[0] == 49
+    s.step(1)
+    assert s.count(1) == 0
+
+
+def test_predprey_prey_alone_follow_their_draws():
+    # 40 prey, no predators, 25 steps: each prey dies when its stream-1 draw is below 0.005 and
+    # otherwise gives birth when its stream-3 draw is below 0.02; children get the next IDs in
+    # parent order (here parent order = list order after the spatial sort, so compare sets and
+    # counts) and their values come from the parent's draws.  Speeds above 1 are capped at 1.
+    seed, n = 11, 40
+    rng = np.random.default_rng(0)
+    x = rng.uniform(0, 1000, n).astype(np.float32)
+    y = rng.uniform(0, 1000, n).astype(np.float32)
+    vx = rng.uniform(-0.6, 0.6, n).astype(np.float32)
+    vy = rng.uniform(-0.6, 0.6, n).astype(np.float32)
+    s = _pp_sim(seed, [x, y, vx, vy], [[], [], [], [], []])
+    alive = {i + 1: (x[i], y[i], vx[i], vy[i]) for i in range(n)}
+    next_id = n + 1
+    W = np.float32(1000.0)
+
+    def wrap(v):
+        v = np.float32(v)
+        if v < 0:
+            v = np.float32(v + W)
+        if v >= W:
+            v = np.float32(v - W)
+        return v
+    for step in range(25):
+        s.step(1)
+        survivors = {}
+        for aid, (px, py, pvx, pvy) in alive.items():
+            if _uniform(seed, aid, step, 1, 0) < 0.005:
+                continue
+            sp = np.sqrt(np.float32(pvx * pvx + pvy * pvy), dtype=np.float32)
+            if sp > 1.0:   # the flee function caps the speed at 1 (children start at up to sqrt 2)
+                pvx, pvy = np.float32(pvx / sp), np.float32(pvy / sp)
+            survivors[aid] = (wrap(px + pvx), wrap(py + pvy), pvx, pvy)
+        got = s.get_agents(0, [np.float32] * 4)
+        (GX, GY, GVX, GVY), gids = got
+        parents = [aid for aid in gids if aid in survivors]   # parents in list order at birth time
+        born = {}
+        for aid in parents:
+            if _uniform(seed, aid, step, 3, 0) < 0.02:
+                px, py = survivors[aid][:2]
+                u = [_uniform(seed, aid, step, 3, k) for k in range(5)]
+                born[next_id] = (wrap(np.float32(px) + np.float32((np.float32(u[1]) - np.float32(0.5)) * np.float32(1.0))),
+                                 wrap(np.float32(py) + np.float32((np.float32(u[2]) - np.float32(0.5)) * np.float32(1.0))),
+                                 np.float32(np.float32(u[3]) * np.float32(2.0) - np.float32(1.0)),
+                                 np.float32(np.float32(u[4]) * np.float32(2.0) - np.float32(1.0)))
+                next_id += 1
+        alive = {**survivors, **born}
+        assert sorted(gids.tolist()) == sorted(alive), f"step {step}"
+        for i, aid in enumerate(gids):
+            np.testing.assert_allclose([GX[i], GY[i], GVX[i], GVY[i]], alive[aid], rtol=0, atol=1e-4)
+    assert next_id > n + 1 and len(alive) != n   # both births and deaths happened
