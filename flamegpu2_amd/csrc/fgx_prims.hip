@@ -600,13 +600,15 @@ struct CompactArgs {
     uint32_t* d_dst_count;
     uint32_t dst_capacity;
     uint32_t* d_next_id;       // may be NULL (no IDs)
+    uint32_t* d_id_slots;      // may be NULL; nonzero: ID = next ID + parent index (cleared here)
     uint32_t* d_overflow;
     // item ranges (device-resident, NULL = 0): items are [*d_src_off, n); item k has
     // flag k - *d_keep_prefix, and the first *d_keep_prefix items are kept unconditionally
     const uint32_t* d_src_off;
     const uint32_t* d_keep_prefix;
     // scratch: the inputs as read by the first pass, then the per-tile kept counts
-    uint32_t* stash;           // [0] items n, [1] source offset, [2] kept prefix, [3] dst count, [4] next ID
+    uint32_t* stash;           // [0] items n, [1] source offset, [2] kept prefix, [3] dst count, [4] next ID,
+                               // [5] ID by parent index
     uint32_t* counts;          // num_tiles words
 };
 
@@ -637,6 +639,7 @@ __global__ void __launch_bounds__(kBlock) k_compact_counts(CompactArgs a) {
         a.stash[2] = pre;
         a.stash[3] = a.births ? __hip_atomic_load(a.d_dst_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
         a.stash[4] = a.births && a.d_next_id ? __hip_atomic_load(a.d_next_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        a.stash[5] = a.births && a.d_id_slots ? __hip_atomic_load(a.d_id_slots, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
     }
     uint64_t masks[kItems];
     compact_masks(a, blockIdx.x, n, pre, masks);
@@ -660,6 +663,7 @@ __global__ void __launch_bounds__(kBlock) k_compact_scatter(CompactArgs a, VarTa
     __shared__ uint32_t s_cnt[kItems * (kBlock / 64)];
     const uint32_t tile = blockIdx.x;
     const uint32_t n = a.stash[0], off = a.stash[1], pre = a.stash[2], dst0 = a.stash[3], id0 = a.stash[4];
+    const bool by_slot = a.stash[5] != 0u;
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint64_t lanes_below = (1ull << lane) - 1ull;
     uint64_t masks[kItems];
@@ -693,7 +697,8 @@ __global__ void __launch_bounds__(kBlock) k_compact_scatter(CompactArgs a, VarTa
                     if (a.d_overflow) *a.d_overflow = 1u;
                 }
                 *a.d_dst_count = nd;
-                if (a.d_next_id) *a.d_next_id = id0 + kept;
+                if (a.d_next_id) *a.d_next_id = id0 + (by_slot ? n : kept);
+                if (a.d_id_slots) *a.d_id_slots = 0u;   // pass 1 stashed it; nothing reads it after
             }
         }
     }
@@ -710,7 +715,7 @@ __global__ void __launch_bounds__(kBlock) k_compact_scatter(CompactArgs a, VarTa
             const uint32_t d = dst0 + r;
             if (d >= a.dst_capacity) continue;
             copy_element(vt, off + i, d, a.id_var);
-            if (a.id_var >= 0) reinterpret_cast<uint32_t*>(vt.out[a.id_var])[d] = id0 + r;
+            if (a.id_var >= 0) reinterpret_cast<uint32_t*>(vt.out[a.id_var])[d] = id0 + (by_slot ? i : r);
         }
     }
 }
@@ -1286,7 +1291,7 @@ int fgx_compact(fgx_ctx* ctx, const uint32_t* flags, uint32_t keep, const uint32
 int fgx_append_births(fgx_ctx* ctx, const uint32_t* flags, const uint32_t* d_n_parents, uint32_t n_parents_cap,
                       uint32_t nvars, const void* const* newbuf, void* const* dst, const uint32_t* var_bytes,
                       int32_t id_var, uint32_t* d_dst_count, uint32_t dst_capacity, uint32_t* d_next_id,
-                      uint32_t* d_overflow, void* scratch, size_t scratch_bytes, void* stream) {
+                      uint32_t* d_id_slots, uint32_t* d_overflow, void* scratch, size_t scratch_bytes, void* stream) {
     (void)ctx;
     if (!d_dst_count || !d_next_id) return fail(FGX_ERR_ARG, "fgx_append_births: NULL device counters");
     if (id_var >= (int32_t)nvars) return fail(FGX_ERR_ARG, "fgx_append_births: id_var out of range");
@@ -1306,6 +1311,7 @@ int fgx_append_births(fgx_ctx* ctx, const uint32_t* flags, const uint32_t* d_n_p
     a.d_dst_count = d_dst_count;
     a.dst_capacity = dst_capacity;
     a.d_next_id = d_next_id;
+    a.d_id_slots = d_id_slots;
     a.d_overflow = d_overflow;
     return launch_compact(a, vt, scratch, scratch_bytes, stream);
 }

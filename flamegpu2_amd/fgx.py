@@ -75,7 +75,7 @@ _SIGS = {
                        vp, u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), vp, sz, vp],
     "fgx_compact": [vp, vp, u32, vp, u32, u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), vp, vp, sz, vp],
     "fgx_append_births": [vp, vp, vp, u32, u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), i32, vp, u32, vp,
-                          vp, vp, sz, vp],
+                          vp, vp, vp, sz, vp],
     "fgx_exclusive_scan_u32": [vp, vp, vp, u32, vp, sz, vp],
     "fgx_assign_ids": [vp, u32, u32, vp],
     "fgx_reduce_sum": [vp, _c.c_int, vp, vp, u32, vp, vp, sz, vp],
@@ -202,11 +202,11 @@ def compact(ctx, flags, keep, n_cap, vars_in, vars_out, var_bytes, d_count_out, 
 
 
 def append_births(ctx, flags, n_parents_cap, newbuf, dst, var_bytes, id_var, d_dst_count, dst_capacity, d_next_id,
-                  scratch, scratch_len, stream=0, d_n_parents=None, d_overflow=None):
+                  scratch, scratch_len, stream=0, d_n_parents=None, d_overflow=None, d_id_slots=None):
     nv = len(dst)
     check(lib().fgx_append_births(ctx.h, flags, d_n_parents, n_parents_cap, nv, _arr(vp, newbuf), _arr(vp, dst),
-                                  _arr(u32, var_bytes), id_var, d_dst_count, dst_capacity, d_next_id, d_overflow,
-                                  scratch, scratch_len, stream))
+                                  _arr(u32, var_bytes), id_var, d_dst_count, dst_capacity, d_next_id, d_id_slots,
+                                  d_overflow, scratch, scratch_len, stream))
 
 
 def exclusive_scan(ctx, d_in, d_out, n, scratch, scratch_len, stream=0):

@@ -193,7 +193,6 @@ enum DeviceErrorCode : uint32_t {
     DERR_RESERVED_NAME = 9,
     DERR_UNKNOWN_ENV = 10,
     DERR_ENV_TYPE = 11,
-    DERR_AGENT_OUT_ID = 12,
     DERR_ARRAY_INDEX = 13,
     DERR_RANDOM_RANGE = 14,
     DERR_MSG_OUTPUT_DISABLED = 15,
@@ -220,7 +219,6 @@ inline const char* device_error_text(uint32_t code) {
         case DERR_RESERVED_NAME: return "variable names starting with '_' are reserved for internal use";
         case DERR_UNKNOWN_ENV: return "unknown environment property";
         case DERR_ENV_TYPE: return "environment property accessed with the wrong type or length";
-        case DERR_AGENT_OUT_ID: return "AgentOut::getID() is not available: new agent IDs are assigned by the ordered birth scan";
         case DERR_ARRAY_INDEX: return "array variable index out of bounds";
         case DERR_RANDOM_RANGE: return "invalid arguments passed to AgentRandom::uniform()";
         case DERR_MSG_OUTPUT_DISABLED: return "message output used by a function without message output";
@@ -240,6 +238,8 @@ struct FnArgs {
     uint32_t* death_flags;       // non-null when agent death is enabled
     uint32_t* msg_out_flags;     // non-null with message output
     uint32_t* birth_flags;       // non-null with agent output
+    const uint32_t* d_new_next_id;  // agent output: the output agent's next ID (constant during the launch)
+    uint32_t* d_id_slots;        // agent output: set to 1 when a parent reads its child's ID
     const char* env;             // environment property buffer (device)
     const uint32_t* d_msg_in_count;
     const uint32_t* ids;         // the agent list's "_id" array

@@ -1244,11 +1244,19 @@ class DeviceAPI {
                                   detail::DERR_UNKNOWN_AGENT_VAR, detail::DERR_AGENT_VAR_TYPE);
             if (on) c_.a.birth_flags[c_.lidx] = 1u;
         }
-        // New-agent IDs are nextID + the parent's rank among this launch's births, which is
-        // only known after the launch; the ID can be read from the new agent next step.
+        // Marks the birth (as setVariable does) and returns the child's ID.  Children of a
+        // launch normally get nextID + their parent's rank among the parents that gave
+        // birth, known only after the launch; once a parent reads the ID here, the launch
+        // uses nextID + the parent's index among the launch's agents for every child
+        // instead (the append after the launch reserves one ID per parent), so the value
+        // returned here is the child's ID whatever the scheduling.
         __device__ __forceinline__ id_t getID() const {
-            c_.err.note(true, detail::DERR_AGENT_OUT_ID, 0);
-            return ID_NOT_SET;
+            const bool on = c_.a.birth_flags != nullptr;
+            c_.err.note(!on, detail::DERR_AGENT_OUTPUT_DISABLED, 0);
+            if (!on) return ID_NOT_SET;
+            c_.a.birth_flags[c_.lidx] = 1u;
+            *c_.a.d_id_slots = 1u;
+            return *c_.a.d_new_next_id + c_.lidx;
         }
 
      private:

@@ -242,6 +242,7 @@ struct FnRT {
     uint32_t* cond_flags = nullptr;
     uint32_t* d_first = nullptr;
     uint32_t* d_work = nullptr;
+    uint32_t* d_id_slots = nullptr;   // agent output: set when a parent read its child's ID in-kernel
     void* cond_scratch = nullptr;
     size_t cond_scratch_bytes = 0;
     void* stream = nullptr;
@@ -990,6 +991,7 @@ inline void CUDASimulation::ensure_init() {
                 if (r.min->meta.dims == 2 && isf("x") && isf("y")) r.sort_dims = 2;
             }
             r.d_launch_count = counter();
+            if (r.out_agent) r.d_id_slots = counter();
             if (f->condition) {
                 r.d_first = counter();
                 r.d_work = counter();
@@ -1610,6 +1612,8 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
             defs.insert(defs.end(), f.newbuf[i].def.begin(), f.newbuf[i].def.end());
         }
         a.birth_flags = f.birth_flags;
+        a.d_new_next_id = f.out_agent->d_next_id;
+        a.d_id_slots = f.d_id_slots;
         // broadcastInit: new agents start from the variable defaults.
         detail::fgx_check(fgx_fill_vars((uint32_t)ptrs.size(), ptrs.data(), bytes.data(), defs.data(), offs.data(), l.cap, st),
                           "new-agent defaults");
@@ -1739,7 +1743,7 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
             }
             detail::fgx_check(fgx_append_births(ctx_, f.msg_flags, f.launch_count, l.cap, (uint32_t)m.vars.size(),
                                                 src.data(), dst.data(), by.data(), -1, m.d_count, m.cap, d_dummy_,
-                                                d_overflow_, f.append_scratch, f.append_scratch_bytes, st),
+                                                nullptr, d_overflow_, f.append_scratch, f.append_scratch_bytes, st),
                               "message append");
         }
         tick("msgout:" + f.d->name, st, false);
@@ -1791,8 +1795,8 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
             detail::fgx_check(fgx_memcpy_d2d(l.d_count, f.d_first, 4, st), "state transition count");
         } else {
             detail::fgx_check(fgx_append_births(ctx_, nullptr, l.d_count, l.cap, (uint32_t)l.vars.size(), src.data(),
-                                                dst.data(), by.data(), -1, dl.d_count, dl.cap, d_dummy_, d_overflow_,
-                                                f.trans_scratch, f.trans_scratch_bytes, st),
+                                                dst.data(), by.data(), -1, dl.d_count, dl.cap, d_dummy_, nullptr,
+                                                d_overflow_, f.trans_scratch, f.trans_scratch_bytes, st),
                               "state transition");
             detail::fgx_check(fgx_fill_u32(l.d_count, 0, 1, st), "state transition count");
         }
@@ -1811,7 +1815,7 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
         tick("births:" + f.d->name, st, true);
         detail::fgx_check(fgx_append_births(ctx_, f.birth_flags, f.launch_count, f.cap, (uint32_t)dl.vars.size(), src.data(),
                                             dst.data(), by.data(), dl.id_slot, dl.d_count, dl.cap, f.out_agent->d_next_id,
-                                            d_overflow_, f.births_scratch, f.births_scratch_bytes, st),
+                                            f.d_id_slots, d_overflow_, f.births_scratch, f.births_scratch_bytes, st),
                           "agent births");
         tick("births:" + f.d->name, st, false);
     }

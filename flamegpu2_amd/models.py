@@ -117,10 +117,12 @@ class Simulation:
             out.append((name.value.decode(), _DT.get(t.value, np.uint8), e.value, b.value))
         return out
 
-    def set_agents(self, agent, arrays, ids=None, state=None):
+    def set_agents(self, agent, arrays, ids=None, state=None, n=None):
+        """n: the population size, needed only for an agent without variables."""
         info = self.var_info(agent)
         assert len(arrays) == len(info), f"{agent} has {len(info)} variables"
-        n = len(arrays[0])
+        n = len(arrays[0]) if arrays else (len(ids) if ids is not None else n)
+        assert n is not None, "set_agents: give n for an agent without variables"
         keep = []
         for a, (name, dt, e, b) in zip(arrays, info):
             a = np.ascontiguousarray(a, dtype=dt)

@@ -211,11 +211,17 @@ int fgx_compact(fgx_ctx* ctx, const uint32_t* flags, uint32_t keep, const uint32
  * append newbuf[*][i] after *d_dst_count entries of dst, in parent order;
  * variable id_var (-1: none) receives *d_next_id + rank.  *d_dst_count and
  * *d_next_id advance by the number of births.  Births beyond dst_capacity
- * are dropped and reported through *d_overflow (set to 1) when non-NULL. */
+ * are dropped and reported through *d_overflow (set to 1) when non-NULL.
+ * d_id_slots (may be NULL) is a device word that an agent function sets to 1 when
+ * a parent reads its child's ID in-kernel (AgentOut::getID): the launch then
+ * reserved one ID per parent, so every child of the append receives
+ * *d_next_id + its parent's index, *d_next_id advances by the parent count, and
+ * the append clears the word.  Either way IDs are unique and do not depend on
+ * scheduling. */
 int fgx_append_births(fgx_ctx* ctx, const uint32_t* flags, const uint32_t* d_n_parents, uint32_t n_parents_cap,
                       uint32_t nvars, const void* const* newbuf, void* const* dst, const uint32_t* var_bytes,
                       int32_t id_var, uint32_t* d_dst_count, uint32_t dst_capacity, uint32_t* d_next_id,
-                      uint32_t* d_overflow, void* scratch, size_t scratch_bytes, void* stream);
+                      uint32_t* d_id_slots, uint32_t* d_overflow, void* scratch, size_t scratch_bytes, void* stream);
 
 /* ---------------------------------------------------------------- agent function conditions
  * CUDAFatAgent::processFunctionCondition (src/flamegpu/simulation/detail/CUDAFatAgent.cu:186-236):

@@ -94,6 +94,13 @@ uint32_t orc_compact(uint32_t n, const uint32_t* flags, uint32_t keep,
 uint32_t orc_births(uint32_t n_parents, const uint32_t* flags,
                     uint32_t nvars, const void* const* newbuf, const uint32_t* var_bytes,
                     void* const* dst, uint32_t dst_count, uint32_t id_var, uint32_t* next_id);
+/* The same, with by_parent_index != 0: the launch's parents reserved one ID each, so a
+ * child's ID is next_id + its parent's index and *next_id advances by n_parents (the
+ * library's rule once a parent reads AgentOut::getID() in-kernel). */
+uint32_t orc_births_ex(uint32_t n_parents, const uint32_t* flags,
+                       uint32_t nvars, const void* const* newbuf, const uint32_t* var_bytes,
+                       void* const* dst, uint32_t dst_count, uint32_t id_var, uint32_t* next_id,
+                       int32_t by_parent_index);
 /* Auto spatial sort key (unclamped) masked to the sort's bit range. */
 void orc_agent_sort_keys(int dims, const float* mn, const float* mx, float radius, uint32_t n,
                          const float* x, const float* y, const float* z, uint32_t* keys, uint32_t* key_bits);

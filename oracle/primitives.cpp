@@ -251,9 +251,10 @@ extern "C" uint32_t orc_compact(uint32_t n, const uint32_t* flags, uint32_t keep
 
 // scatterNew (CUDAAgentStateList.cu:189-250): flagged new agents appended after the
 // existing agents in parent order.  ID = next_id + rank (replaces atomicInc, DeviceAPI.cuh:552).
-extern "C" uint32_t orc_births(uint32_t n_parents, const uint32_t* flags,
-                               uint32_t nvars, const void* const* newbuf, const uint32_t* var_bytes,
-                               void* const* dst, uint32_t dst_count, uint32_t id_var, uint32_t* next_id) {
+extern "C" uint32_t orc_births_ex(uint32_t n_parents, const uint32_t* flags,
+                                  uint32_t nvars, const void* const* newbuf, const uint32_t* var_bytes,
+                                  void* const* dst, uint32_t dst_count, uint32_t id_var, uint32_t* next_id,
+                                  int32_t by_parent_index) {
     uint32_t rank = 0;
     for (uint32_t i = 0; i < n_parents; ++i) {
         if (!flags[i]) continue;
@@ -261,7 +262,7 @@ extern "C" uint32_t orc_births(uint32_t n_parents, const uint32_t* flags,
         for (uint32_t v = 0; v < nvars; ++v) {
             const uint32_t sz = var_bytes[v];
             if (v == id_var) {
-                const uint32_t id = *next_id + rank;
+                const uint32_t id = *next_id + (by_parent_index ? i : rank);
                 std::memcpy(static_cast<char*>(dst[v]) + (size_t)d * sz, &id, sizeof(id));
             } else {
                 std::memcpy(static_cast<char*>(dst[v]) + (size_t)d * sz,
@@ -270,8 +271,14 @@ extern "C" uint32_t orc_births(uint32_t n_parents, const uint32_t* flags,
         }
         ++rank;
     }
-    *next_id += rank;
+    *next_id += by_parent_index ? n_parents : rank;
     return dst_count + rank;
+}
+
+extern "C" uint32_t orc_births(uint32_t n_parents, const uint32_t* flags,
+                               uint32_t nvars, const void* const* newbuf, const uint32_t* var_bytes,
+                               void* const* dst, uint32_t dst_count, uint32_t id_var, uint32_t* next_id) {
+    return orc_births_ex(n_parents, flags, nvars, newbuf, var_bytes, dst, dst_count, id_var, next_id, 0);
 }
 
 // calculateSpatialHash (CUDASimulation.cu:372-408) with the grid of spatialSortAgent_async

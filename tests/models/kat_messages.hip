@@ -8,6 +8,10 @@
 //       mode 0 clamped Moore, 1 wrapped Moore, 2 von Neumann, 3 wrapped von Neumann.
 //       Each agent records the value read through at() of its own element and an
 //       order-sensitive hash and count of its neighbourhood.
+//   truncate / append_keep / optional_truncate / optional_append / append_resize
+//       tests/test_cases/runtime/messaging/test_append_truncate.cu:72-310
+//   persistence_off / persistence_on
+//       tests/test_cases/runtime/messaging/test_brute_force.cu:440-600
 #include <string>
 
 #include "flamegpu/flamegpu.h"
@@ -16,8 +20,92 @@ using flamegpu::ALIVE;
 using flamegpu::MessageArray;
 using flamegpu::MessageArray2D;
 using flamegpu::MessageArray3D;
+using flamegpu::MessageBruteForce;
 using flamegpu::MessageBucket;
 using flamegpu::MessageNone;
+
+// ---------------------------------------------------------------- append / truncate
+// (test_append_truncate.cu:22-67, 229-256)
+FLAMEGPU_AGENT_FUNCTION(Out_AppendTruncate, MessageNone, MessageBruteForce) {
+    FLAMEGPU->message_out.setVariable("x", 0);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(Out_AppendTruncate2, MessageNone, MessageBruteForce) {
+    FLAMEGPU->message_out.setVariable("x", 1);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(In_AppendTruncate, MessageBruteForce, MessageNone) {
+    unsigned int count = 0;
+    for (auto& message : FLAMEGPU->message_in) {
+        (void)message;
+        count++;
+    }
+    FLAMEGPU->setVariable<unsigned int>("count", count);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(In_AppendTruncate2, MessageBruteForce, MessageNone) {
+    unsigned int count0 = 0, count1 = 0;
+    for (auto& message : FLAMEGPU->message_in) {
+        const int x = message.getVariable<int>("x");
+        if (x == 0) count0++;
+        else if (x == 1) count1++;
+    }
+    FLAMEGPU->setVariable<unsigned int>("count0", count0);
+    FLAMEGPU->setVariable<unsigned int>("count1", count1);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(OptionalOut_AppendTruncate, MessageNone, MessageBruteForce) {
+    if (FLAMEGPU->getVariable<unsigned int>("do_out") > 0) {
+        FLAMEGPU->message_out.setVariable("x", 0);
+        FLAMEGPU->setVariable<unsigned int>("do_out", 0);
+    } else {
+        FLAMEGPU->setVariable<unsigned int>("do_out", 1);
+    }
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(OptionalOut_AppendTruncate2, MessageNone, MessageBruteForce) {
+    if (FLAMEGPU->getVariable<unsigned int>("do_out") > 0) FLAMEGPU->message_out.setVariable("x", 1);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(Out_1, MessageNone, MessageBruteForce) {
+    FLAMEGPU->message_out.setVariable("x", 1);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(Out_2, MessageNone, MessageBruteForce) {
+    FLAMEGPU->message_out.setVariable("x", 2);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(In_012, MessageBruteForce, MessageNone) {
+    unsigned int c0 = 0, c1 = 0, c2 = 0;
+    for (auto& message : FLAMEGPU->message_in) {
+        const int x = message.getVariable<int>("x");
+        if (x == 1) c1++;
+        else if (x == 2) c2++;
+        else c0++;
+    }
+    FLAMEGPU->setVariable<unsigned int>("0", c0);
+    FLAMEGPU->setVariable<unsigned int>("1", c1);
+    FLAMEGPU->setVariable<unsigned int>("2", c2);
+    return ALIVE;
+}
+
+// ---------------------------------------------------------------- persistence
+// (test_brute_force.cu:447-470): output on even steps only, read every step
+FLAMEGPU_AGENT_FUNCTION_CONDITION(EvenOnlyCondition) { return FLAMEGPU->getStepCounter() % 2 == 0; }
+FLAMEGPU_AGENT_FUNCTION(out_simple, MessageNone, MessageBruteForce) {
+    FLAMEGPU->message_out.setVariable<int>("id", FLAMEGPU->getVariable<int>("id"));
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(in_simple, MessageBruteForce, MessageNone) {
+    unsigned int count = 0, sum = 0;
+    for (auto& m : FLAMEGPU->message_in) {
+        count++;
+        sum += m.getVariable<int>("id");
+    }
+    FLAMEGPU->setVariable<unsigned int>("count", count);
+    FLAMEGPU->setVariable<unsigned int>("sum", sum);
+    return ALIVE;
+}
 
 // ---------------------------------------------------------------- bucket (test_bucket.cu:60-98)
 FLAMEGPU_AGENT_FUNCTION(bucket_out_mandatory, MessageNone, MessageBucket) {
@@ -216,6 +304,70 @@ flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* 
         if (param(6, 0.0f) != 0.0f) out.setMessageOutputOptional(true);
         in.setMessageInput("array");
         model.newLayer().addAgentFunction(out);
+        model.newLayer().addAgentFunction(in);
+    } else if (v == "truncate" || v == "append_keep" || v == "optional_truncate" || v == "optional_append") {
+        const bool optional = v == "optional_truncate" || v == "optional_append";
+        const bool two = v == "append_keep" || v == "optional_append";
+        auto message = model.newMessage("Message");
+        message.newVariable<int>("x");
+        auto a = model.newAgent("Agent");
+        if (two) {
+            a.newVariable<unsigned int>("count0");
+            a.newVariable<unsigned int>("count1");
+        } else {
+            a.newVariable<unsigned int>("count");
+        }
+        if (optional) a.newVariable<unsigned int>("do_out");
+        flamegpu::AgentFunctionDescription fo = optional ? a.newFunction("Out_AppendTruncate", OptionalOut_AppendTruncate)
+                                                         : a.newFunction("Out_AppendTruncate", Out_AppendTruncate);
+        fo.setMessageOutputOptional(optional);
+        fo.setMessageOutput(message);
+        model.newLayer("OutLayer").addAgentFunction(fo);
+        if (two) {
+            flamegpu::AgentFunctionDescription fo2 = optional
+                                                         ? a.newFunction("Out_AppendTruncate2", OptionalOut_AppendTruncate2)
+                                                         : a.newFunction("Out_AppendTruncate2", Out_AppendTruncate2);
+            fo2.setMessageOutputOptional(optional);
+            fo2.setMessageOutput(message);
+            model.newLayer("OutLayer2").addAgentFunction(fo2);
+        }
+        flamegpu::AgentFunctionDescription fi = two ? a.newFunction("In_AppendTruncate", In_AppendTruncate2)
+                                                    : a.newFunction("In_AppendTruncate", In_AppendTruncate);
+        fi.setMessageInput(message);
+        model.newLayer("InLayer").addAgentFunction(fi);
+    } else if (v == "append_resize") {
+        auto message = model.newMessage("Message");
+        message.newVariable<int>("x");
+        auto a = model.newAgent("a");
+        auto fa = a.newFunction("Out_1", Out_1);
+        fa.setMessageOutput(message);
+        auto b = model.newAgent("b");
+        auto fb = b.newFunction("Out_2", Out_2);
+        fb.setMessageOutput(message);
+        auto c = model.newAgent("c");
+        auto fc = c.newFunction("In", In_012);
+        fc.setMessageInput(message);
+        c.newVariable<unsigned int>("0", 0);
+        c.newVariable<unsigned int>("1", 0);
+        c.newVariable<unsigned int>("2", 0);
+        model.newLayer("OutLayer").addAgentFunction(fa);
+        model.newLayer("OutLayer2").addAgentFunction(fb);
+        model.newLayer("InLayer").addAgentFunction(fc);
+    } else if (v == "persistence_off" || v == "persistence_on") {
+        auto message = model.newMessage<MessageBruteForce>("msg");
+        message.newVariable<int>("id");
+        message.setPersistent(v == "persistence_on");
+        auto agent = model.newAgent("agent");
+        agent.newVariable<int>("id");
+        agent.newVariable<unsigned int>("count", 0);
+        agent.newVariable<unsigned int>("sum", 0);
+        auto af = agent.newFunction("out", out_simple);
+        af.setMessageOutput("msg");
+        af.setMessageOutputOptional(true);
+        af.setFunctionCondition(EvenOnlyCondition);
+        auto in = agent.newFunction("in", in_simple);
+        in.setMessageInput("msg");
+        model.newLayer().addAgentFunction(af);
         model.newLayer().addAgentFunction(in);
     } else {
         throw flamegpu::exception::InvalidArgument("unknown test model variant '" + v + "'");

@@ -13,6 +13,8 @@
 //   bruteforce          tests/test_cases/runtime/messaging/test_brute_force.cu:73-200
 //   random              tests/test_cases/runtime/random/test_agent_random.cu:28-185
 //   death_disabled / wrap_out_of_bounds   device errors (test_spatial_3d.cu:976-1039)
+//   agent_id_states / agent_id_two_types  in-kernel child IDs (AgentOut::getID)
+//       tests/test_cases/runtime/agent/test_device_agent_creation.cu:1195-1340
 #include <string>
 
 #include "flamegpu/flamegpu.h"
@@ -300,6 +302,17 @@ FLAMEGPU_STEP_FUNCTION(step_fn1) {
     FLAMEGPU->environment.setProperty<unsigned int, 4>("uint_prop_array", {c[0] + 1, c[1] + 1, c[2] + 1, c[3] + 1});
 }
 
+// test_device_agent_creation.cu:1195-1203
+FLAMEGPU_AGENT_FUNCTION(CopyID, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<flamegpu::id_t>("id_copy", FLAMEGPU->getID());
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(DeviceBirth, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<flamegpu::id_t>("id_other", FLAMEGPU->agent_out.getID());
+    FLAMEGPU->agent_out.setVariable<flamegpu::id_t>("id_other", FLAMEGPU->getID());
+    return ALIVE;
+}
+
 flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* p, uint32_t np) {
     const std::string v = variant_c ? variant_c : "";
     flamegpu::ModelDescription model("kat_" + v);
@@ -463,6 +476,41 @@ flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* 
         agent.newVariable<int>("c");
         agent.newVariable<float>("d");
         model.newLayer().addAgentFunction(agent.newFunction("draw", draw_random));
+    } else if (v == "agent_id_states") {
+        // AgentID_MultipleStatesUniqueIDs (test_device_agent_creation.cu:1204-1244)
+        auto agent = model.newAgent("agent");
+        agent.newVariable<flamegpu::id_t>("id_copy", flamegpu::ID_NOT_SET);
+        agent.newVariable<flamegpu::id_t>("id_other", flamegpu::ID_NOT_SET);
+        agent.newState("a");
+        agent.newState("b");
+        auto af1_a = agent.newFunction("birth", DeviceBirth);
+        af1_a.setAgentOutput(agent, "a");
+        af1_a.setInitialState("a");
+        af1_a.setEndState("a");
+        auto af1_b = agent.newFunction("birth2", DeviceBirth);
+        af1_b.setAgentOutput(agent, "b");
+        af1_b.setInitialState("b");
+        af1_b.setEndState("b");
+        auto af_a = agent.newFunction("copy_id", CopyID);
+        af_a.setInitialState("a");
+        af_a.setEndState("a");
+        auto af_b = agent.newFunction("copy_id2", CopyID);
+        af_b.setInitialState("b");
+        af_b.setEndState("b");
+        model.newLayer().addAgentFunction(af1_a);
+        model.newLayer().addAgentFunction(af1_b);
+        auto layer2 = model.newLayer();
+        layer2.addAgentFunction(af_a);
+        layer2.addAgentFunction(af_b);
+    } else if (v == "agent_id_two_types") {
+        // AgentID_DeviceBirth_MultipleAgents (test_device_agent_creation.cu:1297-1339)
+        auto agentA = model.newAgent("agentA");
+        agentA.newVariable<flamegpu::id_t>("id_other", flamegpu::ID_NOT_SET);
+        auto agentB = model.newAgent("agentB");
+        agentB.newVariable<flamegpu::id_t>("id_other", flamegpu::ID_NOT_SET);
+        auto af1_a = agentA.newFunction("birth", DeviceBirth);
+        af1_a.setAgentOutput(agentB);
+        model.newLayer().addAgentFunction(af1_a);
     } else if (v == "death_disabled") {
         auto agent = model.newAgent("agent");
         agent.newVariable<float>("x");

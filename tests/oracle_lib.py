@@ -140,7 +140,7 @@ def compact(flags, keep, arrays):
     return c, [o[:c] for o in outs]
 
 
-def births(flags, newbuf, dst, dst_count, id_var, next_id):
+def births(flags, newbuf, dst, dst_count, id_var, next_id, by_parent_index=False):
     """dst: list of arrays with spare capacity; newbuf[id_var] is ignored."""
     flags = _u32(flags)
     nv = len(dst)
@@ -148,8 +148,9 @@ def births(flags, newbuf, dst, dst_count, id_var, next_id):
     dp = (ctypes.c_void_p * nv)(*[a.ctypes.data for a in dst])
     by = (ctypes.c_uint32 * nv)(*[a.itemsize for a in dst])
     nid = ctypes.c_uint32(next_id)
-    c = lib().orc_births(ctypes.c_uint32(len(flags)), ptr(flags), ctypes.c_uint32(nv), nbp, by, dp,
-                         ctypes.c_uint32(dst_count), ctypes.c_uint32(id_var), ctypes.byref(nid))
+    c = lib().orc_births_ex(ctypes.c_uint32(len(flags)), ptr(flags), ctypes.c_uint32(nv), nbp, by, dp,
+                            ctypes.c_uint32(dst_count), ctypes.c_uint32(id_var), ctypes.byref(nid),
+                            ctypes.c_int32(1 if by_parent_index else 0))
     return c, nid.value
 
 

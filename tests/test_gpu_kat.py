@@ -141,6 +141,47 @@ def test_device_agent_creation(variant, n_total, n1, n12):
     assert np.array_equal(born, np.sort(born)) and born.min() == n + 1
 
 
+def test_agent_id_multiple_states_unique_ids():
+    # DeviceAgentCreationTest.AgentID_MultipleStatesUniqueIDs (test_device_agent_creation.cu:1204-1296):
+    # every agent of states a and b gives birth and stores its child's ID (agent_out.getID());
+    # the child stores its parent's ID.  IDs must be unique, the in-model copies must match
+    # the exported IDs, and parent/child pairings must agree both ways.
+    n = 100
+    s = _sim("agent_id_states")
+    z = np.zeros(n, np.uint32)
+    s.set_agents("agent", [z, z], state="a")
+    s.set_agents("agent", [z, z], state="b")
+    s.step(1)
+    pops = [s.get_agents("agent", st) for st in ("a", "b")]
+    assert [len(p[1]) for p in pops] == [2 * n, 2 * n]
+    ids = np.concatenate([p[1] for p in pops])
+    copy = np.concatenate([p[0][0] for p in pops])
+    other = np.concatenate([p[0][1] for p in pops])
+    assert len(set(ids.tolist())) == len(ids)
+    assert np.array_equal(ids, copy)
+    pair = {int(i): int(o) for i, o in zip(ids, other) if o != 0}
+    assert len(pair) == len(ids)
+    for i, o in pair.items():
+        assert pair[o] == i
+
+
+def test_agent_id_birth_into_other_agent_type():
+    # AgentID_DeviceBirth_MultipleAgents (test_device_agent_creation.cu:1297-1339): agents A give
+    # birth to agents B, whose IDs must come from B's counter (no collision with B's own)
+    s = _sim("agent_id_two_types")
+    s.set_agents("agentA", [np.zeros(100, np.uint32)])
+    s.set_agents("agentB", [np.zeros(200, np.uint32)])
+    s.step(1)
+    (bo,), bids = s.get_agents("agentB")
+    (ao,), aids = s.get_agents("agentA")
+    assert len(bids) == 300 and len(set(bids.tolist())) == 300
+    # the children carry their parents' IDs, and each parent holds its child's ID
+    child_of = dict(zip(aids.tolist(), ao.tolist()))
+    for bid, parent in zip(bids.tolist(), bo.tolist()):
+        if parent:
+            assert child_of[parent] == bid
+
+
 def test_state_transition():
     n = 1000
     s = _sim("transition")

@@ -166,3 +166,93 @@ def test_array_index_out_of_bounds_raises():
     s.set_agents("agent", [np.arange(n, dtype=np.uint32), write, z, z, z])
     with pytest.raises(FgxError, match="DeviceError"):
         s.step(1)
+
+
+# ---------------------------------------------------------------- append / truncate, persistence
+# tests/test_cases/runtime/messaging/test_append_truncate.cu:72-310 and
+# test_brute_force.cu:517-600, through tests/models/kat_messages.hip.
+def _kat(variant):
+    return Simulation("kat_messages", [], seed=1, variant=variant)
+
+
+def test_truncate():
+    n = 1024
+    s = _kat("truncate")
+    s.set_agents("Agent", [np.zeros(n, np.uint32)])
+    for _ in range(2):
+        s.step(1)
+        (count,), _ = s.get_agents("Agent")
+        assert np.all(count == n)
+
+
+def test_append_keep_data():
+    # two output layers append to one list: every agent reads both sets, each step
+    n = 1024
+    s = _kat("append_keep")
+    z = np.zeros(n, np.uint32)
+    s.set_agents("Agent", [z, z])
+    for _ in range(2):
+        s.step(1)
+        (c0, c1), _ = s.get_agents("Agent")
+        assert np.all(c0 == n) and np.all(c1 == n)
+
+
+def test_optional_truncate():
+    n = 1024
+    do_out = (np.random.default_rng(5).uniform(size=n) < 0.7).astype(np.uint32)
+    k = int(do_out.sum())
+    s = _kat("optional_truncate")
+    s.set_agents("Agent", [np.zeros(n, np.uint32), do_out])
+    s.step(1)
+    (count, _), _ = s.get_agents("Agent")
+    assert np.all(count == k)
+    s.step(1)   # the outputting agents flipped do_out: the others output now
+    (count, _), _ = s.get_agents("Agent")
+    assert np.all(count == n - k)
+
+
+def test_optional_append_keep_data():
+    n = 1024
+    do_out = (np.random.default_rng(6).uniform(size=n) < 0.7).astype(np.uint32)
+    k = int(do_out.sum())
+    s = _kat("optional_append")
+    z = np.zeros(n, np.uint32)
+    s.set_agents("Agent", [z, z, do_out])
+    s.step(1)
+    (c0, c1, _), _ = s.get_agents("Agent")
+    assert np.all(c0 == k) and np.all(c1 == n - k)
+    s.step(1)
+    (c0, c1, _), _ = s.get_agents("Agent")
+    assert np.all(c0 == n - k) and np.all(c1 == k)
+
+
+def test_append_keep_data_resize():
+    # the second output list is twice the first, forcing the list to grow before the append
+    # (the reference's issue #725): the first output must survive the resize
+    n = 1024
+    s = _kat("append_resize")
+    s.set_agents("a", [], n=n)
+    s.set_agents("b", [], n=2 * n)
+    s.set_agents("c", [np.zeros(1, np.uint32)] * 3)
+    s.step(1)
+    (c0, c1, c2), _ = s.get_agents("c")
+    assert (int(c0[0]), int(c1[0]), int(c2[0])) == (0, n, 2 * n)
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_message_persistence(persistent):
+    # output on even steps only: a non-persistent list is empty on odd steps, a persistent
+    # one still holds the even step's messages
+    n = 128
+    s = _kat("persistence_on" if persistent else "persistence_off")
+    z = np.zeros(n, np.uint32)
+    s.set_agents("agent", [np.arange(n, dtype=np.int32), z, z])
+    s.step(1)
+    (_, count, total), _ = s.get_agents("agent")
+    assert np.all(count == n) and np.all(total == n * (n - 1) // 2)
+    s.step(1)
+    (_, count, total), _ = s.get_agents("agent")
+    if persistent:
+        assert np.all(count == n) and np.all(total == n * (n - 1) // 2)
+    else:
+        assert np.all(count == 0) and np.all(total == 0)

@@ -363,6 +363,40 @@ def test_births_bit_exact(ctx, n_par, existing, p):
     assert np.array_equal(host(d_id, np.uint32, c_o), oid[:c_o])
 
 
+@pytest.mark.parametrize("n_par", [5000, 3_000_000])
+def test_births_ids_by_parent_index(ctx, n_par):
+    # the ID word set (a parent read its child's ID in-kernel): child ID = next ID + parent
+    # index, the next ID advances by the parent count, and the append clears the word
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(n_par + 1)
+    existing = 77
+    flags = (rng.uniform(size=n_par) < 0.1).astype(np.uint32)
+    newx = rng.uniform(size=n_par).astype(np.float32)
+    cap = existing + n_par
+    ox, oid = np.zeros(cap, np.float32), np.zeros(cap, np.uint32)
+    c_o, nid_o = O.births(flags, [newx, None], [ox, oid], existing, 1, 1000, by_parent_index=True)
+    d_f, d_new = dev(flags), dev(newx)
+    d_x, d_id = empty(cap, np.float32), empty(cap, np.uint32)
+    d_cnt = dev(np.array([existing], np.uint32))
+    d_nid = dev(np.array([1000], np.uint32))
+    d_slots = dev(np.array([1], np.uint32))
+    sb = fgx.scratch_bytes(fgx.FGX_OP_BIRTHS, n_par)
+    s = scratch(sb)
+    fgx.append_births(ctx, ptr(d_f), n_par, [ptr(d_new), 0], [ptr(d_x), ptr(d_id)], [4, 4], 1, ptr(d_cnt), cap,
+                      ptr(d_nid), ptr(s), sb, d_id_slots=ptr(d_slots))
+    sync()
+    assert int(host(d_cnt, np.uint32)[0]) == c_o
+    assert int(host(d_nid, np.uint32)[0]) == nid_o == 1000 + n_par
+    assert int(host(d_slots, np.uint32)[0]) == 0
+    assert np.array_equal(host(d_x, np.float32, c_o)[existing:], ox[existing:c_o])
+    assert np.array_equal(host(d_id, np.uint32, c_o)[existing:], oid[existing:c_o])
+    # the word is clear: the next append numbers by rank again
+    fgx.append_births(ctx, ptr(d_f), n_par, [ptr(d_new), 0], [ptr(d_x), ptr(d_id)], [4, 4], 1, ptr(d_cnt), cap,
+                      ptr(d_nid), ptr(s), sb, d_id_slots=ptr(d_slots))
+    sync()
+    assert int(host(d_nid, np.uint32)[0]) == 1000 + n_par + int(flags.sum())
+
+
 def test_births_overflow_flag(ctx):
     from flamegpu2_amd import fgx
     n_par, existing, cap = 1000, 900, 950
