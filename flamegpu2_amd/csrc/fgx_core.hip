@@ -149,6 +149,10 @@ int fgx_ctx_create(int device, fgx_ctx** out) {
     FGX_HIP(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(FGX_ERR_ARG, std::string("fgx_ctx_create: built for gfx950, device is ") + prop.gcnArchName);
+    {
+        const int rc = noalloc_scratch_init(device);
+        if (rc) return rc;
+    }
     auto* c = new fgx_ctx();
     c->c.device = device;
     c->c.cu_count = prop.multiProcessorCount;

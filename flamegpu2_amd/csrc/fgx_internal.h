@@ -104,6 +104,10 @@ __device__ __forceinline__ uint32_t wave_inclusive_u32(uint32_t v) {
 // between blocks.  Their scratch starts with this 256-byte header (unused words kept
 // so that the per-tile words after it stay 256-byte aligned), then 8 bytes per tile:
 // tile sums, or a compaction's stashed inputs and per-tile counts.
+// Looks up, on the current device, the address of the library's own build working space
+// (used by fgx_spatial_build_noalloc); called by fgx_ctx_create once per device.
+int noalloc_scratch_init(int device);
+
 struct ScratchHeader {
     uint32_t pad[64];
 };

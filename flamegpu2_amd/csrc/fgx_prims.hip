@@ -947,6 +947,28 @@ static uint32_t big_bin_grid(const Ctx* c, uint32_t bins) {
     return std::max(1u, std::min(bins, cap));
 }
 
+// ---------------------------------------------------------------- library-owned build space
+// fgx_spatial_build_noalloc works in this array instead of a caller buffer.  Each GPU holds
+// its own copy for the life of the process.  In host code the array's name is not a device
+// address: its address on a device comes from hipGetSymbolAddress (noalloc_scratch_init,
+// once per device, that device current) and only that address is used.
+static __device__ __align__(256) char noalloc_scratch[FGX_NOALLOC_BUILD_BYTES];
+constexpr int kMaxDevices = 64;
+static void* g_noalloc_ptr[kMaxDevices];
+static std::mutex g_noalloc_mutex;
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(FGX_ERR_ARG, "noalloc_scratch_init: device out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mutex);
+    if (g_noalloc_ptr[device]) return FGX_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+    if (e != hipSuccess || !p)
+        return fail(FGX_ERR_HIP, std::string("hipGetSymbolAddress(noalloc_scratch): ") + hipGetErrorString(e));
+    g_noalloc_ptr[device] = p;
+    return FGX_OK;
+}
+
 static uint32_t stream_grid(const Ctx* c, uint32_t n) {
     // Enough waves to cover HBM latency on every CU; grid-stride beyond that.
     const uint32_t cap = (uint32_t)std::max(c ? c->cu_count : 256, 1) * 8u;
@@ -964,10 +986,12 @@ static int run_scan(const uint32_t* in, uint32_t* out, uint32_t n, char* scratch
     return scan_counts(in, out, n, hdr, status, (hipStream_t)stream);
 }
 
-// Shared by the message build and the agent sort.
+// Shared by the message build and the agent sort.  shared_space: the working space is the
+// library's own per-GPU array (fgx_spatial_build_noalloc), which every such call shares:
+// its histogram is always cleared and no clean record is kept for it.
 static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const uint32_t* d_n, uint32_t n_cap,
                         const float* x, const float* y, const float* z, const VarTable& vt, uint32_t* start_out,
-                        void* scratch, size_t scratch_bytes, void* stream) {
+                        void* scratch, size_t scratch_bytes, void* stream, bool shared_space = false) {
     const BuildLayout L = build_layout(n_cap, bins);
     if (!scratch || scratch_bytes < L.total) return fail(FGX_ERR_CAPACITY, "scratch too small for bin sort");
     char* s = static_cast<char*>(scratch);
@@ -987,7 +1011,8 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
     FGX_HIP(hipStreamIsCapturing((hipStream_t)stream, &cap));
     const bool capturing = cap != hipStreamCaptureStatusNone;
     bool clean = false;
-    if (ctx && ctx->c.keep_zero) {
+    const bool record = ctx && ctx->c.keep_zero && !shared_space;
+    if (record) {
         std::lock_guard<std::mutex> lk(clean_mutex());
         auto it = clean_record().find(scratch);
         clean = !capturing && it != clean_record().end() && (it->second == bins || it->second == kAllZero);
@@ -1023,7 +1048,7 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
                            unsorted_read);
         FGX_HIP(hipGetLastError());
     }
-    if (ctx && ctx->c.keep_zero && !capturing) {
+    if (record && !capturing) {
         std::lock_guard<std::mutex> lk(clean_mutex());
         clean_record()[scratch] = bins;
     }
@@ -1160,6 +1185,43 @@ int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t
         ka.grid[a] = meta->grid[a] ? meta->grid[a] : 1u;
     }
     return run_bin_sort(ctx, ka, meta->bins, d_n, n_cap, x, y, z, vt, pbm, scratch, scratch_bytes, stream);
+}
+
+int fgx_spatial_build_noalloc(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t* d_n, uint32_t n_cap,
+                              const float* x, const float* y, const float* z,
+                              uint32_t nvars, const void* const* vars_in, void* const* vars_out,
+                              const uint32_t* var_bytes, uint32_t* pbm, void* stream) {
+    static_assert((size_t)FGX_NOALLOC_MAX_BINS + 3 <= 0xFFFFFFFFull, "bin limit fits the layout arithmetic");
+    if (!ctx) return fail(FGX_ERR_ARG, "fgx_spatial_build_noalloc: NULL context");
+    if (!meta || !pbm) return fail(FGX_ERR_ARG, "fgx_spatial_build_noalloc: NULL meta or pbm");
+    if (n_cap > FGX_NOALLOC_MAX_MESSAGES || meta->bins > FGX_NOALLOC_MAX_BINS)
+        return fail(FGX_ERR_CAPACITY, "fgx_spatial_build_noalloc: " + std::to_string(n_cap) + " messages / " +
+                                          std::to_string(meta->bins) + " bins exceed the limits of " +
+                                          std::to_string(FGX_NOALLOC_MAX_MESSAGES) + " / " +
+                                          std::to_string(FGX_NOALLOC_MAX_BINS) + "; use fgx_spatial_build");
+    const int dev = ctx->c.device;
+    void* space = dev >= 0 && dev < kMaxDevices ? g_noalloc_ptr[dev] : nullptr;
+    if (!space) return fail(FGX_ERR_HIP, "fgx_spatial_build_noalloc: no build space on this device");
+    if (build_layout(n_cap, meta->bins).total > FGX_NOALLOC_BUILD_BYTES)
+        return fail(FGX_ERR_CAPACITY, "fgx_spatial_build_noalloc: working space too small");
+    if (meta->dims != 2 && meta->dims != 3) return fail(FGX_ERR_ARG, "fgx_spatial_build_noalloc: dims must be 2 or 3");
+    if (n_cap && (!x || !y || (meta->dims == 3 && !z)))
+        return fail(FGX_ERR_ARG, "fgx_spatial_build_noalloc: missing location array");
+    VarTable vt;
+    int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
+    if (rc) return rc;
+    KeyArgs ka;
+    std::memset(&ka, 0, sizeof(ka));
+    ka.mode = 0;
+    ka.dims = meta->dims;
+    ka.radius = meta->cell > 0.0f ? meta->cell : meta->radius;
+    for (int a = 0; a < 3; ++a) {
+        ka.min[a] = meta->min[a];
+        ka.width[a] = meta->width[a];
+        ka.grid[a] = meta->grid[a] ? meta->grid[a] : 1u;
+    }
+    return run_bin_sort(ctx, ka, meta->bins, d_n, n_cap, x, y, z, vt, pbm, space, FGX_NOALLOC_BUILD_BYTES, stream,
+                        true);
 }
 
 int fgx_bucket_build(fgx_ctx* ctx, int32_t key_min, uint32_t keys, const uint32_t* d_n, uint32_t n_cap, const int32_t* key,

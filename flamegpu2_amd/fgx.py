@@ -67,6 +67,8 @@ _SIGS = {
                                    _c.POINTER(SpatialMeta)],
     "fgx_spatial_build": [vp, _c.POINTER(SpatialMeta), vp, u32, vp, vp, vp, u32, _c.POINTER(vp), _c.POINTER(vp),
                           _c.POINTER(u32), vp, vp, sz, vp],
+    "fgx_spatial_build_noalloc": [vp, _c.POINTER(SpatialMeta), vp, u32, vp, vp, vp, u32, _c.POINTER(vp),
+                                  _c.POINTER(vp), _c.POINTER(u32), vp, vp],
     "fgx_bucket_build": [vp, i32, u32, vp, u32, vp, u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), vp, vp, sz,
                          vp],
     "fgx_agent_sort_key_bits": [_c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float,
@@ -167,6 +169,13 @@ def spatial_build(ctx, meta, n_cap, x, y, z, vars_in, vars_out, var_bytes, pbm, 
     nv = len(vars_in)
     check(lib().fgx_spatial_build(ctx.h, _c.byref(meta), d_n, n_cap, x, y, z, nv, _arr(vp, vars_in),
                                   _arr(vp, vars_out), _arr(u32, var_bytes), pbm, scratch, scratch_len, stream))
+
+
+def spatial_build_noalloc(ctx, meta, n_cap, x, y, z, vars_in, vars_out, var_bytes, pbm, stream=0, d_n=None):
+    """fgx_spatial_build working in the library's own per-GPU space (no scratch argument)."""
+    nv = len(vars_in)
+    check(lib().fgx_spatial_build_noalloc(ctx.h, _c.byref(meta), d_n, n_cap, x, y, z, nv, _arr(vp, vars_in),
+                                          _arr(vp, vars_out), _arr(u32, var_bytes), pbm, stream))
 
 
 def bucket_build(ctx, key_min, keys, n_cap, key, vars_in, vars_out, var_bytes, pbm, scratch, scratch_len, stream=0,

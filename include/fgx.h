@@ -155,6 +155,23 @@ int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t
                       uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
                       uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * fgx_spatial_build with its working space taken from a device array that the library
+ * holds on each GPU for the life of the process (FGX_NOALLOC_BUILD_BYTES) instead of a
+ * caller buffer.  The caller passes the inputs and outputs only.  n_cap is limited to
+ * FGX_NOALLOC_MAX_MESSAGES and the bin count to FGX_NOALLOC_MAX_BINS (FGX_ERR_CAPACITY
+ * beyond).  Allocates nothing.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other host threads) race and must be ordered by the
+ * caller.  Results are those of fgx_spatial_build. */
+#define FGX_NOALLOC_MAX_MESSAGES 2097152u
+#define FGX_NOALLOC_MAX_BINS 2097152u
+#define FGX_NOALLOC_BUILD_BYTES (34u << 20)
+int fgx_spatial_build_noalloc(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t* d_n, uint32_t n_cap,
+                              const float* x, const float* y, const float* z,
+                              uint32_t nvars, const void* const* vars_in, void* const* vars_out,
+                              const uint32_t* var_bytes, uint32_t* pbm, void* stream);
+
 /* Bucket message index (src/flamegpu/runtime/messaging/MessageBucket.cu:58-150,
  * MessageBucket::CUDAModelHandler::buildIndex): a stable sort of the messages by integer
  * key into `keys` buckets [key_min, key_min + keys), with pbm[0..keys] the bucket start
