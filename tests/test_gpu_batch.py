@@ -202,6 +202,39 @@ def test_batched_half_width_boids_against_oracle():
             assert np.array_equal(g, e)
 
 
+@pytest.mark.parametrize("bins_per_radius", [1, 2])
+def test_ensemble_configuration_one_step_against_oracle(bins_per_radius):
+    # the batched-ensemble workload in full (256 circles simulations, 11.7 M agents, one
+    # batch), one step, every simulation against its own oracle run: IDs and order exact,
+    # FP32 state within 1e-5 relative
+    import bench
+    from concurrent.futures import ThreadPoolExecutor
+    plans = bench.ensemble_plans(256)
+    made = [workloads.circles(n, seed=s, radius=r) for (n, r, s) in plans]
+    b = Batch("circles_spatial3D", made[0][0])
+    if bins_per_radius == 2:
+        b.set_option("bins_per_radius", 2)
+    hs = [b.add(p, s, 1, {"Circle": a}) for (p, a), (_, _, s) in zip(made, plans)]
+    print(f"ensemble: {len(hs)} simulations, {sum(n for n, _, _ in plans)} agents, bins per radius {bins_per_radius}",
+          flush=True)
+    b.step(1)
+    assert sorted(b.finished()) == sorted(hs)
+
+    def oracle(k):
+        (p, a), (_, _, s) = made[k], plans[k]
+        ref = O.Sim(0, p, s, bin_mult=1.0 / bins_per_radius)
+        ref.set_agents(0, a)
+        ref.step(1)
+        return ref.get_agents(0, [np.float32] * 4)
+    with ThreadPoolExecutor(16) as ex:
+        refs = list(ex.map(oracle, range(len(plans))))
+    for h, (exp, eid) in zip(hs, refs):
+        got, gid = b.get_agents(h, "Circle")
+        assert np.array_equal(gid, eid), f"simulation {h}: IDs / order"
+        for g, e in zip(got, exp):
+            np.testing.assert_allclose(g, e, rtol=1e-5, atol=1e-6, err_msg=f"simulation {h}")
+
+
 def test_batched_join_and_leave_same_as_alone():
     sims = _circles_sims()
     b = Batch("circles_spatial3D", sims[0][0])
@@ -270,6 +303,26 @@ def test_serving_results_same_alone_and_in_the_mix():
         for g, e in zip(rec["agents"], exp):
             assert np.array_equal(g, e), f"request {r.id} differs from its run alone"
         assert rec["latency"] >= 0.0
+        _against_oracle(r, rec)
+
+
+def _against_oracle(r, rec):
+    """A served run against the oracle's run of the same request: IDs and order exact; boids
+    state bit-exact, circles (sinf) within 1e-5 relative."""
+    from flamegpu2_amd import serving
+    lib, agent, make = serving.MODELS[r.model]
+    params, pops = make(r.n, r.seed)
+    model = 0 if r.model == "circles" else 1
+    ref = O.Sim(model, params, r.seed)
+    ref.set_agents(0, pops[agent])
+    ref.step(r.steps)
+    exp, eid = ref.get_agents(0, [a.dtype for a in rec["agents"]])
+    assert np.array_equal(rec["ids"], eid), f"request {r.id}: IDs / order against the oracle"
+    for g, e in zip(rec["agents"], exp):
+        if model == 1:
+            assert np.array_equal(g, e), f"request {r.id} differs from the oracle"
+        else:
+            np.testing.assert_allclose(g, e, rtol=1e-5, atol=1e-6, err_msg=f"request {r.id}")
 
 
 def test_serving_hbm_budget_admits_in_order():
@@ -300,3 +353,4 @@ def test_serving_shortest_first_under_agent_cap():
         assert np.array_equal(rec["ids"], eid), f"request {r.id}"
         for g, e in zip(rec["agents"], exp):
             assert np.array_equal(g, e), f"request {r.id} differs from its run alone"
+        _against_oracle(r, rec)

@@ -72,6 +72,8 @@ def _close(got, exp, what):
 
 
 def test_boids_500_steps_statistics():
+    if "boids" not in GOLD:
+        pytest.skip("no boids trace in tests/golden/long_traces.json (make_long_traces.py boids)")
     g = GOLD["boids"]
     params, arrs = workloads.boids(g["agents"], seed=g["seed"])
     sim = Simulation("boids_spatial3D", params, seed=g["seed"])
