@@ -5,9 +5,13 @@ Requests are admitted while the HBM they need (agent, message, bin-table and
 scratch bytes, from the batch's own estimate) fits the budget, and, when a cap
 on resident agents is set, while the agents fit the cap.  Policy "fifo" admits
 in arrival order; policy "shortest" admits the queued run with the least work
-(agents x steps) first, which shortens typical latency under overload at the
-cost of the longest runs' latency;
-an admitted run joins its model's batch at the next step and shares every launch
+(agents x steps) first, which shortens typical latency under overload.  So that
+long runs are not starved, a run that has waited longer than `max_wait` seconds
+is admitted before any other, oldest first, and while such a run does not fit
+nothing else is admitted (the capacity that frees up is kept for it).  Under
+"shortest", a run that does not fit the agent cap does not block the queue: the
+next candidate that fits is admitted instead.
+An admitted run joins its model's batch at the next step and shares every launch
 with the runs already resident, so runs that arrive while others are on the GPU
 overlap with them.  The batches of different models run on their own streams.
 A run leaves its batch once its steps are done and its final population has been
@@ -64,14 +68,17 @@ class Server:
     POLICIES = ("fifo", "shortest")
 
     def __init__(self, hbm_budget=160 << 30, max_chunk=16, tuning=None, device=0, policy="fifo",
-                 max_active_agents=None):
+                 max_active_agents=None, max_wait=None):
         if policy not in self.POLICIES:
             raise ValueError(f"unknown admission policy '{policy}' (one of {', '.join(self.POLICIES)})")
         if max_active_agents is not None and max_active_agents <= 0:
             raise ValueError("max_active_agents must be positive")
+        if max_wait is not None and max_wait < 0:
+            raise ValueError("max_wait must be non-negative")
         self.budget = hbm_budget
         self.policy = policy
         self.max_active_agents = max_active_agents
+        self.max_wait = max_wait
         self.max_chunk = max_chunk
         self.tuning = tuning or {}
         self.device = device
@@ -87,11 +94,28 @@ class Server:
             self.batches[model] = b
         return self.batches[model]
 
-    def run(self, requests, collect=False, clock=time.perf_counter, progress=None):
+    def _candidates(self, queue, now):
+        """Queued runs in admission order, and how many of the first ones are overdue
+        (waited longer than max_wait)."""
+        overdue = [] if self.max_wait is None else [q for q in queue if now - q.arrival > self.max_wait]
+        overdue.sort(key=lambda q: (q.arrival, q.id))
+        late = {q.id for q in overdue}
+        rest = [q for q in queue if q.id not in late]
+        if self.policy == "fifo":
+            rest.sort(key=lambda q: (q.arrival, q.id))
+        else:
+            rest.sort(key=lambda q: (q.n * q.steps, q.arrival, q.id))
+        return overdue + rest, len(overdue)
+
+    def run(self, requests, collect=False, clock=time.perf_counter, progress=None, stop_after=None):
         """Serves `requests` (sorted by arrival, seconds from the start).  Returns one
-        record per request: dict(id, model, n, steps, arrival, admitted, completed,
-        latency[, agents, ids]).  `progress(text)` is called every ~5 s of serving."""
+        record per completed request: dict(id, model, n, steps, arrival, admitted,
+        completed, latency[, agents, ids]).  `progress(text)` is called every ~5 s of
+        serving.  With `stop_after` (seconds), serving ends at that time even if runs are
+        still queued or active; `self.unfinished` then lists them as dict(id, n, steps,
+        arrival, admitted or None, steps_done)."""
         self.phase = collections.Counter()
+        self.unfinished = []
         last_report = clock()
         reqs = sorted(requests, key=lambda r: r.arrival)
         queue = collections.deque()
@@ -104,15 +128,19 @@ class Server:
         t0 = clock()
         while nxt < len(reqs) or queue or active:
             now = clock() - t0
+            if stop_after is not None and now >= stop_after:
+                break
             while nxt < len(reqs) and reqs[nxt].arrival <= now:
                 queue.append(reqs[nxt])
                 nxt += 1
             # admission, by policy, while the HBM budget and the agent cap allow; a run
             # that does not fit waits (it is always admitted when nothing is active)
-            while queue:
-                r = queue[0] if self.policy == "fifo" else min(queue, key=lambda q: (q.n * q.steps, q.arrival))
+            order, n_overdue = self._candidates(queue, now)
+            for pos, r in enumerate(order):
                 if active and self.max_active_agents is not None and agents + r.n > self.max_active_agents:
-                    break
+                    if pos < n_overdue or self.policy == "fifo":
+                        break      # keep the freed capacity for the overdue / first-come run
+                    continue       # shortest: a smaller run further on may fit
                 lib, agent, make = MODELS[r.model]
                 b = self._batch(r.model)
                 tp = clock()
@@ -124,7 +152,7 @@ class Server:
                 self.phase["admit_s"] += clock() - tp
                 if used + need > self.budget and active:
                     b.remove(h)
-                    break
+                    break          # HBM is the hard limit: wait for completions
                 del made[r.id]
                 used += need
                 agents += r.n
@@ -174,18 +202,42 @@ class Server:
                 last_report = clock()
                 progress(f"serving t={clock() - t0:.1f}s arrived={nxt}/{len(reqs)} queued={len(queue)} "
                          f"active={len(active)} done={len(results)} {dict(self.phase)}")
+        for v in active.values():
+            r = v[0]
+            self.unfinished.append({"id": r.id, "n": r.n, "steps": r.steps, "arrival": r.arrival, "admitted": v[2],
+                                    "steps_done": v[3]})
+        for r in list(queue) + list(reqs[nxt:]):
+            self.unfinished.append({"id": r.id, "n": r.n, "steps": r.steps, "arrival": r.arrival, "admitted": None,
+                                    "steps_done": 0})
         return results
 
 
-def summarize(results, wall):
+def summarize(results, wall, unfinished=()):
+    """Latency percentiles of the completed runs, the longest wait for admission, and the
+    agent-steps served per second of wall time (steps of unfinished runs included).  With
+    unfinished runs (a run cut off by stop_after) it also gives the percentiles over every
+    arrived request, counting an unfinished one at its wait so far: a lower bound."""
     lat = np.array([r["latency"] for r in results]) if results else np.zeros(1)
+    waits = [r["admitted"] - r["arrival"] for r in results]
+    waits += [(u["admitted"] if u["admitted"] is not None else wall) - u["arrival"] for u in unfinished
+              if u["arrival"] <= wall]
     agent_steps = float(sum(r["n"] * r["steps"] for r in results))
-    return {
+    partial = float(sum(u["n"] * u["steps_done"] for u in unfinished))
+    out = {
         "requests": len(results),
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "mean_latency_s": float(lat.mean()),
-        "agent_steps": agent_steps,
-        "agent_steps_per_s": agent_steps / wall if wall > 0 else 0.0,
+        "max_wait_s": float(max(waits)) if waits else 0.0,
+        "agent_steps": agent_steps + partial,
+        "agent_steps_per_s": (agent_steps + partial) / wall if wall > 0 else 0.0,
         "wall_s": wall,
     }
+    if unfinished:
+        arrived = [u for u in unfinished if u["arrival"] <= wall]
+        low = np.concatenate([lat if results else np.zeros(0), np.array([wall - u["arrival"] for u in arrived])])
+        out["unfinished"] = len(unfinished)
+        out["unfinished_arrived"] = len(arrived)
+        out["p50_latency_lower_bound_s"] = float(np.percentile(low, 50)) if len(low) else 0.0
+        out["p99_latency_lower_bound_s"] = float(np.percentile(low, 99)) if len(low) else 0.0
+    return out

@@ -95,3 +95,48 @@ def test_bad_arguments():
         serving.Server(policy="lifo")
     with pytest.raises(ValueError):
         serving.Server(max_active_agents=0)
+
+
+def test_shortest_does_not_block_on_a_run_that_does_not_fit(stub):
+    # cap 50: R0 (30 agents) is admitted; R1 (30) does not fit beside it, R2 (15) does and is
+    # admitted at once instead of waiting behind R1
+    reqs = _reqs([(30, 2), (30, 3), (15, 10)])   # work 60, 90, 150
+    res = serving.Server(max_chunk=1, policy="shortest", max_active_agents=50).run(reqs)
+    adm = {r["id"]: r["admitted"] for r in res}
+    assert adm[2] < adm[1]
+    assert stub.max_resident <= 50
+
+
+def test_overdue_runs_go_first_and_hold_capacity(stub):
+    # with max_wait 0 every queued run is overdue at once: admission falls back to arrival
+    # order, and a first run that does not fit holds the queue (no smaller run overtakes)
+    reqs = _reqs([(30, 2), (30, 3), (15, 10)])
+    res = serving.Server(max_chunk=1, policy="shortest", max_active_agents=50, max_wait=0.0).run(reqs)
+    adm = {r["id"]: r["admitted"] for r in res}
+    assert adm[0] <= adm[1] <= adm[2]
+    # without aging, shortest-first admits by work
+    reqs = _reqs([(50, 9), (10, 2), (30, 5), (10, 40)])
+    res = serving.Server(max_chunk=4, policy="shortest", max_active_agents=50, max_wait=None).run(reqs)
+    assert _order(res) == [1, 2, 3, 0]
+
+
+def test_stop_after_reports_unfinished_runs(stub):
+    reqs = _reqs([(30, 2), (30, 3)])
+    srv = serving.Server(max_chunk=1)
+    res = srv.run(reqs, stop_after=0.0)
+    assert res == [] and sorted(u["id"] for u in srv.unfinished) == [0, 1]
+    s = serving.summarize(res, 1.0, srv.unfinished)
+    assert s["requests"] == 0 and s["unfinished"] == 2 and s["max_wait_s"] >= 1.0
+    assert s["p99_latency_lower_bound_s"] >= 1.0
+
+
+def test_summary_reports_longest_wait():
+    res = [{"id": 0, "n": 10, "steps": 2, "arrival": 0.0, "admitted": 0.5, "completed": 1.0, "latency": 1.0},
+           {"id": 1, "n": 10, "steps": 2, "arrival": 0.2, "admitted": 3.0, "completed": 4.0, "latency": 3.8}]
+    s = serving.summarize(res, 4.0)
+    assert abs(s["max_wait_s"] - 2.8) < 1e-9 and s["agent_steps"] == 40.0
+
+
+def test_bad_max_wait():
+    with pytest.raises(ValueError):
+        serving.Server(max_wait=-1.0)
