@@ -87,6 +87,11 @@ class BatchSimulation {
     uint32_t pendingCount() const { return (uint32_t)pending_.size(); }
     // Agents over all resident simulations (agent-steps per step).
     uint64_t residentAgents() const;
+    // Per-wave unit tables for spatial iteration (CUDASimulation::setUnitTables).
+    void setUnitTables(bool on) {
+        unit_tables_ = on;
+        clear_graphs();
+    }
     void setGraphs(bool on) {
         graphs_on_ = on;
         clear_graphs();
@@ -236,6 +241,7 @@ class BatchSimulation {
     std::vector<GraphEntry> graphs_;
     float bin_mult_ = 1.0f;
     bool graphs_on_ = true;
+    bool unit_tables_ = true;
     uint32_t stable_steps_ = 0;   // steps since the last repack
 };
 
@@ -868,6 +874,7 @@ inline void BatchSimulation::launch(FnB& f) {
     std::memset(&a, 0, sizeof(a));
     List& l = f.agent->list;
     a.n_cap = l.total;
+    a.unit_tables = unit_tables_ ? 1 : 0;
     a.zero_page = (const char*)d_pages_;
     a.sink_page = (char*)d_pages_ + 128;
     a.stream_id = f.d->stream_id;

@@ -245,6 +245,7 @@ struct FnArgs {
     const uint32_t* ids;         // the agent list's "_id" array
     const uint32_t* d_first;     // function condition: agents [*d_first, n) run (NULL: all)
     int32_t msg_out_optional;
+    int32_t unit_tables;         // spatial input: build the per-wave unit tables in LDS (0: bin table from HBM)
     uint32_t* d_msg_count_out;   // mandatory output: the launch writes the message count (agent count) here
     uint32_t agent_mul, msg_in_mul, msg_out_mul, new_mul, env_mul;
     // neighbourhood staging (spatial message input)

@@ -178,6 +178,7 @@ FGXM_EXPORT int fgxm_set_option(fgxm_sim* s, const char* key, int value) {
         else if (k == "concurrency") s->sim->CUDAConfig().inLayerConcurrency = value != 0;
         else if (k == "kernel_timing") s->sim->setKernelTiming(value != 0);
         else if (k == "staging") s->sim->setNeighbourhoodStaging(value != 0);
+        else if (k == "unit_tables") s->sim->setUnitTables(value != 0);
         else if (k == "bins_per_radius") s->sim->setBinWidthMultiplier(bins_per_radius_mult(value));
         else throw flamegpu::exception::InvalidArgument("unknown option " + k);
     });
@@ -572,6 +573,7 @@ FGXM_EXPORT int fgxm_batch_set_option(fgxm_batch* b, const char* key, int value)
     return fgxm_detail::guard([&] {
         const std::string k = key ? key : "";
         if (k == "graphs") b->batch->setGraphs(value != 0);
+        else if (k == "unit_tables") b->batch->setUnitTables(value != 0);
         else if (k == "bins_per_radius") b->batch->setBinWidthMultiplier(bins_per_radius_mult(value));
         else throw flamegpu::exception::InvalidArgument("unknown batch option " + k);
     });

@@ -52,7 +52,18 @@ struct ThreadCtx {
     uint32_t msg_n;
     uint32_t lidx;              // index among the agents this launch runs (output slots, flags)
     uint32_t stream_id;         // RNG stream of the call
+    __attribute__((address_space(3))) uint32_t* wtab;   // this wave's LDS region for spatial unit tables (null: none)
 };
+
+// Per-wave LDS region of the spatial iterators (kWaveTabWords words):
+//   [0]            claim word: 0 until the first spatial filter of the call takes the table
+//   [4 + 4g ..]    group g (g < kTabGroups): first x cell, last x cell, table word offset, row key
+//   [kTabHead ..]  the unit table: per group, per neighbour row, the bin-table entries of the
+//                  group's x range (wrapping iterators: a start/end pair per cell; clamped
+//                  ones: the bin boundaries of the row).
+constexpr uint32_t kWaveTabWords = 1536;
+constexpr uint32_t kTabGroups = 4;
+constexpr uint32_t kTabHead = 32;
 
 // Global-memory pointers (address space 1), so loads through pointers rebuilt from
 // integers stay global loads instead of flat loads.
@@ -451,6 +462,8 @@ class SpatialFilter {
         id_mask_ = id_ok ? 0xFFFFFFFFu : 0u;
         pbm_ = m.pbm;
         sv_ = nullptr;
+        tab_ = nullptr;
+        tm_ = 0u;
         // LDS source when the whole wave is covered by the block's staged neighbourhood
         if (allow_stage && c.stage) {
             const detail::StageHeader* sh = c.stage->sh;
@@ -471,6 +484,7 @@ class SpatialFilter {
             }
         }
         plan(m, cx, cy, cz, 0, 0, 0);
+        if (c.wtab) build_table(cx, cy, cz);
     }
 
     class iterator;
@@ -529,9 +543,17 @@ class SpatialFilter {
         // first message
         __device__ __forceinline__ iterator(const SpatialFilter& f, int) : f_(f), m_(f, 0, 0.f, 0.f, 0.f, 0) {
             pu_ = 0;
-            f_.first_unit(k_);
-            f_.load_unit(k_, ps_, pe_, po_);
-            next_unit();
+            if (f_.tm_) {
+                ta_ = f_.tlane_;
+                read_table(ps_, pe_);
+                po_ = 0;
+                l_ = e_ = 0;
+                take_units();
+            } else {
+                f_.first_unit(k_);
+                f_.load_unit(k_, ps_, pe_, po_);
+                next_unit();
+            }
             fetch();
             shift();
             step();
@@ -565,9 +587,39 @@ class SpatialFilter {
                 }
             }
         }
+        // Unit table: the range of the prefetched unit (entry ta_; past the last unit the read
+        // stays at entry 0).
+        __device__ __forceinline__ void read_table(uint32_t& s, uint32_t& e) const {
+            const uint32_t at = pu_ < f_.units_ ? ta_ : 0u;
+            const volatile __attribute__((address_space(3))) uint32_t* t = f_.tab_;
+            s = t[at];
+            e = t[WRAP ? at + 1u : (pu_ < f_.units_ ? at + f_.tstride_ : 0u)];
+        }
+        // Unit table: the lanes whose look-ahead left its unit take the prefetched one (and the
+        // next, while they take empty ones) with selects, not per-lane branches.  A lane's units
+        // are consecutive table entries, so the next unit's range is one LDS read.
+        __device__ __forceinline__ void take_one(bool& need) {
+            const bool take = need && pu_ < f_.units_;
+            l_ = take ? ps_ : l_;
+            e_ = take ? pe_ : e_;
+            pu_ += take ? 1 : 0;
+            ta_ += take ? (WRAP ? 2u : 1u) : 0u;
+            uint32_t s, e;
+            read_table(s, e);
+            ps_ = take ? s : ps_;
+            pe_ = take ? e : pe_;
+            need = take && l_ >= e_;
+        }
+        __device__ __forceinline__ void take_units() {
+            bool need = l_ >= e_;
+            take_one(need);
+            while (__ballot(need)) take_one(need);   // a lane took an empty unit
+            more_ = l_ < e_;
+        }
         __device__ __forceinline__ void step() {
             ++l_;
-            if (l_ >= e_) next_unit();
+            if (f_.tm_) take_units();
+            else if (l_ >= e_) next_unit();
             fetch();
         }
         // Unconditional (index 0 past the last message) so the loads issue without a branch.
@@ -592,6 +644,8 @@ class SpatialFilter {
         int pu_ = 0;
         Unit k_;
         uint32_t ps_ = 0, pe_ = 0, po_ = 0;
+        // unit table: that unit's entry
+        uint32_t ta_ = 0;
     };
     __device__ __forceinline__ iterator begin() const { return iterator(*this, 0); }
     __device__ __forceinline__ iterator end() const { return iterator(*this); }
@@ -648,6 +702,140 @@ class SpatialFilter {
             s0_ = s1_ = 1;
         }
     }
+    // The wave's unit table (see the class comment).  Called by every lane that builds the
+    // filter, after plan().  Leaves tab_ null (units from HBM) when the table is taken or
+    // the rows do not fit.  Lanes exchange the group bounds through registers (shuffles and
+    // readlane), and the table through volatile LDS accesses, which the compiler keeps in
+    // program order (LDS executes one wave's accesses in order).
+    __device__ __forceinline__ void build_table(int cx, int cy, int cz) {
+        volatile __attribute__((address_space(3))) uint32_t* w = c_.wtab;
+        const uint32_t claimed = __builtin_amdgcn_readfirstlane(w[0]);
+        if (claimed) return;
+        w[0] = 1u;
+        const int R = last_ / 2;
+        const int reach = last_ + 1;                        // cells per axis in a neighbourhood
+        const int nrows = DIMS == 3 ? reach * reach : reach;
+        const int rk = DIMS == 3 ? cz * gy_ + cy : cy;      // the cell row
+        const uint32_t lane = __lane_id();
+        const uint64_t me = 1ull << lane;
+        const uint64_t act = __ballot(1);
+        // groups: runs of adjacent active lanes with the same row
+        const int rk_prev = __shfl(rk, lane ? (int)lane - 1 : 0, 64);
+        const bool lead = lane == 0 || !((act >> (lane - 1)) & 1ull) || rk_prev != rk;
+        const uint64_t leads = __ballot(lead);
+        const uint32_t ng = (uint32_t)__builtin_popcountll(leads);
+        if (ng > detail::kTabGroups) return;
+        const uint32_t g = (uint32_t)__builtin_popcountll(leads & (me | (me - 1ull))) - 1u;
+        const int first = 63 - __builtin_clzll(leads & (me | (me - 1ull)));   // lane of this run's leader
+        // x range of each run: an inclusive max/min scan within the run, read at its last lane
+        int lo = cx, hi = cx;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int lo2 = __shfl_up(lo, o, 64), hi2 = __shfl_up(hi, o, 64);
+            const bool in = (int)lane - o >= first;
+            lo = in && lo2 < lo ? lo2 : lo;
+            hi = in && hi2 > hi ? hi2 : hi;
+        }
+        const uint64_t lasts = __ballot(lane == 63 || !((act >> (lane + 1)) & 1ull) || ((leads >> (lane + 1)) & 1ull));
+        constexpr uint32_t wpc = WRAP ? 2u : 1u;           // words per entry
+        int gx0[detail::kTabGroups], growk[detail::kTabGroups];
+        uint32_t gper[detail::kTabGroups], gcells[detail::kTabGroups], gbase[detail::kTabGroups];
+        uint32_t off = detail::kTabHead, total = 0;
+        uint64_t rest = lasts;
+#pragma unroll
+        for (uint32_t k = 0; k < detail::kTabGroups; ++k) {
+            const bool on = k < ng;
+            const int L = on ? __builtin_ctzll(rest) : 0;
+            rest &= rest - 1ull;
+            const int glo = __builtin_amdgcn_readlane(lo, L), ghi = __builtin_amdgcn_readlane(hi, L);
+            gx0[k] = glo - R;
+            growk[k] = on ? __builtin_amdgcn_readlane(rk, L) : 0;
+            // wrapping: the cells [lo - R, hi + R]; clamped: the boundaries [lo - R, hi + R + 1]
+            gper[k] = on ? (uint32_t)(ghi - glo + 2 * R + 1) + (WRAP ? 0u : 1u) : 0u;
+            gcells[k] = gper[k] * (uint32_t)nrows;
+            gbase[k] = off;
+            off += gcells[k] * wpc;
+            total += gcells[k];
+        }
+        if (off > detail::kWaveTabWords) return;
+        // fill: the active lanes take the entries in turn, four loads in flight each
+        const uint32_t rank = (uint32_t)__builtin_popcountll(act & (me - 1ull));
+        const uint32_t nact = (uint32_t)__builtin_popcountll(act);
+        for (uint32_t q0 = rank; q0 < total; q0 += 4u * nact) {
+            uint32_t dst[4], src[4], v0[4], v1[4];
+            bool ok[4];
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                uint32_t q = q0 + j * nact;
+                ok[j] = q < total;
+                q = ok[j] ? q : 0u;
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t t = 0; t + 1 < detail::kTabGroups; ++t) {
+                    const bool past = k == t && q >= gcells[t];
+                    q -= past ? gcells[t] : 0u;
+                    k += past ? 1u : 0u;
+                }
+                int x0 = gx0[0], rkk = growk[0];
+                uint32_t per = gper[0], base = gbase[0];
+#pragma unroll
+                for (uint32_t t = 1; t < detail::kTabGroups; ++t) {
+                    x0 = k == t ? gx0[t] : x0;
+                    rkk = k == t ? growk[t] : rkk;
+                    per = k == t ? gper[t] : per;
+                    base = k == t ? gbase[t] : base;
+                }
+                const uint32_t row = q / per, v = q - row * per;
+                const int ry = DIMS == 3 ? rkk % gy_ : rkk, rz = DIMS == 3 ? rkk / gy_ : 0;
+                const int dy = (DIMS == 3 ? (int)row / reach : (int)row) - R;
+                const int dz = DIMS == 3 ? (int)row % reach - R : 0;
+                const int vx = x0 + (int)v;
+                uint32_t bin;
+                bool in_grid = true;
+                if (WRAP) {
+                    const int y = detail::wrap_mod(ry, dy, gy_), z = DIMS == 3 ? detail::wrap_mod(rz, dz, gz_) : 0;
+                    bin = (uint32_t)((z * gy_ + y) * gx_ + detail::wrap_mod(vx, 0, gx_));
+                } else {
+                    const int y = ry + dy, z = rz + dz;
+                    in_grid = y >= 0 && y < gy_ && (DIMS == 2 || (z >= 0 && z < gz_));
+                    const int xc = vx < 0 ? 0 : (vx > gx_ ? gx_ : vx);   // a boundary: [0, gx]
+                    bin = in_grid ? (uint32_t)((z * gy_ + y) * gx_ + xc) : 0u;
+                }
+                dst[j] = base + (v * (uint32_t)nrows + row) * wpc;
+                src[j] = bin;
+                ok[j] = ok[j] && in_grid;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                v0[j] = ok[j] ? pbm_[src[j]] : 0u;
+                v1[j] = (WRAP && ok[j]) ? pbm_[src[j] + 1u] : 0u;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                if (q0 + j * nact < total) {
+                    w[dst[j]] = v0[j];
+                    if (WRAP) w[dst[j] + 1u] = v1[j];
+                }
+            }
+        }
+        // this lane's row 0 entry for its first unit, and the words per row of its group
+        uint32_t base = gbase[0], per = gper[0];
+        int x0 = gx0[0];
+#pragma unroll
+        for (uint32_t t = 1; t < detail::kTabGroups; ++t) {
+            base = g == t ? gbase[t] : base;
+            per = g == t ? gper[t] : per;
+            x0 = g == t ? gx0[t] : x0;
+        }
+        // Entries are stored column-major (x outer, neighbour row inner), so this lane's units
+        // are consecutive entries: wrapping, unit u = (x cell, row) is entry tlane_ + u of cell
+        // cx - R's column; clamped, unit = row, with the strip's end boundary tstride_ entries on.
+        tlane_ = base + (uint32_t)((WRAP ? cx - R : xs_) - x0) * (uint32_t)nrows * wpc;
+        tstride_ = WRAP ? 0u : (uint32_t)(span_ + 1) * (uint32_t)nrows;
+        tab_ = c_.wtab;
+        tm_ = 1u;
+    }
+    // Unit table: c0 counts neighbour rows, c1 cells along x (wrapping) and p0 is the entry.
     __device__ __forceinline__ void first_unit(Unit& k) const {
         k.c0 = k.c1 = k.c2 = 0;
         k.p0 = st0_;
@@ -715,6 +903,10 @@ class SpatialFilter {
                 s = pbm_[first];
                 t = pbm_[first + (uint32_t)span_ + 1u];
             }
+            // use the loaded range here (the empty asm needs its registers), so that no global
+            // load is outstanding for the range where the walk uses it: a wait there would also
+            // wait for the message loads in flight, on every unit change
+            asm volatile("" ::"v"(s), "v"(t));
             o = 0;
             e = ok ? t : s;
         }
@@ -737,6 +929,9 @@ class SpatialFilter {
 
     const detail::ThreadCtx& c_;
     const detail::StageView* sv_;
+    const __attribute__((address_space(3))) uint32_t* tab_;   // the wave's unit table, or null
+    uint32_t tm_;                   // 1 when units come from the table (wave-uniform)
+    uint32_t tlane_, tstride_;      // table: this lane's first unit word; clamped: words to its strip end
     float loc_[3];
     float width_[3];
     int gx_, gy_, gz_;              // grid
@@ -1348,6 +1543,7 @@ __device__ __forceinline__ void run_agent(const ThreadCtx& c) {
     const uint32_t o = c.lidx;
     if (a.msg_out_flags) a.msg_out_flags[o] = a.msg_out_optional ? 0u : 1u;
     if (a.birth_flags) a.birth_flags[o] = 0u;
+    if (c.wtab) *static_cast<volatile __attribute__((address_space(3))) uint32_t*>(c.wtab) = 0u;   // free for this call's first spatial filter
     DeviceAPI<typename F::MsgIn, typename F::MsgOut> api(c);
     const AGENT_STATUS s = F()(&api);
     if (a.death_flags)
@@ -1370,6 +1566,28 @@ __device__ __forceinline__ uint32_t xcd_block() {
 
 // Agents [first, n) of the list run, first = the agents a function condition disabled.
 template <typename F>
+constexpr bool reads_spatial() {
+    return F::MsgIn::kind == MsgKind::Spatial2D || F::MsgIn::kind == MsgKind::Spatial3D;
+}
+
+// Dynamic LDS of the non-staged kernels: one unit-table region per wave for spatial input.
+template <typename F>
+constexpr uint32_t fn_lds_bytes(uint32_t block) {
+    return reads_spatial<F>() ? (block / 64u) * kWaveTabWords * 4u : 0u;
+}
+
+// This wave's unit-table region, or null (no spatial input).
+template <typename F>
+__device__ __forceinline__ __attribute__((address_space(3))) uint32_t* wave_table() {
+    if constexpr (reads_spatial<F>()) {
+        extern __shared__ uint32_t fgx_stage_smem[];
+        return (__attribute__((address_space(3))) uint32_t*)fgx_stage_smem +
+               __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * kWaveTabWords;
+    }
+    return nullptr;
+}
+
+template <typename F>
 __global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
     const uint32_t first = a.d_first ? load_count(a.d_first) : 0u;
     const uint32_t n = a.d_n ? load_count(a.d_n) : a.n_cap;
@@ -1379,8 +1597,8 @@ __global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
     const uint32_t i = first + o;
     uint32_t mn = 0u;
     if constexpr (F::MsgIn::kind == MsgKind::BruteForce) mn = a.d_msg_in_count ? load_count(a.d_msg_in_count) : 0u;
-    const ThreadCtx c{a, ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step, a.seed, nullptr,
-                      0u, mn, o, a.stream_id};
+    const ThreadCtx c{a,  ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step, a.seed, nullptr,
+                      0u, mn,                     o,           a.stream_id, a.unit_tables ? wave_table<F>() : nullptr};
     run_agent<F>(c);
 }
 
@@ -1410,7 +1628,8 @@ __global__ void __launch_bounds__(1024) agent_function_kernel_seg(const FnArgs a
             if (local < s.n) {
                 const uint32_t i = s.agent_off + local;
                 const ThreadCtx c{a,        ErrState{s.d_error, i}, a.zero_page, a.sink_page, i, s.meta, s.env, s.d_step,
-                                  s.seed,   nullptr,                s.msg_off,   s.msg_n,     i,     a.stream_id};
+                                  s.seed,   nullptr,                s.msg_off,   s.msg_n,     i,     a.stream_id,
+                                  a.unit_tables ? wave_table<F>() : nullptr};
                 run_agent<F>(c);
             }
         }
@@ -1586,7 +1805,7 @@ __global__ void __launch_bounds__(256) agent_function_kernel_staged(const FnArgs
     const StageView sv{fgx_stage_smem, fgx_stage_smem + maxe + 1, fgx_stage_smem + 2 * (maxe + 1), a.stage_cap,
                        a.stage_nvars > (uint32_t)F::MsgIn::In::dims ? 1u : 0u, &sh};
     const ThreadCtx c{a,         ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step,
-                      a.seed,    staged ? &sv : nullptr, 0u, 0u, i, a.stream_id};
+                      a.seed,    staged ? &sv : nullptr, 0u, 0u, i, a.stream_id, nullptr};
     run_agent<F>(c);
 }
 
@@ -1598,7 +1817,7 @@ __global__ void __launch_bounds__(256) agent_condition_kernel(const FnArgs a, ui
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const ThreadCtx c{a, ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step, a.seed, nullptr,
-                      0u, 0u, i, a.stream_id | 0x80000000u};
+                      0u, 0u, i, a.stream_id | 0x80000000u, nullptr};
     ReadOnlyDeviceAPI api(c);
     flags[i] = C()(&api) ? 0u : 1u;
 }
@@ -1618,14 +1837,15 @@ void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, uint3
         }
     }
     (void)smem;
-    hipLaunchKernelGGL(agent_function_kernel<F>, dim3(grid), dim3(block), 0, (hipStream_t)stream, a);
+    hipLaunchKernelGGL(agent_function_kernel<F>, dim3(grid), dim3(block), fn_lds_bytes<F>(block), (hipStream_t)stream,
+                       a);
 }
 
 template <typename F>
 void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t apt,
                                uint32_t grid, uint32_t block, void* stream) {
-    hipLaunchKernelGGL(agent_function_kernel_seg<F>, dim3(grid), dim3(block), 0, (hipStream_t)stream, a, segs, nsegs,
-                       ntiles, apt);
+    hipLaunchKernelGGL(agent_function_kernel_seg<F>, dim3(grid), dim3(block), fn_lds_bytes<F>(block), (hipStream_t)stream,
+                       a, segs, nsegs, ntiles, apt);
 }
 }  // namespace detail
 

@@ -222,6 +222,7 @@ struct FnRT {
     AgentRT* out_agent = nullptr;
     std::string out_state;
     int sort_dims = 0;              // 2/3 when the function triggers the auto spatial sort
+    bool swap_transition = false;   // this step: the end state is known empty, so the lists swap
     uint32_t cap = 0;               // per-launch buffers sized for this many agents
     uint32_t* death_flags = nullptr;
     uint32_t* birth_flags = nullptr;
@@ -619,6 +620,13 @@ class CUDASimulation {
         stage_enabled_ = on;
         clear_graphs();
     }
+    // Spatial message iteration reads each wave's bin-table ranges from a table it copies to
+    // LDS when the iterator is built (default on); off reads the bin table from HBM at every
+    // unit change.  Results are identical either way.
+    void setUnitTables(bool on) {
+        unit_tables_ = on;
+        clear_graphs();
+    }
     // Spatial message bins of width radius x mult, mult = 1 (the reference's grid) or 0.5.
     // With half-width bins the iterators search 2 bins either side: the same in-radius
     // neighbours from 5^3 bins of 1/8 the volume (125/216 of the candidates), visited in
@@ -711,6 +719,12 @@ class CUDASimulation {
     void size_function(detail::FnRT& f, uint32_t cap);
     bool ensure_capacity();
     bool model_can_grow() const;
+    void plan_transitions(bool counted);
+    std::vector<uint8_t> swap_plan() const {
+        std::vector<uint8_t> p;
+        for (auto& f : fns_) p.push_back(f.swap_transition ? 1 : 0);
+        return p;
+    }
     void enqueue_step();
     void enqueue_layer(const LayerData& layer);
     void launch_function(detail::FnRT& f, void* st);
@@ -808,6 +822,7 @@ class CUDASimulation {
     // graphs
     struct GraphEntry {
         std::vector<void*> start, end;
+        std::vector<uint8_t> swaps;   // the step's transition plan (which transitions swap lists)
         fgx_graph* g = nullptr;
         uint64_t last_use = 0;
     };
@@ -831,6 +846,7 @@ class CUDASimulation {
     // kernel timing
     bool timing_ = false;
     bool stage_enabled_ = false;
+    bool unit_tables_ = true;
     float bin_mult_ = 1.0f;
     struct PendingTime {
         std::string label;
@@ -1196,6 +1212,47 @@ inline void CUDASimulation::sync_counts() {
     if (h_small_[0] != 0) check_device_error();
 }
 
+// Which state transitions of the coming step swap lists instead of copying
+// (CUDAFatAgent::transitionState swaps when the end state holds no agent and no function
+// condition left agents behind). The end state is known empty when its upper bound at that
+// point of the step is zero: the bounds start from the counts read back at step start and
+// grow as births and transitions are enqueued. A transition swaps only when no other
+// function of its layer touches either list (they may run concurrently) and the two lists
+// have equal capacities; nothing is known after a host layer function, which may add agents.
+inline void CUDASimulation::plan_transitions(bool counted) {
+    for (auto& f : fns_) f.swap_transition = false;
+    if (!counted) return;
+    std::map<const detail::StateList*, uint64_t> ub;
+    for (auto& kv : agents_)
+        for (auto& s : kv.second.states) ub[&s.second] = s.second.h_count;
+    for (auto& layer : m_->layers) {
+        if (!layer->host_functions.empty()) return;
+        std::vector<detail::FnRT*> fl;
+        for (auto& fd : layer->agent_functions) fl.push_back(&fns_[fn_index_.at(fd.get())]);
+        for (auto* f : fl) {
+            const auto* in = &f->agent->states.at(f->in_state);
+            const uint64_t n = ub[in];
+            if (f->in_state != f->end_state) {
+                const auto* dst = &f->agent->states.at(f->end_state);
+                bool alone = true;
+                for (auto* g : fl) {
+                    if (g == f) continue;
+                    for (const std::string* s : {&f->in_state, &f->end_state}) {
+                        if (g->agent == f->agent && (g->in_state == *s || g->end_state == *s)) alone = false;
+                        if (g->out_agent == f->agent && g->out_state == *s) alone = false;
+                    }
+                }
+                f->swap_transition = alone && !f->d->condition && ub[dst] == 0 && in->cap == dst->cap;
+            }
+            if (f->out_agent) ub[&f->out_agent->states.at(f->out_state)] += n;
+            if (f->in_state != f->end_state) {
+                ub[&f->agent->states.at(f->end_state)] += n;
+                if (!f->d->condition) ub[in] = 0;
+            }
+        }
+    }
+}
+
 // Upper bounds of every list for the coming step (births append, transitions move).
 inline bool CUDASimulation::ensure_capacity() {
     std::map<detail::StateList*, uint64_t> ub, peak;
@@ -1234,9 +1291,29 @@ inline bool CUDASimulation::ensure_capacity() {
         if (c > 0xFFFFFFFFull / 2) throw exception::InvalidOperation("population exceeds the 32-bit index range");
         return (uint32_t)c;
     };
+    std::map<detail::StateList*, uint32_t> target;
     for (auto& kv : peak) {
-        if (uint32_t c = need(kv.second, kv.first->cap)) {
-            grow_list(*kv.first, c);
+        const uint32_t c = need(kv.second, kv.first->cap);
+        target[kv.first] = c ? c : kv.first->cap;
+    }
+    // the two lists of an unconditional transition keep equal capacities, so that the
+    // transition can swap them when its end state is empty
+    for (bool again = true; again;) {
+        again = false;
+        for (auto& f : fns_) {
+            if (f.in_state == f.end_state || f.d->condition) continue;
+            auto* a = &f.agent->states.at(f.in_state);
+            auto* b = &f.agent->states.at(f.end_state);
+            const uint32_t m = std::max(target[a], target[b]);
+            if (target[a] != m || target[b] != m) {
+                target[a] = target[b] = m;
+                again = true;
+            }
+        }
+    }
+    for (auto& kv : target) {
+        if (kv.second > kv.first->cap) {
+            grow_list(*kv.first, kv.second);
             changed = true;
         }
     }
@@ -1551,6 +1628,7 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
     a.zero_page = (const char*)d_pages_;
     a.sink_page = (char*)d_pages_ + 128;
     a.stream_id = f.d->stream_id;
+    a.unit_tables = unit_tables_ ? 1 : 0;
     a.d_n = l.d_count;
     a.d_step = d_step_;
     a.seed = config_.random_seed;
@@ -1775,9 +1853,22 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
         tick("death:" + f.d->name, st, false);
         for (auto& v : l.vars) std::swap(v.cur, v.alt);
     }
-    // 3. state transition (CUDAFatAgent::transitionState): append all to the end state
-    if (f.in_state != f.end_state) {
+    // 3. state transition (CUDAFatAgent::transitionState): into a list known to be empty
+    // the two lists swap buffers (equal capacities) and only the count moves; otherwise
+    // every agent is appended to the end state
+    if (f.in_state != f.end_state && f.swap_transition) {
         auto& dl = f.agent->states.at(f.end_state);
+        tick("transition-swap:" + f.d->name, st, true);
+        for (size_t i = 0; i < l.vars.size(); ++i) {
+            std::swap(l.vars[i].cur, dl.vars[i].cur);
+            std::swap(l.vars[i].alt, dl.vars[i].alt);
+        }
+        detail::fgx_check(fgx_memcpy_d2d(dl.d_count, l.d_count, 4, st), "state transition count");
+        detail::fgx_check(fgx_fill_u32(l.d_count, 0, 1, st), "state transition count");
+        tick("transition-swap:" + f.d->name, st, false);
+    } else if (f.in_state != f.end_state) {
+        auto& dl = f.agent->states.at(f.end_state);
+        tick("transition:" + f.d->name, st, true);
         std::vector<const void*> src;
         std::vector<void*> dst;
         std::vector<uint32_t> by;
@@ -1800,6 +1891,7 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
                               "state transition");
             detail::fgx_check(fgx_fill_u32(l.d_count, 0, 1, st), "state transition count");
         }
+        tick("transition:" + f.d->name, st, false);
     }
     // 4. births (CUDAAgentStateList::scatterNew): appended after the survivors, ordered IDs
     if (f.out_agent) {
@@ -2028,11 +2120,14 @@ inline bool CUDASimulation::step() {
     ensure_init();
     if (!init_functions_ran_) run_init_functions();
     bool changed = false;
+    bool counted = false;
     if (model_can_grow() || step_ == 0 || populations_set_) {
         populations_set_ = false;
         sync_counts();
         changed = ensure_capacity();
+        counted = true;
     }
+    plan_transitions(counted);
     upload_env();
     const bool use_graph = cuda_config_.graphs && graph_eligible_ && !timing_;
     if (use_graph) {
@@ -2040,9 +2135,10 @@ inline bool CUDASimulation::step() {
         // One single-step graph per double-buffer arrangement at step start; a step
         // swaps buffers, so a model alternates between a small number of arrangements.
         const auto start = buffer_state();
+        const auto swaps = swap_plan();
         GraphEntry* hit = nullptr;
         for (auto& g : graphs_)
-            if (g.start == start) hit = &g;
+            if (g.start == start && g.swaps == swaps) hit = &g;
         if (!hit && graphs_.size() >= kMaxStepGraphs) {
             size_t lru = 0;
             for (size_t k = 1; k < graphs_.size(); ++k)
@@ -2060,6 +2156,7 @@ inline bool CUDASimulation::step() {
         if (!hit) {
             GraphEntry e;
             e.start = start;
+            e.swaps = swaps;
             detail::fgx_check(fgx_graph_begin(stream_), "graph begin");
             enqueue_step();
             detail::fgx_check(fgx_graph_end(stream_, &e.g), "graph end");

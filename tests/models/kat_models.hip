@@ -10,6 +10,8 @@
 //   birth_mandatory / birth_optional / birth_mandatory_death / birth_optional_death
 //       tests/test_cases/runtime/agent/test_device_agent_creation.cu:23-300
 //   transition          tests/test_cases/runtime/agent/detail/test_agent_state_transition.cu:67-174
+//   transition_src0_dest10 / transition_src10_dest0 / transition_src10_dest10
+//                       test_agent_state_transition.cu:67-250
 //   bruteforce          tests/test_cases/runtime/messaging/test_brute_force.cu:73-200
 //   random              tests/test_cases/runtime/random/test_agent_random.cu:28-185
 //   death_disabled / wrap_out_of_bounds   device errors (test_spatial_3d.cu:976-1039)
@@ -213,6 +215,30 @@ FLAMEGPU_AGENT_FUNCTION(OptionalOutputWithDeath, MessageNone, MessageNone) {
     }
     return ALIVE;
 }
+// test_agent_state_transition.cu:29-62
+FLAMEGPU_AGENT_FUNCTION(AgentGood, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<int>("x", 11);
+    for (int k = 0; k < 4; ++k) FLAMEGPU->setVariable<int, 4>("y", k, 23 + k);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(AgentBad, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<int>("x", 13);
+    for (int k = 0; k < 4; ++k) FLAMEGPU->setVariable<int, 4>("y", k, 3 + k);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(AgentDecrement, MessageNone, MessageNone) {
+    const unsigned int x = FLAMEGPU->getVariable<unsigned int>("x");
+    FLAMEGPU->setVariable<unsigned int>("x", x == 0 ? 0 : x - 1);
+    for (int k = 0; k < 4; ++k) FLAMEGPU->setVariable<int, 4>("z", k, 23 + k);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(AgentNull, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<unsigned int>("x", 0xFFFFFFFFu);
+    for (int k = 0; k < 4; ++k) FLAMEGPU->setVariable<int, 4>("z", k, 3 + k);
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION_CONDITION(Zero_X) { return FLAMEGPU->getVariable<unsigned int>("x") == 0; }
+
 FLAMEGPU_AGENT_FUNCTION(AddOne, MessageNone, MessageNone) {
     FLAMEGPU->setVariable<unsigned int>("x", FLAMEGPU->getVariable<unsigned int>("x") + 1);
     return ALIVE;
@@ -456,6 +482,45 @@ flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* 
         f.setInitialState("a");
         f.setEndState("b");
         model.newLayer().addAgentFunction(f);
+    } else if (v == "transition_src0_dest10" || v == "transition_src10_dest0") {
+        // test_agent_state_transition.cu:67-171 (Src_0_Dest_10, Src_10_Dest_0)
+        auto agent = model.newAgent("agent");
+        agent.newState("Start");
+        agent.newState("End");
+        agent.newState("End2");
+        agent.setInitialState("Start");
+        agent.newVariable<int>("x");
+        agent.newVariable<int, 4>("y");
+        auto f1 = agent.newFunction("Function1", AgentGood);
+        f1.setInitialState("Start");
+        f1.setEndState("End");
+        if (v == "transition_src0_dest10") {
+            model.newLayer().addAgentFunction(f1);
+        } else {
+            auto f2 = agent.newFunction("Function2", AgentBad);
+            f2.setInitialState("End");
+            f2.setEndState("End2");
+            model.newLayer().addAgentFunction(f2);
+            model.newLayer().addAgentFunction(f1);
+        }
+    } else if (v == "transition_src10_dest10") {
+        // test_agent_state_transition.cu:172-250: a conditional transition, so lists never swap
+        auto agent = model.newAgent("agent");
+        agent.newState("Start");
+        agent.newState("End");
+        agent.setInitialState("Start");
+        agent.newVariable<unsigned int>("x");
+        agent.newVariable<unsigned int>("y");
+        agent.newVariable<int, 4>("z");
+        auto f1 = agent.newFunction("Function1", AgentDecrement);
+        f1.setInitialState("Start");
+        f1.setEndState("Start");
+        auto f2 = agent.newFunction("Function2", AgentNull);
+        f2.setInitialState("Start");
+        f2.setEndState("End");
+        f2.setFunctionCondition(Zero_X);
+        model.newLayer().addAgentFunction(f1);
+        model.newLayer().addAgentFunction(f2);
     } else if (v == "bruteforce") {
         auto message = model.newMessage("msg");
         message.newVariable<int>("x");

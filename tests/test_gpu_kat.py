@@ -28,13 +28,16 @@ def _expected_counts(P, dims, grid, mask=None):
     return nb[tuple(cells)]
 
 
+# spatial tests run with the per-wave unit tables (default) and with the per-lane unit walk
+@pytest.mark.parametrize("tables", [1, 0])
 @pytest.mark.parametrize("variant", ["mandatory3d", "optional3d", "optionalnone3d"])
-def test_spatial3d_message_counts(variant):
+def test_spatial3d_message_counts(variant, tables):
     rng = np.random.default_rng(11)
     n = 2049
     P = rng.uniform(0, 5, size=(3, n)).astype(np.float32)
     do_output = (rng.uniform(size=n) < 0.8).astype(np.int32)
     s = _sim(variant)
+    s.set_option("unit_tables", tables)
     z = np.zeros(n, np.uint32)
     s.set_agents("agent", [P[0], P[1], P[2], do_output, z, z])
     s.step(1)
@@ -50,11 +53,13 @@ def test_spatial3d_message_counts(variant):
     assert np.all(bad == 0)
 
 
-def test_spatial2d_message_counts():
+@pytest.mark.parametrize("tables", [1, 0])
+def test_spatial2d_message_counts(tables):
     rng = np.random.default_rng(12)
     n = 2049
     P = rng.uniform(0, 5, size=(2, n)).astype(np.float32)
     s = _sim("mandatory2d")
+    s.set_option("unit_tables", tables)
     z = np.zeros(n, np.uint32)
     s.set_agents("agent", [P[0], P[1], z, z])
     s.step(1)
@@ -63,8 +68,9 @@ def test_spatial2d_message_counts():
     assert np.all(bad == 0)
 
 
+@pytest.mark.parametrize("tables", [1, 0])
 @pytest.mark.parametrize("offset", [(0.0, 0.0, 0.0), (141.0, -540.0, 200.0), (-1401.5, 5640.3, -2008.8)])
-def test_spatial3d_wrapped(offset):
+def test_spatial3d_wrapped(offset, tables):
     off = np.array(offset, np.float32)
     i, j, k = np.meshgrid(np.arange(35), np.arange(35), np.arange(35), indexing="ij")
     P = [(i.ravel() * np.float32(2.0)).astype(np.float32) + off[0],
@@ -72,6 +78,7 @@ def test_spatial3d_wrapped(offset):
          (k.ravel() * np.float32(2.0)).astype(np.float32) + off[2]]
     n = len(P[0])
     s = _sim("wrapped3d", list(offset))
+    s.set_option("unit_tables", tables)
     zf, zu = np.zeros(n, np.float32), np.zeros(n, np.uint32)
     s.set_agents("agent", P + [zf, zf, zf, zu, zu])
     s.step(1)
@@ -81,14 +88,16 @@ def test_spatial3d_wrapped(offset):
     assert np.all(count == 27)
 
 
+@pytest.mark.parametrize("tables", [1, 0])
 @pytest.mark.parametrize("offset", [(0.0, 0.0), (141.0, -540.0), (-1401.5, 5640.3)])
-def test_spatial2d_wrapped(offset):
+def test_spatial2d_wrapped(offset, tables):
     off = np.array(offset, np.float32)
     i, j = np.meshgrid(np.arange(35), np.arange(35), indexing="ij")
     P = [(i.ravel() * np.float32(2.0)).astype(np.float32) + off[0],
          (j.ravel() * np.float32(2.0)).astype(np.float32) + off[1]]
     n = len(P[0])
     s = _sim("wrapped2d", list(offset))
+    s.set_option("unit_tables", tables)
     zf, zu = np.zeros(n, np.float32), np.zeros(n, np.uint32)
     s.set_agents("agent", P + [zf, zf, zu, zu])
     s.step(1)
@@ -190,6 +199,91 @@ def test_state_transition():
     assert s.count("agent", "a") == 0 and s.count("agent", "b") == n
     (x,), _ = s.get_agents("agent", "b")
     assert np.array_equal(x, np.arange(n, dtype=np.uint32) + 1)
+
+
+# tests/test_cases/runtime/agent/detail/test_agent_state_transition.cu: a transition into an
+# empty state swaps the two lists (CUDAFatAgent.cu:139-184); kernel timing labels the
+# transitions "transition-swap:" (lists swapped) or "transition:" (agents appended)
+
+
+def _transitions(s):
+    kt = s.kernel_times(reset=True)
+    return {k: v[1] for k, v in kt.items() if k.startswith("transition")}
+
+
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_transition_src0_dest10(graphs):
+    n = 10
+    s = _sim("transition_src0_dest10")
+    s.set_option("graphs", graphs)
+    y0 = np.tile(np.array([13, 14, 15, 16], np.int32), n)
+    s.set_agents("agent", [np.full(n, 12, np.int32), y0], state="Start")
+    for step in range(2):
+        s.step(1)
+        assert s.count("agent", "Start") == 0 and s.count("agent", "End") == n and s.count("agent", "End2") == 0
+        (x, y), _ = s.get_agents("agent", "End")
+        assert np.all(x == 11) and np.array_equal(y.reshape(n, 4), np.tile([23, 24, 25, 26], (n, 1)))
+
+
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_transition_src10_dest0(graphs):
+    n = 10
+    s = _sim("transition_src10_dest0")
+    s.set_option("graphs", graphs)
+    y0 = np.tile(np.array([13, 14, 15, 16], np.int32), n)
+    s.set_agents("agent", [np.full(n, 12, np.int32), y0], state="Start")
+    s.step(1)
+    assert [s.count("agent", st) for st in ("Start", "End", "End2")] == [0, n, 0]
+    (x, y), _ = s.get_agents("agent", "End")
+    assert np.all(x == 11) and np.array_equal(y.reshape(n, 4), np.tile([23, 24, 25, 26], (n, 1)))
+    s.step(1)
+    assert [s.count("agent", st) for st in ("Start", "End", "End2")] == [0, 0, n]
+    (x, y), ids = s.get_agents("agent", "End2")
+    assert np.all(x == 13) and np.array_equal(y.reshape(n, 4), np.tile([3, 4, 5, 6], (n, 1)))
+    assert sorted(ids.tolist()) == list(range(1, n + 1))
+
+
+def test_transition_swaps_only_into_an_empty_state():
+    n = 1000
+    s = _sim("transition_src10_dest0")
+    s.set_option("kernel_timing", 1)
+    s.set_agents("agent", [np.arange(n, dtype=np.int32), np.zeros(4 * n, np.int32)], state="Start")
+    s.step(1)
+    # End2 <- End (nothing to move, End2 empty), then End <- Start (End empty): both swap
+    assert _transitions(s) == {"transition-swap:Function2": 1, "transition-swap:Function1": 1}
+    # agents placed in Start again, End holds n: Function2 empties End into the empty End2
+    # first (swap); Function1 then moves Start into the now-empty End (swap)
+    (x, y), ids = s.get_agents("agent", "End")
+    s.set_agents("agent", [x, y], ids=ids + n, state="Start")
+    s.step(1)
+    assert _transitions(s) == {"transition-swap:Function2": 1, "transition-swap:Function1": 1}
+    assert [s.count("agent", st) for st in ("Start", "End", "End2")] == [0, n, n]
+    # End2 now holds n: the next End -> End2 transition appends
+    s.step(1)
+    assert _transitions(s) == {"transition:Function2": 1, "transition-swap:Function1": 1}
+    assert [s.count("agent", st) for st in ("Start", "End", "End2")] == [0, 0, 2 * n]
+    (x, y), ids = s.get_agents("agent", "End2")
+    assert np.all(x == 13) and len(set(ids.tolist())) == 2 * n
+
+
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_transition_src10_dest10_conditional(graphs):
+    """Src_10_Dest_10: agents leave Start as their counter reaches zero, 10 per step."""
+    n, rounds = 10, 3
+    s = _sim("transition_src10_dest10")
+    s.set_option("graphs", graphs)
+    val = (1 + np.arange(rounds * n) % rounds).astype(np.uint32)
+    z = np.tile(np.array([13, 14, 15, 16], np.int32), rounds * n)
+    s.set_agents("agent", [val, val.copy(), z], state="Start")
+    for i in range(1, rounds + 1):
+        s.step(1)
+        assert s.count("agent", "Start") == (rounds - i) * n and s.count("agent", "End") == i * n
+        (_, ys, zs), _ = s.get_agents("agent", "Start")
+        assert np.array_equal(zs.reshape(-1, 4), np.tile([23, 24, 25, 26], (len(ys), 1)))
+        assert sorted(set(ys.tolist())) == list(range(i + 1, rounds + 1))
+        (_, ye, ze), _ = s.get_agents("agent", "End")
+        assert np.array_equal(ze.reshape(-1, 4), np.tile([3, 4, 5, 6], (len(ye), 1)))
+        assert np.array_equal(np.bincount(ye, minlength=rounds + 1)[1:i + 1], [n] * i)
 
 
 def test_bruteforce_sum():

@@ -15,10 +15,11 @@ from flamegpu2_amd.models import Simulation
 pytestmark = pytest.mark.gpu
 
 
-def _run_pair(model, oracle_model, params, pops, agent_names, steps, seed, graphs=True, staging=0):
+def _run_pair(model, oracle_model, params, pops, agent_names, steps, seed, graphs=True, staging=0, tables=1):
     sim = Simulation(model, params, seed=seed)
     sim.set_option("graphs", graphs)
     sim.set_option("staging", staging)
+    sim.set_option("unit_tables", tables)
     ref = O.Sim(oracle_model, params, seed)
     for t, (name, arrs) in enumerate(zip(agent_names, pops)):
         sim.set_agents(name, arrs)
@@ -69,20 +70,25 @@ def test_circles_long_run_statistics():
     assert abs(float(got[3].mean()) - float(exp[3].mean())) <= 0.01 * max(float(exp[3].mean()), 1e-6) + 1e-6
 
 
-@pytest.mark.parametrize("n,steps,graphs,staging", [(32768, 5, True, 0), (32768, 5, False, 0), (1 << 20, 1, True, 0),
-                                                    (32768, 5, True, 1), (1 << 20, 1, True, 1)])
-def test_boids_bit_exact(n, steps, graphs, staging):
+# staging: the LDS-staged neighbourhood; tables: per-wave unit tables (0 = per-lane unit walk)
+@pytest.mark.parametrize("n,steps,graphs,staging,tables", [(32768, 5, True, 0, 1), (32768, 5, False, 0, 1),
+                                                           (1 << 20, 1, True, 0, 1), (32768, 5, True, 1, 1),
+                                                           (1 << 20, 1, True, 1, 1), (32768, 5, True, 0, 0),
+                                                           (1 << 20, 1, True, 0, 0)])
+def test_boids_bit_exact(n, steps, graphs, staging, tables):
     params, arrs = workloads.boids(n, seed=7)
-    sim, ref = _run_pair("boids_spatial3D", 1, params, [arrs], ["Boid"], steps, 7, graphs=graphs, staging=staging)
+    sim, ref = _run_pair("boids_spatial3D", 1, params, [arrs], ["Boid"], steps, 7, graphs=graphs, staging=staging,
+                         tables=tables)
     assert np.array_equal(sim.pbm("location"), ref.pbm(0))
     _compare(sim, ref, 0, "Boid", exact=True)
 
 
-@pytest.mark.parametrize("staging", [0, 1])
-def test_predprey_small_bit_exact_traces(staging):
+@pytest.mark.parametrize("staging,tables", [(0, 1), (1, 1), (0, 0)])
+def test_predprey_small_bit_exact_traces(staging, tables):
     params, prey, pred = workloads.predprey(40000, 10000, seed=3, width=200.0)
     sim = Simulation("predator_prey", params, seed=3)
     sim.set_option("staging", staging)
+    sim.set_option("unit_tables", tables)
     ref = O.Sim(2, params, 3)
     sim.set_agents("Prey", prey)
     sim.set_agents("Predator", pred)
