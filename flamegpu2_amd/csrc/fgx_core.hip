@@ -158,11 +158,22 @@ int fgx_ctx_create(int device, fgx_ctx** out) {
     c->c.cu_count = prop.multiProcessorCount;
     c->c.hbm_bytes = prop.totalGlobalMem;
     c->c.xcd_count = 8;
+    // fused-layer image space, allocated here so that no later call allocates (calls may be captured)
+    if (hipMalloc((void**)&c->c.layers.dev, FGX_LAYER_SPACE_BYTES) != hipSuccess ||
+        hipHostMalloc((void**)&c->c.layers.host, FGX_LAYER_SPACE_BYTES, hipHostMallocDefault) != hipSuccess) {
+        fgx_ctx_destroy(c);
+        return fail(FGX_ERR_HIP, "fgx_ctx_create: layer image space allocation failed");
+    }
     *out = c;
     return FGX_OK;
 }
 
 int fgx_ctx_destroy(fgx_ctx* ctx) {
+    if (ctx) {
+        if (ctx->c.layers.dev || ctx->c.layers.host) hipSetDevice(ctx->c.device);
+        if (ctx->c.layers.dev) hipFree(ctx->c.layers.dev);
+        if (ctx->c.layers.host) hipHostFree(ctx->c.layers.host);
+    }
     delete ctx;
     return FGX_OK;
 }
@@ -372,6 +383,64 @@ int fgx_graph_destroy(fgx_graph* g) {
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
     delete g;
+    return FGX_OK;
+}
+
+// Fused layer launch: assemble the image (header + launch records), find it in the
+// context's image space or upload it there, then one launch over all functions' blocks.
+int fgx_layer_launch(fgx_ctx* ctx, const void* kernel, uint32_t block, const fgx_fn_launch* fns, uint32_t nfns,
+                     uint32_t args_bytes, void* stream) {
+    if (!ctx || !kernel || !fns) return fail(FGX_ERR_ARG, "fgx_layer_launch: NULL context, kernel or functions");
+    if (nfns == 0 || nfns > FGX_LAYER_MAX_FNS)
+        return fail(FGX_ERR_ARG, "fgx_layer_launch: 1 to " + std::to_string(FGX_LAYER_MAX_FNS) + " functions per launch");
+    if (args_bytes == 0 || args_bytes > FGX_LAYER_MAX_ARGS_BYTES)
+        return fail(FGX_ERR_ARG, "fgx_layer_launch: launch record size out of range");
+    if (block == 0 || block > 1024 || block % 64 != 0) return fail(FGX_ERR_ARG, "fgx_layer_launch: block must be 64..1024, a multiple of 64");
+    static_assert(sizeof(fgx_layer_header) <= FGX_LAYER_ARGS_OFFSET, "layer header must fit before the launch records");
+    fgx_layer_header h;
+    std::memset(&h, 0, sizeof(h));
+    h.nfns = nfns;
+    h.stride = (args_bytes + 15u) & ~15u;
+    uint32_t lds = 0;
+    uint64_t grid = 0;
+    for (uint32_t k = 0; k < nfns; ++k) {
+        if (!fns[k].args || fns[k].blocks == 0) return fail(FGX_ERR_ARG, "fgx_layer_launch: function without launch record or blocks");
+        h.begin[k] = (uint32_t)grid;
+        h.fn[k] = fns[k].fn;
+        grid += fns[k].blocks;
+        lds = std::max(lds, fns[k].lds_bytes);
+    }
+    if (grid >= (1ull << 31)) return fail(FGX_ERR_ARG, "fgx_layer_launch: too many blocks");
+    h.begin[nfns] = (uint32_t)grid;
+    const size_t bytes = FGX_LAYER_ARGS_OFFSET + (size_t)nfns * h.stride;
+    std::string img(bytes, '\0');
+    std::memcpy(&img[0], &h, sizeof(h));
+    for (uint32_t k = 0; k < nfns; ++k) std::memcpy(&img[FGX_LAYER_ARGS_OFFSET + (size_t)k * h.stride], fns[k].args, args_bytes);
+    LayerSpace& L = ctx->c.layers;
+    const char* d_img = nullptr;
+    auto it = L.images.find(img);
+    if (it != L.images.end()) {
+        d_img = it->second;
+    } else {
+        if (!L.dev || !L.host) return fail(FGX_ERR_STATE, "fgx_layer_launch: context has no layer image space");
+        const size_t at = (L.used + 255) & ~(size_t)255;
+        if (at + bytes > FGX_LAYER_SPACE_BYTES) return fail(FGX_ERR_CAPACITY, "fgx_layer_launch: layer image space full");
+        std::memcpy(L.host + at, img.data(), bytes);
+        FGX_HIP(hipMemcpyAsync(L.dev + at, L.host + at, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+        L.used = at + bytes;
+        d_img = L.dev + at;
+        L.images.emplace(std::move(img), d_img);
+    }
+    void* arg = const_cast<char*>(d_img);
+    void* kargs[] = {&arg};
+    FGX_HIP(hipLaunchKernel(kernel, dim3((uint32_t)grid), dim3(block), kargs, lds, (hipStream_t)stream));
+    return FGX_OK;
+}
+
+int fgx_layer_space_info(fgx_ctx* ctx, uint32_t* images, size_t* bytes) {
+    if (!ctx) return fail(FGX_ERR_ARG, "fgx_layer_space_info: NULL context");
+    if (images) *images = (uint32_t)ctx->c.layers.images.size();
+    if (bytes) *bytes = ctx->c.layers.used;
     return FGX_OK;
 }
 

@@ -22,6 +22,14 @@ int fail(int code, const std::string& msg);
             return ::fgx::fail(FGX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Device space for fused-layer images: immutable once written, found by content.
+struct LayerSpace {
+    char* dev = nullptr;        // FGX_LAYER_SPACE_BYTES of device memory
+    char* host = nullptr;       // pinned staging copy (the source of each image's upload)
+    size_t used = 0;
+    std::unordered_map<std::string, const char*> images;
+};
+
 struct Ctx {
     int device = 0;
     int cu_count = 0;
@@ -29,6 +37,7 @@ struct Ctx {
     size_t hbm_bytes = 0;
     // FGX_OPT_KEEP_SCRATCH_ZERO: bin sorts leave their histogram zeroed
     int keep_zero = 0;
+    LayerSpace layers;
 };
 
 // Host-side record, per scratch buffer, of the bin count whose histogram region a sort

@@ -55,6 +55,13 @@ typedef CONDITION_RESULT (*FLAMEGPU_EXIT_CONDITION_POINTER)(HostAPI*);
 namespace detail {
 typedef void (*LaunchFn)(const FnArgs& args, uint32_t grid, uint32_t block, uint32_t smem, void* stream);
 typedef void (*CondLaunchFn)(const FnArgs& args, uint32_t* flags, uint32_t grid, void* stream);
+// The layer kernel of a translation unit: one __global__ function that runs any of the
+// unit's agent functions, chosen per block from a fused-layer image (fgx_layer_launch).
+struct LayerKernel {
+    const void* kernel = nullptr;                        // host stub of the layer kernel
+    int (*index_of)(LaunchFn launch) = nullptr;          // a function's index in it, or -1
+    uint32_t (*lds_bytes)(int index, uint32_t block) = nullptr;   // its dynamic LDS per block
+};
 typedef void (*LaunchSegFn)(const FnArgs& args, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t apt,
                             uint32_t grid, uint32_t block, void* stream);
 template <typename C>

@@ -26,6 +26,11 @@ struct fgxm_sim {
 namespace fgxm_detail {
 inline thread_local std::string last_error;
 
+// The layer kernel of this translation unit: every agent function defined above.
+inline flamegpu::detail::LayerKernel layer_kernel() {
+    return flamegpu::detail::LayerKernelOf<flamegpu::detail::fn_collect<0, __COUNTER__>::type>::get();
+}
+
 template <typename F>
 int guard(F&& f) {
     try {
@@ -98,6 +103,7 @@ FGXM_EXPORT int fgxm_create(const char* variant, const float* params, uint32_t n
             s->sim->SimulationConfig().random_seed = seed;
             s->sim->CUDAConfig().device_id = device;
             s->sim->applyConfig();
+            s->sim->setLayerKernel(fgxm_detail::layer_kernel());
         } catch (...) {
             delete s;
             throw;
@@ -179,6 +185,7 @@ FGXM_EXPORT int fgxm_set_option(fgxm_sim* s, const char* key, int value) {
         else if (k == "kernel_timing") s->sim->setKernelTiming(value != 0);
         else if (k == "staging") s->sim->setNeighbourhoodStaging(value != 0);
         else if (k == "unit_tables") s->sim->setUnitTables(value != 0);
+        else if (k == "layer_fusion") s->sim->setLayerFusion(value != 0);
         else if (k == "bins_per_radius") s->sim->setBinWidthMultiplier(bins_per_radius_mult(value));
         else throw flamegpu::exception::InvalidArgument("unknown option " + k);
     });
@@ -232,6 +239,15 @@ FGXM_EXPORT int fgxm_timed_steps(fgxm_sim* s, uint32_t steps, double* ms) {
 
 FGXM_EXPORT int fgxm_step_counter(fgxm_sim* s, uint32_t* out) {
     return fgxm_detail::guard([&] { *out = s->sim->getStepCounter(); });
+}
+
+FGXM_EXPORT int fgxm_stat(fgxm_sim* s, const char* key, uint64_t* out) {
+    return fgxm_detail::guard([&] {
+        const std::string k = key ? key : "";
+        if (k == "fused_layer_launches") *out = s->sim->fusedLayerLaunches();
+        else if (k == "layer_images") *out = s->sim->layerImages();
+        else throw flamegpu::exception::InvalidArgument("unknown counter " + k);
+    });
 }
 
 FGXM_EXPORT int fgxm_simulate(fgxm_sim* s, uint32_t steps) {

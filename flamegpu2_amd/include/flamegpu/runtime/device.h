@@ -17,6 +17,7 @@
 #include <cmath>
 #include <limits>
 #include <type_traits>
+#include <utility>
 
 #include "flamegpu/model.h"
 
@@ -1587,11 +1588,15 @@ __device__ __forceinline__ __attribute__((address_space(3))) uint32_t* wave_tabl
     return nullptr;
 }
 
+// One block of an agent-function launch: block `blk` of the function runs agents
+// [first + blk * blockDim, ...) of its list.  Shared by the function's own kernel (its
+// launch record in the kernel-argument segment) and the layer kernel (its record in
+// device memory).
 template <typename F>
-__global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
+__device__ __forceinline__ void run_function_block(const FnArgs& a, uint32_t blk) {
     const uint32_t first = a.d_first ? load_count(a.d_first) : 0u;
     const uint32_t n = a.d_n ? load_count(a.d_n) : a.n_cap;
-    const uint32_t o = xcd_block() * blockDim.x + threadIdx.x;
+    const uint32_t o = blk * blockDim.x + threadIdx.x;
     if (a.d_msg_count_out && o == 0) *a.d_msg_count_out = n;   // one message per agent (no condition here)
     if (first + o >= n) return;
     const uint32_t i = first + o;
@@ -1600,6 +1605,84 @@ __global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
     const ThreadCtx c{a,  ErrState{a.d_error, i}, a.zero_page, a.sink_page, i, a.in_meta, a.env, a.d_step, a.seed, nullptr,
                       0u, mn,                     o,           a.stream_id, a.unit_tables ? wave_table<F>() : nullptr};
     run_agent<F>(c);
+}
+
+template <typename F>
+__global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
+    run_function_block<F>(a, xcd_block());
+}
+
+// ---------------------------------------------------------------- layer kernel
+// Compile-time registry of a translation unit's agent functions.  FLAMEGPU_AGENT_FUNCTION
+// takes a __COUNTER__ value N and instantiates fn_put<N, F>, whose friend definitions give
+// fn_key<N> a definition; fn_collect<0, End> then lists, in definition order, every F
+// registered below End.  The layer kernel is instantiated over that list where the
+// model library closes its translation unit (flamegpu/model_abi.h).
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wundefined-inline"
+#pragma clang diagnostic ignored "-Wundefined-internal"
+template <int N>
+struct fn_key {
+    friend constexpr int fn_defined(fn_key<N>);
+    friend auto fn_type(fn_key<N>);
+};
+template <int N, typename F>
+struct fn_put {
+    friend constexpr int fn_defined(fn_key<N>) { return 1; }
+    friend auto fn_type(fn_key<N>) { return static_cast<F*>(nullptr); }
+    static constexpr bool ok = true;
+};
+template <int N, int = fn_defined(fn_key<N>{})>
+constexpr bool fn_registered(int) { return true; }
+template <int N>
+constexpr bool fn_registered(...) { return false; }
+#pragma clang diagnostic pop
+
+template <typename... Fs>
+struct fn_list {};
+template <typename A, typename B>
+struct fn_concat;
+template <typename... A, typename... B>
+struct fn_concat<fn_list<A...>, fn_list<B...>> {
+    using type = fn_list<A..., B...>;
+};
+template <int N, bool On = fn_registered<N>(0)>
+struct fn_at {
+    using type = fn_list<>;
+};
+template <int N>
+struct fn_at<N, true> {
+    using type = fn_list<std::remove_pointer_t<decltype(fn_type(fn_key<N>{}))>>;
+};
+// the slots [Lo, Hi), split in halves so that instantiation depth stays logarithmic
+template <int Lo, int Hi, bool One = (Hi - Lo <= 1)>
+struct fn_collect {
+    using type = typename fn_concat<typename fn_collect<Lo, (Lo + Hi) / 2>::type,
+                                    typename fn_collect<(Lo + Hi) / 2, Hi>::type>::type;
+};
+template <int Lo, int Hi>
+struct fn_collect<Lo, Hi, true> {
+    using type = std::conditional_t<(Hi > Lo), typename fn_at<Lo>::type, fn_list<>>;
+};
+
+template <typename... Fs, size_t... I>
+__device__ __forceinline__ void run_layer_fn(uint32_t fn, const FnArgs& a, uint32_t blk, std::index_sequence<I...>) {
+    ((fn == I ? run_function_block<Fs>(a, blk) : (void)0), ...);
+}
+
+// One launch for the functions of a layer (fgx_layer_launch builds the image): logical
+// block b belongs to function k with begin[k] <= b < begin[k + 1] and runs as that
+// function's block b - begin[k], with the function's launch record from the image.
+template <typename... Fs>
+__global__ void __launch_bounds__(256) layer_kernel(const char* __restrict__ image) {
+    const fgx_layer_header& h = *reinterpret_cast<const fgx_layer_header*>(image);
+    const uint32_t b = xcd_block();
+    const uint32_t nf = h.nfns;
+    uint32_t k = 0;
+#pragma unroll
+    for (uint32_t j = 1; j < FGX_LAYER_MAX_FNS; ++j) k += (j < nf && b >= h.begin[j]) ? 1u : 0u;
+    const FnArgs& a = *reinterpret_cast<const FnArgs*>(image + FGX_LAYER_ARGS_OFFSET + (size_t)k * h.stride);
+    run_layer_fn<Fs...>(h.fn[k], a, b - h.begin[k], std::index_sequence_for<Fs...>{});
 }
 
 // Batched launch: the simulations of a batch share one launch.  Tile t (blockDim x
@@ -1847,6 +1930,30 @@ void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nse
     hipLaunchKernelGGL(agent_function_kernel_seg<F>, dim3(grid), dim3(block), fn_lds_bytes<F>(block), (hipStream_t)stream,
                        a, segs, nsegs, ntiles, apt);
 }
+
+// Host view of the layer kernel over the functions Fs (see LayerKernel).
+template <typename L>
+struct LayerKernelOf;
+template <typename... Fs>
+struct LayerKernelOf<fn_list<Fs...>> {
+    static int index_of(LaunchFn launch) {
+        const LaunchFn fns[] = {&launch_agent_function<Fs>..., nullptr};
+        for (int k = 0; k < (int)sizeof...(Fs); ++k)
+            if (fns[k] == launch) return k;
+        return -1;
+    }
+    static uint32_t lds_bytes(int index, uint32_t block) {
+        const uint32_t b[] = {fn_lds_bytes<Fs>(block)..., 0u};
+        return index >= 0 && index < (int)sizeof...(Fs) ? b[index] : 0u;
+    }
+    static LayerKernel get() {
+        LayerKernel k;
+        if constexpr (sizeof...(Fs) > 0) k.kernel = reinterpret_cast<const void*>(&layer_kernel<Fs...>);
+        k.index_of = &index_of;
+        k.lds_bytes = &lds_bytes;
+        return k;
+    }
+};
 }  // namespace detail
 
 }  // namespace flamegpu
@@ -1857,9 +1964,11 @@ void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nse
     struct funcName##_impl {                                                                                        \
         typedef message_in MsgIn;                                                                                   \
         typedef message_out MsgOut;                                                                                 \
+        static constexpr int fgx_slot = __COUNTER__;                                                                \
         __device__ __forceinline__ flamegpu::AGENT_STATUS operator()(                                               \
             flamegpu::DeviceAPI<message_in, message_out>* FLAMEGPU) const;                                          \
     };                                                                                                              \
+    static_assert(flamegpu::detail::fn_put<funcName##_impl::fgx_slot, funcName##_impl>::ok, "registry");            \
     [[maybe_unused]] constexpr funcName##_impl funcName{};                                                           \
     __device__ __forceinline__ flamegpu::AGENT_STATUS funcName##_impl::operator()(                                  \
         flamegpu::DeviceAPI<message_in, message_out>* FLAMEGPU) const

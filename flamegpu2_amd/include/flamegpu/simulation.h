@@ -222,6 +222,7 @@ struct FnRT {
     AgentRT* out_agent = nullptr;
     std::string out_state;
     int sort_dims = 0;              // 2/3 when the function triggers the auto spatial sort
+    int fused_fn = -2;              // index in the layer kernel (-1: launched on its own, -2: not looked up)
     bool swap_transition = false;   // this step: the end state is known empty, so the lists swap
     uint32_t cap = 0;               // per-launch buffers sized for this many agents
     uint32_t* death_flags = nullptr;
@@ -627,6 +628,26 @@ class CUDASimulation {
         unit_tables_ = on;
         clear_graphs();
     }
+    // The layer kernel of the model's translation unit (set by the model library): layers
+    // of several independent functions then run as one launch when in-layer concurrency
+    // is on.  setLayerFusion(false) keeps one launch per function, on forked streams.
+    void setLayerKernel(const detail::LayerKernel& k) {
+        layer_kernel_ = k;
+        for (auto& f : fns_) f.fused_fn = -2;
+        clear_graphs();
+    }
+    void setLayerFusion(bool on) {
+        layer_fusion_ = on;
+        clear_graphs();
+    }
+    // Fused layer launches enqueued so far (graph replays not counted).
+    uint64_t fusedLayerLaunches() const { return fused_launches_; }
+    uint64_t layerImages() {
+        ensure_init();
+        uint32_t n = 0;
+        detail::fgx_check(fgx_layer_space_info(ctx_, &n, nullptr), "layer images");
+        return n;
+    }
     // Spatial message bins of width radius x mult, mult = 1 (the reference's grid) or 0.5.
     // With half-width bins the iterators search 2 bins either side: the same in-radius
     // neighbours from 5^3 bins of 1/8 the volume (125/216 of the candidates), visited in
@@ -719,6 +740,9 @@ class CUDASimulation {
     void size_function(detail::FnRT& f, uint32_t cap);
     bool ensure_capacity();
     bool model_can_grow() const;
+    uint32_t prepare_function(detail::FnRT& f, void* st, detail::FnArgs& a);
+    bool launch_layer_fused(const LayerData& layer);
+    static constexpr uint32_t kFnBlock = 256;   // threads per block of every agent-function launch
     void plan_transitions(bool counted);
     std::vector<uint8_t> swap_plan() const {
         std::vector<uint8_t> p;
@@ -823,6 +847,8 @@ class CUDASimulation {
     struct GraphEntry {
         std::vector<void*> start, end;
         std::vector<uint8_t> swaps;   // the step's transition plan (which transitions swap lists)
+        bool stale = false;           // recorded a layer image upload: captured again next time
+        bool concurrent = true;       // captured with in-layer concurrency on
         fgx_graph* g = nullptr;
         uint64_t last_use = 0;
     };
@@ -847,6 +873,11 @@ class CUDASimulation {
     bool timing_ = false;
     bool stage_enabled_ = false;
     bool unit_tables_ = true;
+    detail::LayerKernel layer_kernel_{};
+    bool layer_fusion_ = true;
+    bool layer_space_warned_ = false;
+    bool new_layer_images_ = false;   // the enqueue being captured uploaded a layer image
+    uint64_t fused_launches_ = 0;
     float bin_mult_ = 1.0f;
     struct PendingTime {
         std::string label;
@@ -1619,10 +1650,21 @@ inline void CUDASimulation::collect_timings() {
 }
 
 inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
+    detail::FnArgs a;
+    const uint32_t grid = prepare_function(f, st, a);
+    const std::string label = "fn:" + f.agent->d->name + "::" + f.d->name;
+    tick(label, st, true);
+    f.d->launch(a, grid, kFnBlock, stage_smem_bytes(a), st);
+    tick(label, st, false);
+    if (hipGetLastError() != hipSuccess) throw exception::CUDAError("agent function launch failed: " + f.d->name);
+}
+
+// Launch record and grid of one agent function; the device-side preparation (new-agent
+// defaults, the launch count) is enqueued on `st`.
+inline uint32_t CUDASimulation::prepare_function(detail::FnRT& f, void* st, detail::FnArgs& a) {
     auto& l = f.agent->states.at(f.in_state);
     if (f.cap < l.cap)
         throw exception::InvalidOperation("function '" + f.d->name + "' buffers are smaller than its agent list");
-    detail::FnArgs a;
     std::memset(&a, 0, sizeof(a));
     a.n_cap = l.cap;
     a.zero_page = (const char*)d_pages_;
@@ -1726,13 +1768,53 @@ inline void CUDASimulation::launch_function(detail::FnRT& f, void* st) {
     // mandatory output into an emptied list: the kernel sets the message count
     f.msg_count_in_kernel = f.mout && f.mout->truncate && !f.d->message_output_optional && !f.d->condition;
     a.d_msg_count_out = f.msg_count_in_kernel ? f.mout->d_count : nullptr;
-    const uint32_t block = 256;
-    const uint32_t grid = std::max<uint32_t>(1, (l.cap + block - 1) / block);
-    const std::string label = "fn:" + f.agent->d->name + "::" + f.d->name;
-    tick(label, st, true);
-    f.d->launch(a, grid, block, stage_smem_bytes(a), st);
-    tick(label, st, false);
-    if (hipGetLastError() != hipSuccess) throw exception::CUDAError("agent function launch failed: " + f.d->name);
+    return std::max<uint32_t>(1, (l.cap + kFnBlock - 1) / kFnBlock);
+}
+
+// In-layer concurrency as one launch: the layer kernel of the model's translation unit
+// runs every function of the layer, each over its own blocks, from launch records held
+// in device memory (fgx_layer_launch).  Layers it cannot take (a function outside the
+// kernel, LDS staging) return false and run on forked streams instead.
+inline bool CUDASimulation::launch_layer_fused(const LayerData& layer) {
+    if (!layer_fusion_ || !layer_kernel_.kernel || layer.agent_functions.size() > FGX_LAYER_MAX_FNS) return false;
+    std::vector<detail::FnRT*> fl;
+    for (auto& fd : layer.agent_functions) {
+        auto& f = fns_[fn_index_.at(fd.get())];
+        if (f.fused_fn == -2) f.fused_fn = layer_kernel_.index_of(f.d->launch);
+        if (f.fused_fn < 0) return false;
+        if (stage_enabled_ && f.min && f.min->spatial) return false;
+        fl.push_back(&f);
+    }
+    std::vector<detail::FnArgs> args(fl.size());
+    std::vector<fgx_fn_launch> L(fl.size());
+    std::vector<uint32_t> grids(fl.size());
+    for (size_t k = 0; k < fl.size(); ++k) {
+        grids[k] = prepare_function(*fl[k], stream_, args[k]);
+        L[k].args = &args[k];
+        L[k].fn = (uint32_t)fl[k]->fused_fn;
+        L[k].blocks = grids[k];
+        L[k].lds_bytes = layer_kernel_.lds_bytes(fl[k]->fused_fn, kFnBlock);
+    }
+    uint32_t before = 0;
+    detail::fgx_check(fgx_layer_space_info(ctx_, &before, nullptr), "layer images");
+    const int rc = fgx_layer_launch(ctx_, layer_kernel_.kernel, kFnBlock, L.data(), (uint32_t)L.size(),
+                                    (uint32_t)sizeof(detail::FnArgs), stream_);
+    if (rc == FGX_ERR_CAPACITY) {
+        if (!layer_space_warned_) {
+            std::fprintf(stderr, "flamegpu: fused layer images fill their device space; further new layers launch per function\n");
+            layer_space_warned_ = true;
+        }
+        for (size_t k = 0; k < fl.size(); ++k) fl[k]->d->launch(args[k], grids[k], kFnBlock, 0, stream_);
+        if (hipGetLastError() != hipSuccess) throw exception::CUDAError("agent function launch failed");
+    } else {
+        detail::fgx_check(rc, "fused layer launch");
+        uint32_t after = 0;
+        detail::fgx_check(fgx_layer_space_info(ctx_, &after, nullptr), "layer images");
+        if (after != before) new_layer_images_ = true;   // this enqueue uploaded an image
+        ++fused_launches_;
+    }
+    for (auto* f : fl) unmap_function(*f, stream_);
+    return true;
 }
 
 
@@ -2036,6 +2118,7 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
         }
         return;
     }
+    if (launch_layer_fused(layer)) return;
     // fork: each function of the layer on its own stream, joined back into the main stream
     detail::fgx_check(fgx_event_record(fork_event_, stream_), "fork");
     size_t k = 0;
@@ -2134,11 +2217,23 @@ inline bool CUDASimulation::step() {
         if (changed) clear_graphs();
         // One single-step graph per double-buffer arrangement at step start; a step
         // swaps buffers, so a model alternates between a small number of arrangements.
+        // a graph whose capture uploaded layer images repeats the upload on every replay:
+        // drop it once it has run, so the arrangement is captured again without it
+        bool stale = false;
+        for (auto& g : graphs_) stale = stale || g.stale;
+        if (stale) {
+            synchronize();
+            for (size_t k = graphs_.size(); k-- > 0;)
+                if (graphs_[k].stale) {
+                    fgx_graph_destroy(graphs_[k].g);
+                    graphs_.erase(graphs_.begin() + (std::ptrdiff_t)k);
+                }
+        }
         const auto start = buffer_state();
         const auto swaps = swap_plan();
         GraphEntry* hit = nullptr;
         for (auto& g : graphs_)
-            if (g.start == start && g.swaps == swaps) hit = &g;
+            if (g.start == start && g.swaps == swaps && g.concurrent == cuda_config_.inLayerConcurrency) hit = &g;
         if (!hit && graphs_.size() >= kMaxStepGraphs) {
             size_t lru = 0;
             for (size_t k = 1; k < graphs_.size(); ++k)
@@ -2157,9 +2252,12 @@ inline bool CUDASimulation::step() {
             GraphEntry e;
             e.start = start;
             e.swaps = swaps;
+            e.concurrent = cuda_config_.inLayerConcurrency;
+            new_layer_images_ = false;
             detail::fgx_check(fgx_graph_begin(stream_), "graph begin");
             enqueue_step();
             detail::fgx_check(fgx_graph_end(stream_, &e.g), "graph end");
+            e.stale = new_layer_images_;
             e.end = buffer_state();
             set_buffer_state(start);   // capture recorded the work; nothing has run yet
             graphs_.push_back(std::move(e));

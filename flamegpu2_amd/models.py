@@ -41,6 +41,7 @@ def _lib(model):
         "fgxm_kernel_times": [vp, c.c_char_p, c.c_uint32, c.c_int],
         "fgxm_timed_steps": [vp, c.c_uint32, c.POINTER(c.c_double)],
         "fgxm_step_counter": [vp, u32p],
+        "fgxm_stat": [vp, c.c_char_p, c.POINTER(c.c_uint64)],
         "fgxm_simulate": [vp, c.c_uint32],
         "fgxm_log_config": [vp, c.c_int, c.c_char_p, c.c_char_p, c.c_char_p, c.c_char_p, c.c_uint32],
         "fgxm_export_log": [vp, c.c_char_p, c.c_int, c.c_int, c.c_int, c.c_int],
@@ -183,6 +184,11 @@ class Simulation:
         ms = ctypes.c_double()
         self._chk(self.L.fgxm_timed_steps(self.h, n, ctypes.byref(ms)))
         return ms.value
+
+    def stat(self, key):
+        n = ctypes.c_uint64()
+        self._chk(self.L.fgxm_stat(self.h, _b(key), ctypes.byref(n)))
+        return n.value
 
     def step_counter(self):
         n = ctypes.c_uint32()

@@ -101,6 +101,45 @@ int fgx_graph_end(void* stream, fgx_graph** out);
 int fgx_graph_launch(fgx_graph* g, void* stream);
 int fgx_graph_destroy(fgx_graph* g);
 
+/* ---------------------------------------------------------------- fused layer launch
+ * One launch for the independent agent functions of a layer (replaces the
+ * reference's one launch per function, each on its own stream:
+ * src/flamegpu/simulation/CUDASimulation.cu:931-1038, streams at :2036-2048).
+ *
+ * `kernel` is a layer kernel of the model library (the host stub of a __global__
+ * function taking one argument, `const void* image`): it finds its function from its
+ * block index in the image header below and runs it with that function's launch
+ * record.  Each fgx_fn_launch gives the function's index among the kernel's
+ * functions, its blocks (of `block` threads), its dynamic LDS per block, and its
+ * launch record in host memory (`args_bytes` bytes, the same size for every function).
+ *
+ * Images are content-addressed: the first call with a given image copies it (with
+ * hipMemcpyAsync on `stream`, from a pinned copy that is never rewritten) into device
+ * space that fgx_ctx_create allocates and the context keeps until it is destroyed; later calls with the same bytes reuse it, and an image is never changed
+ * once written, so the call may be captured into a graph and replayed.  When that space
+ * (FGX_LAYER_SPACE_BYTES) is full the call returns FGX_ERR_CAPACITY and launches
+ * nothing: the caller launches the functions one by one instead. */
+#define FGX_LAYER_MAX_FNS 8
+#define FGX_LAYER_MAX_ARGS_BYTES 8192u
+#define FGX_LAYER_ARGS_OFFSET 128u      /* launch record k at ARGS_OFFSET + k * stride */
+#define FGX_LAYER_SPACE_BYTES (4u << 20)   /* per context, device and pinned host */
+typedef struct {
+    uint32_t nfns;
+    uint32_t stride;                          /* bytes between launch records (16-aligned) */
+    uint32_t begin[FGX_LAYER_MAX_FNS + 1];    /* first block of function k; begin[nfns] = grid */
+    uint32_t fn[FGX_LAYER_MAX_FNS];           /* the functions' indices */
+} fgx_layer_header;
+typedef struct {
+    const void* args;      /* host: the function's launch record */
+    uint32_t fn;           /* index among the layer kernel's functions */
+    uint32_t blocks;       /* blocks of `block` threads (at least 1) */
+    uint32_t lds_bytes;    /* dynamic LDS per block the function needs */
+} fgx_fn_launch;
+int fgx_layer_launch(fgx_ctx* ctx, const void* kernel, uint32_t block, const fgx_fn_launch* fns, uint32_t nfns,
+                     uint32_t args_bytes, void* stream);
+/* Layer images held by the context: count and bytes used (tests, diagnostics). */
+int fgx_layer_space_info(fgx_ctx* ctx, uint32_t* images, size_t* bytes);
+
 /* ---------------------------------------------------------------- scratch sizing
  * CUB-style two-phase sizing (replaces CubTemporaryMemory,
  * src/flamegpu/simulation/detail/CubTemporaryMemory.cu:23-32 and the

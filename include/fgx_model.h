@@ -59,6 +59,10 @@ int fgxm_kernel_times(fgxm_sim* sim, char* buf, uint32_t buflen, int reset);
  * returns the elapsed milliseconds (inputs already resident on the device). */
 int fgxm_timed_steps(fgxm_sim* sim, uint32_t steps, double* ms);
 int fgxm_step_counter(fgxm_sim* sim, uint32_t* out);
+/* Runtime counters (no counterpart in the reference): "fused_layer_launches" (layers
+ * enqueued as one launch, graph replays not counted), "layer_images" (fused-layer images
+ * the simulation's context holds). */
+int fgxm_stat(fgxm_sim* sim, const char* key, uint64_t* out);
 /* CUDASimulation::simulate() for `steps` steps: init functions, step log frame 0,
  * the steps, exit functions, exit log (src/flamegpu/simulation/CUDASimulation.cu:1220-1344). */
 int fgxm_simulate(fgxm_sim* sim, uint32_t steps);

@@ -12,6 +12,7 @@
 //   transition          tests/test_cases/runtime/agent/detail/test_agent_state_transition.cu:67-174
 //   transition_src0_dest10 / transition_src10_dest0 / transition_src10_dest10
 //                       test_agent_state_transition.cu:67-250
+//   concurrency         tests/test_cases/simulation/test_cuda_simulation_concurrency.cu:322-390
 //   bruteforce          tests/test_cases/runtime/messaging/test_brute_force.cu:73-200
 //   random              tests/test_cases/runtime/random/test_agent_random.cu:28-185
 //   death_disabled / wrap_out_of_bounds   device errors (test_spatial_3d.cu:976-1039)
@@ -238,6 +239,15 @@ FLAMEGPU_AGENT_FUNCTION(AgentNull, MessageNone, MessageNone) {
     return ALIVE;
 }
 FLAMEGPU_AGENT_FUNCTION_CONDITION(Zero_X) { return FLAMEGPU->getVariable<unsigned int>("x") == 0; }
+
+// tests/test_cases/simulation/test_cuda_simulation_concurrency.cu:322-334: slow, uninteresting work
+FLAMEGPU_AGENT_FUNCTION(SlowAgentFunction, MessageNone, MessageNone) {
+    for (int i = 0; i < 65536; i++) {
+        const float v = FLAMEGPU->getVariable<float>("v");
+        FLAMEGPU->setVariable<float>("v", v + v);
+    }
+    return ALIVE;
+}
 
 FLAMEGPU_AGENT_FUNCTION(AddOne, MessageNone, MessageNone) {
     FLAMEGPU->setVariable<unsigned int>("x", FLAMEGPU->getVariable<unsigned int>("x") + 1);
@@ -521,6 +531,16 @@ flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* 
         f2.setFunctionCondition(Zero_X);
         model.newLayer().addAgentFunction(f1);
         model.newLayer().addAgentFunction(f2);
+    } else if (v == "concurrency") {
+        // test_cuda_simulation_concurrency.cu:353-390 (LayerConcurrency): one layer, one slow
+        // function for each of 4 agent types
+        auto layer = model.newLayer();
+        for (int i = 0; i < 4; ++i) {
+            const std::string name = "agent_" + std::to_string(i);
+            auto a = model.newAgent(name);
+            a.newVariable<float>("v");
+            layer.addAgentFunction(a.newFunction(name + "_slowAgentFunction", SlowAgentFunction));
+        }
     } else if (v == "bruteforce") {
         auto message = model.newMessage("msg");
         message.newVariable<int>("x");

@@ -286,6 +286,43 @@ def test_transition_src10_dest10_conditional(graphs):
         assert np.array_equal(np.bincount(ye, minlength=rounds + 1)[1:i + 1], [n] * i)
 
 
+# tests/test_cases/simulation/test_cuda_simulation_concurrency.cu:76-107,353-390
+# (LayerConcurrency): 4 agent types x 512 agents, one slow function each in one layer;
+# the step must run at least 1.5x faster with in-layer concurrency than without.  With
+# layer fusion the layer is one launch; without it, one launch per function on forked streams.
+
+
+def _mean_step_ms(s, pops, reps=3):
+    total = 0.0
+    for r in range(reps + 1):
+        for name, arr in pops:
+            s.set_agents(name, [arr])
+        ms = s.timed_steps(1)
+        if r:
+            total += ms
+    return total / reps
+
+
+@pytest.mark.parametrize("fusion", [1, 0])
+def test_layer_concurrency_speedup(fusion):
+    s = _sim("concurrency")
+    s.set_option("layer_fusion", fusion)
+    pops = [(f"agent_{i}", np.arange(512, dtype=np.float32)) for i in range(4)]
+    s.set_option("concurrency", 0)
+    seq = _mean_step_ms(s, pops)
+    s.set_option("concurrency", 1)
+    before = s.stat("fused_layer_launches")
+    con = _mean_step_ms(s, pops)
+    fused = s.stat("fused_layer_launches") - before
+    print(f"fusion {fusion}: sequential {seq:.3f} ms, concurrent {con:.3f} ms, speedup {seq / con:.2f}, "
+          f"fused launches {fused}")
+    assert (fused > 0) == bool(fusion)
+    assert seq / con >= 1.5
+    for i in range(4):
+        (v,), _ = s.get_agents(f"agent_{i}")
+        assert v[0] == 0 and np.all(np.isinf(v[1:]))
+
+
 def test_bruteforce_sum():
     n = 2000
     rng = np.random.default_rng(4)

@@ -83,12 +83,14 @@ def test_boids_bit_exact(n, steps, graphs, staging, tables):
     _compare(sim, ref, 0, "Boid", exact=True)
 
 
-@pytest.mark.parametrize("staging,tables", [(0, 1), (1, 1), (0, 0)])
-def test_predprey_small_bit_exact_traces(staging, tables):
+# fusion: the two functions of each layer as one launch (default) or one launch each
+@pytest.mark.parametrize("staging,tables,fusion", [(0, 1, 1), (1, 1, 1), (0, 0, 1), (0, 1, 0)])
+def test_predprey_small_bit_exact_traces(staging, tables, fusion):
     params, prey, pred = workloads.predprey(40000, 10000, seed=3, width=200.0)
     sim = Simulation("predator_prey", params, seed=3)
     sim.set_option("staging", staging)
     sim.set_option("unit_tables", tables)
+    sim.set_option("layer_fusion", fusion)
     ref = O.Sim(2, params, 3)
     sim.set_agents("Prey", prey)
     sim.set_agents("Predator", pred)
@@ -101,6 +103,8 @@ def test_predprey_small_bit_exact_traces(staging, tables):
         assert sim.count("Predator") == ref.count(1), f"step {step}: predator population"
     _compare(sim, ref, 0, "Prey", exact=True)
     _compare(sim, ref, 1, "Predator", exact=True)
+    # layers of two functions run as one launch with fusion on (counted when enqueued)
+    assert (sim.stat("fused_layer_launches") > 0) == bool(fusion)
 
 
 def test_predprey_full_config_three_steps():
