@@ -142,11 +142,13 @@ def bench_batched(args):
     ms_step_b = b2.timed_steps(4) / 4
     b2.close()
     b.close()
-    # plain: each simulation with its own launches, one after another
+    # plain: each simulation with its own launches, one after another, with the same bin
+    # width as the batched run (like for like)
     t0 = time.perf_counter()
     gpu_ms = 0.0
     for (p, a), (_, _, s) in zip(made, plans):
         sim = Simulation("circles_spatial3D", p, seed=s)
+        sim.set_option("bins_per_radius", bpr)
         sim.set_agents("Circle", a)
         gpu_ms += sim.timed_steps(steps)
         sim.get_agents("Circle")
@@ -156,7 +158,8 @@ def bench_batched(args):
     return {
         "workload": f"{len(plans)} circles simulations, N log-uniform [1024, 262144] (seeded), r ~ U[1, 3], "
                     f"density 1, {steps} steps each; batched run autotuned over block {{64..1024}} x agents per "
-                    f"thread {{1,2,4}} x bin width {{1.0, 0.5}} x radius; plain runs use bins as wide as the radius",
+                    f"thread {{1,2,4}} (+ persistent grids of 16/24/32 waves per CU) x bin width {{1.0, 0.5}} x "
+                    f"radius; plain runs use the batched run's bin width",
         "agents": agents,
         "batched_agent_steps_per_s": total / wall_b,
         "plain_agent_steps_per_s": total / wall_p,

@@ -6,9 +6,11 @@ shape.  So the tuner measures candidate shapes on the real run's first steps
 instead of on a copy, then keeps the fastest for the rest of the run.
 
 Search space: block size {64, 128, 256, 512, 1024} x agents per thread {1, 2, 4},
-plus persistent grids (blocks per CU) for the best one-block-per-tile shape.  Each
-candidate runs two untimed steps (repack, graph capture of both buffer
-arrangements) and `steps` timed steps.
+plus persistent grids for the best one-block-per-tile shape.  A persistent grid is
+sized in resident waves per CU (16, 24 or 32 of them: blocks per CU = waves x 64 /
+block size), so that it holds as many waves as the one-block-per-tile grid keeps
+resident.  Each candidate runs two untimed steps (repack, graph capture of both
+buffer arrangements) and `steps` timed steps.
 
 The bin-width multiplier (1 or 0.5: spatial bins as wide as the radius, or half
 as wide and searched two bins either side) is the third dimension.  It changes the
@@ -19,21 +21,26 @@ before the run starts, and the run is then built with the winner.
 
 BLOCKS = (64, 128, 256, 512, 1024)
 AGENTS_PER_THREAD = (1, 2, 4)
-PERSISTENT = (2, 4, 8)
+PERSISTENT_WAVES = (16, 24, 32)   # resident waves per CU of a persistent grid
 BINS_PER_RADIUS = (1, 2)   # bin-width multipliers 1.0 and 0.5
 
 
-def candidates(blocks=BLOCKS, apts=AGENTS_PER_THREAD, persistent=PERSISTENT):
+def persistent_blocks_per_cu(block, waves):
+    """Blocks per CU of a persistent grid holding `waves` wave64s per CU."""
+    return max(1, (waves * 64) // block)
+
+
+def candidates(blocks=BLOCKS, apts=AGENTS_PER_THREAD, persistent=PERSISTENT_WAVES):
     out = [(b, a, 0) for b in blocks for a in apts]
     return out, list(persistent)
 
 
-def steps_needed(steps=2, blocks=BLOCKS, apts=AGENTS_PER_THREAD, persistent=PERSISTENT):
+def steps_needed(steps=2, blocks=BLOCKS, apts=AGENTS_PER_THREAD, persistent=PERSISTENT_WAVES):
     """Simulation steps one tuning pass consumes."""
     return (len(blocks) * len(apts) + len(persistent)) * (steps + 2)
 
 
-def autotune(batch, steps=2, blocks=BLOCKS, apts=AGENTS_PER_THREAD, persistent=PERSISTENT):
+def autotune(batch, steps=2, blocks=BLOCKS, apts=AGENTS_PER_THREAD, persistent=PERSISTENT_WAVES):
     """Times every candidate on `batch` (a models.Batch) and leaves the fastest set.
     Returns (best (block, agents_per_thread, blocks_per_cu), {shape: ms per step})."""
     timings = {}
@@ -42,8 +49,10 @@ def autotune(batch, steps=2, blocks=BLOCKS, apts=AGENTS_PER_THREAD, persistent=P
         batch.set_tuning(*shape)
         timings[shape] = batch.timed_steps(steps) / steps
     best = min(timings, key=timings.get)
-    for bpc in pers:
-        shape = (best[0], best[1], bpc)
+    for waves in pers:
+        shape = (best[0], best[1], persistent_blocks_per_cu(best[0], waves))
+        if shape in timings:
+            continue
         batch.set_tuning(*shape)
         timings[shape] = batch.timed_steps(steps) / steps
     best = min(timings, key=timings.get)

@@ -58,9 +58,14 @@ class BatchSimulation {
         std::map<std::string, std::shared_ptr<AgentVector>> populations;   // by agent name (initial state)
     };
 
-    explicit BatchSimulation(const ModelDescription& model, int device = 0) : m_(model.data()), device_(device) {
-        validate();
+    // host_phases_outside: the caller runs the model's init and exit functions itself
+    // (CUDAEnsemble does, on a simulation of its own per run), so the model may have them.
+    explicit BatchSimulation(const ModelDescription& model, int device = 0, bool host_phases_outside = false)
+        : m_(model.data()), device_(device) {
+        check_model(*m_, host_phases_outside);
     }
+    // Throws InvalidOperation naming the first feature of the model a batch cannot run.
+    static void check_model(const ModelData& m, bool host_phases_outside);
     BatchSimulation(const BatchSimulation&) = delete;
     BatchSimulation& operator=(const BatchSimulation&) = delete;
     ~BatchSimulation() { release(); }
@@ -180,7 +185,6 @@ class BatchSimulation {
         if (it == sims_.end()) throw exception::InvalidArgument("BatchSimulation: unknown simulation handle");
         return it->second;
     }
-    void validate();
     void init();
     void release();
     void alloc(void** p, size_t bytes) { detail::fgx_check(fgx_malloc(ctx_, bytes ? bytes : 4, p), "device allocation"); }
@@ -246,13 +250,14 @@ class BatchSimulation {
 };
 
 // ------------------------------------------------------------------ implementation
-inline void BatchSimulation::validate() {
-    if (!m_->step_functions.empty() || !m_->exit_conditions.empty() || !m_->init_functions.empty() ||
-        !m_->exit_functions.empty())
-        throw exception::InvalidOperation("BatchSimulation: models with host init/step/exit functions cannot be batched");
+inline void BatchSimulation::check_model(const ModelData& m, bool host_phases_outside) {
+    if (!m.step_functions.empty() || !m.exit_conditions.empty())
+        throw exception::InvalidOperation("BatchSimulation: models with host step functions or exit conditions cannot be batched");
+    if (!host_phases_outside && (!m.init_functions.empty() || !m.exit_functions.empty()))
+        throw exception::InvalidOperation("BatchSimulation: models with host init/exit functions cannot be batched");
     std::map<std::string, int> writers;
     std::set<std::string> written;
-    for (auto& layer : m_->layers) {
+    for (auto& layer : m.layers) {
         if (!layer->host_functions.empty())
             throw exception::InvalidOperation("BatchSimulation: layers with host functions cannot be batched");
         for (auto& f : layer->agent_functions) {
@@ -275,14 +280,14 @@ inline void BatchSimulation::validate() {
             }
         }
     }
-    for (auto& mm : m_->messages)
+    for (auto& mm : m.messages)
         if (mm.second->kind == MsgKind::Bucket || is_array_kind(mm.second->kind))
             throw exception::InvalidOperation("BatchSimulation: bucket and array message lists ('" + mm.first +
                                               "') cannot be batched");
-    for (auto& mm : m_->messages)
+    for (auto& mm : m.messages)
         if (mm.second->persistent)
             throw exception::InvalidOperation("BatchSimulation: persistent message lists cannot be batched");
-    for (auto& a : m_->agents)
+    for (auto& a : m.agents)
         if (a.second->sortPeriod > 1)
             throw exception::InvalidOperation("BatchSimulation: agent '" + a.first +
                                               "' sorts every few steps; batched simulations sort every step or never");
@@ -1083,5 +1088,317 @@ inline void BatchSimulation::getPopulationData(int h, AgentVector& pop) {
                               "population download");
     synchronize();
 }
+
+
+// include/flamegpu/simulation/CUDAEnsemble.h:40-129.  simulate() runs the plans in one
+// BatchSimulation when the model can be batched (the default; Config().batched = false
+// opts out): every run's init functions run first on a simulation of its own, the
+// batch steps all runs together, and each run's exit functions and exit log run on a
+// simulation holding its final populations.  Other models, and ensembles with a step
+// log, run each plan as its own CUDASimulation on `concurrent_runs` host threads
+// sharing the device (src/flamegpu/simulation/detail/SimRunner.cu:51-90).
+class CUDAEnsemble {
+ public:
+    struct EnsembleConfig {
+        std::string out_directory;
+        std::string out_format = "json";
+        unsigned int concurrent_runs = 4;
+        std::set<int> devices;
+        bool quiet = false;
+        bool verbose = false;
+        bool timing = false;
+        enum ErrorLevel { Off = 0, Slow = 1, Fast = 2 };
+        ErrorLevel error_level = Slow;
+        bool truncate_log_files = false;
+        bool batched = true;          // run batchable models as one batch (no counterpart in the reference)
+        size_t batch_bytes = 0;       // HBM the batch may hold at once (0: three quarters of the device's)
+    };
+    explicit CUDAEnsemble(const ModelDescription& model, int argc = 0, const char** argv = nullptr) : model_(model) {
+        for (int i = 1; i < argc; ++i) {
+            const std::string a = argv[i];
+            if ((a == "-c" || a == "--concurrent") && i + 1 < argc) config_.concurrent_runs = (unsigned)std::stoul(argv[++i]);
+            else if (a == "-q" || a == "--quiet") config_.quiet = true;
+            else if (a == "-t" || a == "--timing") config_.timing = true;
+        }
+    }
+    EnsembleConfig& Config() { return config_; }
+    const EnsembleConfig& getConfig() const { return config_; }
+    // Population initialiser called for every run before it starts.
+    void setPopulationInit(std::function<void(CUDASimulation&, const RunPlan&, unsigned int)> f) { init_ = std::move(f); }
+    // src/flamegpu/simulation/CUDAEnsemble.cu: setStepLog / setExitLog
+    void setStepLog(const StepLoggingConfig& cfg) {
+        if (cfg.model().get() != model_.data().get())
+            throw exception::InvalidArgument("Model descriptions attached to LoggingConfig and CUDAEnsemble do not match, in CUDAEnsemble::setStepLog()");
+        step_log_ = std::make_shared<StepLoggingConfig>(cfg);
+    }
+    void setExitLog(const LoggingConfig& cfg) {
+        if (cfg.model().get() != model_.data().get())
+            throw exception::InvalidArgument("Model descriptions attached to LoggingConfig and CUDAEnsemble do not match, in CUDAEnsemble::setExitLog()");
+        exit_log_ = std::make_shared<LoggingConfig>(cfg);
+    }
+    unsigned int simulate(const RunPlanVector& plans) {
+        for (auto& p : plans)
+            if (p.model() != model_.data())
+                throw exception::InvalidArgument("CUDAEnsemble::simulate: the run plans are for a different model");
+        logs_.clear();
+        logs_.resize(plans.size());
+        run_logs_.clear();
+        const auto t0 = std::chrono::steady_clock::now();
+        unsigned int errors = 0;
+        batched_ = config_.batched && batchable(plans);
+        if (batched_) {
+            try {
+                errors = simulate_batched(plans);
+            } catch (const NotBatchable&) {
+                // an init function left agents outside their initial state: run every plan alone
+                logs_.assign(plans.size(), RunRecord());
+                run_logs_.clear();
+                batched_ = false;
+            }
+        }
+        if (!batched_) errors = simulate_pool(plans);
+        elapsed_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (errors && config_.error_level != EnsembleConfig::Off) {
+            throw exception::EnsembleError(detail::format("%u/%zu ensemble runs failed: %s", errors, plans.size(),
+                                                          first_error().c_str()));
+        }
+        return errors;
+    }
+    // Whether the last simulate() ran its plans as one batch.
+    bool lastRunBatched() const { return batched_; }
+    // include/flamegpu/simulation/CUDAEnsemble.h: getLogs(), keyed by plan index
+    const std::map<unsigned int, RunLog>& getLogs() const { return run_logs_; }
+    const std::vector<RunRecord>& getRunRecords() const { return logs_; }
+    double getEnsembleElapsedTime() const { return elapsed_; }
+
+ private:
+
+    struct NotBatchable {};
+    bool batchable(const RunPlanVector& plans) const {
+        if (step_log_) return false;
+        for (auto& p : plans)
+            if (p.getSteps() == 0) return false;   // runs until an exit condition: not a batch
+        try {
+            BatchSimulation::check_model(*model_.data(), true);
+        } catch (const exception::InvalidOperation&) {
+            return false;
+        }
+        return true;
+    }
+    int device() const { return config_.devices.empty() ? 0 : *config_.devices.begin(); }
+    static void plan_env(CUDASimulation& sim, const RunPlan& p) {
+        for (auto& kv : p.properties()) sim_set_env(sim, kv.first, kv.second);
+    }
+    // A run's batch plan: its populations and environment once its init functions ran.
+    BatchSimulation::Plan prologue(const RunPlan& p, unsigned int i) {
+        BatchSimulation::Plan bp;
+        bp.seed = p.getRandomSimulationSeed();
+        bp.steps = p.getSteps();
+        const auto& m = *model_.data();
+        if (!init_ && m.init_functions.empty()) {
+            bp.env = m.env.buffer;
+            for (auto& kv : p.properties()) {
+                const auto& prop = m.env.props.at(kv.first);
+                std::memcpy(bp.env.data() + prop.offset, kv.second.data(), kv.second.size());
+            }
+            return bp;
+        }
+        CUDASimulation sim(model_);
+        sim.CUDAConfig().is_ensemble = true;
+        sim.CUDAConfig().device_id = device();
+        sim.SimulationConfig().random_seed = bp.seed;
+        sim.SimulationConfig().steps = bp.steps;
+        plan_env(sim, p);
+        if (init_) init_(sim, p, i);
+        sim.runInitPhase();
+        bp.env = sim.env_host();
+        for (auto& a : m.agents) {
+            for (auto& st : a.second->states)
+                if (st != a.second->initial_state && sim_count(sim, a.first, st) != 0) throw NotBatchable();
+            auto pop = std::make_shared<AgentVector>(AgentDescription(a.second));
+            sim.getPopulationData(*pop, a.second->initial_state);
+            bp.populations[a.first] = pop;
+        }
+        return bp;
+    }
+    // A finished run: final populations, then exit functions and the exit log.
+    void epilogue(BatchSimulation& batch, int h, const RunPlan& p, const std::vector<char>& env, RunRecord& lg,
+                  unsigned int i) {
+        const auto& m = *model_.data();
+        std::map<std::string, std::shared_ptr<AgentVector>> pops;
+        for (auto& a : m.agents) {
+            auto pop = std::make_shared<AgentVector>(AgentDescription(a.second));
+            batch.getPopulationData(h, *pop);
+            lg.final_counts[a.first] = (unsigned int)pop->size();
+            pops[a.first] = pop;
+        }
+        RunLog rl;
+        if (!m.exit_functions.empty() || exit_log_) {
+            CUDASimulation sim(model_);
+            sim.CUDAConfig().is_ensemble = true;
+            sim.CUDAConfig().device_id = device();
+            sim.SimulationConfig().random_seed = p.getRandomSimulationSeed();
+            sim.SimulationConfig().steps = p.getSteps();
+            sim.env_host() = env;
+            sim.mark_env_dirty();
+            if (exit_log_) sim.setExitLog(*exit_log_);
+            for (auto& a : m.agents)
+                if (pops[a.first]->size()) sim.setPopulationData(*pops[a.first], a.second->initial_state);
+            sim.runExitPhase(p.getSteps());
+            rl = sim.getRunLog();
+        }
+        run_logs_[i] = rl;
+    }
+    // Runs join the batch while its HBM estimate stays within the budget; finished runs
+    // leave after every step, and queued runs take their place.
+    unsigned int simulate_batched(const RunPlanVector& plans) {
+        BatchSimulation batch(model_, device(), true);
+        size_t budget = config_.batch_bytes;
+        if (!budget) {
+            size_t free_b = 0, total_b = 0;
+            budget = hipMemGetInfo(&free_b, &total_b) == hipSuccess ? free_b / 4 * 3 : (size_t)64 << 30;
+        }
+        struct Running {
+            size_t plan;
+            size_t bytes;
+            std::vector<char> env;
+            std::chrono::steady_clock::time_point start;
+        };
+        std::map<int, Running> running;
+        size_t used = 0, next = 0;
+        unsigned int errors = 0;
+        bool stop = false;
+        while (!stop && (next < plans.size() || !running.empty())) {
+            while (!stop && next < plans.size()) {
+                const size_t i = next;
+                RunRecord& lg = logs_[i];
+                lg.started = true;
+                lg.plan_index = (unsigned int)i;
+                lg.seed = plans[i].getRandomSimulationSeed();
+                lg.steps = plans[i].getSteps();
+                const auto r0 = std::chrono::steady_clock::now();
+                BatchSimulation::Plan bp;
+                try {
+                    bp = prologue(plans[i], (unsigned int)i);
+                } catch (const NotBatchable&) {
+                    throw;
+                } catch (const std::exception& e) {
+                    lg.error = e.what();
+                    ++errors;
+                    lg.wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - r0).count();
+                    if (config_.error_level == EnsembleConfig::Fast) stop = true;
+                    ++next;
+                    continue;
+                }
+                const size_t est = batch.estimateBytes(bp);
+                if (!running.empty() && used + est > budget) break;   // waits for runs to finish
+                const int h = batch.add(bp);
+                running[h] = Running{i, est, bp.env, r0};
+                used += est;
+                ++next;
+            }
+            if (stop || running.empty()) continue;
+            try {
+                batch.step();
+            } catch (const std::exception& e) {
+                // a launch or host error of the batch: every resident run fails with it
+                for (auto& kv : running) {
+                    RunRecord& lg = logs_[kv.second.plan];
+                    lg.error = e.what();
+                    ++errors;
+                }
+                running.clear();
+                if (config_.error_level == EnsembleConfig::Fast) stop = true;
+                batch.synchronize();
+                continue;
+            }
+            for (int h : batch.finished()) {
+                Running& r = running.at(h);
+                RunRecord& lg = logs_[r.plan];
+                try {
+                    epilogue(batch, h, plans[r.plan], r.env, lg, (unsigned int)r.plan);
+                } catch (const std::exception& e) {
+                    lg.error = e.what();
+                    ++errors;
+                    if (config_.error_level == EnsembleConfig::Fast) stop = true;
+                }
+                lg.wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - r.start).count();
+                used -= r.bytes;
+                batch.remove(h);
+                running.erase(h);
+            }
+        }
+        return errors;
+    }
+    unsigned int simulate_pool(const RunPlanVector& plans) {
+        std::mutex log_mutex;
+        std::atomic<unsigned> next{0}, errors{0};
+        std::atomic<bool> stop{false};
+        auto worker = [&]() {
+            while (!stop) {
+                const unsigned i = next++;
+                if (i >= plans.size()) break;
+                RunRecord& lg = logs_[i];
+                lg.started = true;
+                lg.plan_index = i;
+                lg.seed = plans[i].getRandomSimulationSeed();
+                lg.steps = plans[i].getSteps();
+                const auto r0 = std::chrono::steady_clock::now();
+                try {
+                    CUDASimulation sim(model_);
+                    sim.CUDAConfig().is_ensemble = true;
+                    sim.CUDAConfig().device_id = device();
+                    sim.SimulationConfig().random_seed = plans[i].getRandomSimulationSeed();
+                    sim.SimulationConfig().steps = plans[i].getSteps();
+                    for (auto& kv : plans[i].properties()) sim_set_env(sim, kv.first, kv.second);
+                    if (init_) init_(sim, plans[i], i);
+                    if (step_log_) sim.setStepLog(*step_log_);
+                    if (exit_log_) sim.setExitLog(*exit_log_);
+                    sim.simulate();
+                    for (auto& a : model_.data()->agents) lg.final_counts[a.first] = sim_count(sim, a.first);
+                    const RunLog& rl = sim.getRunLog();
+                    std::lock_guard<std::mutex> lock(log_mutex);
+                    run_logs_[i] = rl;
+                } catch (const std::exception& e) {
+                    lg.error = e.what();
+                    ++errors;
+                    if (config_.error_level == EnsembleConfig::Fast) stop = true;
+                }
+                lg.wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - r0).count();
+            }
+        };
+        std::vector<std::thread> th;
+        const unsigned nt = std::max(1u, std::min<unsigned>(config_.concurrent_runs, (unsigned)plans.size()));
+        for (unsigned t = 0; t < nt; ++t) th.emplace_back(worker);
+        for (auto& t : th) t.join();
+        return errors;
+    }
+    std::string first_error() const {
+        for (auto& l : logs_)
+            if (!l.error.empty()) return l.error;
+        return "";
+    }
+    static void sim_set_env(CUDASimulation& s, const std::string& name, const std::vector<char>& b) {
+        auto& props = s.model()->env.props;
+        auto it = props.find(name);
+        if (it == props.end()) throw exception::InvalidEnvProperty("unknown property " + name);
+        std::memcpy(s.env_host().data() + it->second.offset, b.data(), b.size());
+        s.mark_env_dirty();
+    }
+    static unsigned int sim_count(CUDASimulation& s, const std::string& agent, const std::string& state = "") {
+        HostAPI api(s, 0);
+        return api.agent(agent, state).count();
+    }
+    ModelDescription model_;
+    EnsembleConfig config_;
+    std::vector<RunRecord> logs_;
+    std::map<unsigned int, RunLog> run_logs_;
+    std::shared_ptr<StepLoggingConfig> step_log_;
+    std::shared_ptr<LoggingConfig> exit_log_;
+    std::function<void(CUDASimulation&, const RunPlan&, unsigned int)> init_;
+    double elapsed_ = 0.0;
+    bool batched_ = false;
+};
+
 
 }  // namespace flamegpu

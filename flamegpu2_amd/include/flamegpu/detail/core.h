@@ -78,6 +78,7 @@ FGX_DERIVED_EXCEPTION(AgentIDCollision)
 FGX_DERIVED_EXCEPTION(ArrayMessageWriteConflict)
 FGX_DERIVED_EXCEPTION(InvalidMessage)
 FGX_DERIVED_EXCEPTION(InvalidInputFile)
+FGX_DERIVED_EXCEPTION(UnknownInternalError)
 FGX_DERIVED_EXCEPTION(InvalidHostFunc)
 FGX_DERIVED_EXCEPTION(InvalidPopulationData)
 FGX_DERIVED_EXCEPTION(InvalidConfig)

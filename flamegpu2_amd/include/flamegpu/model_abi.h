@@ -3,11 +3,13 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <sstream>
 #include <string>
+#include <typeindex>
 
 #include "fgx_model.h"
 #include "flamegpu/flamegpu.h"
@@ -239,6 +241,96 @@ FGXM_EXPORT int fgxm_timed_steps(fgxm_sim* s, uint32_t steps, double* ms) {
 
 FGXM_EXPORT int fgxm_step_counter(fgxm_sim* s, uint32_t* out) {
     return fgxm_detail::guard([&] { *out = s->sim->getStepCounter(); });
+}
+
+FGXM_EXPORT int fgxm_ensemble_simulate(const char* variant, const float* params, uint32_t nparams, uint32_t nplans,
+                                       const uint64_t* seeds, const uint32_t* steps, const char* options,
+                                       const char* sum_agent, const char* sum_var, int32_t* out_status,
+                                       uint32_t* out_counts, double* out_sums, int32_t* out_batched) {
+    return fgxm_detail::guard([&] {
+        flamegpu::ModelDescription model = fgxm_build_model(variant, params, nparams);
+        flamegpu::RunPlanVector plans(model, nplans);
+        for (uint32_t i = 0; i < nplans; ++i) {
+            plans[i].setRandomSimulationSeed(seeds[i]);
+            plans[i].setSteps(steps[i]);
+        }
+        flamegpu::CUDAEnsemble ens(model);
+        std::istringstream opts(options ? options : "");
+        std::string kv;
+        while (std::getline(opts, kv, ';')) {
+            const auto eq = kv.find('=');
+            if (kv.empty() || eq == std::string::npos) continue;
+            const std::string k = kv.substr(0, eq);
+            const long v = std::stol(kv.substr(eq + 1));
+            if (k == "concurrent") ens.Config().concurrent_runs = (unsigned)v;
+            else if (k == "error_level") ens.Config().error_level = (flamegpu::CUDAEnsemble::EnsembleConfig::ErrorLevel)v;
+            else if (k == "batched") ens.Config().batched = v != 0;
+            else throw flamegpu::exception::InvalidArgument("unknown ensemble option " + k);
+        }
+        // sum_agent empty: sum_var names an environment property, logged at exit
+        const bool sums = sum_agent && *sum_agent && sum_var && *sum_var;
+        const bool env_value = !(sum_agent && *sum_agent) && sum_var && *sum_var;
+        std::type_index sum_type = typeid(void);
+        if (env_value) {
+            auto pit = model.data()->env.props.find(sum_var);
+            if (pit == model.data()->env.props.end())
+                throw flamegpu::exception::InvalidEnvProperty(std::string("unknown environment property ") + sum_var);
+            sum_type = pit->second.type;
+            if (sum_type != typeid(uint32_t) && sum_type != typeid(int32_t) && sum_type != typeid(float))
+                throw flamegpu::exception::UnsupportedVarType("logged environment values of 4-byte types only");
+            flamegpu::LoggingConfig cfg(model);
+            cfg.logEnvironment(sum_var);
+            ens.setExitLog(cfg);
+        }
+        if (sums) {
+            flamegpu::LoggingConfig cfg(model);
+            auto ait = model.data()->agents.find(sum_agent);
+            if (ait == model.data()->agents.end()) throw flamegpu::exception::InvalidAgentName(std::string("unknown agent ") + sum_agent);
+            auto vit = ait->second->variables.find(sum_var);
+            if (vit == ait->second->variables.end()) throw flamegpu::exception::InvalidAgentVar(std::string("unknown variable ") + sum_var);
+            sum_type = vit->second.type;
+            if (sum_type == typeid(uint32_t)) cfg.agent(sum_agent).logSum<uint32_t>(sum_var);
+            else if (sum_type == typeid(int32_t)) cfg.agent(sum_agent).logSum<int32_t>(sum_var);
+            else if (sum_type == typeid(float)) cfg.agent(sum_agent).logSum<float>(sum_var);
+            else throw flamegpu::exception::UnsupportedVarType("exit sums of 4-byte variables only");
+            ens.setExitLog(cfg);
+        }
+        std::string err;
+        try {
+            ens.simulate(plans);
+        } catch (const flamegpu::exception::EnsembleError& e) {
+            err = e.what();
+        }
+        const auto& recs = ens.getRunRecords();
+        const auto& logs = ens.getLogs();
+        for (uint32_t i = 0; i < nplans; ++i) {
+            const auto& r = recs[i];
+            out_status[i] = !r.started ? 2 : (r.error.empty() ? 0 : 1);
+            uint32_t k = 0;
+            for (auto& a : model.data()->agents) {
+                auto it = r.final_counts.find(a.first);
+                if (out_counts) out_counts[(size_t)i * model.data()->agents.size() + k] = it == r.final_counts.end() ? 0u : it->second;
+                ++k;
+            }
+            if (out_sums) {
+                out_sums[i] = std::nan("");
+                auto lt = logs.find(i);
+                if (sums && out_status[i] == 0 && lt != logs.end()) {
+                    const auto f = lt->second.getExitLog().getAgent(sum_agent);
+                    if (sum_type == typeid(uint32_t)) out_sums[i] = (double)f.getSum<uint32_t>(sum_var);
+                    else if (sum_type == typeid(int32_t)) out_sums[i] = (double)f.getSum<int32_t>(sum_var);
+                    else out_sums[i] = (double)f.getSum<float>(sum_var);
+                } else if (env_value && out_status[i] == 0 && lt != logs.end()) {
+                    const auto& f = lt->second.getExitLog();
+                    if (sum_type == typeid(uint32_t)) out_sums[i] = (double)f.getEnvironmentProperty<uint32_t>(sum_var);
+                    else if (sum_type == typeid(int32_t)) out_sums[i] = (double)f.getEnvironmentProperty<int32_t>(sum_var);
+                    else out_sums[i] = (double)f.getEnvironmentProperty<float>(sum_var);
+                }
+            }
+        }
+        if (out_batched) *out_batched = ens.lastRunBatched() ? 1 : 0;
+        if (!err.empty()) throw flamegpu::exception::EnsembleError(err);
+    });
 }
 
 FGXM_EXPORT int fgxm_stat(fgxm_sim* s, const char* key, uint64_t* out) {

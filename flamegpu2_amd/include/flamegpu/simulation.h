@@ -275,6 +275,10 @@ class HostAgentAPI {
     std::pair<double, double> meanStandardDeviation(const std::string& variable);
     template <typename T>
     std::vector<T> getVariableValues(const std::string& variable);
+    // include/flamegpu/runtime/agent/HostAgentAPI.cuh: newAgent (HostNewAgentAPI).  The agent
+    // starts from the variable defaults with the next free ID, and joins its state list when
+    // the host function returns.
+    AgentVector::Agent newAgent();
 
  private:
     template <typename T>
@@ -306,7 +310,9 @@ class HostEnvironment {
 
 class HostRandom {
  public:
-    explicit HostRandom(uint64_t seed) : g_(seed) {}
+    explicit HostRandom(uint64_t seed) : g_(seed), seed_(seed) {}
+    // include/flamegpu/runtime/random/HostRandom.cuh: getSeed
+    uint64_t getSeed() const { return seed_; }
     template <typename T>
     T uniform() {
         return std::uniform_real_distribution<T>(0, 1)(g_);
@@ -322,10 +328,14 @@ class HostRandom {
     T normal() {
         return std::normal_distribution<T>(0, 1)(g_);
     }
-    void setSeed(uint64_t s) { g_.seed(s); }
+    void setSeed(uint64_t s) {
+        g_.seed(s);
+        seed_ = s;
+    }
 
  private:
     std::mt19937_64 g_;
+    uint64_t seed_ = 0;
 };
 
 class HostAPI {
@@ -392,6 +402,7 @@ class RunPlan {
     }
     uint64_t getRandomSimulationSeed() const { return seed_; }
     unsigned int getSteps() const { return steps_; }
+    const std::shared_ptr<ModelData>& model() const { return m_; }
     std::string getOutputSubdirectory() const { return subdir_; }
     const std::map<std::string, std::vector<char>>& properties() const { return props_; }
 
@@ -533,6 +544,25 @@ class CUDASimulation {
         synchronize();
         check_device_error();
         elapsed_sim_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        process_exit_log();
+    }
+    // The two host phases of simulate() around steps run elsewhere (an ensemble's batch):
+    // init functions, then, once the run's final populations are set, exit functions and
+    // the exit log as after `steps` steps.
+    void runInitPhase() {
+        ensure_init();
+        reset_log();
+        run_init_functions();
+        synchronize();
+        check_device_error();
+    }
+    void runExitPhase(unsigned int steps) {
+        ensure_init();
+        step_ = steps;
+        detail::fgx_check(fgx_fill_u32(d_step_, steps, 1, stream_), "step counter");
+        run_exit_functions();
+        synchronize();
+        check_device_error();
         process_exit_log();
     }
     void simulate(const RunPlan& plan) {
@@ -699,6 +729,21 @@ class CUDASimulation {
     void* reduce_out() { return d_reduce_out_; }
     void check_device_error();
     void mark_env_dirty() { env_dirty_ = true; }
+    // agents created by host functions (HostAgentAPI::newAgent), appended when the function returns
+    AgentVector& host_new_agents(detail::AgentRT& a, detail::StateList& l) {
+        auto& p = host_new_[&l];
+        if (!p.v) {
+            p.a = &a;
+            p.v.reset(new AgentVector(AgentDescription(a.d)));
+        }
+        return *p.v;
+    }
+    bool host_agents_pending(const detail::AgentRT* a = nullptr) const {
+        for (auto& kv : host_new_)
+            if (kv.second.v && !kv.second.v->empty() && (!a || kv.second.a == a)) return true;
+        return false;
+    }
+    void flush_host_agents();
     std::vector<char>& env_host() { return env_host_; }
     const std::shared_ptr<ModelData>& model() const { return m_; }
     void sync_counts();
@@ -861,6 +906,11 @@ class CUDASimulation {
     bool graph_evict_warned_ = false;
     bool graph_eligible_ = false;
     bool populations_set_ = false;   // setPopulationData since the last step: re-check capacities
+    struct HostNew {
+        detail::AgentRT* a = nullptr;
+        std::unique_ptr<AgentVector> v;
+    };
+    std::map<detail::StateList*, HostNew> host_new_;   // pending host-created agents, by state list
     void clear_graphs() {
         for (auto& g : graphs_) fgx_graph_destroy(g.g);
         graphs_.clear();
@@ -1233,7 +1283,9 @@ inline void CUDASimulation::sync_counts() {
     detail::fgx_check(fgx_memcpy_d2h(h_small_, d_small_, (size_t)small_used_ * 4, stream_), "count readback");
     synchronize();
     for (auto& kv : agents_) {
-        kv.second.h_next_id = h_small_[kv.second.d_next_id - d_small_];
+        // IDs given to host-created agents not yet uploaded stay taken
+        const uint32_t dev_next = h_small_[kv.second.d_next_id - d_small_];
+        kv.second.h_next_id = host_agents_pending(&kv.second) ? std::max(dev_next, kv.second.h_next_id) : dev_next;
         for (auto& s : kv.second.states) s.second.h_count = h_small_[s.second.d_count - d_small_];
     }
     (void)k;
@@ -1998,7 +2050,10 @@ inline void CUDASimulation::unmap_function(detail::FnRT& f, void* st) {
 inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
     if (!layer.host_functions.empty()) {
         synchronize();
-        for (auto fn : layer.host_functions) fn(host_api_.get());
+        for (auto fn : layer.host_functions) {
+            fn(host_api_.get());
+            flush_host_agents();
+        }
         upload_env();
         return;
     }
@@ -2273,6 +2328,7 @@ inline bool CUDASimulation::step() {
     for (auto fn : m_->step_functions) {
         synchronize();
         fn(host_api_.get());
+        flush_host_agents();
         upload_env();
     }
     bool cont = true;
@@ -2294,9 +2350,39 @@ inline void CUDASimulation::run_init_functions() {
     for (auto fn : m_->init_functions) {
         synchronize();
         fn(host_api_.get());
+        flush_host_agents();
     }
     upload_env();
     elapsed_init_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Host-created agents join their lists after the survivors, in creation order, with the
+// IDs they were given at creation (HostAgentAPI::newAgent).
+inline void CUDASimulation::flush_host_agents() {
+    if (!host_agents_pending()) return;
+    sync_counts();
+    for (auto& kv : host_new_) {
+        if (!kv.second.v || kv.second.v->empty()) continue;
+        detail::StateList& l = *kv.first;
+        AgentVector& v = *kv.second.v;
+        const uint32_t k = (uint32_t)v.size(), n = l.h_count;
+        if ((uint64_t)n + k > 0x7FFFFFFFull) throw exception::InvalidOperation("population exceeds the 32-bit index range");
+        if (n + k > l.cap) grow_list(l, std::max<uint32_t>(n + k, l.cap + l.cap / 2));
+        for (auto& var : l.vars)
+            detail::fgx_check(fgx_memcpy_h2d((char*)var.cur + (size_t)n * var.bytes, v.raw(var.name).data(),
+                                             (size_t)k * var.bytes, stream_),
+                              "host agent upload");
+        l.h_count = n + k;
+        detail::fgx_check(fgx_memcpy_h2d(l.d_count, &l.h_count, 4, stream_), "count upload");
+        synchronize();
+        v.clear();
+    }
+    for (auto& kv : agents_)
+        detail::fgx_check(fgx_memcpy_h2d(kv.second.d_next_id, &kv.second.h_next_id, 4, stream_), "next id upload");
+    synchronize();
+    clear_graphs();
+    populations_set_ = true;
+    ensure_capacity();   // a host layer function may add agents in the middle of a step
 }
 
 inline void CUDASimulation::run_exit_functions() {
@@ -2304,6 +2390,7 @@ inline void CUDASimulation::run_exit_functions() {
     for (auto fn : m_->exit_functions) {
         synchronize();
         fn(host_api_.get());
+        flush_host_agents();
     }
     elapsed_exit_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -2488,6 +2575,15 @@ inline unsigned int HostAgentAPI::count() {
     return l_.h_count;
 }
 
+inline AgentVector::Agent HostAgentAPI::newAgent() {
+    if (!s_.host_agents_pending()) s_.sync_counts();   // the device's next ID, advanced by births since the last read
+    AgentVector& v = s_.host_new_agents(a_, l_);
+    v.push_back();
+    AgentVector::Agent ag = v.back();
+    ag.setVariable<id_t>("_id", a_.h_next_id++);
+    return ag;
+}
+
 template <typename T>
 inline const detail::VarBuf& HostAgentAPI::var(const std::string& name) {
     auto vit = a_.d->variables.find(name);
@@ -2629,133 +2725,13 @@ inline std::array<T, N> HostEnvironment::setProperty(const std::string& name, co
 // ------------------------------------------------------------------ ensembles
 // Per-run bookkeeping of an ensemble (plan, seed, wall time, final counts, error).
 struct RunRecord {
+    bool started = false;         // the run began (Fast error handling stops the others)
     unsigned int plan_index = 0;
     uint64_t seed = 0;
     unsigned int steps = 0;
     double wall_seconds = 0.0;
     std::map<std::string, unsigned int> final_counts;
     std::string error;
-};
-
-// include/flamegpu/simulation/CUDAEnsemble.h:40-129: independent simulations on
-// `concurrent_runs` host threads sharing the device (the plain per-simulation path).
-class CUDAEnsemble {
- public:
-    struct EnsembleConfig {
-        std::string out_directory;
-        std::string out_format = "json";
-        unsigned int concurrent_runs = 4;
-        std::set<int> devices;
-        bool quiet = false;
-        bool verbose = false;
-        bool timing = false;
-        enum ErrorLevel { Off = 0, Slow = 1, Fast = 2 };
-        ErrorLevel error_level = Slow;
-        bool truncate_log_files = false;
-    };
-    explicit CUDAEnsemble(const ModelDescription& model, int argc = 0, const char** argv = nullptr) : model_(model) {
-        for (int i = 1; i < argc; ++i) {
-            const std::string a = argv[i];
-            if ((a == "-c" || a == "--concurrent") && i + 1 < argc) config_.concurrent_runs = (unsigned)std::stoul(argv[++i]);
-            else if (a == "-q" || a == "--quiet") config_.quiet = true;
-            else if (a == "-t" || a == "--timing") config_.timing = true;
-        }
-    }
-    EnsembleConfig& Config() { return config_; }
-    const EnsembleConfig& getConfig() const { return config_; }
-    // Population initialiser called for every run before it starts.
-    void setPopulationInit(std::function<void(CUDASimulation&, const RunPlan&, unsigned int)> f) { init_ = std::move(f); }
-    // src/flamegpu/simulation/CUDAEnsemble.cu: setStepLog / setExitLog
-    void setStepLog(const StepLoggingConfig& cfg) {
-        if (cfg.model().get() != model_.data().get())
-            throw exception::InvalidArgument("Model descriptions attached to LoggingConfig and CUDAEnsemble do not match, in CUDAEnsemble::setStepLog()");
-        step_log_ = std::make_shared<StepLoggingConfig>(cfg);
-    }
-    void setExitLog(const LoggingConfig& cfg) {
-        if (cfg.model().get() != model_.data().get())
-            throw exception::InvalidArgument("Model descriptions attached to LoggingConfig and CUDAEnsemble do not match, in CUDAEnsemble::setExitLog()");
-        exit_log_ = std::make_shared<LoggingConfig>(cfg);
-    }
-    unsigned int simulate(const RunPlanVector& plans) {
-        logs_.clear();
-        logs_.resize(plans.size());
-        run_logs_.clear();
-        std::mutex log_mutex;
-        std::atomic<unsigned> next{0}, errors{0};
-        std::atomic<bool> stop{false};
-        const auto t0 = std::chrono::steady_clock::now();
-        auto worker = [&]() {
-            while (!stop) {
-                const unsigned i = next++;
-                if (i >= plans.size()) break;
-                RunRecord& lg = logs_[i];
-                lg.plan_index = i;
-                lg.seed = plans[i].getRandomSimulationSeed();
-                lg.steps = plans[i].getSteps();
-                const auto r0 = std::chrono::steady_clock::now();
-                try {
-                    CUDASimulation sim(model_);
-                    sim.CUDAConfig().is_ensemble = true;
-                    sim.SimulationConfig().random_seed = plans[i].getRandomSimulationSeed();
-                    sim.SimulationConfig().steps = plans[i].getSteps();
-                    for (auto& kv : plans[i].properties()) sim_set_env(sim, kv.first, kv.second);
-                    if (init_) init_(sim, plans[i], i);
-                    if (step_log_) sim.setStepLog(*step_log_);
-                    if (exit_log_) sim.setExitLog(*exit_log_);
-                    sim.simulate();
-                    for (auto& a : model_.data()->agents) lg.final_counts[a.first] = sim_count(sim, a.first);
-                    const RunLog& rl = sim.getRunLog();
-                    std::lock_guard<std::mutex> lock(log_mutex);
-                    run_logs_[i] = rl;
-                } catch (const std::exception& e) {
-                    lg.error = e.what();
-                    ++errors;
-                    if (config_.error_level == EnsembleConfig::Fast) stop = true;
-                }
-                lg.wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - r0).count();
-            }
-        };
-        std::vector<std::thread> th;
-        const unsigned nt = std::max(1u, std::min<unsigned>(config_.concurrent_runs, (unsigned)plans.size()));
-        for (unsigned t = 0; t < nt; ++t) th.emplace_back(worker);
-        for (auto& t : th) t.join();
-        elapsed_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (errors && config_.error_level != EnsembleConfig::Off) {
-            throw exception::EnsembleError(detail::format("%u/%zu ensemble runs failed: %s", errors.load(), plans.size(),
-                                                          first_error().c_str()));
-        }
-        return errors;
-    }
-    // include/flamegpu/simulation/CUDAEnsemble.h: getLogs(), keyed by plan index
-    const std::map<unsigned int, RunLog>& getLogs() const { return run_logs_; }
-    const std::vector<RunRecord>& getRunRecords() const { return logs_; }
-    double getEnsembleElapsedTime() const { return elapsed_; }
-
- private:
-    std::string first_error() const {
-        for (auto& l : logs_)
-            if (!l.error.empty()) return l.error;
-        return "";
-    }
-    static void sim_set_env(CUDASimulation& s, const std::string& name, const std::vector<char>& b) {
-        auto& props = s.model()->env.props;
-        auto it = props.find(name);
-        if (it == props.end()) throw exception::InvalidEnvProperty("unknown property " + name);
-        std::memcpy(s.env_host().data() + it->second.offset, b.data(), b.size());
-        s.mark_env_dirty();
-    }
-    static unsigned int sim_count(CUDASimulation& s, const std::string& agent) {
-        HostAPI api(s, 0);
-        return api.agent(agent).count();
-    }
-    ModelDescription model_;
-    EnsembleConfig config_;
-    std::vector<RunRecord> logs_;
-    std::map<unsigned int, RunLog> run_logs_;
-    std::shared_ptr<StepLoggingConfig> step_log_;
-    std::shared_ptr<LoggingConfig> exit_log_;
-    std::function<void(CUDASimulation&, const RunPlan&, unsigned int)> init_;
-    double elapsed_ = 0.0;
 };
 
 namespace util {

@@ -42,6 +42,9 @@ def _lib(model):
         "fgxm_timed_steps": [vp, c.c_uint32, c.POINTER(c.c_double)],
         "fgxm_step_counter": [vp, u32p],
         "fgxm_stat": [vp, c.c_char_p, c.POINTER(c.c_uint64)],
+        "fgxm_ensemble_simulate": [c.c_char_p, c.POINTER(c.c_float), c.c_uint32, c.c_uint32, c.POINTER(c.c_uint64),
+                                   u32p, c.c_char_p, c.c_char_p, c.c_char_p, c.POINTER(c.c_int32), u32p,
+                                   c.POINTER(c.c_double), c.POINTER(c.c_int32)],
         "fgxm_simulate": [vp, c.c_uint32],
         "fgxm_log_config": [vp, c.c_int, c.c_char_p, c.c_char_p, c.c_char_p, c.c_char_p, c.c_uint32],
         "fgxm_export_log": [vp, c.c_char_p, c.c_int, c.c_int, c.c_int, c.c_int],
@@ -216,6 +219,30 @@ class Simulation:
 
     def export_log(self, path, steps=True, exit=True, step_time=False, exit_time=False):
         self._chk(self.L.fgxm_export_log(self.h, _b(str(path)), int(steps), int(exit), int(step_time), int(exit_time)))
+
+
+def ensemble(model, params, seeds, steps, variant=None, options="", sum_agent=None, sum_var=None, nagents=1):
+    """CUDAEnsemble::simulate over len(seeds) runs (fgxm_ensemble_simulate).  Returns a dict:
+    status per run (0 done, 1 failed, 2 not started), final counts (runs x agents), the
+    exit-log sums (sum_agent.sum_var, or the environment property sum_var when sum_agent is
+    None), whether the runs ran as one batch, and the error message ("" when none)."""
+    L = _lib(model)
+    n = len(seeds)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+    st = np.ascontiguousarray(steps, dtype=np.uint32)
+    status = np.zeros(n, np.int32)
+    counts = np.zeros((n, nagents), np.uint32)
+    sums = np.zeros(n, np.float64)
+    batched = ctypes.c_int32(-1)
+    rc = L.fgxm_ensemble_simulate(_b(variant), p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(p), n,
+                                  sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                  st.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _b(options), _b(sum_agent),
+                                  _b(sum_var), status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                  counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                  sums.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(batched))
+    err = "" if rc == 0 else L.fgxm_last_error().decode()
+    return {"status": status, "counts": counts, "sums": sums, "batched": batched.value, "error": err}
 
 
 def state_summary(model, path, params=(), variant=None):

@@ -63,6 +63,19 @@ int fgxm_step_counter(fgxm_sim* sim, uint32_t* out);
  * enqueued as one launch, graph replays not counted), "layer_images" (fused-layer images
  * the simulation's context holds). */
 int fgxm_stat(fgxm_sim* sim, const char* key, uint64_t* out);
+/* CUDAEnsemble::simulate (src/flamegpu/simulation/CUDAEnsemble.cu) over `nplans` runs of
+ * the model `variant`: run i with seed seeds[i] for steps[i] steps.  options: "key=value"
+ * pairs separated by ';': concurrent (runs at once on the per-run path), error_level
+ * (0 Off, 1 Slow, 2 Fast), batched (0: every run as its own simulation).  When sum_agent
+ * and sum_var are given, every run's exit log sums that variable into out_sums[i] (NaN for
+ * a run that failed or did not run).  out_status[i]: 0 done, 1 failed, 2 not started;
+ * out_counts: final population per run and agent (agents in name order).  Returns the
+ * EnsembleError code when runs failed and the error level is not Off, after filling the
+ * outputs; *out_batched = 1 when the runs ran as one batch. */
+int fgxm_ensemble_simulate(const char* variant, const float* params, uint32_t nparams, uint32_t nplans,
+                           const uint64_t* seeds, const uint32_t* steps, const char* options, const char* sum_agent,
+                           const char* sum_var, int32_t* out_status, uint32_t* out_counts, double* out_sums,
+                           int32_t* out_batched);
 /* CUDASimulation::simulate() for `steps` steps: init functions, step log frame 0,
  * the steps, exit functions, exit log (src/flamegpu/simulation/CUDASimulation.cu:1220-1344). */
 int fgxm_simulate(fgxm_sim* sim, uint32_t steps);

@@ -13,6 +13,8 @@
 //   transition_src0_dest10 / transition_src10_dest0 / transition_src10_dest10
 //                       test_agent_state_transition.cu:67-250
 //   concurrency         tests/test_cases/simulation/test_cuda_simulation_concurrency.cu:322-390
+//   ensemble_simulate / ensemble_errors
+//                       tests/test_cases/simulation/test_cuda_ensemble.cu:251-322, 630-748
 //   bruteforce          tests/test_cases/runtime/messaging/test_brute_force.cu:73-200
 //   random              tests/test_cases/runtime/random/test_agent_random.cu:28-185
 //   death_disabled / wrap_out_of_bounds   device errors (test_spatial_3d.cu:976-1039)
@@ -247,6 +249,27 @@ FLAMEGPU_AGENT_FUNCTION(SlowAgentFunction, MessageNone, MessageNone) {
         FLAMEGPU->setVariable<float>("v", v + v);
     }
     return ALIVE;
+}
+
+// tests/test_cases/simulation/test_cuda_ensemble.cu:251-278, 630-652
+FLAMEGPU_AGENT_FUNCTION(simulateAgentFn, MessageNone, MessageNone) {
+    FLAMEGPU->setVariable<uint32_t>("counter", FLAMEGPU->getVariable<uint32_t>("counter") + 1);
+    return ALIVE;
+}
+FLAMEGPU_INIT_FUNCTION(simulateInit) {
+    const uint32_t n = FLAMEGPU->environment.getProperty<uint32_t>("POPULATION_TO_GENERATE");
+    auto agent = FLAMEGPU->agent("Agent");
+    for (uint32_t i = 0; i < n; ++i) agent.newAgent().setVariable<uint32_t>("counter", 0);
+}
+// the reference adds the sum to a file-scope atomic; here it goes to an environment
+// property, which the exit log reports per run
+FLAMEGPU_EXIT_FUNCTION(simulateExit) {
+    const uint32_t total = FLAMEGPU->agent("Agent").sum<uint32_t, uint32_t>("counter");
+    FLAMEGPU->environment.setProperty<uint32_t>("EXIT_SUM", total);
+}
+// every second run fails (the reference counts calls in a static; here the run's seed decides)
+FLAMEGPU_STEP_FUNCTION(throwException) {
+    if (FLAMEGPU->random.getSeed() % 2 == 1) throw flamegpu::exception::UnknownInternalError("Dummy Exception");
 }
 
 FLAMEGPU_AGENT_FUNCTION(AddOne, MessageNone, MessageNone) {
@@ -540,6 +563,20 @@ flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* 
             auto a = model.newAgent(name);
             a.newVariable<float>("v");
             layer.addAgentFunction(a.newFunction(name + "_slowAgentFunction", SlowAgentFunction));
+        }
+    } else if (v == "ensemble_simulate" || v == "ensemble_errors") {
+        // test_cuda_ensemble.cu:279-322 (simulate), 653-748 (ErrorOff / ErrorSlow / ErrorFast)
+        model.Environment().newProperty<uint32_t>("POPULATION_TO_GENERATE", v == "ensemble_simulate" ? 32u : 1u, true);
+        model.Environment().newProperty<uint32_t>("EXIT_SUM", 0u);
+        auto agent = model.newAgent("Agent");
+        agent.newVariable<uint32_t>("counter", 0);
+        model.addInitFunction(simulateInit);
+        if (v == "ensemble_simulate") {
+            auto f = agent.newFunction("simulateAgentFn", simulateAgentFn);
+            model.newLayer().addAgentFunction(f);
+            model.addExitFunction(simulateExit);
+        } else {
+            model.addStepFunction(throwException);
         }
     } else if (v == "bruteforce") {
         auto message = model.newMessage("msg");
