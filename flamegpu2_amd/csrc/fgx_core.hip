@@ -19,6 +19,10 @@ std::unordered_map<const void*, uint32_t>& clean_record() {
     static std::unordered_map<const void*, uint32_t> r;
     return r;
 }
+std::unordered_map<const void*, uint32_t>& bins_history() {
+    static std::unordered_map<const void*, uint32_t> r;
+    return r;
+}
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
@@ -120,6 +124,157 @@ __global__ void k_copy_ranges(VarTable vt, const uint32_t* __restrict__ ranges, 
     }
 }
 
+// ---------------------------------------------------------------- batches whose populations change
+// Per-simulation counts live on the device; these kernels recompute a packed list's
+// per-simulation layout and patch the tables that record it.
+constexpr uint32_t kSegBlock = 1024;
+
+struct SegTargets {
+    uint32_t n;
+    fgx_seg_target t[FGX_SEG_MAX_TARGETS];
+};
+
+// Exclusive prefix of v over the block (1024 threads); *sum receives the block total.
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* s_w, uint32_t* sum) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint32_t inc = wave_inclusive_u32(v);
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t k = 0; k < kSegBlock / 64; ++k) {
+        const uint32_t x = s_w[k];
+        before += k < w ? x : 0u;
+        total += x;
+    }
+    __syncthreads();
+    *sum = total;
+    return before + inc - v;
+}
+
+__device__ __forceinline__ void put_u32(const fgx_seg_target& t, uint32_t rec, int32_t field, uint32_t v) {
+    if (field >= 0) *reinterpret_cast<uint32_t*>(static_cast<char*>(t.table) + (size_t)rec * t.stride + field) = v;
+}
+
+// One block: c[s] = cnt[s] (+ add[s], stored back) -> off[s], *total, and every target's
+// offset / count / first-tile fields (plus the sentinel record nsegs when asked).
+__global__ void __launch_bounds__(kSegBlock) k_seg_layout(uint32_t nsegs, uint32_t* __restrict__ cnt,
+                                                          const uint32_t* __restrict__ add, uint32_t* __restrict__ off,
+                                                          uint32_t* __restrict__ total, SegTargets tg) {
+    __shared__ uint32_t s_w[kSegBlock / 64];
+    uint32_t elems = 0;                          // running offset before this chunk
+    uint32_t tiles[FGX_SEG_MAX_TARGETS];         // running first tile per target
+    for (uint32_t k = 0; k < FGX_SEG_MAX_TARGETS; ++k) tiles[k] = 0;
+    for (uint32_t base = 0; base < nsegs; base += kSegBlock) {
+        const uint32_t s = base + threadIdx.x;
+        const bool on = s < nsegs;
+        const uint32_t c = on ? cnt[s] + (add ? add[s] : 0u) : 0u;
+        uint32_t chunk = 0;
+        const uint32_t eo = elems + block_exclusive(c, s_w, &chunk);
+        if (on) {
+            if (add) cnt[s] = c;
+            off[s] = eo;
+        }
+        for (uint32_t k = 0; k < FGX_SEG_MAX_TARGETS; ++k) {
+            if (k >= tg.n) break;
+            const fgx_seg_target& t = tg.t[k];
+            const uint32_t nt = (t.tile_field >= 0 && t.tile_elems) ? (c + t.tile_elems - 1u) / t.tile_elems : 0u;
+            uint32_t tsum = 0;
+            const uint32_t to = tiles[k] + block_exclusive(nt, s_w, &tsum);
+            if (on) {
+                put_u32(t, s, t.off_field, eo);
+                put_u32(t, s, t.n_field, c);
+                put_u32(t, s, t.tile_field, to);
+            }
+            tiles[k] += tsum;
+        }
+        elems += chunk;
+    }
+    if (threadIdx.x == 0) {
+        *total = elems;
+        for (uint32_t k = 0; k < tg.n && k < FGX_SEG_MAX_TARGETS; ++k) {
+            const fgx_seg_target& t = tg.t[k];
+            if (!t.sentinel) continue;
+            put_u32(t, nsegs, t.off_field, elems);
+            put_u32(t, nsegs, t.n_field, 0u);
+            put_u32(t, nsegs, t.tile_field, tiles[k]);
+        }
+    }
+}
+
+// Block s: out[s] = flags equal to `keep` in [off[s], off[s] + cnt[s]).  out may be cnt:
+// each block reads its own count before the barrier that precedes its write.
+__global__ void __launch_bounds__(256) k_seg_count(const uint32_t* __restrict__ off, const uint32_t* cnt,
+                                                   const uint32_t* __restrict__ flags, uint32_t keep, uint32_t* out) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t s = blockIdx.x;
+    const uint32_t o = off[s], n = cnt[s];
+    uint32_t c = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256u) c += flags[o + i] == keep ? 1u : 0u;
+    c = wave_sum_u32(c);
+    if ((threadIdx.x & 63u) == 0) s_w[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) out[s] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// One block: work = [new_off[0..S) | new_total | boff[0..S) | born_total], new_off the
+// exclusive scan of cnt + born, boff that of born.
+__global__ void __launch_bounds__(kSegBlock) k_seg_append_scan(uint32_t nsegs, const uint32_t* __restrict__ cnt,
+                                                               const uint32_t* __restrict__ born,
+                                                               uint32_t* __restrict__ work) {
+    __shared__ uint32_t s_w[kSegBlock / 64];
+    uint32_t e = 0, b = 0;
+    for (uint32_t base = 0; base < nsegs; base += kSegBlock) {
+        const uint32_t s = base + threadIdx.x;
+        const bool on = s < nsegs;
+        const uint32_t bs = on ? born[s] : 0u, c = on ? cnt[s] + bs : 0u;
+        uint32_t ce = 0, cb = 0;
+        const uint32_t eo = e + block_exclusive(c, s_w, &ce);
+        const uint32_t bo = b + block_exclusive(bs, s_w, &cb);
+        if (on) {
+            work[s] = eo;
+            work[nsegs + 1 + s] = bo;
+        }
+        e += ce;
+        b += cb;
+    }
+    if (threadIdx.x == 0) {
+        work[nsegs] = e;
+        work[2 * nsegs + 1] = b;
+    }
+}
+
+// Output element q of the new layout: simulation s (last new_off[s] <= q), then either
+// its existing element q - new_off[s] or its new agent r = q - new_off[s] - cnt[s]
+// (born[boff[s] + r], ID next_id[s] + r).
+__global__ void k_seg_append_copy(uint32_t nsegs, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt,
+                                  const uint32_t* __restrict__ work, uint32_t n_cap_out, VarTable list, VarTable born,
+                                  int32_t id_var, const uint32_t* __restrict__ next_id) {
+    const uint32_t total = work[nsegs] < n_cap_out ? work[nsegs] : n_cap_out;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+        uint32_t lo = 0, hi = nsegs;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (work[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t local = q - work[lo];
+        const uint32_t c = cnt[lo];
+        if (local < c) {
+            copy_element(list, off[lo] + local, q);
+        } else {
+            const uint32_t r = local - c;
+            copy_element(born, work[nsegs + 1 + lo] + r, q, id_var);
+            if (id_var >= 0) reinterpret_cast<uint32_t*>(born.out[id_var])[q] = next_id[lo] + r;
+        }
+    }
+}
+
+__global__ void k_add_n_u32(uint32_t* __restrict__ p, const uint32_t* __restrict__ d, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] += d[i];
+}
+
 __global__ void k_assign_ids(uint32_t* __restrict__ ids, uint32_t n, uint32_t next_id) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && ids[i] == 0u) ids[i] = next_id + i;
@@ -209,6 +364,7 @@ int fgx_free(fgx_ctx* ctx, void* p) {
         // a later allocation at this address starts with unknown contents
         std::lock_guard<std::mutex> lk(clean_mutex());
         clean_record().erase(p);
+        bins_history().erase(p);
     }
     FGX_HIP(hipFree(p));
     return FGX_OK;
@@ -469,6 +625,64 @@ int fgx_copy_ranges(uint32_t nvars, const void* const* src, void* const* dst, co
     const uint32_t blocks = std::min<uint32_t>(ceil_div(n_total, 256), 8192);
     hipLaunchKernelGGL(k_copy_ranges, dim3(blocks), dim3(256), 0, (hipStream_t)stream, vt, d_ranges, nranges, n_total);
     FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+int fgx_seg_layout(uint32_t nsegs, uint32_t* d_cnt, const uint32_t* d_add, uint32_t* d_off, uint32_t* d_total,
+                   const fgx_seg_target* targets, uint32_t ntargets, void* stream) {
+    if (!d_cnt || !d_off || !d_total) return fail(FGX_ERR_ARG, "fgx_seg_layout: NULL count, offset or total array");
+    if (ntargets > FGX_SEG_MAX_TARGETS) return fail(FGX_ERR_ARG, "fgx_seg_layout: too many targets");
+    if (ntargets && !targets) return fail(FGX_ERR_ARG, "fgx_seg_layout: NULL targets");
+    SegTargets tg;
+    std::memset(&tg, 0, sizeof(tg));
+    tg.n = ntargets;
+    for (uint32_t k = 0; k < ntargets; ++k) {
+        const fgx_seg_target& t = targets[k];
+        if (!t.table || t.stride < 4) return fail(FGX_ERR_ARG, "fgx_seg_layout: target without table or stride");
+        for (int32_t f : {t.off_field, t.n_field, t.tile_field})
+            if (f >= 0 && ((uint32_t)f + 4u > t.stride || (f & 3))) return fail(FGX_ERR_ARG, "fgx_seg_layout: field outside its record");
+        if (t.tile_field >= 0 && !t.tile_elems) return fail(FGX_ERR_ARG, "fgx_seg_layout: tile field without tile size");
+        tg.t[k] = t;
+    }
+    hipLaunchKernelGGL(k_seg_layout, dim3(1), dim3(kSegBlock), 0, (hipStream_t)stream, nsegs, d_cnt, d_add, d_off, d_total, tg);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+int fgx_seg_count(uint32_t nsegs, const uint32_t* d_off, const uint32_t* d_cnt, const uint32_t* flags, uint32_t keep,
+                  uint32_t* d_out, void* stream) {
+    if (!nsegs) return FGX_OK;
+    if (!d_off || !d_cnt || !flags || !d_out) return fail(FGX_ERR_ARG, "fgx_seg_count: NULL argument");
+    hipLaunchKernelGGL(k_seg_count, dim3(nsegs), dim3(256), 0, (hipStream_t)stream, d_off, d_cnt, flags, keep, d_out);
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+int fgx_seg_append(uint32_t nsegs, const uint32_t* d_off, const uint32_t* d_cnt, const uint32_t* d_born, uint32_t n_cap_out,
+                   uint32_t nvars, const void* const* list_in, const void* const* born_in, void* const* list_out,
+                   const uint32_t* var_bytes, int32_t id_var, uint32_t* d_next_id, uint32_t* d_work, void* stream) {
+    if (!nsegs) return FGX_OK;
+    if (!d_off || !d_cnt || !d_born || !d_work) return fail(FGX_ERR_ARG, "fgx_seg_append: NULL table");
+    if (id_var >= (int32_t)nvars) return fail(FGX_ERR_ARG, "fgx_seg_append: ID variable out of range");
+    if (id_var >= 0 && (!d_next_id || var_bytes[id_var] != 4)) return fail(FGX_ERR_ARG, "fgx_seg_append: IDs need next-ID counters and 4-byte IDs");
+    VarTable list, born;
+    int rc = make_var_table(nvars, list_in, list_out, var_bytes, &list);
+    if (rc) return rc;
+    rc = make_var_table(nvars, born_in, list_out, var_bytes, &born);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_seg_append_scan, dim3(1), dim3(kSegBlock), 0, st, nsegs, d_cnt, d_born, d_work);
+    FGX_HIP(hipGetLastError());
+    if (n_cap_out) {
+        const uint32_t blocks = std::min<uint32_t>(ceil_div(n_cap_out, 256), 8192);
+        hipLaunchKernelGGL(k_seg_append_copy, dim3(blocks), dim3(256), 0, st, nsegs, d_off, d_cnt, d_work, n_cap_out, list,
+                           born, id_var, d_next_id);
+        FGX_HIP(hipGetLastError());
+    }
+    if (id_var >= 0) {
+        hipLaunchKernelGGL(k_add_n_u32, dim3(ceil_div(nsegs, 256)), dim3(256), 0, st, d_next_id, d_born, nsegs);
+        FGX_HIP(hipGetLastError());
+    }
     return FGX_OK;
 }
 

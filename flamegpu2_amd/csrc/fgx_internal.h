@@ -45,6 +45,10 @@ struct Ctx {
 constexpr uint32_t kAllZero = 0xFFFFFFFFu;
 std::mutex& clean_mutex();
 std::unordered_map<const void*, uint32_t>& clean_record();
+// Per scratch buffer: the one bin count every sort of it has used so far (kMixedBins once
+// two differ).  Consulted only with FGX_OPT_KEEP_SCRATCH_ZERO.
+constexpr uint32_t kMixedBins = 0xFFFFFFFEu;
+std::unordered_map<const void*, uint32_t>& bins_history();
 
 // ---------------------------------------------------------------- variable tables
 // Passed by value in the kernel argument segment, so every field is wave-uniform

@@ -1005,8 +1005,10 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
     // With FGX_OPT_KEEP_SCRATCH_ZERO every sort leaves [0, bins + 1] zero (the scan clears
     // the counts, k_tile_sums moves the flag), so a buffer last sorted with this bin count
     // needs no clearing launch.  The record describes the buffer after the work already
-    // enqueued eagerly; a captured graph replays later, in an order the record cannot
-    // follow, so a sort under capture always clears and leaves no record behind.
+    // enqueued eagerly.  A captured graph replays later, in an order the record cannot
+    // follow: under capture the clearing is skipped only for a buffer that every earlier
+    // sort used with this same bin count (each of them left the region zero, and the first
+    // one cleared it), since another bin count moves the key arrays over the histogram.
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     FGX_HIP(hipStreamIsCapturing((hipStream_t)stream, &cap));
     const bool capturing = cap != hipStreamCaptureStatusNone;
@@ -1015,8 +1017,12 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
     if (record) {
         std::lock_guard<std::mutex> lk(clean_mutex());
         auto it = clean_record().find(scratch);
-        clean = !capturing && it != clean_record().end() && (it->second == bins || it->second == kAllZero);
+        auto hit = bins_history().find(scratch);
+        const bool single = hit != bins_history().end() && hit->second == bins;
+        clean = capturing ? single : (it != clean_record().end() && (it->second == bins || it->second == kAllZero));
         if (it != clean_record().end()) clean_record().erase(it);   // re-recorded once every launch is enqueued
+        if (hit == bins_history().end()) bins_history()[scratch] = bins;
+        else if (hit->second != bins) hit->second = kMixedBins;
     }
     int rc = FGX_OK;
     if (!clean) {
@@ -1057,7 +1063,8 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
 
 // Segmented: the keys come from k_bin_hist_seg; scan and scatters run over the packed batch.
 static int run_bin_sort_seg(fgx_ctx* ctx, int32_t mode, const fgx_seg_key* d_segs, uint32_t nsegs, uint32_t tiles,
-                            uint32_t n_total, uint32_t keys_total, const float* x, const float* y, const float* z,
+                            uint32_t n_total, const uint32_t* d_n_total, uint32_t keys_total, const float* x,
+                            const float* y, const float* z,
                             const VarTable& vt, uint32_t* start_out, void* scratch, size_t scratch_bytes, void* stream) {
     const BuildLayout L = build_layout(n_total, keys_total);
     if (!scratch || scratch_bytes < L.total) return fail(FGX_ERR_CAPACITY, "scratch too small for segmented bin sort");
@@ -1083,13 +1090,13 @@ static int run_bin_sort_seg(fgx_ctx* ctx, int32_t mode, const fgx_seg_key* d_seg
     }
     if (n_total) {
         const uint32_t g = stream_grid(ctx ? &ctx->c : nullptr, n_total);
-        hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, nullptr, n_total, key, sub, start_out, tmp,
+        hipLaunchKernelGGL(k_index_scatter, dim3(g), dim3(kBlock), 0, st, d_n_total, n_total, key, sub, start_out, tmp,
                            unsorted);
         FGX_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_sort_big_bins, dim3(big_bin_grid(ctx ? &ctx->c : nullptr, keys_total)), dim3(kBlock), 0, st,
                            start_out, keys_total, tmp, sub, unsorted);
         FGX_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, nullptr, n_total, key, start_out, tmp, sub, vt,
+        hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, d_n_total, n_total, key, start_out, tmp, sub, vt,
                            unsorted);
         FGX_HIP(hipGetLastError());
     }
@@ -1099,7 +1106,8 @@ static int run_bin_sort_seg(fgx_ctx* ctx, int32_t mode, const fgx_seg_key* d_seg
 extern "C" {
 
 int fgx_spatial_build_seg(fgx_ctx* ctx, const fgx_seg_key* d_segs, uint32_t nsegs, uint32_t tiles_total,
-                          uint32_t n_total, uint32_t keys_total, const float* x, const float* y, const float* z,
+                          uint32_t n_total, const uint32_t* d_n_total, uint32_t keys_total, const float* x,
+                          const float* y, const float* z,
                           uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
                           uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream) {
     if (!pbm) return fail(FGX_ERR_ARG, "fgx_spatial_build_seg: NULL pbm");
@@ -1108,12 +1116,13 @@ int fgx_spatial_build_seg(fgx_ctx* ctx, const fgx_seg_key* d_segs, uint32_t nseg
     VarTable vt;
     int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
     if (rc) return rc;
-    return run_bin_sort_seg(ctx, 0, d_segs, nsegs, tiles_total, n_total, keys_total, x, y, z, vt, pbm, scratch,
+    return run_bin_sort_seg(ctx, 0, d_segs, nsegs, tiles_total, n_total, d_n_total, keys_total, x, y, z, vt, pbm, scratch,
                             scratch_bytes, stream);
 }
 
 int fgx_agent_sort_seg(fgx_ctx* ctx, const fgx_seg_key* d_segs, uint32_t nsegs, uint32_t tiles_total,
-                       uint32_t n_total, uint32_t keys_total, const float* x, const float* y, const float* z,
+                       uint32_t n_total, const uint32_t* d_n_total, uint32_t keys_total, const float* x,
+                       const float* y, const float* z,
                        uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
                        void* scratch, size_t scratch_bytes, void* stream) {
     if (nsegs && !d_segs) return fail(FGX_ERR_ARG, "fgx_agent_sort_seg: NULL segment table");
@@ -1125,7 +1134,7 @@ int fgx_agent_sort_seg(fgx_ctx* ctx, const fgx_seg_key* d_segs, uint32_t nsegs, 
     if (!scratch || scratch_bytes < L.total + align256((size_t)(keys_total + 1) * 4))
         return fail(FGX_ERR_CAPACITY, "fgx_agent_sort_seg: scratch too small");
     auto* start = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) + L.total);
-    return run_bin_sort_seg(ctx, 1, d_segs, nsegs, tiles_total, n_total, keys_total, x, y, z, vt, start, scratch,
+    return run_bin_sort_seg(ctx, 1, d_segs, nsegs, tiles_total, n_total, d_n_total, keys_total, x, y, z, vt, start, scratch,
                             L.total, stream);
 }
 
@@ -1159,6 +1168,7 @@ int fgx_scratch_clear(void* scratch, size_t bytes, void* stream) {
     if (rc == FGX_OK) {
         std::lock_guard<std::mutex> lk(clean_mutex());
         clean_record()[scratch] = kAllZero;
+        bins_history().erase(scratch);   // the next sort is this buffer's first (it clears under capture)
     }
     return rc;
 }

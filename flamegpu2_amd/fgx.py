@@ -85,10 +85,14 @@ _SIGS = {
     "fgx_seg_key_init": [_c.c_int, _c.c_int, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float), _c.c_float,
                          _c.POINTER(SegKey)],
     "fgx_segments_pack": [_c.POINTER(SegKey), u32, _c.POINTER(u32), _c.POINTER(u32), _c.POINTER(u32)],
-    "fgx_spatial_build_seg": [vp, vp, u32, u32, u32, u32, vp, vp, vp, u32, _c.POINTER(vp), _c.POINTER(vp),
+    "fgx_spatial_build_seg": [vp, vp, u32, u32, u32, vp, u32, vp, vp, vp, u32, _c.POINTER(vp), _c.POINTER(vp),
                               _c.POINTER(u32), vp, vp, sz, vp],
-    "fgx_agent_sort_seg": [vp, vp, u32, u32, u32, u32, vp, vp, vp, u32, _c.POINTER(vp), _c.POINTER(vp),
+    "fgx_agent_sort_seg": [vp, vp, u32, u32, u32, vp, u32, vp, vp, vp, u32, _c.POINTER(vp), _c.POINTER(vp),
                            _c.POINTER(u32), vp, sz, vp],
+    "fgx_seg_layout": [u32, vp, vp, vp, vp, vp, u32, vp],
+    "fgx_seg_count": [u32, vp, vp, vp, u32, vp, vp],
+    "fgx_seg_append": [u32, vp, vp, vp, u32, u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32),
+                       _c.c_int32, vp, vp, vp],
     "fgx_add_u32_n": [vp, u32, u32, vp],
     "fgx_copy_ranges": [u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), vp, u32, u32, vp],
 }
@@ -254,17 +258,40 @@ def segments_pack(keys):
 
 
 def spatial_build_seg(ctx, d_segs, nsegs, tiles, n_total, keys_total, x, y, z, vars_in, vars_out, var_bytes, pbm,
-                      scratch, scratch_len, stream=0):
+                      scratch, scratch_len, stream=0, d_n_total=None):
     nv = len(vars_in)
-    check(lib().fgx_spatial_build_seg(ctx.h, d_segs, nsegs, tiles, n_total, keys_total, x, y, z, nv, _arr(vp, vars_in),
-                                      _arr(vp, vars_out), _arr(u32, var_bytes), pbm, scratch, scratch_len, stream))
+    check(lib().fgx_spatial_build_seg(ctx.h, d_segs, nsegs, tiles, n_total, d_n_total, keys_total, x, y, z, nv,
+                                      _arr(vp, vars_in), _arr(vp, vars_out), _arr(u32, var_bytes), pbm, scratch,
+                                      scratch_len, stream))
 
 
 def agent_sort_seg(ctx, d_segs, nsegs, tiles, n_total, keys_total, x, y, z, vars_in, vars_out, var_bytes, scratch,
-                   scratch_len, stream=0):
+                   scratch_len, stream=0, d_n_total=None):
     nv = len(vars_in)
-    check(lib().fgx_agent_sort_seg(ctx.h, d_segs, nsegs, tiles, n_total, keys_total, x, y, z, nv, _arr(vp, vars_in),
-                                   _arr(vp, vars_out), _arr(u32, var_bytes), scratch, scratch_len, stream))
+    check(lib().fgx_agent_sort_seg(ctx.h, d_segs, nsegs, tiles, n_total, d_n_total, keys_total, x, y, z, nv,
+                                   _arr(vp, vars_in), _arr(vp, vars_out), _arr(u32, var_bytes), scratch, scratch_len,
+                                   stream))
+
+
+class SegTarget(_c.Structure):
+    _fields_ = [("table", vp), ("stride", u32), ("off_field", _c.c_int32), ("n_field", _c.c_int32),
+                ("tile_field", _c.c_int32), ("tile_elems", u32), ("sentinel", _c.c_int32)]
+
+
+def seg_layout(nsegs, d_cnt, d_off, d_total, targets=(), d_add=None, stream=0):
+    arr = (SegTarget * max(len(targets), 1))(*targets)
+    check(lib().fgx_seg_layout(nsegs, d_cnt, d_add, d_off, d_total, arr, len(targets), stream))
+
+
+def seg_count(nsegs, d_off, d_cnt, flags, keep, d_out, stream=0):
+    check(lib().fgx_seg_count(nsegs, d_off, d_cnt, flags, keep, d_out, stream))
+
+
+def seg_append(nsegs, d_off, d_cnt, d_born, n_cap_out, list_in, born_in, list_out, var_bytes, id_var, d_next_id, d_work,
+               stream=0):
+    check(lib().fgx_seg_append(nsegs, d_off, d_cnt, d_born, n_cap_out, len(list_in), _arr(vp, list_in),
+                               _arr(vp, born_in), _arr(vp, list_out), _arr(u32, var_bytes), id_var, d_next_id, d_work,
+                               stream))
 
 
 def add_u32_n(p, n, delta, stream=0):

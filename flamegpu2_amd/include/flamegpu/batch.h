@@ -15,10 +15,18 @@
 // RNG streams (keyed by seed, ID, step), sort order and message order are those of
 // the same simulation run alone.
 //
-// Supported models: fixed populations (no agent death, agent output or state
-// transitions), mandatory message output, every message list written by one agent
-// function before it is read in the step, no host functions inside layers and no
-// step/exit functions (conditions need per-simulation host logic).
+// Populations may change: agent death and agent output (births into an agent's initial
+// state) run over the whole batch.  A stable compaction of the packed list keeps each
+// simulation's survivors together and in order, each simulation's new agents are placed
+// after its own agents (IDs from its own counter), and per-simulation counts stay on the
+// device, where fgx_seg_layout patches every segment table after each change.  The host
+// reads the totals back once per step to keep capacities ahead of the step's bound.
+//
+// Supported models: agents in their initial state (no state transitions), mandatory
+// message output, every message list written by one agent function before it is read
+// in the step, no function conditions, no host functions inside layers and no step
+// functions or exit conditions (they need per-simulation host logic).  AgentOut::getID()
+// reports a device error in a batch: children are numbered per simulation after the launch.
 #pragma once
 
 #include <algorithm>
@@ -120,7 +128,8 @@ class BatchSimulation {
         unsigned int steps_target = 0, steps_done = 0;
         std::vector<char> env;
         std::map<std::string, fgx_spatial_meta> meta;            // spatial message lists
-        std::map<std::string, uint32_t> n;                        // agent name -> population
+        std::map<std::string, uint32_t> n;                        // agent name -> population (as of the last repack)
+        std::map<std::string, uint32_t> next_id;                  // agent name -> next agent ID (as of the last repack)
         std::map<std::string, std::shared_ptr<AgentVector>> pop;  // until uploaded
         bool resident = false;
     };
@@ -128,8 +137,12 @@ class BatchSimulation {
         std::vector<detail::VarBuf> vars;
         uint32_t cap = 0;
         uint32_t total = 0;
-        std::vector<uint32_t> off;   // per resident simulation (batch order)
+        std::vector<uint32_t> off;   // per resident simulation (batch order), host copy as of the last repack
         std::vector<uint32_t> cnt;
+        // device copies, kept by fgx_seg_layout while births and deaths change the populations
+        uint32_t* d_off = nullptr;
+        uint32_t* d_cnt = nullptr;
+        uint32_t* d_total = nullptr;
         int id_slot = -1;
         int var_index(const std::string& n) const {
             for (size_t i = 0; i < vars.size(); ++i)
@@ -162,6 +175,10 @@ class BatchSimulation {
         uint32_t keys_total = 0, tiles = 0;
         void* scratch = nullptr;
         size_t scratch_bytes = 0;
+        // changing populations: per-simulation next IDs, new agents and append work space
+        uint32_t* d_next_id = nullptr;
+        uint32_t* d_born = nullptr;
+        uint32_t* d_work = nullptr;
     };
     struct FnB {
         std::shared_ptr<AgentFunctionData> d;
@@ -173,6 +190,16 @@ class BatchSimulation {
         uint32_t tiles = 0;
         uint32_t* msg_flags = nullptr;   // message-output flags, one per agent of the list
         uint32_t flags_cap = 0;
+        // agent death and agent output
+        AgentB* out_agent = nullptr;
+        uint32_t* death_flags = nullptr;
+        uint32_t* birth_flags = nullptr;
+        std::vector<detail::VarBuf> newbuf;   // new agents by parent index (cur) and compacted (alt)
+        uint32_t pop_cap = 0;                 // capacity of the buffers above
+        void* death_scratch = nullptr;
+        size_t death_scratch_bytes = 0;
+        void* birth_scratch = nullptr;
+        size_t birth_scratch_bytes = 0;
     };
 
     Sim& sim(int h) {
@@ -200,6 +227,19 @@ class BatchSimulation {
     }
     void rebuild();
     void repack(List& l, const std::vector<uint32_t>& keep_old_slot, const std::vector<uint32_t>& new_cnt);
+    // changing populations
+    void read_counts();
+    void ensure_step_capacity();
+    void grow_agent_list(AgentB& a, uint32_t cap);
+    void size_pop_buffers(FnB& f);
+    void size_sort_scratch(AgentB& a);
+    void size_build_scratch(MsgB& m);
+    void upload_layout(List& l, uint32_t S);
+    std::vector<fgx_seg_target> agent_targets(AgentB& a);
+    std::vector<fgx_seg_target> message_targets(MsgB& m);
+    void relayout(List& l, const std::vector<fgx_seg_target>& t, const uint32_t* d_add);
+    uint32_t tile_bound(uint32_t cap, uint32_t tile) const { return (cap + tile - 1) / tile + (uint32_t)order_.size(); }
+    void after_launch(FnB& f);
     void enqueue_step();
     void launch(FnB& f);
     void clear_graphs() {
@@ -247,6 +287,8 @@ class BatchSimulation {
     bool graphs_on_ = true;
     bool unit_tables_ = true;
     uint32_t stable_steps_ = 0;   // steps since the last repack
+    bool dynamic_ = false;        // some function removes or adds agents
+    uint32_t* d_word_ = nullptr;  // a count nobody reads (the births compaction's total)
 };
 
 // ------------------------------------------------------------------ implementation
@@ -261,10 +303,13 @@ inline void BatchSimulation::check_model(const ModelData& m, bool host_phases_ou
         if (!layer->host_functions.empty())
             throw exception::InvalidOperation("BatchSimulation: layers with host functions cannot be batched");
         for (auto& f : layer->agent_functions) {
-            if (f->has_agent_death || f->agent_output.lock() || f->initial_state != f->end_state ||
-                f->initial_state != f->parent->initial_state)
+            if (f->initial_state != f->end_state || f->initial_state != f->parent->initial_state)
                 throw exception::InvalidOperation("BatchSimulation: agent function '" + f->name +
-                                                  "' changes the population or its state; batches need fixed populations");
+                                                  "' changes agent state; batched agents stay in their initial state");
+            if (auto oa = f->agent_output.lock())
+                if (f->agent_output_state != oa->initial_state)
+                    throw exception::InvalidOperation("BatchSimulation: agent function '" + f->name +
+                                                      "' creates agents outside their initial state");
             if (f->message_output_optional)
                 throw exception::InvalidOperation("BatchSimulation: optional message output ('" + f->name + "') cannot be batched");
             if (f->condition)
@@ -340,6 +385,15 @@ inline void BatchSimulation::init() {
                 r.out = &msgs_.at(mo->name);
                 r.out->src_agent = f->parent->name;
             }
+            if (auto oa = f->agent_output.lock()) r.out_agent = &agents_.at(oa->name);
+            if (f->has_agent_death || r.out_agent) dynamic_ = true;
+            for (auto& kv : r.out_agent ? r.out_agent->d->variables : std::map<std::string, Variable>()) {
+                detail::VarBuf b;
+                b.name = kv.first;
+                b.bytes = (uint32_t)kv.second.bytes();
+                b.def = kv.second.default_value;
+                r.newbuf.push_back(b);
+            }
             if (r.in && r.in->spatial && f->parent->sortPeriod != 0) {
                 const auto& vars = f->parent->variables;
                 auto isf = [&](const char* n) {
@@ -352,6 +406,19 @@ inline void BatchSimulation::init() {
             fns_.push_back(std::move(r));
         }
     env_stride_ = std::max<size_t>((m_->env.buffer.size() + 63) / 64 * 64, 64);
+    {
+        void* p = nullptr;
+        alloc(&p, 4);
+        d_word_ = (uint32_t*)p;
+        for (auto& kv : agents_) {
+            alloc(&p, 4);
+            kv.second.list.d_total = (uint32_t*)p;
+        }
+        for (auto& kv : msgs_) {
+            alloc(&p, 4);
+            kv.second.list.d_total = (uint32_t*)p;
+        }
+    }
     initialised_ = true;
 }
 
@@ -366,6 +433,12 @@ inline void BatchSimulation::release() {
         }
         free_typed(kv.second.d_keys);
         free_dev(kv.second.scratch);
+        free_typed(kv.second.list.d_off);
+        free_typed(kv.second.list.d_cnt);
+        free_typed(kv.second.list.d_total);
+        free_typed(kv.second.d_next_id);
+        free_typed(kv.second.d_born);
+        free_typed(kv.second.d_work);
     }
     for (auto& kv : msgs_) {
         for (auto& v : kv.second.list.vars) {
@@ -375,11 +448,23 @@ inline void BatchSimulation::release() {
         free_typed(kv.second.pbm);
         free_typed(kv.second.d_keys);
         free_dev(kv.second.scratch);
+        free_typed(kv.second.list.d_off);
+        free_typed(kv.second.list.d_cnt);
+        free_typed(kv.second.list.d_total);
     }
     for (auto& f : fns_) {
         free_typed(f.d_segs);
         free_typed(f.msg_flags);
+        free_typed(f.death_flags);
+        free_typed(f.birth_flags);
+        for (auto& v : f.newbuf) {
+            free_dev(v.cur);
+            free_dev(v.alt);
+        }
+        free_dev(f.death_scratch);
+        free_dev(f.birth_scratch);
     }
+    free_typed(d_word_);
     free_typed(d_env_);
     free_typed(d_steps_);
     free_typed(d_errors_);
@@ -447,6 +532,8 @@ inline int BatchSimulation::add(const Plan& p) {
                 std::memcpy(ids.data() + i * 4, &v, 4);
             }
         }
+        // the next ID after the population (CUDASimulation::setPopulationData reserves n from `next`)
+        s.next_id[kv.first] = next + (id_t)n;
         s.n[kv.first] = (uint32_t)n;
         s.pop[kv.first] = pop;
     }
@@ -602,6 +689,7 @@ inline void BatchSimulation::rebuild() {
     init();
     synchronize();
     clear_graphs();
+    if (dynamic_) read_counts();
     // new batch order: kept simulations in their order, then joining ones
     std::vector<int> neworder;
     std::vector<uint32_t> old_slot;
@@ -787,7 +875,32 @@ inline void BatchSimulation::rebuild() {
         d_steps_ = (uint32_t*)p;
         alloc(&p, (size_t)sim_cap_ * 16);
         d_errors_ = (uint32_t*)p;
+        auto per_sim = [&](uint32_t*& q, size_t words) {
+            free_typed(q);
+            void* r = nullptr;
+            alloc(&r, words * 4);
+            q = (uint32_t*)r;
+        };
+        for (auto& kv : agents_) {
+            per_sim(kv.second.list.d_off, sim_cap_);
+            per_sim(kv.second.list.d_cnt, sim_cap_);
+            per_sim(kv.second.d_next_id, sim_cap_);
+            per_sim(kv.second.d_born, sim_cap_);
+            per_sim(kv.second.d_work, 2 * (size_t)sim_cap_ + 2);
+        }
+        for (auto& kv : msgs_) {
+            per_sim(kv.second.list.d_off, sim_cap_);
+            per_sim(kv.second.list.d_cnt, sim_cap_);
+        }
     }
+    // device copies of every list's layout and of the agents' next IDs
+    for (auto& kv : agents_) {
+        upload_layout(kv.second.list, S);
+        std::vector<uint32_t> ids(S);
+        for (uint32_t k = 0; k < S; ++k) ids[k] = sims_.at(neworder[k]).next_id.at(kv.first);
+        if (S) detail::fgx_check(fgx_memcpy_h2d(kv.second.d_next_id, ids.data(), (size_t)S * 4, stream_), "next IDs");
+    }
+    for (auto& kv : msgs_) upload_layout(kv.second.list, S);
     if (S) {
         std::vector<char> env((size_t)S * env_stride_, 0);
         std::vector<uint32_t> steps(S);
@@ -807,7 +920,7 @@ inline void BatchSimulation::rebuild() {
     // per-function segment tables
     const uint32_t tile = tuning_.block * tuning_.agents_per_thread;
     for (auto& f : fns_) {
-        std::vector<detail::SegDev> segs(S);
+        std::vector<detail::SegDev> segs(S + 1);
         uint32_t t = 0;
         for (uint32_t s = 0; s < S; ++s) {
             const Sim& sm = sims_.at(neworder[s]);
@@ -841,6 +954,13 @@ inline void BatchSimulation::rebuild() {
             }
         }
         f.tiles = t;
+        {
+            // sentinel record: the batch's tile count (the kernel's loop bound)
+            detail::SegDev& d = segs[S];
+            std::memset(&d, 0, sizeof(d));
+            d.agent_off = f.agent->list.total;
+            d.tile_off = t;
+        }
         if (f.out && f.agent->list.cap > f.flags_cap) {
             free_typed(f.msg_flags);
             f.flags_cap = f.agent->list.cap;
@@ -848,14 +968,16 @@ inline void BatchSimulation::rebuild() {
             alloc(&p, (size_t)f.flags_cap * 4);
             f.msg_flags = (uint32_t*)p;
         }
-        if (S > f.segs_cap) {
+        if (S + 1 > f.segs_cap) {
             free_typed(f.d_segs);
-            f.segs_cap = S * 2;
+            f.segs_cap = (S + 1) * 2;
             void* p = nullptr;
             alloc(&p, (size_t)f.segs_cap * sizeof(detail::SegDev));
             f.d_segs = (detail::SegDev*)p;
         }
-        if (S) detail::fgx_check(fgx_memcpy_h2d(f.d_segs, segs.data(), (size_t)S * sizeof(detail::SegDev), stream_), "function segments");
+        detail::fgx_check(fgx_memcpy_h2d(f.d_segs, segs.data(), (size_t)(S + 1) * sizeof(detail::SegDev), stream_),
+                          "function segments");
+        if (dynamic_) size_pop_buffers(f);
     }
     order_ = neworder;
     // resident bytes: lists (both buffers), bin tables, scratch, per-simulation tables
@@ -897,9 +1019,27 @@ inline void BatchSimulation::launch(FnB& f) {
     if (f.in) fill_table(a.msg_in, &a.msg_in_mul, f.in->list.vars, false);
     if (f.out) {
         fill_table(a.msg_out, &a.msg_out_mul, f.out->list.vars, true);
-        if (f.flags_cap < l.total) throw exception::InvalidOperation("BatchSimulation: message flags below the list size");
+        if (f.flags_cap < (dynamic_ ? l.cap : l.total)) throw exception::InvalidOperation("BatchSimulation: message flags below the list size");
         a.msg_out_flags = f.msg_flags;   // written 1 per agent (mandatory output)
         a.msg_out_optional = 0;
+    }
+    if (f.d->has_agent_death) a.death_flags = f.death_flags;
+    if (f.out_agent) {
+        // new agents start from the variable defaults (broadcastInit), one slot per parent
+        fill_table(a.new_agent, &a.new_mul, f.newbuf, false);
+        a.birth_flags = f.birth_flags;
+        std::vector<void*> ptrs;
+        std::vector<uint32_t> bytes, offs;
+        std::vector<char> defs;
+        for (auto& v : f.newbuf) {
+            ptrs.push_back(v.cur);
+            bytes.push_back(v.bytes);
+            offs.push_back((uint32_t)defs.size());
+            defs.insert(defs.end(), v.def.begin(), v.def.end());
+        }
+        detail::fgx_check(fgx_fill_vars((uint32_t)ptrs.size(), ptrs.data(), bytes.data(), defs.data(), offs.data(), l.cap,
+                                        stream_),
+                          "new-agent defaults");
     }
     {
         std::vector<uint32_t> hs;
@@ -913,10 +1053,13 @@ inline void BatchSimulation::launch(FnB& f) {
         }
     }
     const uint32_t S = (uint32_t)order_.size();
-    if (!S || !f.tiles) return;
-    uint32_t grid = f.tiles;
+    // changing populations: the tile count is on the device (sentinel record); the grid
+    // covers the list's capacity
+    const uint32_t tiles = dynamic_ ? tile_bound(l.cap, tuning_.block * tuning_.agents_per_thread) : f.tiles;
+    if (!S || !tiles) return;
+    uint32_t grid = tiles;
     if (tuning_.blocks_per_cu) grid = std::min<uint32_t>(grid, tuning_.blocks_per_cu * (uint32_t)cu_count_);
-    f.d->launch_seg(a, f.d_segs, S, f.tiles, tuning_.agents_per_thread, grid, tuning_.block, stream_);
+    f.d->launch_seg(a, f.d_segs, S, tuning_.agents_per_thread, grid, tuning_.block, stream_);
     if (hipGetLastError() != hipSuccess) throw exception::CUDAError("batched agent function launch failed: " + f.d->name);
 }
 
@@ -927,7 +1070,7 @@ inline void BatchSimulation::enqueue_step() {
             FnB& f = fns_[fn_index_.at(fd.get())];
             if (!f.in || !f.in->spatial) continue;
             AgentB& a = *f.agent;
-            if (!a.sort_msg.empty() && a.d->sortPeriod == 1 && a.list.total) {
+            if (!a.sort_msg.empty() && a.d->sortPeriod == 1 && (dynamic_ ? a.list.cap : a.list.total)) {
                 List& l = a.list;
                 const int ix = l.var_index("x"), iy = l.var_index("y"), iz = l.var_index("z");
                 std::vector<const void*> src;
@@ -939,7 +1082,9 @@ inline void BatchSimulation::enqueue_step() {
                     by.push_back(v.bytes);
                 }
                 const MsgB& sm = msgs_.at(a.sort_msg);
-                detail::fgx_check(fgx_agent_sort_seg(ctx_, a.d_keys, (uint32_t)order_.size(), a.tiles, l.total, a.keys_total,
+                detail::fgx_check(fgx_agent_sort_seg(ctx_, a.d_keys, (uint32_t)order_.size(),
+                                                     dynamic_ ? tile_bound(l.cap, 2048) : a.tiles, dynamic_ ? l.cap : l.total,
+                                                     dynamic_ ? l.d_total : nullptr, a.keys_total,
                                                      (const float*)l.vars[ix].cur, (const float*)l.vars[iy].cur,
                                                      sm.dims == 3 ? (const float*)l.vars[iz].cur : nullptr,
                                                      (uint32_t)l.vars.size(), src.data(), dst.data(), by.data(), a.scratch,
@@ -959,7 +1104,10 @@ inline void BatchSimulation::enqueue_step() {
                     dst.push_back(v.alt);
                     by.push_back(v.bytes);
                 }
-                detail::fgx_check(fgx_spatial_build_seg(ctx_, m.d_keys, (uint32_t)order_.size(), m.tiles, l.total, m.keys_total,
+                detail::fgx_check(fgx_spatial_build_seg(ctx_, m.d_keys, (uint32_t)order_.size(),
+                                                        dynamic_ ? tile_bound(l.cap, 2048) : m.tiles,
+                                                        dynamic_ ? l.cap : l.total, dynamic_ ? l.d_total : nullptr,
+                                                        m.keys_total,
                                                         (const float*)l.vars[ix].cur, (const float*)l.vars[iy].cur,
                                                         m.dims == 3 ? (const float*)l.vars[iz].cur : nullptr,
                                                         (uint32_t)l.vars.size(), src.data(), dst.data(), by.data(), m.pbm,
@@ -972,10 +1120,7 @@ inline void BatchSimulation::enqueue_step() {
         for (auto& fd : layer->agent_functions) {
             FnB& f = fns_[fn_index_.at(fd.get())];
             launch(f);
-            if (f.out) {
-                for (auto& v : f.out->list.vars) std::swap(v.cur, v.alt);
-                f.out->dirty = true;
-            }
+            after_launch(f);
         }
     }
     for (auto& kv : msgs_) kv.second.dirty = true;
@@ -1024,6 +1169,7 @@ inline void BatchSimulation::step() {
     }
     if (!all_active)
         throw exception::InvalidOperation("BatchSimulation::step: finished simulations are still resident; remove() them first");
+    if (dynamic_) ensure_step_capacity();
     // Graphs only once the batch has been stable for two steps: under churn the
     // capture would be thrown away at the next repack.
     if (graphs_on_ && stable_steps_ >= 2) {
@@ -1055,6 +1201,285 @@ inline void BatchSimulation::step() {
     ++stable_steps_;
 }
 
+// ------------------------------------------------------------------ changing populations
+inline void BatchSimulation::upload_layout(List& l, uint32_t S) {
+    if (S) {
+        detail::fgx_check(fgx_memcpy_h2d(l.d_cnt, l.cnt.data(), (size_t)S * 4, stream_), "list counts");
+        detail::fgx_check(fgx_memcpy_h2d(l.d_off, l.off.data(), (size_t)S * 4, stream_), "list offsets");
+    }
+    detail::fgx_check(fgx_memcpy_h2d(l.d_total, &l.total, 4, stream_), "list total");
+    synchronize();   // the host vectors may change before the copies run otherwise
+}
+
+// Per-simulation counts, offsets and next IDs as the device holds them (before a repack).
+inline void BatchSimulation::read_counts() {
+    const uint32_t S = (uint32_t)order_.size();
+    if (!S) return;
+    for (auto& kv : agents_) {
+        List& l = kv.second.list;
+        std::vector<uint32_t> cnt(S), off(S), ids(S);
+        detail::fgx_check(fgx_memcpy_d2h(cnt.data(), l.d_cnt, (size_t)S * 4, stream_), "count readback");
+        detail::fgx_check(fgx_memcpy_d2h(off.data(), l.d_off, (size_t)S * 4, stream_), "offset readback");
+        detail::fgx_check(fgx_memcpy_d2h(ids.data(), kv.second.d_next_id, (size_t)S * 4, stream_), "next ID readback");
+        synchronize();
+        uint64_t t = 0;
+        for (uint32_t k = 0; k < S; ++k) {
+            Sim& sm = sims_.at(order_[k]);
+            sm.n[kv.first] = cnt[k];
+            sm.next_id[kv.first] = ids[k];
+            t += cnt[k];
+        }
+        l.cnt = cnt;
+        l.off = off;
+        l.total = (uint32_t)t;
+    }
+}
+
+inline std::vector<fgx_seg_target> BatchSimulation::agent_targets(AgentB& a) {
+    std::vector<fgx_seg_target> t;
+    const uint32_t tile = tuning_.block * tuning_.agents_per_thread;
+    for (auto& f : fns_) {
+        if (f.agent != &a || !f.d_segs) continue;
+        t.push_back({f.d_segs, (uint32_t)sizeof(detail::SegDev), (int32_t)offsetof(detail::SegDev, agent_off),
+                     (int32_t)offsetof(detail::SegDev, n), (int32_t)offsetof(detail::SegDev, tile_off), tile, 1});
+    }
+    if (!a.sort_msg.empty() && a.d_keys)
+        t.push_back({a.d_keys, (uint32_t)sizeof(fgx_seg_key), (int32_t)offsetof(fgx_seg_key, elem_off),
+                     (int32_t)offsetof(fgx_seg_key, n), (int32_t)offsetof(fgx_seg_key, tile_off), 2048u, 0});
+    return t;
+}
+
+inline std::vector<fgx_seg_target> BatchSimulation::message_targets(MsgB& m) {
+    std::vector<fgx_seg_target> t;
+    if (m.spatial && m.d_keys)
+        t.push_back({m.d_keys, (uint32_t)sizeof(fgx_seg_key), (int32_t)offsetof(fgx_seg_key, elem_off),
+                     (int32_t)offsetof(fgx_seg_key, n), (int32_t)offsetof(fgx_seg_key, tile_off), 2048u, 0});
+    for (auto& f : fns_) {
+        if (f.in != &m || !f.d_segs) continue;
+        t.push_back({f.d_segs, (uint32_t)sizeof(detail::SegDev), (int32_t)offsetof(detail::SegDev, msg_off),
+                     (int32_t)offsetof(detail::SegDev, msg_n), -1, 0u, 0});
+    }
+    return t;
+}
+
+// New per-simulation counts (plus d_add) -> offsets, total and every table of the list.
+inline void BatchSimulation::relayout(List& l, const std::vector<fgx_seg_target>& t, const uint32_t* d_add) {
+    const uint32_t S = (uint32_t)order_.size();
+    size_t k = 0;
+    do {
+        const uint32_t nt = (uint32_t)std::min<size_t>(t.size() - k, FGX_SEG_MAX_TARGETS);
+        detail::fgx_check(fgx_seg_layout(S, l.d_cnt, k == 0 ? d_add : nullptr, l.d_off, l.d_total, nt ? t.data() + k : nullptr,
+                                         nt, stream_),
+                          "batch layout");
+        k += nt;
+    } while (k < t.size());
+}
+
+// After an agent function of a batch whose populations change (CUDASimulation's unmap
+// order: message output, death, births): the message list takes the layout the agents had
+// during the launch; births are counted and compacted against the parents' layout before
+// the deaths change it; then each simulation's survivors and new agents are packed.
+inline void BatchSimulation::after_launch(FnB& f) {
+    const uint32_t S = (uint32_t)order_.size();
+    List& A = f.agent->list;
+    if (f.out) {
+        for (auto& v : f.out->list.vars) std::swap(v.cur, v.alt);
+        f.out->dirty = true;
+        if (dynamic_ && S) {
+            detail::fgx_check(fgx_memcpy_d2d(f.out->list.d_cnt, A.d_cnt, (size_t)S * 4, stream_), "message layout");
+            relayout(f.out->list, message_targets(*f.out), nullptr);
+        }
+    }
+    if (!dynamic_ || !S) return;
+    auto ptrs = [](std::vector<detail::VarBuf>& vars, bool alt) {
+        std::vector<void*> p;
+        for (auto& v : vars) p.push_back(alt ? v.alt : v.cur);
+        return p;
+    };
+    auto bytes = [](const std::vector<detail::VarBuf>& vars) {
+        std::vector<uint32_t> b;
+        for (auto& v : vars) b.push_back(v.bytes);
+        return b;
+    };
+    if (f.out_agent) {
+        AgentB& B = *f.out_agent;
+        detail::fgx_check(fgx_seg_count(S, A.d_off, A.d_cnt, f.birth_flags, 1u, B.d_born, stream_), "birth counts");
+        const auto src = ptrs(f.newbuf, false);
+        const auto dst = ptrs(f.newbuf, true);
+        const auto by = bytes(f.newbuf);
+        std::vector<const void*> csrc(src.begin(), src.end());
+        detail::fgx_check(fgx_compact(ctx_, f.birth_flags, 1, A.d_total, A.cap, (uint32_t)by.size(), csrc.data(), dst.data(),
+                                      by.data(), d_word_, f.birth_scratch, f.birth_scratch_bytes, stream_),
+                          "new agents");
+    }
+    if (f.d->has_agent_death) {
+        detail::fgx_check(fgx_seg_count(S, A.d_off, A.d_cnt, f.death_flags, 1u, A.d_cnt, stream_), "survivor counts");
+        const auto src = ptrs(A.vars, false);
+        const auto dst = ptrs(A.vars, true);
+        const auto by = bytes(A.vars);
+        std::vector<const void*> csrc(src.begin(), src.end());
+        detail::fgx_check(fgx_compact(ctx_, f.death_flags, 1, A.d_total, A.cap, (uint32_t)by.size(), csrc.data(), dst.data(),
+                                      by.data(), A.d_total, f.death_scratch, f.death_scratch_bytes, stream_),
+                          "death compaction");
+        for (auto& v : A.vars) std::swap(v.cur, v.alt);
+        relayout(A, agent_targets(*f.agent), nullptr);
+    }
+    if (f.out_agent) {
+        AgentB& B = *f.out_agent;
+        List& L = B.list;
+        const auto cur = ptrs(L.vars, false);
+        const auto alt = ptrs(L.vars, true);
+        const auto born = ptrs(f.newbuf, true);
+        const auto by = bytes(L.vars);
+        std::vector<const void*> ccur(cur.begin(), cur.end()), cborn(born.begin(), born.end());
+        detail::fgx_check(fgx_seg_append(S, L.d_off, L.d_cnt, B.d_born, L.cap, (uint32_t)by.size(), ccur.data(), cborn.data(),
+                                         alt.data(), by.data(), L.id_slot, B.d_next_id, B.d_work, stream_),
+                          "birth append");
+        for (auto& v : L.vars) std::swap(v.cur, v.alt);
+        relayout(L, agent_targets(B), B.d_born);
+    }
+}
+
+inline void BatchSimulation::grow_agent_list(AgentB& a, uint32_t cap) {
+    List& l = a.list;
+    if (cap <= l.cap) return;
+    synchronize();
+    for (auto& v : l.vars) {
+        void *nc = nullptr, *na = nullptr;
+        alloc(&nc, (size_t)cap * v.bytes);
+        alloc(&na, (size_t)cap * v.bytes);
+        if (v.cur && l.cap) detail::fgx_check(fgx_memcpy_d2d(nc, v.cur, (size_t)l.cap * v.bytes, stream_), "list growth");
+        synchronize();
+        free_dev(v.cur);
+        free_dev(v.alt);
+        v.cur = nc;
+        v.alt = na;
+    }
+    l.cap = cap;
+}
+
+inline void BatchSimulation::size_pop_buffers(FnB& f) {
+    const uint32_t cap = std::max<uint32_t>(f.agent->list.cap, 1);
+    if (f.out && cap > f.flags_cap) {
+        free_typed(f.msg_flags);
+        f.flags_cap = cap;
+        void* p = nullptr;
+        alloc(&p, (size_t)cap * 4);
+        f.msg_flags = (uint32_t*)p;
+    }
+    if (cap <= f.pop_cap) return;
+    void* p = nullptr;
+    size_t sb = 0;
+    detail::fgx_check(fgx_scratch_bytes(FGX_OP_COMPACT, cap, 0, &sb), "compaction scratch");
+    if (f.d->has_agent_death) {
+        free_typed(f.death_flags);
+        alloc(&p, (size_t)cap * 4);
+        f.death_flags = (uint32_t*)p;
+        free_dev(f.death_scratch);
+        alloc(&f.death_scratch, sb);
+        f.death_scratch_bytes = sb;
+    }
+    if (f.out_agent) {
+        free_typed(f.birth_flags);
+        alloc(&p, (size_t)cap * 4);
+        f.birth_flags = (uint32_t*)p;
+        for (auto& v : f.newbuf) {
+            free_dev(v.cur);
+            free_dev(v.alt);
+            alloc(&v.cur, (size_t)cap * v.bytes);
+            alloc(&v.alt, (size_t)cap * v.bytes);
+        }
+        free_dev(f.birth_scratch);
+        alloc(&f.birth_scratch, sb);
+        f.birth_scratch_bytes = sb;
+    }
+    f.pop_cap = cap;
+}
+
+inline void BatchSimulation::size_sort_scratch(AgentB& a) {
+    if (a.sort_msg.empty()) return;
+    size_t sb = 0;
+    detail::fgx_check(fgx_scratch_bytes(FGX_OP_AGENT_SORT, std::max<uint32_t>(a.list.cap, 1), a.keys_total, &sb),
+                      "scratch sizing");
+    if (sb <= a.scratch_bytes) return;
+    free_dev(a.scratch);
+    sb += sb / 2;
+    alloc(&a.scratch, sb);
+    detail::fgx_check(fgx_scratch_clear(a.scratch, sb, stream_), "scratch clear");
+    a.scratch_bytes = sb;
+}
+
+inline void BatchSimulation::size_build_scratch(MsgB& m) {
+    if (!m.spatial) return;
+    size_t sb = 0;
+    detail::fgx_check(fgx_scratch_bytes(FGX_OP_SPATIAL_BUILD, std::max<uint32_t>(m.list.cap, 1), m.keys_total, &sb),
+                      "scratch sizing");
+    if (sb <= m.scratch_bytes) return;
+    free_dev(m.scratch);
+    sb += sb / 2;
+    alloc(&m.scratch, sb);
+    detail::fgx_check(fgx_scratch_clear(m.scratch, sb, stream_), "scratch clear");
+    m.scratch_bytes = sb;
+}
+
+// Before a step of a batch whose populations change: read the totals back and grow every
+// list to the step's bound (births add at most one agent per parent and function), as
+// CUDASimulation::ensure_capacity does for one simulation.
+inline void BatchSimulation::ensure_step_capacity() {
+    std::map<AgentB*, uint64_t> ub, peak;
+    for (auto& kv : agents_) {
+        uint32_t t = 0;
+        detail::fgx_check(fgx_memcpy_d2h(&t, kv.second.list.d_total, 4, stream_), "total readback");
+        synchronize();
+        kv.second.list.total = t;
+        ub[&kv.second] = t;
+        peak[&kv.second] = t;
+    }
+    std::map<MsgB*, uint64_t> mub;
+    for (auto& layer : m_->layers)
+        for (auto& fd : layer->agent_functions) {
+            FnB& f = fns_[fn_index_.at(fd.get())];
+            const uint64_t n = ub[f.agent];
+            if (f.out) mub[f.out] = std::max(mub[f.out], n);
+            if (f.out_agent) {
+                ub[f.out_agent] += n;
+                peak[f.out_agent] = std::max(peak[f.out_agent], ub[f.out_agent]);
+            }
+        }
+    bool changed = false;
+    for (auto& kv : peak) {
+        AgentB& a = *kv.first;
+        if (kv.second >= (1ull << 31)) throw exception::InvalidOperation("BatchSimulation: batch exceeds the 32-bit index range");
+        if (kv.second > a.list.cap) {
+            grow_agent_list(a, (uint32_t)std::max<uint64_t>(kv.second, (uint64_t)a.list.cap + a.list.cap / 2));
+            changed = true;
+        }
+        size_sort_scratch(a);
+    }
+    for (auto& kv : mub) {
+        MsgB& m = *kv.first;
+        if (kv.second > m.list.cap) {
+            synchronize();
+            const uint32_t cap = (uint32_t)std::max<uint64_t>(kv.second, (uint64_t)m.list.cap + m.list.cap / 2);
+            for (auto& v : m.list.vars) {
+                free_dev(v.cur);
+                free_dev(v.alt);
+                alloc(&v.cur, (size_t)cap * v.bytes);
+                alloc(&v.alt, (size_t)cap * v.bytes);
+            }
+            m.list.cap = cap;
+            changed = true;
+        }
+        size_build_scratch(m);
+    }
+    for (auto& f : fns_) {
+        const uint32_t before = f.pop_cap;
+        size_pop_buffers(f);
+        changed = changed || f.pop_cap != before;
+    }
+    if (changed) clear_graphs();
+}
+
 inline void BatchSimulation::check_errors(const Sim& s, uint32_t slot) {
     uint32_t e[4] = {0, 0, 0, 0};
     detail::fgx_check(fgx_memcpy_d2h(e, d_errors_ + 4 * slot, 16, stream_), "error readback");
@@ -1079,11 +1504,16 @@ inline void BatchSimulation::getPopulationData(int h, AgentVector& pop) {
     auto it = agents_.find(pop.getAgentName());
     if (it == agents_.end()) throw exception::InvalidAgentName("BatchSimulation: unknown agent");
     const List& l = it->second.list;
-    const uint32_t n = l.cnt[slot];
+    uint32_t n = l.cnt[slot], o = l.off[slot];
+    if (dynamic_) {
+        detail::fgx_check(fgx_memcpy_d2h(&n, l.d_cnt + slot, 4, stream_), "count readback");
+        detail::fgx_check(fgx_memcpy_d2h(&o, l.d_off + slot, 4, stream_), "offset readback");
+        synchronize();
+    }
     pop.resize(n);
     for (auto& v : l.vars)
         if (n)
-            detail::fgx_check(fgx_memcpy_d2h(pop.raw(v.name).data(), (const char*)v.cur + (size_t)l.off[slot] * v.bytes,
+            detail::fgx_check(fgx_memcpy_d2h(pop.raw(v.name).data(), (const char*)v.cur + (size_t)o * v.bytes,
                                              (size_t)n * v.bytes, stream_),
                               "population download");
     synchronize();

@@ -201,6 +201,7 @@ enum DeviceErrorCode : uint32_t {
     DERR_ARRAY_MSG_INDEX = 17,
     DERR_ARRAY_CONFLICT = 18,
     DERR_ARRAY_RADIUS = 19,
+    DERR_AGENT_OUT_ID_BATCH = 20,
 };
 
 inline const char* device_error_text(uint32_t code) {
@@ -213,6 +214,7 @@ inline const char* device_error_text(uint32_t code) {
         case DERR_ARRAY_MSG_INDEX: return "array message index out of bounds (MessageArray*::setIndex / iteration origin)";
         case DERR_ARRAY_CONFLICT: return "multiple messages output to the same array message index, or more messages than array elements";
         case DERR_ARRAY_RADIUS: return "invalid radius for array message iteration";
+        case DERR_AGENT_OUT_ID_BATCH: return "AgentOut::getID() is not available in batched simulations";
         case DERR_WRAP_OUT_OF_BOUNDS: return "location exceeds environment bounds, unsupported by the wrapped iterator (MessageSpatial::In::wrap())";
         case DERR_WRAP_NOT_FACTOR: return "spatial messaging radius is not a factor of the environment dimensions, unsupported by the wrapped iterator";
         case DERR_DEATH_DISABLED: return "agent death must be enabled per agent function when defining the model";

@@ -62,8 +62,8 @@ struct LayerKernel {
     int (*index_of)(LaunchFn launch) = nullptr;          // a function's index in it, or -1
     uint32_t (*lds_bytes)(int index, uint32_t block) = nullptr;   // its dynamic LDS per block
 };
-typedef void (*LaunchSegFn)(const FnArgs& args, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t apt,
-                            uint32_t grid, uint32_t block, void* stream);
+typedef void (*LaunchSegFn)(const FnArgs& args, const SegDev* segs, uint32_t nsegs, uint32_t apt, uint32_t grid,
+                            uint32_t block, void* stream);
 template <typename C>
 void launch_condition(const FnArgs& a, uint32_t* flags, uint32_t grid, void* stream);
 template <typename T>
@@ -456,8 +456,8 @@ namespace detail {
 template <typename F>
 void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, uint32_t smem, void* stream);
 template <typename F>
-void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t apt,
-                               uint32_t grid, uint32_t block, void* stream);
+void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nsegs, uint32_t apt, uint32_t grid,
+                               uint32_t block, void* stream);
 }
 
 // ------------------------------------------------------------------ agents

@@ -1450,6 +1450,9 @@ class DeviceAPI {
             const bool on = c_.a.birth_flags != nullptr;
             c_.err.note(!on, detail::DERR_AGENT_OUTPUT_DISABLED, 0);
             if (!on) return ID_NOT_SET;
+            // batched launches number each simulation's children after the launch
+            c_.err.note(c_.a.d_id_slots == nullptr, detail::DERR_AGENT_OUT_ID_BATCH, 0);
+            if (!c_.a.d_id_slots) return ID_NOT_SET;
             c_.a.birth_flags[c_.lidx] = 1u;
             *c_.a.d_id_slots = 1u;
             return *c_.a.d_new_next_id + c_.lidx;
@@ -1693,7 +1696,10 @@ __global__ void __launch_bounds__(256) layer_kernel(const char* __restrict__ ima
 // XCD-contiguous order) runs tiles [b*T/G, (b+1)*T/G).
 template <typename F>
 __global__ void __launch_bounds__(1024) agent_function_kernel_seg(const FnArgs a, const SegDev* __restrict__ segs,
-                                                                  uint32_t nsegs, uint32_t ntiles, uint32_t apt) {
+                                                                  uint32_t nsegs, uint32_t apt) {
+    // segs[nsegs] is the sentinel record: its tile_off is the batch's tile count, kept on
+    // the device when births and deaths change the populations
+    const uint32_t ntiles = segs[nsegs].tile_off;
     const uint32_t lb = xcd_block();
     const uint32_t t0 = (uint32_t)((uint64_t)lb * ntiles / gridDim.x);
     const uint32_t t1 = (uint32_t)((uint64_t)(lb + 1) * ntiles / gridDim.x);
@@ -1925,10 +1931,10 @@ void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, uint3
 }
 
 template <typename F>
-void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nsegs, uint32_t ntiles, uint32_t apt,
-                               uint32_t grid, uint32_t block, void* stream) {
+void launch_agent_function_seg(const FnArgs& a, const SegDev* segs, uint32_t nsegs, uint32_t apt, uint32_t grid,
+                               uint32_t block, void* stream) {
     hipLaunchKernelGGL(agent_function_kernel_seg<F>, dim3(grid), dim3(block), fn_lds_bytes<F>(block), (hipStream_t)stream,
-                       a, segs, nsegs, ntiles, apt);
+                       a, segs, nsegs, apt);
 }
 
 // Host view of the layer kernel over the functions Fs (see LayerKernel).

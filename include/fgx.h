@@ -368,17 +368,22 @@ int fgx_seg_key_init(int kind, int dims, const float* min, const float* max, flo
 int fgx_segments_pack(fgx_seg_key* segs, uint32_t nsegs, uint32_t* n_total, uint32_t* keys_total, uint32_t* tiles_total);
 /* Segmented message-list build: d_segs (device copy of the packed table) with kind
  * FGX_SEG_MESSAGE_BINS; pbm holds keys_total + 1 entries of absolute message indices.
+ * n_total bounds the packed elements (and sizes the scratch); d_n_total, when given, is
+ * the device-resident count (populations that change on the device, whose tables
+ * fgx_seg_layout keeps; tiles_total then bounds the tiles).
  * Scratch: fgx_scratch_bytes(FGX_OP_SPATIAL_BUILD, n_total, keys_total). */
 int fgx_spatial_build_seg(fgx_ctx* ctx, const fgx_seg_key* d_segs, uint32_t nsegs, uint32_t tiles_total,
-                          uint32_t n_total, uint32_t keys_total, const float* x, const float* y, const float* z,
-                          uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
-                          uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream);
-/* Segmented agent sort (kind FGX_SEG_AGENT_SORT).  Scratch:
+                          uint32_t n_total, const uint32_t* d_n_total, uint32_t keys_total, const float* x,
+                          const float* y, const float* z, uint32_t nvars, const void* const* vars_in,
+                          void* const* vars_out, const uint32_t* var_bytes, uint32_t* pbm, void* scratch,
+                          size_t scratch_bytes, void* stream);
+/* Segmented agent sort (kind FGX_SEG_AGENT_SORT), counts as above.  Scratch:
  * fgx_scratch_bytes(FGX_OP_AGENT_SORT, n_total, keys_total). */
 int fgx_agent_sort_seg(fgx_ctx* ctx, const fgx_seg_key* d_segs, uint32_t nsegs, uint32_t tiles_total,
-                       uint32_t n_total, uint32_t keys_total, const float* x, const float* y, const float* z,
-                       uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
-                       void* scratch, size_t scratch_bytes, void* stream);
+                       uint32_t n_total, const uint32_t* d_n_total, uint32_t keys_total, const float* x,
+                       const float* y, const float* z, uint32_t nvars, const void* const* vars_in,
+                       void* const* vars_out, const uint32_t* var_bytes, void* scratch, size_t scratch_bytes,
+                       void* stream);
 /* p[i] += delta for i < n (per-simulation step counters of a batch). */
 int fgx_add_u32_n(uint32_t* p, uint32_t n, uint32_t delta, void* stream);
 /* Copies element ranges of several variables in one launch: range r moves
@@ -387,6 +392,42 @@ int fgx_add_u32_n(uint32_t* p, uint32_t n, uint32_t delta, void* stream);
  * sum of the range lengths (repacking a batch when simulations join or leave). */
 int fgx_copy_ranges(uint32_t nvars, const void* const* src, void* const* dst, const uint32_t* var_bytes,
                     const uint32_t* d_ranges, uint32_t nranges, uint32_t n_total, void* stream);
+
+/* ---------------------------------------------------------------- batches whose populations change
+ * A batch packs its simulations' lists one after another; when deaths and births change
+ * the populations, each simulation's count stays on the device (reference: each
+ * simulation of an ensemble compacts and appends on its own, CUDAFatAgent.cu:105-137 and
+ * CUDAAgentStateList.cu:189-250).  A stable compaction of the whole packed list keeps
+ * every simulation's survivors together and in order; these entries recount the
+ * simulations, append each one's new agents after its own, and patch every table that
+ * records the layout (per-simulation offset, count and first tile). */
+typedef struct {
+    void* table;          /* device records, one per simulation (+ 1 sentinel record when sentinel != 0) */
+    uint32_t stride;      /* bytes per record */
+    int32_t off_field;    /* byte offset of the uint32 element offset in a record, or -1 */
+    int32_t n_field;      /* byte offset of the uint32 element count, or -1 */
+    int32_t tile_field;   /* byte offset of the uint32 first tile, or -1 */
+    uint32_t tile_elems;  /* elements per tile when tile_field >= 0 */
+    int32_t sentinel;     /* record nsegs receives the totals (offset = elements, tile = tiles, count 0) */
+} fgx_seg_target;
+#define FGX_SEG_MAX_TARGETS 8
+/* d_cnt[s] (+ d_add[s] when d_add is given, stored back into d_cnt) -> d_off[s] (exclusive
+ * offsets), *d_total, and the fields of every target record.  One block of 1024 threads. */
+int fgx_seg_layout(uint32_t nsegs, uint32_t* d_cnt, const uint32_t* d_add, uint32_t* d_off, uint32_t* d_total,
+                   const fgx_seg_target* targets, uint32_t ntargets, void* stream);
+/* d_out[s] = flags equal to `keep` in [d_off[s], d_off[s] + d_cnt[s]); d_out may be d_cnt. */
+int fgx_seg_count(uint32_t nsegs, const uint32_t* d_off, const uint32_t* d_cnt, const uint32_t* flags, uint32_t keep,
+                  uint32_t* d_out, void* stream);
+/* Writes the list with each simulation's d_born[s] new agents after its own d_cnt[s]
+ * agents into list_out (capacity n_cap_out elements).  born_in holds the new agents of all
+ * simulations compacted in order (a stable fgx_compact of the new-agent buffers by birth
+ * flag), so simulation s's are at the exclusive prefix of d_born.  With id_var >= 0, new
+ * agent r of simulation s gets ID d_next_id[s] + r and d_next_id[s] advances by d_born[s]
+ * (CUDAAgentStateList::scatterNew numbering, per simulation).  d_work: 2 * nsegs + 2 words.
+ * The caller then runs fgx_seg_layout with d_add = d_born. */
+int fgx_seg_append(uint32_t nsegs, const uint32_t* d_off, const uint32_t* d_cnt, const uint32_t* d_born, uint32_t n_cap_out,
+                   uint32_t nvars, const void* const* list_in, const void* const* born_in, void* const* list_out,
+                   const uint32_t* var_bytes, int32_t id_var, uint32_t* d_next_id, uint32_t* d_work, void* stream);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,7 @@
 //   transition_src0_dest10 / transition_src10_dest0 / transition_src10_dest10
 //                       test_agent_state_transition.cu:67-250
 //   concurrency         tests/test_cases/simulation/test_cuda_simulation_concurrency.cu:322-390
+//   two_radii           one list sorted for two spatial inputs of different radii
 //   ensemble_simulate / ensemble_errors
 //                       tests/test_cases/simulation/test_cuda_ensemble.cu:251-322, 630-748
 //   bruteforce          tests/test_cases/runtime/messaging/test_brute_force.cu:73-200
@@ -270,6 +271,36 @@ FLAMEGPU_EXIT_FUNCTION(simulateExit) {
 // every second run fails (the reference counts calls in a static; here the run's seed decides)
 FLAMEGPU_STEP_FUNCTION(throwException) {
     if (FLAMEGPU->random.getSeed() % 2 == 1) throw flamegpu::exception::UnknownInternalError("Dummy Exception");
+}
+
+// one agent list sorted by two functions whose spatial inputs have different radii (so
+// different bin counts share the list's sort scratch)
+FLAMEGPU_AGENT_FUNCTION(out_loc, MessageNone, MessageSpatial3D) {
+    FLAMEGPU->message_out.setVariable<flamegpu::id_t>("id", FLAMEGPU->getID());
+    FLAMEGPU->message_out.setLocation(FLAMEGPU->getVariable<float>("x"), FLAMEGPU->getVariable<float>("y"),
+                                      FLAMEGPU->getVariable<float>("z"));
+    return ALIVE;
+}
+template <int R10>
+__device__ __forceinline__ unsigned int count_within(flamegpu::DeviceAPI<MessageSpatial3D, MessageNone>* FLAMEGPU) {
+    const float x = FLAMEGPU->getVariable<float>("x"), y = FLAMEGPU->getVariable<float>("y"),
+                z = FLAMEGPU->getVariable<float>("z");
+    const float r = R10 / 10.0f;
+    unsigned int c = 0;
+    for (const auto& m : FLAMEGPU->message_in(x, y, z)) {
+        const float dx = m.getVariable<float>("x") - x, dy = m.getVariable<float>("y") - y,
+                    dz = m.getVariable<float>("z") - z;
+        c += dx * dx + dy * dy + dz * dz <= r * r ? 1u : 0u;
+    }
+    return c;
+}
+FLAMEGPU_AGENT_FUNCTION(in_near, MessageSpatial3D, MessageNone) {
+    FLAMEGPU->setVariable<unsigned int>("near", count_within<10>(FLAMEGPU));
+    return ALIVE;
+}
+FLAMEGPU_AGENT_FUNCTION(in_far, MessageSpatial3D, MessageNone) {
+    FLAMEGPU->setVariable<unsigned int>("far", count_within<25>(FLAMEGPU));
+    return ALIVE;
 }
 
 FLAMEGPU_AGENT_FUNCTION(AddOne, MessageNone, MessageNone) {
@@ -578,6 +609,32 @@ flamegpu::ModelDescription fgxm_build_model(const char* variant_c, const float* 
         } else {
             model.addStepFunction(throwException);
         }
+    } else if (v == "two_radii") {
+        for (auto spec : {std::make_pair("near", 1.0f), std::make_pair("far", 2.5f)}) {
+            auto m = model.newMessage<MessageSpatial3D>(spec.first);
+            m.setMin(0, 0, 0);
+            m.setMax(10, 10, 10);
+            m.setRadius(spec.second);
+            m.newVariable<flamegpu::id_t>("id");
+        }
+        auto agent = model.newAgent("agent");
+        agent.newVariable<float>("x");
+        agent.newVariable<float>("y");
+        agent.newVariable<float>("z");
+        agent.newVariable<unsigned int>("near");
+        agent.newVariable<unsigned int>("far");
+        auto o1 = agent.newFunction("out_near", out_loc);
+        o1.setMessageOutput("near");
+        auto o2 = agent.newFunction("out_far", out_loc);
+        o2.setMessageOutput("far");
+        auto i1 = agent.newFunction("in_near", in_near);
+        i1.setMessageInput("near");
+        auto i2 = agent.newFunction("in_far", in_far);
+        i2.setMessageInput("far");
+        model.newLayer().addAgentFunction(o1);
+        model.newLayer().addAgentFunction(o2);
+        model.newLayer().addAgentFunction(i1);
+        model.newLayer().addAgentFunction(i2);
     } else if (v == "bruteforce") {
         auto message = model.newMessage("msg");
         message.newVariable<int>("x");

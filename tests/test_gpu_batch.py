@@ -279,11 +279,96 @@ def test_batched_graph_and_eager_agree():
             assert np.array_equal(x, y)
 
 
-def test_batch_rejects_growing_models():
+def test_batch_rejects_state_transitions():
     from flamegpu2_amd.fgx import FgxError
-    params, prey, pred = workloads.predprey(100, 10, seed=1, width=100.0)
-    with pytest.raises(FgxError):
-        Batch("predator_prey", params)
+    with pytest.raises(FgxError, match="state"):
+        Batch("kat_models", [], variant="transition")
+
+
+# ---------------------------------------------------------------- populations that change
+# Predator-prey in a batch: deaths (prey eaten, predators starving) and births (both
+# types) over the whole batch, each simulation's counts kept on the device.  Every
+# simulation must give the same bits as alone, whatever it is batched with.
+
+
+def _predprey_sims(k=5):
+    out = []
+    rng = np.random.default_rng(17)
+    for j in range(k):
+        n_prey = int(rng.integers(400, 5000))
+        width = float(rng.choice([80.0, 120.0, 200.0]))
+        params, prey, pred = workloads.predprey(n_prey, max(n_prey // 4, 1), seed=40 + j, width=width)
+        out.append((params, prey, pred, 60 + j))
+    return out
+
+
+def _predprey_alone(p, prey, pred, seed, steps):
+    sim = Simulation("predator_prey", p, seed=seed)
+    sim.set_agents("Prey", prey)
+    sim.set_agents("Predator", pred)
+    sim.step(steps)
+    out = (sim.get_agents("Prey"), sim.get_agents("Predator"))
+    sim.close()
+    return out
+
+
+def _same(got, exp, what):
+    (ga, gid), (ea, eid) = got, exp
+    assert len(gid) == len(eid), f"{what}: population {len(gid)} vs {len(eid)} alone"
+    assert np.array_equal(gid, eid), f"{what}: ID sequence differs from the run alone"
+    for k, (g, e) in enumerate(zip(ga, ea)):
+        assert np.array_equal(g, e), f"{what}: variable {k} differs from the run alone"
+
+
+@pytest.mark.parametrize("graphs", [1, 0])
+def test_batched_predprey_same_as_alone(graphs):
+    sims = _predprey_sims()
+    b = Batch("predator_prey", sims[0][0])
+    b.set_option("graphs", graphs)
+    plan = {}
+    for (p, prey, pred, seed), steps in zip(sims[:3], [12, 25, 7]):
+        plan[b.add(p, seed, steps, {"Prey": prey, "Predator": pred})] = (p, prey, pred, seed, steps)
+    b.step(3)
+    for (p, prey, pred, seed), steps in zip(sims[3:], [15, 9]):
+        plan[b.add(p, seed, steps, {"Prey": prey, "Predator": pred})] = (p, prey, pred, seed, steps)
+    results = {}
+    for _ in range(40):
+        for h in b.finished():
+            results[h] = (b.get_agents(h, "Prey"), b.get_agents(h, "Predator"))
+            b.remove(h)
+        if len(results) == len(plan):
+            break
+        b.step(1)
+    assert len(results) == len(plan)
+    changed = False
+    for h, (p, prey, pred, seed, steps) in plan.items():
+        got = results[h]
+        exp = _predprey_alone(p, prey, pred, seed, steps)
+        _same(got[0], exp[0], f"simulation {h} prey ({steps} steps)")
+        _same(got[1], exp[1], f"simulation {h} predators ({steps} steps)")
+        changed = changed or len(got[0][1]) != len(prey[0]) or len(got[1][1]) != len(pred[0])
+    assert changed   # the populations did change
+
+
+@pytest.mark.parametrize("tuning", [(256, 1, 0), (64, 2, 0), (128, 1, 16)])
+def test_batched_predprey_against_oracle(tuning):
+    sims = _predprey_sims(3)
+    b = Batch("predator_prey", sims[0][0])
+    b.set_tuning(*tuning)
+    hs = [b.add(p, seed, 20, {"Prey": prey, "Predator": pred}) for (p, prey, pred, seed) in sims]
+    b.step(20)
+    assert sorted(b.finished()) == sorted(hs)
+    for h, (p, prey, pred, seed) in zip(hs, sims):
+        ref = O.Sim(2, p, seed)
+        ref.set_agents(0, prey)
+        ref.set_agents(1, pred)
+        ref.step(20)
+        for t, agent in enumerate(("Prey", "Predator")):
+            got, gid = b.get_agents(h, agent)
+            exp, eid = ref.get_agents(t, [a.dtype for a in got])
+            assert np.array_equal(gid, eid), f"{agent}: IDs differ from the oracle"
+            for g, e in zip(got, exp):
+                assert np.array_equal(g, e), f"{agent}: values differ from the oracle"
 
 
 def test_serving_results_same_alone_and_in_the_mix():

@@ -323,6 +323,36 @@ def test_layer_concurrency_speedup(fusion):
         assert v[0] == 0 and np.all(np.isinf(v[1:]))
 
 
+def test_two_radii_sorts_graphs_and_eager():
+    """One agent list sorted, every step, for two spatial inputs of different radii: the
+    two sorts share the list's scratch with different bin counts.  Graph replays (several
+    steps, with eager reads between) must equal eager steps and a brute-force count."""
+    rng = np.random.default_rng(23)
+    n = 3000
+    P = rng.uniform(0, 10, size=(3, n)).astype(np.float32)
+    d = P[:, :, None] - P[:, None, :]
+    d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+    exp_near = (d2 <= np.float32(1.0) * np.float32(1.0)).sum(axis=1)
+    exp_far = (d2 <= np.float32(2.5) * np.float32(2.5)).sum(axis=1)
+    runs = {}
+    for graphs in (1, 0):
+        s = _sim("two_radii")
+        s.set_option("graphs", graphs)
+        z = np.zeros(n, np.uint32)
+        s.set_agents("agent", [P[0], P[1], P[2], z, z])
+        out = []
+        for _ in range(4):
+            s.step(1)
+            (x, y, zz, near, far), ids = s.get_agents("agent")
+            order = np.argsort(ids)
+            out.append((x[order], near[order], far[order]))
+            assert np.array_equal(near[order], exp_near) and np.array_equal(far[order], exp_far)
+        runs[graphs] = out
+    for a, b in zip(runs[1], runs[0]):
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
+
+
 def test_bruteforce_sum():
     n = 2000
     rng = np.random.default_rng(4)
