@@ -558,16 +558,19 @@ int fgx_layer_launch(fgx_ctx* ctx, const void* kernel, uint32_t block, const fgx
     h.nfns = nfns;
     h.stride = (args_bytes + 15u) & ~15u;
     uint32_t lds = 0;
-    uint64_t grid = 0;
+    uint64_t blocks = 0, per_xcd = 0;
     for (uint32_t k = 0; k < nfns; ++k) {
         if (!fns[k].args || fns[k].blocks == 0) return fail(FGX_ERR_ARG, "fgx_layer_launch: function without launch record or blocks");
-        h.begin[k] = (uint32_t)grid;
+        h.begin[k] = (uint32_t)blocks;
         h.fn[k] = fns[k].fn;
-        grid += fns[k].blocks;
+        blocks += fns[k].blocks;
+        per_xcd += (fns[k].blocks + 7u) / 8u;   // the most blocks of function k any XCD runs
         lds = std::max(lds, fns[k].lds_bytes);
     }
+    // the kernel spreads each function's blocks over the 8 XCDs (workgroup w on XCD w % 8)
+    const uint64_t grid = 8u * per_xcd;
     if (grid >= (1ull << 31)) return fail(FGX_ERR_ARG, "fgx_layer_launch: too many blocks");
-    h.begin[nfns] = (uint32_t)grid;
+    h.begin[nfns] = (uint32_t)blocks;
     const size_t bytes = FGX_LAYER_ARGS_OFFSET + (size_t)nfns * h.stride;
     std::string img(bytes, '\0');
     std::memcpy(&img[0], &h, sizeof(h));

@@ -1673,19 +1673,35 @@ __device__ __forceinline__ void run_layer_fn(uint32_t fn, const FnArgs& a, uint3
     ((fn == I ? run_function_block<Fs>(a, blk) : (void)0), ...);
 }
 
-// One launch for the functions of a layer (fgx_layer_launch builds the image): logical
-// block b belongs to function k with begin[k] <= b < begin[k + 1] and runs as that
-// function's block b - begin[k], with the function's launch record from the image.
+// One launch for the functions of a layer (fgx_layer_launch builds the image; function k
+// has begin[k + 1] - begin[k] blocks).  Each function's blocks are split evenly over the 8
+// XCDs, in contiguous chunks as xcd_block() gives a launch of its own (neighbouring agents
+// share an L2): workgroup w runs on XCD x = w % 8 as that XCD's block j = w / 8, and XCD x
+// runs function 0's chunk, then function 1's, and so on.  A whole-grid contiguous split
+// would put a small function's (mostly idle) blocks on some XCDs and the busy ones on others.
 template <typename... Fs>
 __global__ void __launch_bounds__(256) layer_kernel(const char* __restrict__ image) {
+    constexpr uint32_t kXcds = 8;
     const fgx_layer_header& h = *reinterpret_cast<const fgx_layer_header*>(image);
-    const uint32_t b = xcd_block();
+    const uint32_t x = blockIdx.x % kXcds;
+    uint32_t j = blockIdx.x / kXcds;
     const uint32_t nf = h.nfns;
-    uint32_t k = 0;
+    uint32_t k = FGX_LAYER_MAX_FNS, local = 0;
 #pragma unroll
-    for (uint32_t j = 1; j < FGX_LAYER_MAX_FNS; ++j) k += (j < nf && b >= h.begin[j]) ? 1u : 0u;
+    for (uint32_t f = 0; f < FGX_LAYER_MAX_FNS; ++f) {
+        if (f >= nf || k != FGX_LAYER_MAX_FNS) continue;
+        const uint32_t nb = h.begin[f + 1] - h.begin[f];
+        const uint32_t lo = (uint32_t)(((uint64_t)x * nb) / kXcds), hi = (uint32_t)(((uint64_t)(x + 1) * nb) / kXcds);
+        if (j < hi - lo) {
+            k = f;
+            local = lo + j;
+        } else {
+            j -= hi - lo;
+        }
+    }
+    if (k == FGX_LAYER_MAX_FNS) return;   // this XCD's share is done
     const FnArgs& a = *reinterpret_cast<const FnArgs*>(image + FGX_LAYER_ARGS_OFFSET + (size_t)k * h.stride);
-    run_layer_fn<Fs...>(h.fn[k], a, b - h.begin[k], std::index_sequence_for<Fs...>{});
+    run_layer_fn<Fs...>(h.fn[k], a, local, std::index_sequence_for<Fs...>{});
 }
 
 // Batched launch: the simulations of a batch share one launch.  Tile t (blockDim x

@@ -108,8 +108,10 @@ int fgx_graph_destroy(fgx_graph* g);
  *
  * `kernel` is a layer kernel of the model library (the host stub of a __global__
  * function taking one argument, `const void* image`): it finds its function from its
- * block index in the image header below and runs it with that function's launch
- * record.  Each fgx_fn_launch gives the function's index among the kernel's
+ * block index and the image header below, and runs it with that function's launch
+ * record.  The grid is 8 x sum over functions of ceil(blocks / 8): workgroup w runs on XCD
+ * w % 8, and each XCD runs its share of every function (blocks [x*B/8, (x+1)*B/8) of a
+ * function of B blocks), in contiguous chunks.  Each fgx_fn_launch gives the function's index among the kernel's
  * functions, its blocks (of `block` threads), its dynamic LDS per block, and its
  * launch record in host memory (`args_bytes` bytes, the same size for every function).
  *
