@@ -62,7 +62,7 @@ struct ThreadCtx {
 //   [kTabHead ..]  the unit table: per group, per neighbour row, the bin-table entries of the
 //                  group's x range (wrapping iterators: a start/end pair per cell; clamped
 //                  ones: the bin boundaries of the row).
-constexpr uint32_t kWaveTabWords = 1536;
+constexpr uint32_t kWaveTabWords = 1024;
 constexpr uint32_t kTabGroups = 4;
 constexpr uint32_t kTabHead = 32;
 
