@@ -373,13 +373,20 @@ __device__ void sort_one_bin(uint32_t st, uint32_t en, const uint32_t* __restric
 // more than kSmallBin members in LDS and orders them one after another.
 __global__ void __launch_bounds__(kBlock) k_sort_big_bins(const uint32_t* __restrict__ start, uint32_t bins,
                                                           const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
-                                                          const uint32_t* __restrict__ unsorted) {
+                                                          const uint32_t* __restrict__ unsorted,
+                                                          const uint32_t* __restrict__ tile_big, uint32_t ntiles) {
     __shared__ uint32_t s_map[kMapWords];
     __shared__ uint32_t s_gpre[kMapGroups];
     __shared__ uint32_t s_q[kBlock];
     __shared__ uint32_t s_red[2 * (kBlock / 64)];
     __shared__ uint32_t s_qn;
     if (!*unsorted) return;
+    {
+        // no tile has a large bin: nothing to order here
+        uint32_t any = 0;
+        for (uint32_t t = threadIdx.x; t < ntiles; t += kBlock) any |= tile_big[t];
+        if (__syncthreads_or(any) == 0) return;
+    }
     const uint32_t G = gridDim.x, k = blockIdx.x;
     const uint32_t mine = k < bins ? (bins - k + G - 1u) / G : 0u;
     for (uint32_t j0 = 0; j0 < mine; j0 += kBlock) {
@@ -439,26 +446,42 @@ constexpr uint32_t kTwoPassTiles = 1024;
 
 // flag_src/flag_dst (bin sorts; NULL otherwise): block 0 moves the histogram's "unsorted"
 // word to the word the scatters read, leaving the histogram region all zero behind it.
+// tile_big (bin sorts): tile_big[t] = 1 when a count of tile t exceeds kSmallBin, so that
+// k_sort_big_bins can tell at once that no bin needs it.
 __global__ void __launch_bounds__(kBlock) k_tile_sums(const uint32_t* __restrict__ in, uint32_t n,
                                                       uint32_t* __restrict__ tile_sum, uint32_t* flag_src,
-                                                      uint32_t* flag_dst) {
+                                                      uint32_t* flag_dst, uint32_t* __restrict__ tile_big) {
     __shared__ uint32_t s_wave[kBlock / 64];
+    __shared__ uint32_t s_big[kBlock / 64];
     if (flag_src && blockIdx.x == 0 && threadIdx.x == 0) {
         *flag_dst = *flag_src;
         *flag_src = 0u;
     }
     const uint32_t base = blockIdx.x * kTile + threadIdx.x * kItems;
     uint32_t sum = 0;
+    bool big = false;
 #pragma unroll
-    for (uint32_t j = 0; j < kItems; ++j) sum += (base + j < n) ? in[base + j] : 0u;
+    for (uint32_t j = 0; j < kItems; ++j) {
+        const uint32_t c = (base + j < n) ? in[base + j] : 0u;
+        sum += c;
+        big = big || c > kSmallBin;
+    }
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t tot = wave_sum_u32(sum);
-    if (lane_id() == 0) s_wave[w] = tot;
+    const bool wbig = __ballot(big) != 0ull;
+    if (lane_id() == 0) {
+        s_wave[w] = tot;
+        s_big[w] = wbig ? 1u : 0u;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (uint32_t k = 0; k < kBlock / 64; ++k) t += s_wave[k];
+        uint32_t t = 0, b = 0;
+        for (uint32_t k = 0; k < kBlock / 64; ++k) {
+            t += s_wave[k];
+            b |= s_big[k];
+        }
         tile_sum[blockIdx.x] = t;
+        if (tile_big) tile_big[blockIdx.x] = b;
     }
 }
 
@@ -563,14 +586,17 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles_pre(uint32_t* __restrict_
 // tile adds up the sums before it; beyond that, tile sums, one block scanning them,
 // then a pass that reads each tile's prefix.  `status` holds the tile sums.
 // flag_src/flag_dst and zero_in: see k_tile_sums and k_scan_tiles (bin sorts only).
+// tile_big: per-tile "some count exceeds kSmallBin" words for bin sorts (see k_tile_sums);
+// bin_sort_tile_big() places them after the tile sums in `status`.
 static int scan_counts(const uint32_t* in, uint32_t* out, uint32_t n, ScratchHeader* hdr, uint64_t* status, hipStream_t st,
-                       uint32_t zero_in = 0u, uint32_t* flag_src = nullptr, uint32_t* flag_dst = nullptr) {
+                       uint32_t zero_in = 0u, uint32_t* flag_src = nullptr, uint32_t* flag_dst = nullptr,
+                       uint32_t* tile_big = nullptr) {
     (void)hdr;
     const uint32_t tiles = ceil_div(n, kTile);
     if (!tiles) return FGX_OK;
     uint32_t* ts = reinterpret_cast<uint32_t*>(status);   // 8 bytes per tile are reserved; sums use 4
     uint32_t* inw = const_cast<uint32_t*>(in);           // written only when zero_in
-    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kBlock), 0, st, in, n, ts, flag_src, flag_dst);
+    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kBlock), 0, st, in, n, ts, flag_src, flag_dst, tile_big);
     FGX_HIP(hipGetLastError());
     if (tiles <= kTwoPassTiles) {
         hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kBlock), 0, st, inw, out, n, ts, zero_in);
@@ -941,6 +967,12 @@ static size_t tile_scratch_bytes(uint32_t n) {
     return sizeof(ScratchHeader) + align256((size_t)std::max(ceil_div(n, kTile), 1u) * 8);
 }
 
+// The per-tile large-bin words of a bin sort: the second half of the 8 bytes per tile that
+// `status` reserves (the tile sums use the first half).
+static uint32_t* bin_sort_tile_big(uint64_t* status, uint32_t bins) {
+    return reinterpret_cast<uint32_t*>(status) + ceil_div(bins + 1, kTile);
+}
+
 // k_sort_big_bins: at most two blocks per CU (73 KB of LDS each), at most one per bin.
 static uint32_t big_bin_grid(const Ctx* c, uint32_t bins) {
     const uint32_t cap = (uint32_t)std::max(c ? c->cu_count : 256, 1) * 2u;
@@ -1035,11 +1067,12 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
         FGX_HIP(hipGetLastError());
     }
     // scan hist -> start (bins+1 entries, start[bins] = n)
+    auto* status = reinterpret_cast<uint64_t*>(s + L.status);
     {
         auto* hdr = reinterpret_cast<ScratchHeader*>(s + L.hdr);
-        auto* status = reinterpret_cast<uint64_t*>(s + L.status);
         const uint32_t zero = ctx && ctx->c.keep_zero ? 1u : 0u;
-        rc = scan_counts(hist, start_out, bins + 1, hdr, status, st, zero, unsorted, unsorted_read);
+        rc = scan_counts(hist, start_out, bins + 1, hdr, status, st, zero, unsorted, unsorted_read,
+                         bin_sort_tile_big(status, bins));
         if (rc) return rc;
     }
     if (n_cap) {
@@ -1048,7 +1081,8 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
         FGX_HIP(hipGetLastError());
         // sub is free once the index scatter has run: it holds the ordered large bins
         hipLaunchKernelGGL(k_sort_big_bins, dim3(big_bin_grid(ctx ? &ctx->c : nullptr, bins)), dim3(kBlock), 0, st,
-                           start_out, bins, tmp, sub, unsorted_read);
+                           start_out, bins, tmp, sub, unsorted_read, bin_sort_tile_big(status, bins),
+                           ceil_div(bins + 1, kTile));
         FGX_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, d_n, n_cap, key, start_out, tmp, sub, vt,
                            unsorted_read);
@@ -1082,10 +1116,11 @@ static int run_bin_sort_seg(fgx_ctx* ctx, int32_t mode, const fgx_seg_key* d_seg
                            unsorted);
         FGX_HIP(hipGetLastError());
     }
+    auto* status = reinterpret_cast<uint64_t*>(s + L.status);
     {
         auto* hdr = reinterpret_cast<ScratchHeader*>(s + L.hdr);
-        auto* status = reinterpret_cast<uint64_t*>(s + L.status);
-        rc = scan_counts(hist, start_out, keys_total + 1, hdr, status, st);
+        rc = scan_counts(hist, start_out, keys_total + 1, hdr, status, st, 0u, nullptr, nullptr,
+                         bin_sort_tile_big(status, keys_total));
         if (rc) return rc;
     }
     if (n_total) {
@@ -1094,7 +1129,8 @@ static int run_bin_sort_seg(fgx_ctx* ctx, int32_t mode, const fgx_seg_key* d_seg
                            unsorted);
         FGX_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_sort_big_bins, dim3(big_bin_grid(ctx ? &ctx->c : nullptr, keys_total)), dim3(kBlock), 0, st,
-                           start_out, keys_total, tmp, sub, unsorted);
+                           start_out, keys_total, tmp, sub, unsorted, bin_sort_tile_big(status, keys_total),
+                           ceil_div(keys_total + 1, kTile));
         FGX_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_stable_scatter, dim3(g), dim3(kBlock), 0, st, d_n_total, n_total, key, start_out, tmp, sub, vt,
                            unsorted);
