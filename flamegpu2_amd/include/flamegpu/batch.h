@@ -255,6 +255,10 @@ class BatchSimulation {
     bool initialised_ = false;
     fgx_ctx* ctx_ = nullptr;
     void* stream_ = nullptr;
+    // index builds beside the agent sorts of a layer
+    void* build_stream_ = nullptr;
+    void* build_fork_event_ = nullptr;
+    void* build_join_event_ = nullptr;
     Tuning tuning_;
     int cu_count_ = 256;
     std::map<std::string, AgentB> agents_;
@@ -342,6 +346,8 @@ inline void BatchSimulation::init() {
     if (initialised_) return;
     detail::fgx_check(fgx_ctx_create(device_, &ctx_), "fgx_ctx_create");
     detail::fgx_check(fgx_stream_create(&stream_), "stream create");
+    detail::fgx_check(fgx_event_create(&build_fork_event_), "event create");
+    detail::fgx_check(fgx_event_create(&build_join_event_), "event create");
     int cus = 0, xcds = 0;
     size_t hbm = 0;
     if (fgx_device_info(ctx_, &cus, &xcds, &hbm) == FGX_OK && cus > 0) cu_count_ = cus;
@@ -470,6 +476,11 @@ inline void BatchSimulation::release() {
     free_typed(d_errors_);
     free_dev(d_pages_);
     free_typed(d_ranges_);
+    fgx_event_destroy(build_fork_event_);
+    fgx_event_destroy(build_join_event_);
+    build_fork_event_ = build_join_event_ = nullptr;
+    if (build_stream_) fgx_stream_destroy(build_stream_);
+    build_stream_ = nullptr;
     fgx_stream_destroy(stream_);
     fgx_ctx_destroy(ctx_);
     initialised_ = false;
@@ -1065,57 +1076,78 @@ inline void BatchSimulation::launch(FnB& f) {
 
 inline void BatchSimulation::enqueue_step() {
     for (auto& layer : m_->layers) {
-        // agent sorts and message-list builds for the layer's spatial inputs
+        // agent sorts and message-list builds for the layer's spatial inputs.  Sorts move agent
+        // lists and builds message lists: when a layer has both, the builds run on a stream of
+        // their own beside the sorts.
+        std::vector<AgentB*> sorts;
+        std::vector<MsgB*> builds;
         for (auto& fd : layer->agent_functions) {
             FnB& f = fns_[fn_index_.at(fd.get())];
             if (!f.in || !f.in->spatial) continue;
             AgentB& a = *f.agent;
-            if (!a.sort_msg.empty() && a.d->sortPeriod == 1 && (dynamic_ ? a.list.cap : a.list.total)) {
-                List& l = a.list;
-                const int ix = l.var_index("x"), iy = l.var_index("y"), iz = l.var_index("z");
-                std::vector<const void*> src;
-                std::vector<void*> dst;
-                std::vector<uint32_t> by;
-                for (auto& v : l.vars) {
-                    src.push_back(v.cur);
-                    dst.push_back(v.alt);
-                    by.push_back(v.bytes);
-                }
-                const MsgB& sm = msgs_.at(a.sort_msg);
-                detail::fgx_check(fgx_agent_sort_seg(ctx_, a.d_keys, (uint32_t)order_.size(),
-                                                     dynamic_ ? tile_bound(l.cap, 2048) : a.tiles, dynamic_ ? l.cap : l.total,
-                                                     dynamic_ ? l.d_total : nullptr, a.keys_total,
-                                                     (const float*)l.vars[ix].cur, (const float*)l.vars[iy].cur,
-                                                     sm.dims == 3 ? (const float*)l.vars[iz].cur : nullptr,
-                                                     (uint32_t)l.vars.size(), src.data(), dst.data(), by.data(), a.scratch,
-                                                     a.scratch_bytes, stream_),
-                                  "batched agent sort");
-                for (auto& v : l.vars) std::swap(v.cur, v.alt);
+            if (!a.sort_msg.empty() && a.d->sortPeriod == 1 && (dynamic_ ? a.list.cap : a.list.total) &&
+                std::find(sorts.begin(), sorts.end(), &a) == sorts.end())
+                sorts.push_back(&a);
+            if (f.in->dirty && std::find(builds.begin(), builds.end(), f.in) == builds.end()) builds.push_back(f.in);
+        }
+        for (AgentB* ap : sorts) {
+            AgentB& a = *ap;
+            List& l = a.list;
+            const int ix = l.var_index("x"), iy = l.var_index("y"), iz = l.var_index("z");
+            std::vector<const void*> src;
+            std::vector<void*> dst;
+            std::vector<uint32_t> by;
+            for (auto& v : l.vars) {
+                src.push_back(v.cur);
+                dst.push_back(v.alt);
+                by.push_back(v.bytes);
             }
-            MsgB& m = *f.in;
-            if (m.dirty) {
-                List& l = m.list;
-                const int ix = l.var_index("x"), iy = l.var_index("y"), iz = l.var_index("z");
-                std::vector<const void*> src;
-                std::vector<void*> dst;
-                std::vector<uint32_t> by;
-                for (auto& v : l.vars) {
-                    src.push_back(v.cur);
-                    dst.push_back(v.alt);
-                    by.push_back(v.bytes);
-                }
-                detail::fgx_check(fgx_spatial_build_seg(ctx_, m.d_keys, (uint32_t)order_.size(),
-                                                        dynamic_ ? tile_bound(l.cap, 2048) : m.tiles,
-                                                        dynamic_ ? l.cap : l.total, dynamic_ ? l.d_total : nullptr,
-                                                        m.keys_total,
-                                                        (const float*)l.vars[ix].cur, (const float*)l.vars[iy].cur,
-                                                        m.dims == 3 ? (const float*)l.vars[iz].cur : nullptr,
-                                                        (uint32_t)l.vars.size(), src.data(), dst.data(), by.data(), m.pbm,
-                                                        m.scratch, m.scratch_bytes, stream_),
-                                  "batched message-list build");
-                for (auto& v : l.vars) std::swap(v.cur, v.alt);
-                m.dirty = false;
+            const MsgB& sm = msgs_.at(a.sort_msg);
+            detail::fgx_check(fgx_agent_sort_seg(ctx_, a.d_keys, (uint32_t)order_.size(),
+                                                 dynamic_ ? tile_bound(l.cap, 2048) : a.tiles, dynamic_ ? l.cap : l.total,
+                                                 dynamic_ ? l.d_total : nullptr, a.keys_total,
+                                                 (const float*)l.vars[ix].cur, (const float*)l.vars[iy].cur,
+                                                 sm.dims == 3 ? (const float*)l.vars[iz].cur : nullptr,
+                                                 (uint32_t)l.vars.size(), src.data(), dst.data(), by.data(), a.scratch,
+                                                 a.scratch_bytes, stream_),
+                              "batched agent sort");
+            for (auto& v : l.vars) std::swap(v.cur, v.alt);
+        }
+        const bool overlap = !sorts.empty() && !builds.empty();
+        void* bst = stream_;
+        if (overlap) {
+            if (!build_stream_) detail::fgx_check(fgx_stream_create(&build_stream_), "stream create");
+            detail::fgx_check(fgx_event_record(build_fork_event_, stream_), "fork");
+            detail::fgx_check(fgx_stream_wait_event(build_stream_, build_fork_event_), "fork wait");
+            bst = build_stream_;
+        }
+        for (MsgB* mp : builds) {
+            MsgB& m = *mp;
+            List& l = m.list;
+            const int ix = l.var_index("x"), iy = l.var_index("y"), iz = l.var_index("z");
+            std::vector<const void*> src;
+            std::vector<void*> dst;
+            std::vector<uint32_t> by;
+            for (auto& v : l.vars) {
+                src.push_back(v.cur);
+                dst.push_back(v.alt);
+                by.push_back(v.bytes);
             }
+            detail::fgx_check(fgx_spatial_build_seg(ctx_, m.d_keys, (uint32_t)order_.size(),
+                                                    dynamic_ ? tile_bound(l.cap, 2048) : m.tiles,
+                                                    dynamic_ ? l.cap : l.total, dynamic_ ? l.d_total : nullptr,
+                                                    m.keys_total,
+                                                    (const float*)l.vars[ix].cur, (const float*)l.vars[iy].cur,
+                                                    m.dims == 3 ? (const float*)l.vars[iz].cur : nullptr,
+                                                    (uint32_t)l.vars.size(), src.data(), dst.data(), by.data(), m.pbm,
+                                                    m.scratch, m.scratch_bytes, bst),
+                              "batched message-list build");
+            for (auto& v : l.vars) std::swap(v.cur, v.alt);
+            m.dirty = false;
+        }
+        if (overlap) {
+            detail::fgx_check(fgx_event_record(build_join_event_, build_stream_), "join");
+            detail::fgx_check(fgx_stream_wait_event(stream_, build_join_event_), "join wait");
         }
         for (auto& fd : layer->agent_functions) {
             FnB& f = fns_[fn_index_.at(fd.get())];

@@ -919,6 +919,10 @@ class CUDASimulation {
     void set_buffer_state(const std::vector<void*>& st);
     std::vector<void*> join_events_;
     void* fork_event_ = nullptr;
+    // index builds beside the agent sorts of a layer
+    void* build_stream_ = nullptr;
+    void* build_fork_event_ = nullptr;
+    void* build_join_event_ = nullptr;
     // kernel timing
     bool timing_ = false;
     bool stage_enabled_ = false;
@@ -1109,6 +1113,8 @@ inline void CUDASimulation::ensure_init() {
     for (auto& mm : m_->messages)
         if (mm.second->persistent) graph_eligible_ = false;
     detail::fgx_check(fgx_event_create(&fork_event_), "event create");
+    detail::fgx_check(fgx_event_create(&build_fork_event_), "event create");
+    detail::fgx_check(fgx_event_create(&build_join_event_), "event create");
     initialised_ = true;
 }
 
@@ -1155,6 +1161,9 @@ inline void CUDASimulation::release() {
     }
     for (auto e : join_events_) fgx_event_destroy(e);
     if (fork_event_) fgx_event_destroy(fork_event_);
+    if (build_fork_event_) fgx_event_destroy(build_fork_event_);
+    if (build_join_event_) fgx_event_destroy(build_join_event_);
+    if (build_stream_) fgx_stream_destroy(build_stream_);
     for (auto e : free_events_) fgx_event_destroy(e);
     void* p = d_env_;
     free_dev(p);
@@ -2057,7 +2066,25 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
         upload_env();
         return;
     }
-    // auto spatial sort (CUDASimulation.cu:671-682) and lazy index builds, on the main stream
+    // auto spatial sort (CUDASimulation.cu:671-682) and lazy index builds.  Sorts move agent
+    // lists and builds move message lists, so with in-layer concurrency on the builds run on a
+    // stream of their own beside the sorts: both are chains of small latency-bound kernels.
+    bool sorts = false, builds = false;
+    for (auto& fd : layer.agent_functions) {
+        auto& f = fns_[fn_index_.at(fd.get())];
+        const unsigned period = f.d->parent->sortPeriod;
+        sorts = sorts || (f.sort_dims && period && step_ % period == 0);
+        builds = builds || (f.min && f.min->pbm_dirty &&
+                            (f.min->spatial || f.min->kind == MsgKind::Bucket || is_array_kind(f.min->kind)));
+    }
+    const bool overlap = sorts && builds && cuda_config_.inLayerConcurrency && !timing_;
+    void* bst = stream_;
+    if (overlap) {
+        if (!build_stream_) detail::fgx_check(fgx_stream_create(&build_stream_), "stream create");
+        detail::fgx_check(fgx_event_record(build_fork_event_, stream_), "fork");
+        detail::fgx_check(fgx_stream_wait_event(build_stream_, build_fork_event_), "fork wait");
+        bst = build_stream_;
+    }
     for (auto& fd : layer.agent_functions) {
         auto& f = fns_[fn_index_.at(fd.get())];
         const unsigned period = f.d->parent->sortPeriod;
@@ -2096,14 +2123,14 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
                 dst.push_back(v.alt);
                 by.push_back(v.bytes);
             }
-            tick("build:" + m.d->name, stream_, true);
+            tick("build:" + m.d->name, bst, true);
             detail::fgx_check(fgx_spatial_build(ctx_, &m.meta, m.d_count, m.cap, (const float*)m.vars[ix].cur,
                                                 (const float*)m.vars[iy].cur,
                                                 m.meta.dims == 3 ? (const float*)m.vars[iz].cur : nullptr,
                                                 (uint32_t)m.vars.size(), src.data(), dst.data(), by.data(), m.pbm,
-                                                m.build_scratch, m.build_scratch_bytes, stream_),
+                                                m.build_scratch, m.build_scratch_bytes, bst),
                               "message index build");
-            tick("build:" + m.d->name, stream_, false);
+            tick("build:" + m.d->name, bst, false);
             for (auto& v : m.vars) std::swap(v.cur, v.alt);
             m.pbm_dirty = false;
         } else if (f.min && (f.min->kind == MsgKind::Bucket || is_array_kind(f.min->kind)) && f.min->pbm_dirty) {
@@ -2116,26 +2143,30 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
                 dst.push_back(v.alt);
                 by.push_back(v.bytes);
             }
-            tick("build:" + m.d->name, stream_, true);
+            tick("build:" + m.d->name, bst, true);
             if (m.kind == MsgKind::Bucket) {
                 const int ik = m.var_index("_key");
                 detail::fgx_check(fgx_bucket_build(ctx_, m.key_min, m.keys, m.d_count, m.cap, (const int32_t*)m.vars[ik].cur,
                                                    (uint32_t)m.vars.size(), src.data(), dst.data(), by.data(), m.pbm,
-                                                   m.build_scratch, m.build_scratch_bytes, stream_),
+                                                   m.build_scratch, m.build_scratch_bytes, bst),
                                   "bucket message index build");
             } else {
                 const int ii = m.var_index("___INDEX");
                 detail::fgx_check(fgx_array_build(ctx_, m.length, m.d_count, m.cap, (const uint32_t*)m.vars[ii].cur,
                                                   (uint32_t)m.vars.size(), src.data(), dst.data(), by.data(), m.counts,
-                                                  d_error_, stream_),
+                                                  d_error_, bst),
                                   "array message build");
                 // the list now holds every element (MessageArray.cu buildIndex: count = length)
-                detail::fgx_check(fgx_fill_u32(m.d_count, m.length, 1, stream_), "array message count");
+                detail::fgx_check(fgx_fill_u32(m.d_count, m.length, 1, bst), "array message count");
             }
-            tick("build:" + m.d->name, stream_, false);
+            tick("build:" + m.d->name, bst, false);
             for (auto& v : m.vars) std::swap(v.cur, v.alt);
             m.pbm_dirty = false;
         }
+    }
+    if (overlap) {
+        detail::fgx_check(fgx_event_record(build_join_event_, build_stream_), "join");
+        detail::fgx_check(fgx_stream_wait_event(stream_, build_join_event_), "join wait");
     }
     // function conditions (CUDASimulation.cu:687-835): condition kernel, then a stable
     // partition that puts the disabled agents in front; the function runs on the rest
