@@ -438,6 +438,23 @@ __global__ void __launch_bounds__(kBlock) k_stable_scatter(const uint32_t* d_n, 
     }
 }
 
+// 16-byte records of four 4-byte variables (fgx_pack_records); NULL sources give 0 words.
+__global__ void __launch_bounds__(kBlock) k_pack_records(const uint32_t* d_n, uint32_t n_cap,
+                                                         const uint32_t* __restrict__ w0, const uint32_t* __restrict__ w1,
+                                                         const uint32_t* __restrict__ w2, const uint32_t* __restrict__ w3,
+                                                         uint4* __restrict__ out) {
+    const uint32_t n = load_count(d_n, n_cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = xcd_block() * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint4 r;
+        r.x = w0 ? w0[i] : 0u;
+        r.y = w1 ? w1[i] : 0u;
+        r.z = w2 ? w2[i] : 0u;
+        r.w = w3 ? w3[i] : 0u;
+        out[i] = r;
+    }
+}
+
 // ================================================================ kernel-boundary exclusive scan
 // For up to kTwoPassTiles tiles: per-tile sums, then every tile adds the sums of the
 // tiles before it.  The kernel boundary orders the two passes, where a single-pass
@@ -1344,6 +1361,17 @@ int fgx_agent_sort(fgx_ctx* ctx, int dims, const float* mn, const float* mx, flo
         return fail(FGX_ERR_CAPACITY, "fgx_agent_sort: scratch too small");
     auto* start = reinterpret_cast<uint32_t*>(static_cast<char*>(scratch) + L.total);
     return run_bin_sort(ctx, ka, bins, d_n, n_cap, x, y, z, vt, start, scratch, L.total, stream);
+}
+
+int fgx_pack_records(const uint32_t* d_n, uint32_t n_cap, const void* w0, const void* w1, const void* w2,
+                     const void* w3, void* out, void* stream) {
+    if (!n_cap) return FGX_OK;
+    if (!out || (reinterpret_cast<uintptr_t>(out) & 15u)) return fail(FGX_ERR_ARG, "fgx_pack_records: out must be 16-byte aligned");
+    hipLaunchKernelGGL(k_pack_records, dim3(stream_grid(nullptr, n_cap)), dim3(kBlock), 0, (hipStream_t)stream, d_n, n_cap,
+                       static_cast<const uint32_t*>(w0), static_cast<const uint32_t*>(w1),
+                       static_cast<const uint32_t*>(w2), static_cast<const uint32_t*>(w3), static_cast<uint4*>(out));
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
 }
 
 int fgx_exclusive_scan_u32(fgx_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,

@@ -94,6 +94,7 @@ _SIGS = {
     "fgx_seg_append": [u32, vp, vp, vp, u32, u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32),
                        _c.c_int32, vp, vp, vp],
     "fgx_add_u32_n": [vp, u32, u32, vp],
+    "fgx_pack_records": [vp, u32, vp, vp, vp, vp, vp, vp],
     "fgx_copy_ranges": [u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), vp, u32, u32, vp],
 }
 
@@ -296,6 +297,11 @@ def seg_append(nsegs, d_off, d_cnt, d_born, n_cap_out, list_in, born_in, list_ou
 
 def add_u32_n(p, n, delta, stream=0):
     check(lib().fgx_add_u32_n(p, n, delta, stream))
+
+
+def pack_records(d_n, n_cap, w0, w1, w2, w3, out, stream=0):
+    """out[i] = {w0[i], w1[i], w2[i], w3[i]} (16-byte records; None/0 sources give 0 words)."""
+    check(lib().fgx_pack_records(d_n, n_cap, w0, w1, w2, w3, out, stream))
 
 
 def copy_ranges(src, dst, var_bytes, d_ranges, nranges, n_total, stream=0):

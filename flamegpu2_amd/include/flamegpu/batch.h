@@ -164,6 +164,8 @@ class BatchSimulation {
         std::vector<fgx_seg_key> keys;
         void* scratch = nullptr;
         size_t scratch_bytes = 0;
+        void* packed = nullptr;       // spatial: 16-byte {x, y, z, id} record per message
+        uint32_t packed_cap = 0;
         bool dirty = true;
     };
     struct AgentB {
@@ -233,6 +235,7 @@ class BatchSimulation {
     void grow_agent_list(AgentB& a, uint32_t cap);
     void size_pop_buffers(FnB& f);
     void size_sort_scratch(AgentB& a);
+    void size_records(MsgB& m);
     void size_build_scratch(MsgB& m);
     void upload_layout(List& l, uint32_t S);
     std::vector<fgx_seg_target> agent_targets(AgentB& a);
@@ -454,6 +457,8 @@ inline void BatchSimulation::release() {
         free_typed(kv.second.pbm);
         free_typed(kv.second.d_keys);
         free_dev(kv.second.scratch);
+        free_dev(kv.second.packed);
+        kv.second.packed_cap = 0;
         free_typed(kv.second.list.d_off);
         free_typed(kv.second.list.d_cnt);
         free_typed(kv.second.list.d_total);
@@ -792,6 +797,7 @@ inline void BatchSimulation::rebuild() {
             }
             mb.list.cap = cap;
         }
+        size_records(mb);
         mb.list.off.assign(S, 0);
         uint32_t o = 0;
         for (uint32_t s = 0; s < S; ++s) {
@@ -1027,7 +1033,10 @@ inline void BatchSimulation::launch(FnB& f) {
         }
     };
     fill_table(a.agent, &a.agent_mul, l.vars, false);
-    if (f.in) fill_table(a.msg_in, &a.msg_in_mul, f.in->list.vars, false);
+    if (f.in) {
+        fill_table(a.msg_in, &a.msg_in_mul, f.in->list.vars, false);
+        a.in_meta.packed = f.in->packed;   // covers the whole batch, indexed like the message arrays
+    }
     if (f.out) {
         fill_table(a.msg_out, &a.msg_out_mul, f.out->list.vars, true);
         if (f.flags_cap < (dynamic_ ? l.cap : l.total)) throw exception::InvalidOperation("BatchSimulation: message flags below the list size");
@@ -1143,6 +1152,14 @@ inline void BatchSimulation::enqueue_step() {
                                                     m.scratch, m.scratch_bytes, bst),
                               "batched message-list build");
             for (auto& v : l.vars) std::swap(v.cur, v.alt);
+            {
+                const int ii = l.var_index("id");
+                const void* idp = ii >= 0 && l.vars[ii].bytes == 4 ? l.vars[ii].cur : nullptr;
+                detail::fgx_check(fgx_pack_records(dynamic_ ? l.d_total : nullptr, dynamic_ ? l.cap : l.total,
+                                                   l.vars[ix].cur, l.vars[iy].cur, m.dims == 3 ? l.vars[iz].cur : nullptr,
+                                                   idp, m.packed, bst),
+                                  "batched message records");
+            }
             m.dirty = false;
         }
         if (overlap) {
@@ -1441,6 +1458,15 @@ inline void BatchSimulation::size_sort_scratch(AgentB& a) {
     a.scratch_bytes = sb;
 }
 
+// The packed {x, y, z, id} records of a spatial list cover its capacity.  A capacity
+// change also rebuilds the launch arguments (graphs are cleared), which carry the pointer.
+inline void BatchSimulation::size_records(MsgB& m) {
+    if (!m.spatial || m.list.cap <= m.packed_cap) return;
+    free_dev(m.packed);
+    alloc(&m.packed, (size_t)m.list.cap * 16);
+    m.packed_cap = m.list.cap;
+}
+
 inline void BatchSimulation::size_build_scratch(MsgB& m) {
     if (!m.spatial) return;
     size_t sb = 0;
@@ -1502,6 +1528,7 @@ inline void BatchSimulation::ensure_step_capacity() {
             m.list.cap = cap;
             changed = true;
         }
+        size_records(m);
         size_build_scratch(m);
     }
     for (auto& f : fns_) {

@@ -178,6 +178,7 @@ struct SpatialDev {
     uint32_t keys;
     float cell;      // bin width (the radius, or half of it for half-width bins)
     int32_t span;    // bins searched either side of the query's bin (1, or 2 for half-width bins)
+    const void* packed;   // 16-byte {x, y, z, id} record per message of the built list (launch arguments only)
 };
 
 // Device error codes (the device half of exception::DeviceError, replacing DTHROW).

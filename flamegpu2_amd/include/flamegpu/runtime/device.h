@@ -416,14 +416,16 @@ constexpr uint32_t kHashId = name_hash("id", 3);
 // clamped ones), advanced with selects only: no divisions, no per-lane branches.
 //
 // The iterator keeps one candidate of look-ahead: while the body runs on message k,
-// the loads of message k+1's x, y (, z) and id are in flight, and the bin-table
-// range of the unit after the current one is loaded when the current one starts.
-// Messages read through their position / id use those registers; other variables
-// are loaded at access time.
+// the load of message k+1's position and id (one 16-byte record, written with the
+// list's index) is in flight, and on the per-lane walk the bin-table range of the unit
+// after the current one is loaded when the current one starts.  Messages read through
+// their position / id use those registers; other variables are loaded at access time.
+// On the wave's unit table a lane starts with a bit mask of its non-empty units and
+// takes the lowest remaining one whenever its current unit ends.
 //
-// Source: the message list in HBM (bin table pbm), or, in the staged kernel when
-// every lane of the wave is covered by the block's LDS copy, the LDS tables (units
-// then name virtual bin entries and the look-ahead walks LDS slots).
+// Source: the message list in HBM (bin table pbm, packed records), or, in the staged
+// kernel when every lane of the wave is covered by the block's LDS copy, the LDS tables
+// (units then name virtual bin entries and the look-ahead walks LDS slots).
 template <int DIMS, bool WRAP>
 class SpatialFilter {
  public:
@@ -448,20 +450,14 @@ class SpatialFilter {
         const int cx = detail::grid_pos(m, 0, x);
         const int cy = detail::grid_pos(m, 1, y);
         const int cz = DIMS == 3 ? detail::grid_pos(m, 2, z) : 0;
-        // hot variables in HBM
-        const detail::VarBinding& bx = c.a.msg_in[detail::table_slot(detail::kHashX, c.a.msg_in_mul, detail::VAR_SLOT_BITS)];
-        const detail::VarBinding& by = c.a.msg_in[detail::table_slot(detail::kHashY, c.a.msg_in_mul, detail::VAR_SLOT_BITS)];
-        const detail::VarBinding& bz = c.a.msg_in[detail::table_slot(detail::kHashZ, c.a.msg_in_mul, detail::VAR_SLOT_BITS)];
+        // the hot variables come from the list's packed {x, y, z, id} records (an absent or
+        // non-4-byte id reads as 0, and reading it reports the error below)
         const detail::VarBinding& bi = c.a.msg_in[detail::table_slot(detail::kHashId, c.a.msg_in_mul, detail::VAR_SLOT_BITS)];
-        px_ = reinterpret_cast<const float*>(bx.ptr);
-        py_ = reinterpret_cast<const float*>(by.ptr);
-        pz_ = DIMS == 3 ? reinterpret_cast<const float*>(bz.ptr) : px_;
         const bool id_known = bi.hash == detail::kHashId;
         const bool id_ok = id_known && bi.bytes == 4;
         id_err_ = id_ok ? 0u : (id_known ? (uint32_t)detail::DERR_MSG_VAR_TYPE : (uint32_t)detail::DERR_UNKNOWN_MSG_VAR);
-        pid_ = reinterpret_cast<const uint32_t*>(id_ok ? bi.ptr : c.zero_page);
-        id_mask_ = id_ok ? 0xFFFFFFFFu : 0u;
         pbm_ = m.pbm;
+        rec_ = reinterpret_cast<const uint4*>(c.a.in_meta.packed);   // the launch's (a batch: all simulations')
         sv_ = nullptr;
         tab_ = nullptr;
         tm_ = 0u;
@@ -545,11 +541,16 @@ class SpatialFilter {
         __device__ __forceinline__ iterator(const SpatialFilter& f, int) : f_(f), m_(f, 0, 0.f, 0.f, 0.f, 0) {
             pu_ = 0;
             if (f_.tm_) {
-                ta_ = f_.tlane_;
-                read_table(ps_, pe_);
-                po_ = 0;
+                // the lane's non-empty units, one bit each (at most 32 units on the table path)
+                uint32_t mask = 0;
+                for (int u = 0; u < f_.units_; ++u) {
+                    uint32_t s, e;
+                    read_unit((uint32_t)u, s, e);
+                    mask |= (e > s ? 1u : 0u) << u;
+                }
+                nm_ = mask;
                 l_ = e_ = 0;
-                take_units();
+                take_unit();
             } else {
                 f_.first_unit(k_);
                 f_.load_unit(k_, ps_, pe_, po_);
@@ -588,38 +589,30 @@ class SpatialFilter {
                 }
             }
         }
-        // Unit table: the range of the prefetched unit (entry ta_; past the last unit the read
-        // stays at entry 0).
-        __device__ __forceinline__ void read_table(uint32_t& s, uint32_t& e) const {
-            const uint32_t at = pu_ < f_.units_ ? ta_ : 0u;
-            const volatile __attribute__((address_space(3))) uint32_t* t = f_.tab_;
+        // Unit table: the range [s, e) of the lane's unit u.
+        __device__ __forceinline__ void read_unit(uint32_t u, uint32_t& s, uint32_t& e) const {
+            const __attribute__((address_space(3))) uint32_t* t = f_.tab_;
+            const uint32_t at = f_.tlane_ + u * (WRAP ? 2u : 1u);
             s = t[at];
-            e = t[WRAP ? at + 1u : (pu_ < f_.units_ ? at + f_.tstride_ : 0u)];
+            e = t[WRAP ? at + 1u : at + f_.tstride_];
         }
-        // Unit table: the lanes whose look-ahead left its unit take the prefetched one (and the
-        // next, while they take empty ones) with selects, not per-lane branches.  A lane's units
-        // are consecutive table entries, so the next unit's range is one LDS read.
-        __device__ __forceinline__ void take_one(bool& need) {
-            const bool take = need && pu_ < f_.units_;
-            l_ = take ? ps_ : l_;
-            e_ = take ? pe_ : e_;
-            pu_ += take ? 1 : 0;
-            ta_ += take ? (WRAP ? 2u : 1u) : 0u;
+        // Unit table: a lane whose look-ahead left its unit takes its next non-empty unit
+        // (the lowest bit of nm_), with selects, not per-lane branches.  Every unit taken
+        // holds a message, so one take per step is enough; with none left, l_ >= e_ stays.
+        __device__ __forceinline__ void take_unit() {
+            const bool need = l_ >= e_;
+            const uint32_t u = nm_ ? (uint32_t)__builtin_ctz(nm_) : 0u;
             uint32_t s, e;
-            read_table(s, e);
-            ps_ = take ? s : ps_;
-            pe_ = take ? e : pe_;
-            need = take && l_ >= e_;
-        }
-        __device__ __forceinline__ void take_units() {
-            bool need = l_ >= e_;
-            take_one(need);
-            while (__ballot(need)) take_one(need);   // a lane took an empty unit
+            read_unit(u, s, e);
+            const bool take = need && nm_ != 0u;
+            l_ = take ? s : l_;
+            e_ = take ? e : e_;
+            nm_ = need ? (nm_ & (nm_ - 1u)) : nm_;
             more_ = l_ < e_;
         }
         __device__ __forceinline__ void step() {
             ++l_;
-            if (f_.tm_) take_units();
+            if (f_.tm_) take_unit();
             else if (l_ >= e_) next_unit();
             fetch();
         }
@@ -645,8 +638,8 @@ class SpatialFilter {
         int pu_ = 0;
         Unit k_;
         uint32_t ps_ = 0, pe_ = 0, po_ = 0;
-        // unit table: that unit's entry
-        uint32_t ta_ = 0;
+        // unit table: the lane's non-empty units not yet taken, one bit each
+        uint32_t nm_ = 0;
     };
     __device__ __forceinline__ iterator begin() const { return iterator(*this, 0); }
     __device__ __forceinline__ iterator end() const { return iterator(*this); }
@@ -710,6 +703,7 @@ class SpatialFilter {
     // program order (LDS executes one wave's accesses in order).
     __device__ __forceinline__ void build_table(int cx, int cy, int cz) {
         volatile __attribute__((address_space(3))) uint32_t* w = c_.wtab;
+        if (units_ > 32) return;   // the walk keeps one bit per unit
         const uint32_t claimed = __builtin_amdgcn_readfirstlane(w[0]);
         if (claimed) return;
         w[0] = 1u;
@@ -921,10 +915,13 @@ class SpatialFilter {
             z = DIMS == 3 ? v[2 * cap + l] : 0.0f;
             id = sv_->has_id ? sv_->vars[(uint32_t)DIMS * cap + l] : 0u;
         } else {
-            x = px_[l];
-            y = py_[l];
-            z = DIMS == 3 ? pz_[l] : 0.0f;
-            id = pid_[l & id_mask_];
+            // one 16-byte load per candidate: the gathers of a wave touch several bins' lines
+            // at once, and the texture-address unit pays per line and per load instruction
+            const uint4 r = rec_[l];
+            x = __builtin_bit_cast(float, r.x);
+            y = __builtin_bit_cast(float, r.y);
+            z = DIMS == 3 ? __builtin_bit_cast(float, r.z) : 0.0f;
+            id = r.w;
         }
     }
 
@@ -943,9 +940,8 @@ class SpatialFilter {
     int m0_, m1_, m2_, s0_, s1_;    // wrapping: last part value and stride per level
     int ebase_, xbase_, ewidth_;    // LDS source: covering group
     const uint32_t* pbm_;
-    const float *px_, *py_, *pz_;
-    const uint32_t* pid_;
-    uint32_t id_mask_, id_err_;
+    const uint4* rec_;              // packed {x, y, z, id} record per message (HBM source)
+    uint32_t id_err_;
 };
 
 // include/flamegpu/runtime/messaging/MessageSpatial3D/MessageSpatial3DDevice.cuh:503-558

@@ -204,6 +204,7 @@ struct MessageRT {
     uint32_t* pbm = nullptr;        // spatial bin table / bucket start table
     void* build_scratch = nullptr;
     size_t build_scratch_bytes = 0;
+    void* packed = nullptr;         // spatial: 16-byte {x, y, z, id} record per message, rebuilt with the index
     bool truncate = true;
     bool pbm_dirty = true;
     int var_index(const std::string& n) const {
@@ -1141,6 +1142,7 @@ inline void CUDASimulation::release() {
         p = kv.second.counts;
         free_dev(p);
         free_dev(kv.second.build_scratch);
+        free_dev(kv.second.packed);
     }
     for (auto& f : fns_) {
         void* p = f.death_flags;
@@ -1223,6 +1225,8 @@ inline void CUDASimulation::grow_message(detail::MessageRT& m, uint32_t cap) {
     free_dev(m.build_scratch);
     if (m.spatial) {
         m.build_scratch = new_scratch(FGX_OP_SPATIAL_BUILD, cap, m.meta.bins, &m.build_scratch_bytes);
+        free_dev(m.packed);
+        alloc(&m.packed, (size_t)cap * 16);
     } else if (m.kind == MsgKind::Bucket) {
         m.build_scratch = new_scratch(FGX_OP_SPATIAL_BUILD, cap, m.keys, &m.build_scratch_bytes);
     }
@@ -1764,6 +1768,7 @@ inline uint32_t CUDASimulation::prepare_function(detail::FnRT& f, void* st, deta
             a.in_meta.span = m.span;
             a.in_meta.wrap_ok = m.wrap_ok;
             a.in_meta.pbm = f.min->pbm;
+            a.in_meta.packed = f.min->packed;
             stage_neighbourhood(f, l, a);
         } else if (f.min->kind == MsgKind::Bucket) {
             a.in_meta.key_min = f.min->key_min;
@@ -2130,8 +2135,15 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
                                                 (uint32_t)m.vars.size(), src.data(), dst.data(), by.data(), m.pbm,
                                                 m.build_scratch, m.build_scratch_bytes, bst),
                               "message index build");
-            tick("build:" + m.d->name, bst, false);
             for (auto& v : m.vars) std::swap(v.cur, v.alt);
+            {
+                const int ii = m.var_index("id");
+                const void* idp = ii >= 0 && m.vars[ii].bytes == 4 ? m.vars[ii].cur : nullptr;
+                detail::fgx_check(fgx_pack_records(m.d_count, m.cap, m.vars[ix].cur, m.vars[iy].cur,
+                                                   m.meta.dims == 3 ? m.vars[iz].cur : nullptr, idp, m.packed, bst),
+                                  "message records");
+            }
+            tick("build:" + m.d->name, bst, false);
             m.pbm_dirty = false;
         } else if (f.min && (f.min->kind == MsgKind::Bucket || is_array_kind(f.min->kind)) && f.min->pbm_dirty) {
             auto& m = *f.min;

@@ -266,6 +266,28 @@ def test_compact_stable(ctx, n, p, keep):
     assert np.array_equal(host(d_out[2], np.float64, cnt), outs_o[2])
 
 
+@pytest.mark.parametrize("use_dn", [False, True])
+def test_pack_records(ctx, use_dn):
+    # the neighbour walk's {x, y, z, id} records: word k of record i is source k's element i
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(11)
+    cap, n = 100_003, 77_777
+    w = [rng.integers(0, 2**32, size=cap, dtype=np.uint32) for _ in range(4)]
+    d_w = [dev(a) for a in w]
+    out = empty(4 * cap, np.uint32)
+    d_n = dev(np.array([n], np.uint32)) if use_dn else None
+    fgx.pack_records(ptr(d_n) if use_dn else None, cap, ptr(d_w[0]), ptr(d_w[1]), None, ptr(d_w[3]), ptr(out))
+    sync()
+    got = host(out, np.uint32, 4 * cap).reshape(cap, 4)
+    m = n if use_dn else cap
+    assert np.array_equal(got[:m, 0], w[0][:m]) and np.array_equal(got[:m, 1], w[1][:m])
+    assert not got[:m, 2].any()                      # NULL source: 0 words
+    assert np.array_equal(got[:m, 3], w[3][:m])
+    assert not got[m:].any()                         # records past the count are not written
+    with pytest.raises(Exception):
+        fgx.pack_records(None, cap, ptr(d_w[0]), None, None, None, ptr(out) + 4)   # not 16-byte aligned
+
+
 def test_compact_device_count(ctx):
     from flamegpu2_amd import fgx
     rng = np.random.default_rng(3)
