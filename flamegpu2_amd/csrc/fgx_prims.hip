@@ -1058,6 +1058,9 @@ static int run_bin_sort(fgx_ctx* ctx, const KeyArgs& ka, uint32_t bins, const ui
     // follow: under capture the clearing is skipped only for a buffer that every earlier
     // sort used with this same bin count (each of them left the region zero, and the first
     // one cleared it), since another bin count moves the key arrays over the histogram.
+    // A buffer that no sort has used yet still clears under capture: a later sort of the
+    // same graph may use it with another bin count, and would leave this region dirty for
+    // the next replay.
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     FGX_HIP(hipStreamIsCapturing((hipStream_t)stream, &cap));
     const bool capturing = cap != hipStreamCaptureStatusNone;

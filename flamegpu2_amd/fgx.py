@@ -57,6 +57,12 @@ _SIGS = {
     "fgx_memcpy_d2d": [vp, vp, sz, vp],
     "fgx_fill_u32": [vp, u32, sz, vp],
     "fgx_stream_synchronize": [vp],
+    "fgx_stream_create": [_c.POINTER(vp)],
+    "fgx_stream_destroy": [vp],
+    "fgx_graph_begin": [vp],
+    "fgx_graph_end": [vp, _c.POINTER(vp)],
+    "fgx_graph_launch": [vp, vp],
+    "fgx_graph_destroy": [vp],
     "fgx_device_synchronize": [],
     "fgx_scratch_bytes": [_c.c_int, u32, u32, _c.POINTER(sz)],
     "fgx_scratch_clear": [vp, sz, vp],
@@ -227,6 +233,10 @@ def exclusive_scan(ctx, d_in, d_out, n, scratch, scratch_len, stream=0):
     check(lib().fgx_exclusive_scan_u32(ctx.h, d_in, d_out, n, scratch, scratch_len, stream))
 
 
+def fill_u32(dst, value, n, stream=0):
+    check(lib().fgx_fill_u32(dst, value, n, stream))
+
+
 def scratch_clear(scratch, nbytes, stream=0):
     check(lib().fgx_scratch_clear(scratch, nbytes, stream))
 
@@ -297,6 +307,40 @@ def seg_append(nsegs, d_off, d_cnt, d_born, n_cap_out, list_in, born_in, list_ou
 
 def add_u32_n(p, n, delta, stream=0):
     check(lib().fgx_add_u32_n(p, n, delta, stream))
+
+
+def stream_create():
+    h = vp()
+    check(lib().fgx_stream_create(_c.byref(h)))
+    return h.value
+
+
+def stream_destroy(stream):
+    check(lib().fgx_stream_destroy(stream))
+
+
+def stream_synchronize(stream):
+    check(lib().fgx_stream_synchronize(stream))
+
+
+def graph_capture(stream, enqueue):
+    """Captures what enqueue() puts on `stream` into a graph; returns the graph handle."""
+    check(lib().fgx_graph_begin(stream))
+    try:
+        enqueue()
+    finally:
+        g = vp()
+        rc = lib().fgx_graph_end(stream, _c.byref(g))
+    check(rc)
+    return g.value
+
+
+def graph_launch(graph, stream):
+    check(lib().fgx_graph_launch(graph, stream))
+
+
+def graph_destroy(graph):
+    check(lib().fgx_graph_destroy(graph))
 
 
 def pack_records(d_n, n_cap, w0, w1, w2, w3, out, stream=0):

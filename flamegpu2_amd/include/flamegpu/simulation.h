@@ -23,6 +23,7 @@
 #include <set>
 #include <string>
 #include <thread>
+#include <unordered_set>
 #include <vector>
 
 #include "flamegpu/runtime/device.h"
@@ -779,6 +780,7 @@ class CUDASimulation {
         void* p = nullptr;
         alloc(&p, *bytes);
         detail::fgx_check(fgx_scratch_clear(p, *bytes, stream_), "scratch clear");
+        sorted_scratch_.erase(p);
         return p;
     }
     void grow_list(detail::StateList& l, uint32_t cap);
@@ -932,6 +934,13 @@ class CUDASimulation {
     bool layer_fusion_ = true;
     bool layer_space_warned_ = false;
     bool new_layer_images_ = false;   // the enqueue being captured uploaded a layer image
+    // sort scratch buffers that a sort has used: a capture that holds a buffer's first sort
+    // also holds its clearing launch, so the graph is captured again after one run
+    std::unordered_set<const void*> sorted_scratch_;
+    bool first_sorts_ = false;        // the enqueue being captured holds a buffer's first sort
+    void note_sort(const void* scratch) {
+        if (sorted_scratch_.insert(scratch).second) first_sorts_ = true;
+    }
     uint64_t fused_launches_ = 0;
     float bin_mult_ = 1.0f;
     struct PendingTime {
@@ -2105,6 +2114,7 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
                 by.push_back(v.bytes);
             }
             tick("sort:" + f.agent->d->name, stream_, true);
+            note_sort(l.sort_scratch);
             detail::fgx_check(fgx_agent_sort(ctx_, f.sort_dims, f.min->meta.min, f.min->meta.max, f.min->meta.radius, l.d_count,
                                              l.cap, (const float*)l.vars[ix].cur, (const float*)l.vars[iy].cur,
                                              f.sort_dims == 3 ? (const float*)l.vars[iz].cur : nullptr,
@@ -2129,6 +2139,7 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
                 by.push_back(v.bytes);
             }
             tick("build:" + m.d->name, bst, true);
+            note_sort(m.build_scratch);
             detail::fgx_check(fgx_spatial_build(ctx_, &m.meta, m.d_count, m.cap, (const float*)m.vars[ix].cur,
                                                 (const float*)m.vars[iy].cur,
                                                 m.meta.dims == 3 ? (const float*)m.vars[iz].cur : nullptr,
@@ -2158,6 +2169,7 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
             tick("build:" + m.d->name, bst, true);
             if (m.kind == MsgKind::Bucket) {
                 const int ik = m.var_index("_key");
+                note_sort(m.build_scratch);
                 detail::fgx_check(fgx_bucket_build(ctx_, m.key_min, m.keys, m.d_count, m.cap, (const int32_t*)m.vars[ik].cur,
                                                    (uint32_t)m.vars.size(), src.data(), dst.data(), by.data(), m.pbm,
                                                    m.build_scratch, m.build_scratch_bytes, bst),
@@ -2352,10 +2364,11 @@ inline bool CUDASimulation::step() {
             e.swaps = swaps;
             e.concurrent = cuda_config_.inLayerConcurrency;
             new_layer_images_ = false;
+            first_sorts_ = false;
             detail::fgx_check(fgx_graph_begin(stream_), "graph begin");
             enqueue_step();
             detail::fgx_check(fgx_graph_end(stream_, &e.g), "graph end");
-            e.stale = new_layer_images_;
+            e.stale = new_layer_images_ || first_sorts_;
             e.end = buffer_state();
             set_buffer_state(start);   // capture recorded the work; nothing has run yet
             graphs_.push_back(std::move(e));

@@ -57,7 +57,10 @@ int fgx_device_info(fgx_ctx* ctx, int* cu_count, int* xcd_count, size_t* hbm_byt
  * was zeroed when allocated (fgx_scratch_clear) and is written by nothing but those
  * calls.  The bin sorts then leave their histogram zeroed behind them, and a sort on a
  * buffer that fgx_scratch_clear zeroed, or that the last sort used with the same bin
- * count, skips the clearing launch. */
+ * count, skips the clearing launch.  Under stream capture a sort skips it only when
+ * every earlier sort of the buffer used the same bin count; a buffer whose graphs are
+ * replayed must then keep that one bin count in later sorts too (call fgx_scratch_clear
+ * and capture again after changing it). */
 enum { FGX_OPT_KEEP_SCRATCH_ZERO = 1 };
 int fgx_ctx_set_option(fgx_ctx* ctx, int option, int value);
 

@@ -353,6 +353,54 @@ def test_spatial_build_keep_scratch_zero():
     c.close()
 
 
+def test_spatial_build_captured_on_fresh_scratch():
+    """A build captured as the first sort of a freshly cleared buffer (FGX_OPT_KEEP_SCRATCH_ZERO)
+    carries no clearing launch; every replay, with eager builds of the same bin count on the
+    same buffer between the replays, still equals the oracle."""
+    from flamegpu2_amd import fgx
+    c = fgx.Context(0)
+    c.set_option(fgx.FGX_OPT_KEEP_SCRATCH_ZERO, 1)
+    rng = np.random.default_rng(23)
+    n, width, radius = 60000, 4.0, 0.1
+    m = fgx.spatial_meta(3, [0.0] * 3, [width] * 3, radius)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_SPATIAL_BUILD, n, m.bins)
+    s = scratch(sb)   # cleared by fgx_scratch_clear, no sort yet
+    cases = []
+    for _ in range(2):
+        P = rng.uniform(0, width, size=(3, n)).astype(np.float32)
+        pbm_o, _, outs_o = O.spatial_build(O.spatial_meta(3, [0.0] * 3, [width] * 3, radius), P[0], P[1], P[2])
+        d_in = [dev(P[a]) for a in range(3)]
+        d_out = [empty(n, np.float32) for _ in range(3)]
+        cases.append((d_in, d_out, empty(m.bins + 1, np.uint32), pbm_o, outs_o))
+    st = fgx.stream_create()
+
+    def build(case):
+        d_in, d_out, pbm = case[0], case[1], case[2]
+        fgx.spatial_build(c, m, n, ptr(d_in[0]), ptr(d_in[1]), ptr(d_in[2]), [ptr(t) for t in d_in],
+                          [ptr(t) for t in d_out], [4, 4, 4], ptr(pbm), ptr(s), sb, stream=st)
+
+    def check(case):
+        fgx.stream_synchronize(st)
+        assert np.array_equal(host(case[2], np.uint32, m.bins + 1), case[3])
+        for a in range(3):
+            assert np.array_equal(host(case[1][a], np.float32, n), case[4][a])
+        for t in case[1] + [case[2]]:
+            fgx.fill_u32(ptr(t), 0, t.alloc_bytes // 4)   # so that each check sees fresh output
+        sync()
+
+    g = fgx.graph_capture(st, lambda: build(cases[0]))
+    try:
+        for _ in range(3):
+            fgx.graph_launch(g, st)
+            check(cases[0])
+            build(cases[1])
+            check(cases[1])
+    finally:
+        fgx.graph_destroy(g)
+        fgx.stream_destroy(st)
+    c.close()
+
+
 def test_spatial_build_large_bin_table(ctx):
     # 4.1 M bins (> 1,024 scan tiles): the bin counts go through the block-scanned path
     _build_case(ctx, 3, 1 << 20, 16.0, 0.1, 21)
