@@ -1611,6 +1611,30 @@ __global__ void __launch_bounds__(256) agent_function_kernel(const FnArgs a) {
     run_function_block<F>(a, xcd_block());
 }
 
+// The same kernel built for 8 resident waves per SIMD (at most 64 VGPRs), for neighbour
+// walks: their loops wait on gathers, and a kernel a few registers above 64 keeps only 7.
+template <typename F>
+__global__ void __launch_bounds__(256, 8) agent_function_kernel_w8(const FnArgs a) {
+    run_function_block<F>(a, xcd_block());
+}
+
+// Whether a function's launches take agent_function_kernel_w8: when the plain build needs
+// more than 64 and at most 72 VGPRs (so 7 waves per SIMD) and the 8-wave build spills at
+// most 64 bytes per lane.  Read once from the compiled kernels' attributes.
+template <typename F>
+bool use_w8_kernel() {
+    static const bool use = [] {
+        hipFuncAttributes plain{}, w8{};
+        if (hipFuncGetAttributes(&plain, reinterpret_cast<const void*>(&agent_function_kernel<F>)) != hipSuccess ||
+            hipFuncGetAttributes(&w8, reinterpret_cast<const void*>(&agent_function_kernel_w8<F>)) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        return plain.numRegs > 64 && plain.numRegs <= 72 && w8.localSizeBytes <= 64;
+    }();
+    return use;
+}
+
 // ---------------------------------------------------------------- layer kernel
 // Compile-time registry of a translation unit's agent functions.  FLAMEGPU_AGENT_FUNCTION
 // takes a __COUNTER__ value N and instantiates fn_put<N, F>, whose friend definitions give
@@ -1938,6 +1962,13 @@ void launch_agent_function(const FnArgs& a, uint32_t grid, uint32_t block, uint3
         }
     }
     (void)smem;
+    if constexpr (k == MsgKind::Spatial2D || k == MsgKind::Spatial3D) {
+        if (block <= 256 && use_w8_kernel<F>()) {
+            hipLaunchKernelGGL(agent_function_kernel_w8<F>, dim3(grid), dim3(block), fn_lds_bytes<F>(block),
+                               (hipStream_t)stream, a);
+            return;
+        }
+    }
     hipLaunchKernelGGL(agent_function_kernel<F>, dim3(grid), dim3(block), fn_lds_bytes<F>(block), (hipStream_t)stream,
                        a);
 }
