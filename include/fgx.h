@@ -202,7 +202,7 @@ int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t
  * w2[i], w3[i]} for i < n, a NULL source giving 0 words.  Used for the position and id
  * of each message of a built spatial list, which the neighbour walk then reads with one
  * load per candidate instead of four.  No counterpart in the reference (its iterators
- * read each variable separately, MessageSpatial3DDevice.cuh:574-632).  n = *d_n if d_n
+ * read each variable separately, MessageSpatial3DDevice.cuh:592-632).  n = *d_n if d_n
  * != NULL (<= n_cap), else n_cap; out holds 16 * n_cap bytes, 16-byte aligned. */
 int fgx_pack_records(const uint32_t* d_n, uint32_t n_cap, const void* w0, const void* w1, const void* w2,
                      const void* w3, void* out, void* stream);
