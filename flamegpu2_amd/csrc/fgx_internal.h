@@ -58,6 +58,10 @@ struct VarTable {
     uint32_t bytes[FGX_MAX_VARS];
     const char* in[FGX_MAX_VARS];
     char* out[FGX_MAX_VARS];
+    // optional 16-byte records written beside the variables (fgx_spatial_build_records):
+    // word k of record i is 4-byte variable rec_var[k] of element i, or 0 when it is -1
+    char* rec;
+    int32_t rec_var[4];
 };
 
 int make_var_table(uint32_t nvars, const void* const* in, void* const* out, const uint32_t* bytes, VarTable* t);
@@ -80,6 +84,17 @@ __device__ __forceinline__ void copy_element(const VarTable& t, uint32_t src_i, 
             for (uint32_t w = 0; w < b; ++w) d[w] = s[w];
         }
     }
+}
+
+// Record dst_i of vt.rec from element src_i of the input variables (no-op without records).
+__device__ __forceinline__ void write_record(const VarTable& t, uint32_t src_i, uint32_t dst_i) {
+    if (!t.rec) return;
+    uint4 r;
+    r.x = t.rec_var[0] >= 0 ? reinterpret_cast<const uint32_t*>(t.in[t.rec_var[0]])[src_i] : 0u;
+    r.y = t.rec_var[1] >= 0 ? reinterpret_cast<const uint32_t*>(t.in[t.rec_var[1]])[src_i] : 0u;
+    r.z = t.rec_var[2] >= 0 ? reinterpret_cast<const uint32_t*>(t.in[t.rec_var[2]])[src_i] : 0u;
+    r.w = t.rec_var[3] >= 0 ? reinterpret_cast<const uint32_t*>(t.in[t.rec_var[3]])[src_i] : 0u;
+    reinterpret_cast<uint4*>(t.rec)[dst_i] = r;
 }
 
 // ---------------------------------------------------------------- XCD-contiguous block order

@@ -420,7 +420,10 @@ __global__ void __launch_bounds__(kBlock) k_stable_scatter(const uint32_t* d_n, 
     const uint32_t n = load_count(d_n, n_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
     if (!*unsorted) {
-        for (uint32_t s = xcd_block() * blockDim.x + threadIdx.x; s < n; s += stride) copy_element(vt, s, s);
+        for (uint32_t s = xcd_block() * blockDim.x + threadIdx.x; s < n; s += stride) {
+            copy_element(vt, s, s);
+            write_record(vt, s, s);
+        }
         return;
     }
     for (uint32_t s = xcd_block() * blockDim.x + threadIdx.x; s < n; s += stride) {
@@ -430,11 +433,13 @@ __global__ void __launch_bounds__(kBlock) k_stable_scatter(const uint32_t* d_n, 
         const uint32_t en = start[b + 1];
         if (en - st > kSmallBin) {
             copy_element(vt, sorted[s], s);
+            write_record(vt, sorted[s], s);
             continue;
         }
         uint32_t rank = 0;
         for (uint32_t t = st; t < en; ++t) rank += tmp[t] < e ? 1u : 0u;
         copy_element(vt, e, st + rank);
+        write_record(vt, e, st + rank);
     }
 }
 
@@ -1229,10 +1234,11 @@ int fgx_scratch_clear(void* scratch, size_t bytes, void* stream) {
     return rc;
 }
 
-int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t* d_n, uint32_t n_cap,
-                      const float* x, const float* y, const float* z,
-                      uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
-                      uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream) {
+static int spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t* d_n, uint32_t n_cap,
+                         const float* x, const float* y, const float* z,
+                         uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                         const int32_t* rec_vars, void* records,
+                         uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream) {
     if (!meta || !pbm) return fail(FGX_ERR_ARG, "fgx_spatial_build: NULL meta or pbm");
     if (meta->dims != 2 && meta->dims != 3) return fail(FGX_ERR_ARG, "fgx_spatial_build: dims must be 2 or 3");
     if (n_cap && (!x || !y || (meta->dims == 3 && !z)))
@@ -1240,6 +1246,16 @@ int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t
     VarTable vt;
     int rc = make_var_table(nvars, vars_in, vars_out, var_bytes, &vt);
     if (rc) return rc;
+    if (records) {
+        if (!rec_vars || (reinterpret_cast<uintptr_t>(records) & 15u))
+            return fail(FGX_ERR_ARG, "fgx_spatial_build_records: NULL rec_vars, or records not 16-byte aligned");
+        for (int k = 0; k < 4; ++k) {
+            if (rec_vars[k] >= (int32_t)nvars || (rec_vars[k] >= 0 && var_bytes[rec_vars[k]] != 4))
+                return fail(FGX_ERR_ARG, "fgx_spatial_build_records: a record word names no 4-byte variable");
+            vt.rec_var[k] = rec_vars[k] < 0 ? -1 : rec_vars[k];
+        }
+        vt.rec = static_cast<char*>(records);
+    }
     KeyArgs ka;
     std::memset(&ka, 0, sizeof(ka));
     ka.mode = 0;
@@ -1251,6 +1267,24 @@ int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t
         ka.grid[a] = meta->grid[a] ? meta->grid[a] : 1u;
     }
     return run_bin_sort(ctx, ka, meta->bins, d_n, n_cap, x, y, z, vt, pbm, scratch, scratch_bytes, stream);
+}
+
+int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t* d_n, uint32_t n_cap,
+                      const float* x, const float* y, const float* z,
+                      uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                      uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream) {
+    return spatial_build(ctx, meta, d_n, n_cap, x, y, z, nvars, vars_in, vars_out, var_bytes, nullptr, nullptr, pbm,
+                         scratch, scratch_bytes, stream);
+}
+
+int fgx_spatial_build_records(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t* d_n, uint32_t n_cap,
+                              const float* x, const float* y, const float* z, uint32_t nvars,
+                              const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                              const int32_t* rec_vars, void* records, uint32_t* pbm, void* scratch,
+                              size_t scratch_bytes, void* stream) {
+    if (!records) return fail(FGX_ERR_ARG, "fgx_spatial_build_records: NULL records");
+    return spatial_build(ctx, meta, d_n, n_cap, x, y, z, nvars, vars_in, vars_out, var_bytes, rec_vars, records, pbm,
+                         scratch, scratch_bytes, stream);
 }
 
 int fgx_spatial_build_noalloc(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t* d_n, uint32_t n_cap,

@@ -101,6 +101,8 @@ _SIGS = {
                        _c.c_int32, vp, vp, vp],
     "fgx_add_u32_n": [vp, u32, u32, vp],
     "fgx_pack_records": [vp, u32, vp, vp, vp, vp, vp, vp],
+    "fgx_spatial_build_records": [vp, _c.POINTER(SpatialMeta), vp, u32, vp, vp, vp, u32, _c.POINTER(vp),
+                                  _c.POINTER(vp), _c.POINTER(u32), _c.POINTER(_c.c_int32), vp, vp, vp, sz, vp],
     "fgx_copy_ranges": [u32, _c.POINTER(vp), _c.POINTER(vp), _c.POINTER(u32), vp, u32, u32, vp],
 }
 
@@ -180,6 +182,15 @@ def spatial_build(ctx, meta, n_cap, x, y, z, vars_in, vars_out, var_bytes, pbm, 
     nv = len(vars_in)
     check(lib().fgx_spatial_build(ctx.h, _c.byref(meta), d_n, n_cap, x, y, z, nv, _arr(vp, vars_in),
                                   _arr(vp, vars_out), _arr(u32, var_bytes), pbm, scratch, scratch_len, stream))
+
+
+def spatial_build_records(ctx, meta, n_cap, x, y, z, vars_in, vars_out, var_bytes, rec_vars, records, pbm, scratch,
+                          scratch_len, stream=0, d_n=None):
+    """fgx_spatial_build that also writes 16-byte records of vars rec_vars[0..3] (-1: 0 word) in bin order."""
+    nv = len(vars_in)
+    check(lib().fgx_spatial_build_records(ctx.h, _c.byref(meta), d_n, n_cap, x, y, z, nv, _arr(vp, vars_in),
+                                          _arr(vp, vars_out), _arr(u32, var_bytes), _arr(_c.c_int32, rec_vars),
+                                          records, pbm, scratch, scratch_len, stream))
 
 
 def spatial_build_noalloc(ctx, meta, n_cap, x, y, z, vars_in, vars_out, var_bytes, pbm, stream=0, d_n=None):

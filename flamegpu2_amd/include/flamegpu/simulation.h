@@ -2140,20 +2140,18 @@ inline void CUDASimulation::enqueue_layer(const LayerData& layer) {
             }
             tick("build:" + m.d->name, bst, true);
             note_sort(m.build_scratch);
-            detail::fgx_check(fgx_spatial_build(ctx_, &m.meta, m.d_count, m.cap, (const float*)m.vars[ix].cur,
-                                                (const float*)m.vars[iy].cur,
-                                                m.meta.dims == 3 ? (const float*)m.vars[iz].cur : nullptr,
-                                                (uint32_t)m.vars.size(), src.data(), dst.data(), by.data(), m.pbm,
-                                                m.build_scratch, m.build_scratch_bytes, bst),
+            // the walk's {x, y, z, id} records are written by the build's scatter
+            const int ii = m.var_index("id");
+            const int32_t rec_vars[4] = {ix, iy, m.meta.dims == 3 ? iz : -1,
+                                         ii >= 0 && m.vars[ii].bytes == 4 ? ii : -1};
+            detail::fgx_check(fgx_spatial_build_records(ctx_, &m.meta, m.d_count, m.cap, (const float*)m.vars[ix].cur,
+                                                        (const float*)m.vars[iy].cur,
+                                                        m.meta.dims == 3 ? (const float*)m.vars[iz].cur : nullptr,
+                                                        (uint32_t)m.vars.size(), src.data(), dst.data(), by.data(),
+                                                        rec_vars, m.packed, m.pbm, m.build_scratch,
+                                                        m.build_scratch_bytes, bst),
                               "message index build");
             for (auto& v : m.vars) std::swap(v.cur, v.alt);
-            {
-                const int ii = m.var_index("id");
-                const void* idp = ii >= 0 && m.vars[ii].bytes == 4 ? m.vars[ii].cur : nullptr;
-                detail::fgx_check(fgx_pack_records(m.d_count, m.cap, m.vars[ix].cur, m.vars[iy].cur,
-                                                   m.meta.dims == 3 ? m.vars[iz].cur : nullptr, idp, m.packed, bst),
-                                  "message records");
-            }
             tick("build:" + m.d->name, bst, false);
             m.pbm_dirty = false;
         } else if (f.min && (f.min->kind == MsgKind::Bucket || is_array_kind(f.min->kind)) && f.min->pbm_dirty) {

@@ -198,6 +198,16 @@ int fgx_spatial_build(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t
                       const float* x, const float* y, const float* z,
                       uint32_t nvars, const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
                       uint32_t* pbm, void* scratch, size_t scratch_bytes, void* stream);
+/* As fgx_spatial_build, and also writes the list's 16-byte records in bin order while it
+ * scatters: word k of record i is 4-byte variable rec_vars[k] of the sorted element i, or
+ * 0 when rec_vars[k] is -1 (the records fgx_pack_records would write from vars_out,
+ * without its extra pass).  records holds 16 * n_cap bytes, 16-byte aligned.  No
+ * counterpart in the reference (see fgx_pack_records). */
+int fgx_spatial_build_records(fgx_ctx* ctx, const fgx_spatial_meta* meta, const uint32_t* d_n, uint32_t n_cap,
+                              const float* x, const float* y, const float* z, uint32_t nvars,
+                              const void* const* vars_in, void* const* vars_out, const uint32_t* var_bytes,
+                              const int32_t* rec_vars, void* records, uint32_t* pbm, void* scratch,
+                              size_t scratch_bytes, void* stream);
 /* Packs four 4-byte variables of a list into 16-byte records: out[i] = {w0[i], w1[i],
  * w2[i], w3[i]} for i < n, a NULL source giving 0 words.  Used for the position and id
  * of each message of a built spatial list, which the neighbour walk then reads with one

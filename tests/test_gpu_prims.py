@@ -266,6 +266,43 @@ def test_compact_stable(ctx, n, p, keep):
     assert np.array_equal(host(d_out[2], np.float64, cnt), outs_o[2])
 
 
+@pytest.mark.parametrize("dims,n,width,radius,seed", [(3, 200_000, 10.0, 0.5, 31), (2, 50_000, 30.0, 1.0, 32)])
+def test_spatial_build_records(ctx, dims, n, width, radius, seed):
+    # the build's scatter writes the {x, y, z, id} records of the sorted list: word k of
+    # record i is variable rec_vars[k] of sorted element i (z absent in 2-D: 0 words)
+    from flamegpu2_amd import fgx
+    rng = np.random.default_rng(seed)
+    P = rng.uniform(0, width, size=(dims, n)).astype(np.float32)
+    ids = (np.arange(n, dtype=np.uint32) * 5 + 1).astype(np.uint32)
+    mn, mx = [0.0] * dims, [float(width)] * dims
+    pbm_o, _, outs_o = O.spatial_build(O.spatial_meta(dims, mn, mx, radius), P[0], P[1], P[2] if dims == 3 else None,
+                                       extra=[ids])
+    m = fgx.spatial_meta(dims, mn, mx, radius)
+    d_in = [dev(P[a]) for a in range(dims)] + [dev(ids)]
+    d_out = [empty(n, np.float32) for _ in range(dims)] + [empty(n, np.uint32)]
+    rec = empty(4 * n, np.uint32)
+    pbm = empty(m.bins + 1, np.uint32)
+    sb = fgx.scratch_bytes(fgx.FGX_OP_SPATIAL_BUILD, n, m.bins)
+    s = scratch(sb)
+    rec_vars = [0, 1, 2, 3] if dims == 3 else [0, 1, -1, 2]
+    fgx.spatial_build_records(ctx, m, n, ptr(d_in[0]), ptr(d_in[1]), ptr(d_in[2]) if dims == 3 else None,
+                              [ptr(t) for t in d_in], [ptr(t) for t in d_out], [4] * (dims + 1), rec_vars, ptr(rec),
+                              ptr(pbm), ptr(s), sb)
+    sync()
+    assert np.array_equal(host(pbm, np.uint32, m.bins + 1), pbm_o)
+    got = host(rec, np.uint32, 4 * n).reshape(n, 4)
+    for a in range(dims):
+        assert np.array_equal(host(d_out[a], np.float32, n), outs_o[a])
+        assert np.array_equal(got[:, a].view(np.float32), outs_o[a])
+    if dims == 2:
+        assert not got[:, 2].any()
+    assert np.array_equal(got[:, 3], outs_o[dims])
+    with pytest.raises(Exception):   # a record word must name a 4-byte variable
+        fgx.spatial_build_records(ctx, m, n, ptr(d_in[0]), ptr(d_in[1]), ptr(d_in[2]) if dims == 3 else None,
+                                  [ptr(t) for t in d_in], [ptr(t) for t in d_out], [4] * (dims + 1), [0, 1, 7, -1],
+                                  ptr(rec), ptr(pbm), ptr(s), sb)
+
+
 @pytest.mark.parametrize("use_dn", [False, True])
 def test_pack_records(ctx, use_dn):
     # the neighbour walk's {x, y, z, id} records: word k of record i is source k's element i
